@@ -1,0 +1,125 @@
+/* libvxl: MI355X (gfx950) periodic HEX8 effective-stiffness solve path.
+ *
+ * C ABI.  Plain pointers and sizes only; device pointers are HIP device
+ * addresses owned by the caller unless stated otherwise; `stream` arguments
+ * are hipStream_t values passed as void*.  Every function returns VXL_OK (0)
+ * or a negative VXL_ERR_* code; vxl_last_error() then describes the failure
+ * (thread-local).  A handle is bound to the stream it was created with and
+ * is not re-entrant: concurrent solves use separate handles (the reference's
+ * process-level configuration, src/voxel_lmfem/solver.py:13-15).
+ *
+ * Layouts.  Grid (nz, ny, nx), node id (z*ny + y)*nx + x, periodic.
+ * Displacement-like fields are component planes: field[r][c][node] with r the
+ * right-hand side (load case) and c in {x, y, z}.  Material coefficients are
+ * the 21 upper-triangle entries (i <= j, row major) of the engineering-Voigt
+ * 6x6 stiffness in order 11,22,33,23,13,12 (the reference's PAIRS order,
+ * src/voxel_lmfem/_core/materials/material_field.py:14).  Macro strains are
+ * engineering strains in the same order.
+ */
+#ifndef VXL_H
+#define VXL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VXL_OK = 0,
+  VXL_ERR_ARG = -1,      /* invalid argument: the reference raises ValueError */
+  VXL_ERR_HIP = -2,      /* HIP runtime failure: the reference raises RuntimeError */
+  VXL_ERR_CONVERGENCE = -3 /* true residual above tolerance (solver.py:78-79) */
+};
+
+enum { VXL_INDEX_U8 = 1, VXL_INDEX_U16 = 2, VXL_INDEX_I32 = 4 };
+enum { VXL_F32 = 0, VXL_F64 = 1 };
+/* Solve modes.  FP64: FP64 operator, FP64 FFT preconditioner, FP64 CG.
+ * MIXED: FP32 operator, FP32 FFT preconditioner and FP32 CG with FP64
+ * reliable updates (true residual f - K u, dots and stress in FP64). */
+enum { VXL_SOLVE_FP64 = 0, VXL_SOLVE_MIXED = 1 };
+
+typedef struct vxl_cell vxl_cell;
+typedef struct vxl_precond vxl_precond;
+typedef struct vxl_solver vxl_solver;
+
+/* Thread-local text of the last failure ("" if none). */
+const char* vxl_last_error(void);
+/* Library ABI version (major*10000 + minor*100 + patch). */
+int vxl_version(void);
+
+/* Cell: grid, voxel size and material field.
+ * Replaces the operator construction of the reference's hybrid FI storage:
+ *   HybridFIOperator.__init__  src/voxel_lmfem/_core/materials/hybrid_adapter.py:45-68
+ *   build_hybrid_material      src/voxel_lmfem/_core/materials/hybrid_material.py:14-70
+ * index_dev: per-voxel row index (uint8/uint16/int32 per index_kind), device,
+ *   caller-owned, must outlive the cell.
+ * c21_host: rows x 21 FP64 coefficients (host); copied to device in FP64 and
+ *   FP32.
+ * scale_dev: optional per-voxel FP32 factor multiplying the row (NULL = 1),
+ *   device, caller-owned.  The reference has no continuous-modulus field; this
+ *   carries density (SIMP) cells.
+ * h = voxel size (hx, hy, hz) = domain / (nx, ny, nz)
+ *   (hybrid_adapter.py:147). */
+int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy, double hz,
+                    int index_kind, const void* index_dev, int rows, const double* c21_host,
+                    const float* scale_dev, void* stream);
+int vxl_cell_destroy(vxl_cell* cell);
+
+/* out = K u for nrhs fields (precision VXL_F32/VXL_F64), component-plane
+ * layout.  Replaces the pybind `apply`:
+ *   operator_kernels.cu:88-116 (split background + contrast FP64 K.u) and
+ *   hybrid_kernels.cu:117-127 (pool-indexed FP64 K.u). */
+int vxl_apply(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev);
+
+/* f = -V sum_e B0^T C_e E for nload macro strains (host, nload x 6), FP64.
+ * Replaces `rhs`  hybrid_kernels.cu:128-137 (rhs_kernel :49-68). */
+int vxl_rhs(vxl_cell* cell, int nload, const double* macro_host, double* f_dev);
+
+/* out_host (nload x 6) = <C (E + B0 u)> over all elements, FP64.
+ * Replaces `stress`  hybrid_kernels.cu:138-147 (stress_kernel :69-106). */
+int vxl_avg_stress(vxl_cell* cell, int nload, const double* u_dev, const double* macro_host,
+                   double* out_host);
+
+/* Subtract the per-component mean of nrhs FP64 fields in place.
+ * Replaces HybridFIOperator.project_zero_mean  hybrid_adapter.py:78-80. */
+int vxl_project_zero_mean(vxl_cell* cell, int nrhs, double* u_dev);
+
+/* FFT Green preconditioner of the isotropic reference medium (lam, mu):
+ * z = IFFT(K0(k)^-1 FFT(r)), k = 0 mode removed, hand-written LDS pencil FFTs.
+ * Replaces enable_fi_reference/fi_coeff (formula_adapter.py:40-68,
+ * formula_kernels.cu:104-137, 199-208), contest prepare/spectral
+ * (contest_kernels.cu:46-94) and NativeFFT (native_fft.cu:11-58).
+ * precision: VXL_F32 or VXL_F64 (arithmetic and storage of the spectra). */
+int vxl_precond_create(vxl_precond** out, vxl_cell* cell, int precision, double lam, double mu,
+                       int nrhs_max);
+/* r_dev, z_dev: nrhs fields in the precond's precision. */
+int vxl_precond_apply(vxl_precond* pc, int nrhs, const void* r_dev, void* z_dev);
+int vxl_precond_destroy(vxl_precond* pc);
+
+/* Six-load (or nrhs-load) C_eff solver: RHS, block PCG, certification,
+ * stress recovery.  Replaces benchmark.workload/solve
+ * (src/voxel_lmfem/_core/runtime/benchmark.py:77-112) driving the rewritten
+ * solve_pcg_batched (femlib/batched/pcg.py:20-252, chain_adapter.py:80-142).
+ * Workspaces are allocated here, not in vxl_solver_run. */
+int vxl_solver_create(vxl_solver** out, vxl_cell* cell, int mode, int nrhs, double lam, double mu);
+/* macro_host: nrhs x 6 engineering strains.
+ * ch_host: 6 x nrhs, column j = <sigma> of load j (CH[:, j], benchmark.py:98).
+ * iterations/restarts/residuals: nrhs each (host); residuals are the
+ *   certified FP64 true relative residuals ||f - K u||/||f||.
+ * u_dev_out: optional (NULL) device buffer nrhs x 3 x N FP64 receiving the
+ *   zero-mean fluctuation fields.
+ * Returns VXL_ERR_CONVERGENCE if a load does not certify within max_iter. */
+int vxl_solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter,
+                   double* ch_host, int* iterations, int* restarts, double* residuals,
+                   double* u_dev_out);
+/* Per-phase device time of the last run in milliseconds:
+ * [0] setup+rhs, [1] iterations, [2] certification+stress. */
+int vxl_solver_timings(vxl_solver* s, double* ms3);
+int vxl_solver_destroy(vxl_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VXL_H */

@@ -1,0 +1,152 @@
+"""GPU parity of the libvxl kernels against the CPU oracle (FP64 bitwise-close,
+FP32 to its rounding level): stiffness action, RHS, mean stress, FFT Green
+preconditioner."""
+import numpy as np
+import pytest
+
+from oracle import fem, materials as om, spectral
+from voxel_lmfem import materials as pm
+
+pytestmark = pytest.mark.gpu
+
+
+def make_cell(shape, seed, *, labels=3, local=True, big_table=False):
+    rng = np.random.default_rng(seed)
+    phase = rng.integers(0, labels, size=shape).astype(np.int32)
+    table = {}
+    for k in range(labels):
+        X = rng.normal(size=(6, 6))
+        table[k] = dict(kind="anisotropic", C=(X @ X.T + 6 * np.eye(6)).tolist())
+    orientation = None
+    if local:
+        table[labels - 1]["frame"] = "local"
+        a, b = rng.normal(size=shape + (3,)), rng.normal(size=shape + (3,))
+        orientation = dict(axis1=a, axis2=b)
+    if big_table:  # more than 65536 rows -> int32 index
+        pass
+    h = rng.uniform(.3, 1.5, size=3)
+    field = pm.build_field(phase, table, orientation, deduplicate=False)
+    full = om.unpack21(field.rows[field.index.astype(np.int64)])
+    return field, full, h
+
+
+def planes(a):
+    return np.ascontiguousarray(a.T)  # (N,3) -> (3,N)
+
+
+SHAPES = [(2, 2, 2), (4, 4, 4), (3, 4, 5), (5, 7, 6), (24, 20, 40), (9, 33, 17)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_apply_matches_oracle(torch_gpu, shape):
+    torch = torch_gpu
+    from voxel_lmfem.gpu import Cell
+    print(f"case apply shape={shape}", flush=True)
+    field, full, h = make_cell(shape, seed=sum(shape))
+    cell = Cell(field, shape, h)
+    op = fem.MatrixFree(full, shape, h)
+    rng = np.random.default_rng(1)
+    n = int(np.prod(shape))
+    U = rng.normal(size=(2, n, 3))
+    ref = np.stack([op.apply(u) for u in U])
+    u64 = torch.tensor(np.stack([planes(u) for u in U]), device=cell.device)
+    out64 = cell.apply_planes(u64).cpu().numpy()
+    scale = np.abs(ref).max()
+    err64 = np.abs(out64 - np.stack([planes(r) for r in ref])).max() / scale
+    out32 = cell.apply_planes(u64.float()).double().cpu().numpy()
+    err32 = np.abs(out32 - np.stack([planes(r) for r in ref])).max() / scale
+    print(f"  fp64 err {err64:.2e} fp32 err {err32:.2e}", flush=True)
+    assert err64 < 1e-13
+    assert err32 < 2e-6
+    cell.close()
+
+
+def test_apply_symmetry_rigid_and_index_kinds(torch_gpu):
+    torch = torch_gpu
+    from voxel_lmfem.gpu import Cell
+    shape = (6, 5, 7)
+    print(f"case symmetry shape={shape}", flush=True)
+    field, full, h = make_cell(shape, seed=11)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(2)
+    u = torch.tensor(rng.normal(size=(1, 3, n)), device="cuda")
+    v = torch.tensor(rng.normal(size=(1, 3, n)), device="cuda")
+    outs = []
+    for dtype in (np.uint8, np.uint16, np.int32):
+        f = pm.MaterialField(field.rows, field.index.astype(dtype), field.labels, field.tensors, field.local)
+        cell = Cell(f, shape, h)
+        Ku, Kv = cell.apply_planes(u), cell.apply_planes(v)
+        outs.append(Ku.cpu().numpy())
+        a, b = float((v * Ku).sum()), float((u * Kv).sum())
+        assert abs(a - b) < 1e-12 * max(abs(a), 1)
+        rigid = torch.zeros(1, 3, n, dtype=torch.float64, device="cuda")
+        rigid[0, 0], rigid[0, 1], rigid[0, 2] = 1.0, -2.0, 0.5
+        assert float(cell.apply_planes(rigid).abs().max()) < 1e-12 * float(Ku.abs().max())
+        cell.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_apply_scale_field(torch_gpu):
+    torch = torch_gpu
+    from voxel_lmfem.gpu import Cell
+    shape = (5, 6, 7)
+    print(f"case scale shape={shape}", flush=True)
+    field, full, h = make_cell(shape, seed=5, local=False)
+    rng = np.random.default_rng(3)
+    s = rng.uniform(.1, 2., size=shape).astype(np.float32)
+    cell = Cell(field, shape, h, scale=s)
+    op = fem.MatrixFree(full * s.reshape(-1).astype(np.float64)[:, None, None], shape, h)
+    u = rng.normal(size=(int(np.prod(shape)), 3))
+    ref = planes(op.apply(u))
+    out = cell.apply_planes(torch.tensor(planes(u)[None], device="cuda"))[0].cpu().numpy()
+    assert np.abs(out - ref).max() < 1e-13 * np.abs(ref).max()
+    cell.close()
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (3, 5, 7), (24, 20, 40)])
+def test_rhs_and_stress_match_oracle(torch_gpu, shape):
+    torch = torch_gpu
+    from voxel_lmfem.gpu import Cell
+    print(f"case rhs/stress shape={shape}", flush=True)
+    field, full, h = make_cell(shape, seed=7)
+    cell = Cell(field, shape, h)
+    op = fem.MatrixFree(full, shape, h)
+    macro = np.eye(6) + 0.1 * np.random.default_rng(0).normal(size=(6, 6))
+    f = cell.rhs_planes(macro).cpu().numpy()
+    for j in range(6):
+        ref = planes(op.rhs(macro[j]))
+        assert np.abs(f[j] - ref).max() < 1e-13 * np.abs(ref).max()
+    n = int(np.prod(shape))
+    U = np.random.default_rng(9).normal(size=(6, n, 3))
+    u = torch.tensor(np.stack([planes(x) for x in U]), device="cuda")
+    s = cell.avg_stress_planes(u, macro)
+    for j in range(6):
+        ref = op.avg_stress(U[j], macro[j])
+        assert np.abs(s[j] - ref).max() < 1e-12 * np.abs(ref).max()
+    cell.close()
+
+
+PC_SHAPES = [(4, 4, 4), (8, 8, 8), (6, 5, 4), (3, 7, 5), (16, 12, 10), (17, 2, 9), (32, 48, 20)]
+
+
+@pytest.mark.parametrize("shape", PC_SHAPES)
+def test_precond_matches_oracle(torch_gpu, shape):
+    torch = torch_gpu
+    from voxel_lmfem.gpu import Cell, Preconditioner
+    print(f"case precond shape={shape}", flush=True)
+    field, full, h = make_cell(shape, seed=3, local=False)
+    cell = Cell(field, shape, h)
+    lam, mu = 2.5, 1.3
+    n = int(np.prod(shape))
+    R = np.random.default_rng(6).normal(size=(3, n, 3))
+    R -= R.mean(axis=1, keepdims=True)
+    ref = np.stack([planes(spectral.green_apply(r, shape, h, lam, mu)) for r in R])
+    r = torch.tensor(np.stack([planes(x) for x in R]), device="cuda")
+    for precision, tol in (("float64", 1e-11), ("float32", 5e-6)):
+        pc = Preconditioner(cell, lam, mu, precision=precision, nrhs_max=3)
+        z = pc.apply_planes(r.to(pc.dtype)).double().cpu().numpy()
+        err = np.abs(z - ref).max() / np.abs(ref).max()
+        print(f"  {precision} err {err:.2e}", flush=True)
+        assert err < tol
+        pc.close()
+    cell.close()

@@ -1,0 +1,99 @@
+"""End-to-end GPU C_eff against the oracle, the reference's KATs and its
+archived records."""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from oracle import homogenize as oracle
+from test_oracle import gpu_fixture_cell, iso, laminate_expected
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_reference_gpu_fixture(torch_gpu, precision):
+    """tests/test_gpu.py:10-29 of the reference (4^3 rotated anisotropic)."""
+    from voxel_lmfem import homogenize
+    print(f"case gpu-fixture {precision}", flush=True)
+    phase, table, domain, rot = gpu_fixture_cell()
+    ref = oracle.homogenize_direct(phase, table, domain, orientation=rot)
+    out = homogenize(phase, table, domain, orientation=rot, backend="cuda", warmup=True, precision=precision)
+    assert rel(out["CH"], ref["CH"]) < 1e-8
+    assert out["max_true_residual"] <= 1e-8
+    assert set(out) >= {"seconds", "CH", "calls", "stages", "field_statistics", "peak_allocated_bytes",
+                        "peak_reserved_bytes", "backend", "storage", "operator", "scope", "max_true_residual",
+                        "elapsed_seconds", "voigt_order"}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_kats(torch_gpu, precision):
+    from voxel_lmfem import homogenize
+    print(f"case kats {precision}", flush=True)
+    r = homogenize(np.full((2, 3, 4), 17, dtype=np.int32), {17: dict(kind="isotropic", E=3500., nu=.35)},
+                   [2., 3., 5.], precision=precision)
+    np.testing.assert_allclose(r["CH"], iso(3500., .35), rtol=2e-13, atol=2e-10)
+    phase = np.zeros((4, 2, 3), dtype=np.uint8)
+    phase[:2] = 1
+    specs = {0: dict(kind="isotropic", E=3500., nu=.35), 1: dict(kind="isotropic", E=70000., nu=.2)}
+    r = homogenize(phase, specs, [1., 1., 1.], precision=precision)
+    np.testing.assert_allclose(r["CH"], laminate_expected([iso(3500., .35), iso(70000., .2)]), rtol=1e-8, atol=1e-5)
+    R = np.array([[0., -1., 0.], [1., 0., 0.], [0., 0., 1.]])
+    r = homogenize(np.zeros((2, 2, 2), dtype=np.uint8),
+                   {0: dict(kind="anisotropic", C=np.diag([10., 20., 30., 4., 5., 6.]), frame="local")},
+                   [1., 2., 3.], orientation=R, precision=precision)
+    np.testing.assert_allclose(r["CH"], np.diag([20., 10., 30., 5., 4., 6.]), rtol=2e-13, atol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_archived_woven_8(torch_gpu, precision):
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import reference_weave
+    print(f"case woven8 {precision}", flush=True)
+    rec = json.loads((GOLDEN / "woven_records.json").read_text())["8_fp64_r0"]
+    (phase, mats, dom, ori), meta = reference_weave(8)
+    assert meta["input_hash"] == rec["input_hash"]
+    out = homogenize(phase, mats, dom, orientation=ori, precision=precision)
+    assert rel(out["CH"], rec["CH"]) < 1e-8
+    ref = out["operator"]["reference_medium"]
+    assert (ref["lam"], ref["mu"]) == (rec["reference_lam"], rec["reference_mu"])
+    if precision == "fp64":
+        its = [c["iterations"] for c in out["calls"]]
+        print("  iterations", its, "reference", rec["iterations"], flush=True)
+        assert all(abs(a - b) <= 1 for a, b in zip(its, rec["iterations"]))
+
+
+def test_spheres_32_mixed_vs_oracle(torch_gpu):
+    """Config 1 (32^3 random spheres, vf 0.30): mixed GPU vs FP64 CPU oracle."""
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import periodic_spheres
+    print("case spheres32", flush=True)
+    (phase, mats, dom, ori), meta = periodic_spheres(32, seed=1)
+    ref = oracle.homogenize_matfree(phase, mats, dom)
+    for precision in ("fp64", "mixed"):
+        out = homogenize(phase, mats, dom, precision=precision)
+        err = rel(out["CH"], ref["CH"])
+        print(f"  {precision} rel err {err:.2e} its {[c['iterations'] for c in out['calls']]}", flush=True)
+        assert err < 1e-7
+        assert out["max_true_residual"] <= 1e-8
+
+
+@pytest.mark.parametrize("grid", [64])
+def test_archived_woven_certify(torch_gpu, grid):
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import reference_weave
+    print(f"case woven{grid} certify", flush=True)
+    rec = json.loads((GOLDEN / "woven_records.json").read_text())[f"certify_{grid}"]
+    (phase, mats, dom, ori), meta = reference_weave(grid)
+    assert meta["input_hash"] == rec["input_hash"]
+    out = homogenize(phase, mats, dom, orientation=ori, precision="mixed")
+    err = rel(out["CH"], rec["CH"])
+    print(f"  rel err {err:.2e} its {[c['iterations'] for c in out['calls']]} "
+          f"seconds {out['seconds']:.3f}", flush=True)
+    assert err < 1e-7

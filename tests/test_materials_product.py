@@ -1,0 +1,83 @@
+"""Product host bookkeeping (labels, frames, rotated rows, reference medium)
+against the oracle: integer maps and frames bit-exact."""
+import numpy as np
+import pytest
+
+from oracle import materials as om, homogenize as oh
+from voxel_lmfem import materials as pm, inputs, reference_medium
+from voxel_lmfem.cells import reference_weave, periodic_spheres
+from test_oracle import gpu_fixture_cell
+
+
+def test_phase_table_bit_exact():
+    phase, table, domain, rot = gpu_fixture_cell()
+    a = pm.phase_table(table)
+    b = om.phase_table(table)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_frames_bit_exact_both_representations():
+    (phase, mats, dom, ori), _ = reference_weave(8)
+    sel = np.flatnonzero(phase.ravel() == 1)
+    assert np.array_equal(pm.frames(ori, phase.shape, sel), om.orientation_values(ori, phase.shape, sel))
+    _, _, _, rot = gpu_fixture_cell()
+    sel = np.arange(64)
+    assert np.array_equal(pm.frames(rot, (4, 4, 4), sel), om.orientation_values(rot, (4, 4, 4), sel))
+
+
+@pytest.mark.parametrize("dedup", [True, False])
+def test_field_rows_equal_reference_pool(dedup):
+    (phase, mats, dom, ori), _ = reference_weave(8)
+    f = pm.build_field(phase, mats, ori, deduplicate=dedup)
+    pool, index = om.hybrid_pool(phase, mats, ori)
+    # every voxel sees bit-identical coefficients
+    assert np.array_equal(f.rows[f.index.astype(np.int64)], pool[index])
+    if not dedup:
+        assert np.array_equal(f.index.astype(np.int64), index)
+    else:
+        assert len(f.rows) < len(pool)
+
+
+def test_index_dtype_follows_row_count():
+    phase, mats, dom, rot = gpu_fixture_cell()
+    f = pm.build_field(phase, mats, rot)
+    assert f.index.dtype == np.uint8 and f.index_kind == 1
+    big = np.arange(300 * 8).reshape(300, 2, 4) % 300
+    table = {k: dict(kind="isotropic", E=1. + k, nu=.3) for k in range(300)}
+    f = pm.build_field(big, table)
+    assert f.index.dtype == np.uint16 and f.index_kind == 2
+
+
+def test_reference_medium_matches_oracle():
+    (phase, mats, dom, ori), _ = reference_weave(8)
+    f = pm.build_field(phase, mats, ori)
+    lam, mu, meta = reference_medium.select(f, phase, mats)
+    assert (lam, mu) == oh.reference_lame(phase, mats, ori)
+    (phase, mats, dom, ori), _ = periodic_spheres(16, seed=4)
+    f = pm.build_field(phase, mats)
+    lam, mu, _ = reference_medium.select(f, phase, mats)
+    assert (lam, mu) == oh.reference_lame(phase, mats, None)
+
+
+@pytest.mark.parametrize("phase,materials,domain,orientation,message", [
+    (np.zeros((2, 2, 2)), {0: dict(kind="isotropic", E=1., nu=.2)}, [1, 1, 1], None, "integer"),
+    (np.ones((2, 2, 2), dtype=int), {0: dict(kind="isotropic", E=1., nu=.2)}, [1, 1, 1], None, "Undefined"),
+    (np.zeros((2, 2, 2), dtype=int), {0: dict(kind="isotropic", E=1., nu=.2)}, [1, 0, 1], None, "positive"),
+    (np.zeros((2, 2, 2), dtype=int), {0: dict(kind="anisotropic", C=-np.eye(6))}, [1, 1, 1], None, "positive definite"),
+    (np.zeros((2, 2, 2), dtype=int), {0: dict(kind="anisotropic", C=np.eye(6), frame="local")}, [1, 1, 1], None,
+     "require orientation"),
+    (np.zeros((2, 2, 2), dtype=int), {0: dict(kind="anisotropic", C=np.eye(6), frame="local")}, [1, 1, 1],
+     np.diag([1, 1, -1]), "right-handed"),
+])
+def test_homogenize_rejects_invalid_inputs_before_gpu(phase, materials, domain, orientation, message):
+    from voxel_lmfem import homogenize
+    with pytest.raises(ValueError, match=message):
+        homogenize(phase, materials, domain, orientation=orientation, backend="gpu")
+
+
+def test_backend_names():
+    from voxel_lmfem import homogenize
+    with pytest.raises(ValueError, match="backend"):
+        homogenize(np.zeros((2, 2, 2), dtype=int), {0: dict(kind="isotropic", E=1., nu=.2)}, [1, 1, 1],
+                   backend="tpu")

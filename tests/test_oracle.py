@@ -1,0 +1,287 @@
+"""Pins the CPU oracle against the reference's KATs, archived records and the
+mathematics (independent formulations, invariants, closed forms)."""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from oracle import fem, homogenize as H, materials as mat, pcg, reference_medium as rm, spectral
+
+
+def iso(E, nu):
+    mu = E / (2 * (1 + nu))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    C = np.zeros((6, 6))
+    C[:3, :3] = lam
+    C[:3, :3] += 2 * mu * np.eye(3)
+    C[3:, 3:] = mu * np.eye(3)
+    return C
+
+
+def random_spd(rng, n):
+    X = rng.normal(size=(n, 6, 6))
+    return np.einsum("nij,nkj->nik", X, X) + np.eye(6)
+
+
+# ---- materials ---------------------------------------------------------------
+def test_isotropic_closed_form():
+    # tests/test_import.py:35
+    assert mat.material_stiffness(dict(kind="isotropic", E=3500., nu=.35))[0, 0] == pytest.approx(
+        3500 * .65 / (1.35 * .30))
+
+
+def test_orthotropic_reduces_to_isotropic():
+    E, nu = 7., .25
+    G = E / (2 * (1 + nu))
+    C = mat.material_stiffness(dict(kind="orthotropic", E1=E, E2=E, E3=E, nu12=nu, nu13=nu, nu23=nu,
+                                    G12=G, G13=G, G23=G))
+    np.testing.assert_allclose(C, mat.isotropic_to_voigt(E, nu), rtol=1e-13, atol=1e-13)
+
+
+def test_bond_equals_fourth_order_rotation():
+    rng = np.random.default_rng(5)
+    C = random_spd(rng, 1)[0]
+    for _ in range(5):
+        R, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        if np.linalg.det(R) < 0:
+            R[:, 0] *= -1
+        np.testing.assert_allclose(mat.unpack21(mat.rotate_packed(C, R))[0], mat.fourth_order_rotate(C, R),
+                                   rtol=1e-13, atol=1e-12)
+
+
+def test_rotation_permutes_diagonal_and_keeps_isotropy():
+    R = np.array([[0., -1., 0.], [1., 0., 0.], [0., 0., 1.]])
+    np.testing.assert_allclose(mat.fourth_order_rotate(np.diag([10., 20., 30., 4., 5., 6.]), R),
+                               np.diag([20., 10., 30., 5., 4., 6.]), atol=1e-12)
+    rng = np.random.default_rng(1)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    Q[:, 0] *= np.sign(np.linalg.det(Q))
+    np.testing.assert_allclose(mat.fourth_order_rotate(iso(3., .3), Q), iso(3., .3), atol=1e-12)
+
+
+def test_axis_pair_frames_are_rotations():
+    rng = np.random.default_rng(2)
+    a, b = rng.normal(size=(10, 3)), rng.normal(size=(10, 3))
+    R = mat.orientation_values(dict(axis1=a, axis2=b), (10,), np.arange(10))
+    np.testing.assert_allclose(R.swapaxes(1, 2) @ R, np.broadcast_to(np.eye(3), R.shape), atol=1e-14)
+    np.testing.assert_allclose(np.linalg.det(R), 1, atol=1e-14)
+    np.testing.assert_allclose(R[:, :, 0], a / np.linalg.norm(a, axis=1)[:, None], atol=1e-15)
+
+
+# ---- FEM ---------------------------------------------------------------------
+@pytest.mark.parametrize("shape", [(3, 4, 5), (2, 2, 2), (4, 3, 2)])
+def test_matrix_free_equals_assembled(shape):
+    rng = np.random.default_rng(sum(shape))
+    h = np.array([.3, .5, .7])
+    n = int(np.prod(shape))
+    full = random_spd(rng, n)
+    K = fem.assemble_matrix(full, shape, h)
+    op = fem.MatrixFree(full, shape, h)
+    u = rng.normal(size=(n, 3))
+    Ku = K @ u.ravel()
+    np.testing.assert_allclose(op.apply(u).ravel(), Ku, atol=1e-13 * np.abs(Ku).max())
+    assert abs(K - K.T).max() <= 1e-13 * abs(K).max()
+    assert np.abs(op.apply(np.ones((n, 3)) * [1., -2., 3.])).max() < 1e-12 * np.abs(Ku).max()
+    v = rng.normal(size=(n, 3))
+    v -= v.mean(0)
+    assert float(np.vdot(v, op.apply(v))) > 0
+
+
+def test_uniform_stencil_equals_assembled():
+    shape, h = (4, 5, 6), np.array([.2, .3, .45])
+    C = mat.material_stiffness(dict(kind="orthotropic", E1=13., E2=9., E3=7., nu12=.2, nu13=.25, nu23=.3,
+                                    G12=3., G13=2.5, G23=2.))
+    n = int(np.prod(shape))
+    u = np.random.default_rng(0).normal(size=(n, 3))
+    a = fem.apply_uniform_stencil(u, shape, fem.uniform_stencil(C, h))
+    b = fem.MatrixFree(np.broadcast_to(C, (n, 6, 6)).copy(), shape, h).apply(u)
+    np.testing.assert_allclose(a, b, atol=1e-13 * np.abs(b).max())
+
+
+def test_uniform_cell_rhs_zero_and_stress_is_C():
+    shape, h = (3, 3, 4), np.array([.5, 1., 2.])
+    C = iso(7., .3)
+    n = int(np.prod(shape))
+    op = fem.MatrixFree(np.broadcast_to(C, (n, 6, 6)).copy(), shape, h)
+    for j in range(6):
+        E = np.eye(6)[j]
+        assert np.abs(op.rhs(E)).max() < 1e-13
+        np.testing.assert_allclose(op.avg_stress(np.zeros((n, 3)), E), C @ E, atol=1e-13)
+
+
+# ---- KATs of the reference CPU backend (tests/test_solver.py) -------------------
+def test_homogeneous_cell_kat():
+    r = H.homogenize_direct(np.full((2, 3, 4), 17, dtype=np.int32), {17: dict(kind="isotropic", E=3500., nu=.35)},
+                            [2., 3., 5.])
+    np.testing.assert_allclose(r["CH"], iso(3500., .35), rtol=2e-13, atol=2e-10)
+
+
+def laminate_expected(matrices):
+    a, b = [0, 1, 5], [2, 3, 4]
+    A, B, D = [np.zeros((3, 3)) for _ in range(3)]
+    for C in matrices:
+        inv = np.linalg.inv(C[np.ix_(b, b)])
+        ba = C[np.ix_(b, a)]
+        A += .5 * inv
+        B += .5 * inv @ ba
+        D += .5 * (C[np.ix_(a, a)] - C[np.ix_(a, b)] @ inv @ ba)
+    Hm = np.linalg.inv(A)
+    out = np.zeros((6, 6))
+    out[np.ix_(b, b)] = Hm
+    out[np.ix_(b, a)] = Hm @ B
+    out[np.ix_(a, b)] = B.T @ Hm
+    out[np.ix_(a, a)] = D + B.T @ Hm @ B
+    return out
+
+
+def test_laminate_kat_direct_and_matrix_free():
+    phase = np.zeros((4, 2, 3), dtype=np.uint8)
+    phase[:2] = 1
+    specs = {0: dict(kind="isotropic", E=3500., nu=.35), 1: dict(kind="isotropic", E=70000., nu=.2)}
+    expected = laminate_expected([iso(3500., .35), iso(70000., .2)])
+    np.testing.assert_allclose(H.homogenize_direct(phase, specs, [1., 1., 1.])["CH"], expected,
+                               rtol=2e-12, atol=2e-9)
+    np.testing.assert_allclose(H.homogenize_matfree(phase, specs, [1., 1., 1.])["CH"], expected,
+                               rtol=1e-8, atol=1e-5)
+
+
+def test_rotated_anisotropic_kat():
+    R = np.array([[0., -1., 0.], [1., 0., 0.], [0., 0., 1.]])
+    r = H.homogenize_direct(np.zeros((2, 2, 2), dtype=np.uint8),
+                            {0: dict(kind="anisotropic", C=np.diag([10., 20., 30., 4., 5., 6.]), frame="local")},
+                            [1., 2., 3.], orientation=R)
+    np.testing.assert_allclose(r["CH"], np.diag([20., 10., 30., 5., 4., 6.]), rtol=2e-13, atol=1e-12)
+
+
+@pytest.mark.parametrize("phase,materials,domain,orientation,message", [
+    (np.zeros((2, 2, 2)), {0: dict(kind="isotropic", E=1., nu=.2)}, [1, 1, 1], None, "integer"),
+    (np.ones((2, 2, 2), dtype=int), {0: dict(kind="isotropic", E=1., nu=.2)}, [1, 1, 1], None, "Undefined"),
+    (np.zeros((2, 2, 2), dtype=int), {0: dict(kind="isotropic", E=1., nu=.2)}, [1, 0, 1], None, "positive"),
+    (np.zeros((2, 2, 2), dtype=int), {0: dict(kind="anisotropic", C=-np.eye(6))}, [1, 1, 1], None, "positive definite"),
+    (np.zeros((2, 2, 2), dtype=int), {0: dict(kind="anisotropic", C=np.eye(6), frame="local")}, [1, 1, 1], None,
+     "require orientation"),
+    (np.zeros((2, 2, 2), dtype=int), {0: dict(kind="anisotropic", C=np.eye(6), frame="local")}, [1, 1, 1],
+     np.diag([1, 1, -1]), "right-handed"),
+])
+def test_oracle_rejects_invalid_inputs(phase, materials, domain, orientation, message):
+    with pytest.raises(ValueError, match=message):
+        H.homogenize_direct(phase, materials, domain, orientation=orientation)
+
+
+def test_cpu_size_guard():
+    with pytest.raises(ValueError, match="4096"):
+        H.homogenize_direct(np.zeros((17, 17, 17), dtype=np.uint8), {0: dict(kind="isotropic", E=1., nu=.2)},
+                            [1, 1, 1])
+
+
+def gpu_fixture_cell():
+    """tests/test_gpu.py:14-29 of the reference: 4^3 rotated anisotropic cell."""
+    z, y, x = np.indices((4, 4, 4))
+    phase = ((x + 2 * y + z) % 3 == 0).astype(np.uint8)
+    table = {0: dict(kind="isotropic", E=3500., nu=.35),
+             1: dict(kind="anisotropic", C=np.diag([135000., 9000., 9000., 3500., 5000., 5000.]).tolist(),
+                     frame="local")}
+    angle = .2 * x + .1 * y
+    rot = np.zeros(phase.shape + (3, 3))
+    rot[..., 0, 0] = rot[..., 1, 1] = np.cos(angle)
+    rot[..., 0, 1] = -np.sin(angle)
+    rot[..., 1, 0] = np.sin(angle)
+    rot[..., 2, 2] = 1.
+    return phase, table, [1., 1.5, 2.], rot
+
+
+def test_matrix_free_pcg_matches_sparse_direct():
+    phase, table, domain, rot = gpu_fixture_cell()
+    d = H.homogenize_direct(phase, table, domain, orientation=rot)
+    m = H.homogenize_matfree(phase, table, domain, orientation=rot)
+    assert np.linalg.norm(m["CH"] - d["CH"]) / np.linalg.norm(d["CH"]) < 1e-8
+    assert max(d["true_residuals"]) < 1e-10
+
+
+# ---- spectral ----------------------------------------------------------------
+def test_symbol_matches_operator_on_plane_waves():
+    shape, h = (6, 5, 4), np.array([.3, .5, .7])
+    C = iso(3., .3)
+    lam, mu = C[0, 1], C[3, 3]
+    n = int(np.prod(shape))
+    op = fem.MatrixFree(np.broadcast_to(C, (n, 6, 6)).copy(), shape, h)
+    S = spectral.symbol(shape, h, lam, mu)
+    z, y, x = np.indices(shape)
+    rng = np.random.default_rng(3)
+    for kz, ky, kx in [(1, 2, 3), (0, 0, 1), (2, 4, 1), (3, 0, 0), (5, 4, 2)]:
+        ph = np.exp(2j * np.pi * (kx * x / shape[2] + ky * y / shape[1] + kz * z / shape[0])).ravel()
+        amp = rng.normal(size=3)
+        Ku = op.apply(np.real(ph[:, None] * amp)) + 1j * op.apply(np.imag(ph[:, None] * amp))
+        pred = ph[:, None] * (S[kz, ky, kx] @ amp)
+        assert np.abs(Ku - pred).max() < 1e-13 * np.abs(pred).max()
+
+
+@pytest.mark.parametrize("shape", [(6, 5, 4), (8, 8, 8), (3, 7, 5)])
+def test_green_sherman_morrison_equals_dense_inverse(shape):
+    h = np.array([.3, .5, .7])
+    lam, mu = 2.5, 1.3
+    r = np.random.default_rng(4).normal(size=(int(np.prod(shape)), 3))
+    r -= r.mean(0)
+    a = spectral.green_apply(r, shape, h, lam, mu)
+    b = spectral.green_dense(r, shape, h, lam, mu)
+    assert np.abs(a - b).max() < 1e-12 * np.abs(b).max()
+    # K0 G r = r on zero-mean r
+    C = iso(1., .3)
+    C[:3, :3] = lam
+    C[np.arange(3), np.arange(3)] = lam + 2 * mu
+    C[3:, 3:] = mu * np.eye(3)
+    op = fem.MatrixFree(np.broadcast_to(C, (len(r), 6, 6)).copy(), shape, h)
+    np.testing.assert_allclose(op.apply(a), r, atol=1e-12 * np.abs(r).max())
+
+
+# ---- PCG and reference medium -------------------------------------------------
+def test_pcg_reaches_direct_solution():
+    rng = np.random.default_rng(0)
+    A = rng.normal(size=(30, 30))
+    A = A @ A.T + 30 * np.eye(30)
+    b = rng.normal(size=30)
+    d = np.diag(A)
+    u, its, res = pcg.pcg(lambda v: A @ v, lambda v: v / d, b, tol=1e-12)
+    np.testing.assert_allclose(u, np.linalg.solve(A, b), rtol=1e-9)
+    assert res < 1e-12 and its <= 30
+
+
+def test_design_closed_unordered_phases_keep_baseline():
+    # independent_validation.py:191-197
+    def iso_kmu(K, mu):
+        return rm.isotropic_stiffness(K, mu)
+    phases = [iso_kmu(10., 1.), iso_kmu(1., 10.), iso_kmu(4., 3.)]
+    base = iso_kmu(3., 3.)
+    ref, status = rm.design_closed(phases, base)
+    assert status == "unsupported" and np.array_equal(ref, base)
+
+
+def test_design_closed_bound_is_max_generalized_eigenvalue():
+    rng = np.random.default_rng(7)
+    soft = rm.isotropic_stiffness(2., 1.)
+    phases = [soft] + [soft + X @ X.T for X in rng.normal(size=(3, 6, 6))]
+    base = rm.isotropic_stiffness(50., 40.)
+    ref, status = rm.design_closed(phases, base)
+    from scipy.linalg import eigvalsh
+    expected = max(eigvalsh(c, soft)[-1] for c in phases)
+    assert status == "closed_form"
+    assert rm.bound(phases, ref)["condition_bound"] == pytest.approx(expected, rel=1e-10)
+
+
+def test_archived_woven_8_record():
+    """repro/raw/paper/precision/.../8_fp64_r0.json: input hash, reference
+    medium, C_eff and iteration counts of the reference's FP64-FFT solve."""
+    import sys
+    from conftest import ROOT
+    sys.path.insert(0, str(ROOT / "voxel-lmfem_amd"))
+    from voxel_lmfem.cells import reference_weave
+    rec = json.loads((GOLDEN / "woven_records.json").read_text())["8_fp64_r0"]
+    (phase, mats, dom, ori), meta = reference_weave(8)
+    assert meta["input_hash"] == rec["input_hash"]
+    out = H.homogenize_matfree(phase, mats, dom, orientation=ori)
+    assert out["reference"] == (rec["reference_lam"], rec["reference_mu"])
+    CH = np.array(rec["CH"])
+    assert np.linalg.norm(out["CH"] - CH) / np.linalg.norm(CH) < 1e-9
+    its = [c["iterations"] for c in out["calls"]]
+    assert all(abs(a - b) <= 1 for a, b in zip(its, rec["iterations"]))

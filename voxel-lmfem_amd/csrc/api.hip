@@ -1,0 +1,295 @@
+// extern "C" entry points of libvxl (declared in include/vxl.h).
+#include "vxl_cg.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace vxl {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& message) { g_error = message; }
+
+int fail(const std::string& message) {
+  g_error = message;
+  if (message.find("hip") != std::string::npos || message.find("HIP") != std::string::npos)
+    return VXL_ERR_HIP;
+  return VXL_ERR_ARG;
+}
+
+int solver_create(vxl_solver* s);
+int solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter, double* ch_host,
+               int* iterations, int* restarts, double* residuals, double* u_out);
+int solver_release(vxl_solver* s);
+
+namespace {
+
+template <typename Idx>
+__global__ void index_max_kernel(const Idx* index, int64_t n, unsigned int* out) {
+  unsigned int m = 0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const long long v = static_cast<long long>(index[i]);
+    const unsigned int u = v < 0 ? 0xffffffffu : static_cast<unsigned int>(v);
+    m = u > m ? u : m;
+  }
+  atomicMax(out, m);
+}
+
+int check_index(vxl_cell* c) {
+  unsigned int* d = nullptr;
+  VXL_HIP(hipMalloc(&d, sizeof(unsigned int)));
+  unsigned int h = 0;
+  VXL_HIP(hipMemcpyAsync(d, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  const int64_t n = c->grid.n;
+  const int blocks = int(std::min<int64_t>((n + 255) / 256, 1024));
+  switch (c->mat.index_kind) {
+    case VXL_INDEX_U8:
+      hipLaunchKernelGGL(index_max_kernel<uint8_t>, dim3(blocks), dim3(256), 0, c->stream,
+                         static_cast<const uint8_t*>(c->mat.index), n, d);
+      break;
+    case VXL_INDEX_U16:
+      hipLaunchKernelGGL(index_max_kernel<uint16_t>, dim3(blocks), dim3(256), 0, c->stream,
+                         static_cast<const uint16_t*>(c->mat.index), n, d);
+      break;
+    case VXL_INDEX_I32:
+      hipLaunchKernelGGL(index_max_kernel<int32_t>, dim3(blocks), dim3(256), 0, c->stream,
+                         static_cast<const int32_t*>(c->mat.index), n, d);
+      break;
+  }
+  VXL_HIP(hipGetLastError());
+  VXL_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  VXL_HIP(hipStreamSynchronize(c->stream));
+  VXL_HIP(hipFree(d));
+  if (h >= static_cast<unsigned int>(c->rows)) return fail("material row index out of range");
+  return VXL_OK;
+}
+
+}  // namespace
+}  // namespace vxl
+
+using namespace vxl;
+
+extern "C" {
+
+const char* vxl_last_error(void) { return g_error.c_str(); }
+
+int vxl_version(void) { return 100; }
+
+int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy, double hz,
+                    int index_kind, const void* index_dev, int rows, const double* c21_host,
+                    const float* scale_dev, void* stream) {
+  g_error.clear();
+  if (!out) return fail("out handle is NULL");
+  *out = nullptr;
+  if (nz < 2 || ny < 2 || nx < 2) return fail("grid dimensions must be >= 2");
+  if (int64_t(nz) * ny * nx > (int64_t(1) << 40)) return fail("grid too large");
+  if (!(hx > 0 && hy > 0 && hz > 0) || !std::isfinite(hx * hy * hz))
+    return fail("voxel sizes must be finite and positive");
+  if (index_kind != VXL_INDEX_U8 && index_kind != VXL_INDEX_U16 && index_kind != VXL_INDEX_I32)
+    return fail("index_kind must be VXL_INDEX_U8, VXL_INDEX_U16 or VXL_INDEX_I32");
+  if (!index_dev || !c21_host || rows < 1) return fail("index, coefficient table and rows are required");
+  for (int64_t i = 0; i < int64_t(rows) * 21; ++i)
+    if (!std::isfinite(c21_host[i])) return fail("coefficient table contains nonfinite values");
+  auto* c = new vxl_cell();
+  c->grid.nx = nx;
+  c->grid.ny = ny;
+  c->grid.nz = nz;
+  c->grid.n = int64_t(nx) * ny * nz;
+  c->h[0] = hx;
+  c->h[1] = hy;
+  c->h[2] = hz;
+  c->volume = hx * hy * hz;
+  c->rows = rows;
+  c->stream = static_cast<hipStream_t>(stream);
+  c->mat.index_kind = index_kind;
+  c->mat.index = index_dev;
+  c->mat.scale = scale_dev;
+  c->c64_owned = nullptr;
+  c->c32_owned = nullptr;
+  (void)hipGetDevice(&c->device);
+  std::vector<double> t64(size_t(rows) * kRowWidth, 0.0);
+  std::vector<float> t32(size_t(rows) * kRowWidth, 0.0f);
+  for (int r = 0; r < rows; ++r)
+    for (int k = 0; k < 21; ++k) {
+      t64[size_t(r) * kRowWidth + k] = c21_host[size_t(r) * 21 + k];
+      t32[size_t(r) * kRowWidth + k] = float(c21_host[size_t(r) * 21 + k]);
+    }
+  int rc = VXL_OK;
+  auto cleanup = [&]() {
+    if (c->c64_owned) (void)hipFree(c->c64_owned);
+    if (c->c32_owned) (void)hipFree(c->c32_owned);
+    delete c;
+  };
+  if (hipMalloc(&c->c64_owned, t64.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->c32_owned, t32.size() * sizeof(float)) != hipSuccess) {
+    cleanup();
+    g_error = "hipMalloc failed for the coefficient table";
+    return VXL_ERR_HIP;
+  }
+  if (hipMemcpy(c->c64_owned, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->c32_owned, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    cleanup();
+    g_error = "hipMemcpy failed for the coefficient table";
+    return VXL_ERR_HIP;
+  }
+  c->mat.c64 = c->c64_owned;
+  c->mat.c32 = c->c32_owned;
+  rc = check_index(c);
+  if (rc != VXL_OK) {
+    std::string msg = g_error;
+    cleanup();
+    g_error = msg;
+    return rc;
+  }
+  *out = c;
+  return VXL_OK;
+}
+
+int vxl_cell_destroy(vxl_cell* cell) {
+  if (!cell) return VXL_OK;
+  if (cell->c64_owned) (void)hipFree(cell->c64_owned);
+  if (cell->c32_owned) (void)hipFree(cell->c32_owned);
+  delete cell;
+  return VXL_OK;
+}
+
+int vxl_apply(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev) {
+  g_error.clear();
+  if (!cell || !u_dev || !out_dev || nrhs < 1) return fail("vxl_apply: invalid arguments");
+  if (u_dev == out_dev) return fail("vxl_apply: input and output must not alias");
+  if (precision == VXL_F64)
+    return launch_apply<double>(cell, nrhs, static_cast<const double*>(u_dev), static_cast<double*>(out_dev),
+                                cell->stream);
+  if (precision == VXL_F32)
+    return launch_apply<float>(cell, nrhs, static_cast<const float*>(u_dev), static_cast<float*>(out_dev),
+                               cell->stream);
+  return fail("precision must be VXL_F32 or VXL_F64");
+}
+
+int vxl_rhs(vxl_cell* cell, int nload, const double* macro_host, double* f_dev) {
+  g_error.clear();
+  if (!cell || !macro_host || !f_dev || nload < 1) return fail("vxl_rhs: invalid arguments");
+  double* m = nullptr;
+  VXL_HIP(hipMalloc(&m, sizeof(double) * 6 * nload));
+  VXL_HIP(hipMemcpyAsync(m, macro_host, sizeof(double) * 6 * nload, hipMemcpyHostToDevice, cell->stream));
+  int rc = launch_rhs(cell, nload, m, f_dev, cell->stream);
+  VXL_HIP(hipStreamSynchronize(cell->stream));
+  VXL_HIP(hipFree(m));
+  return rc;
+}
+
+int vxl_avg_stress(vxl_cell* cell, int nload, const double* u_dev, const double* macro_host,
+                   double* out_host) {
+  g_error.clear();
+  if (!cell || !macro_host || !u_dev || !out_host || nload < 1) return fail("vxl_avg_stress: invalid arguments");
+  double *m = nullptr, *partial = nullptr, *out = nullptr;
+  VXL_HIP(hipMalloc(&m, sizeof(double) * 6 * nload));
+  VXL_HIP(hipMalloc(&partial, sizeof(double) * size_t(stress_partial_size(cell, nload))));
+  VXL_HIP(hipMalloc(&out, sizeof(double) * 6 * nload));
+  VXL_HIP(hipMemcpyAsync(m, macro_host, sizeof(double) * 6 * nload, hipMemcpyHostToDevice, cell->stream));
+  int rc = launch_avg_stress(cell, nload, u_dev, m, partial, out, cell->stream);
+  if (rc == VXL_OK)
+    VXL_HIP(hipMemcpyAsync(out_host, out, sizeof(double) * 6 * nload, hipMemcpyDeviceToHost, cell->stream));
+  VXL_HIP(hipStreamSynchronize(cell->stream));
+  VXL_HIP(hipFree(m));
+  VXL_HIP(hipFree(partial));
+  VXL_HIP(hipFree(out));
+  return rc;
+}
+
+int vxl_project_zero_mean(vxl_cell* cell, int nrhs, double* u_dev) {
+  g_error.clear();
+  if (!cell || !u_dev || nrhs < 1) return fail("vxl_project_zero_mean: invalid arguments");
+  double* partial = nullptr;
+  const int64_t n = cell->grid.n;
+  VXL_HIP(hipMalloc(&partial, sizeof(double) * 3 * nrhs * size_t(vec_blocks(n))));
+  int rc = launch_project_zero_mean(u_dev, n, nrhs, partial, nullptr, cell->stream);
+  VXL_HIP(hipStreamSynchronize(cell->stream));
+  VXL_HIP(hipFree(partial));
+  return rc;
+}
+
+int vxl_precond_create(vxl_precond** out, vxl_cell* cell, int precision, double lam, double mu,
+                       int nrhs_max) {
+  g_error.clear();
+  if (!out || !cell || nrhs_max < 1) return fail("vxl_precond_create: invalid arguments");
+  if (precision != VXL_F32 && precision != VXL_F64) return fail("precision must be VXL_F32 or VXL_F64");
+  *out = nullptr;
+  auto* pc = new vxl_precond();
+  pc->cell = cell;
+  pc->precision = precision;
+  pc->nrhs_max = nrhs_max;
+  pc->lam = lam;
+  pc->mu = mu;
+  pc->impl = nullptr;
+  int rc = precond_create(pc);
+  if (rc != VXL_OK) {
+    precond_release(pc);
+    delete pc;
+    return rc;
+  }
+  *out = pc;
+  return VXL_OK;
+}
+
+int vxl_precond_apply(vxl_precond* pc, int nrhs, const void* r_dev, void* z_dev) {
+  g_error.clear();
+  if (!pc || !r_dev || !z_dev) return fail("vxl_precond_apply: invalid arguments");
+  return precond_apply(pc, nrhs, r_dev, z_dev, pc->cell->stream);
+}
+
+int vxl_precond_destroy(vxl_precond* pc) {
+  if (!pc) return VXL_OK;
+  precond_release(pc);
+  delete pc;
+  return VXL_OK;
+}
+
+int vxl_solver_create(vxl_solver** out, vxl_cell* cell, int mode, int nrhs, double lam, double mu) {
+  g_error.clear();
+  if (!out || !cell || nrhs < 1 || nrhs > 64) return fail("vxl_solver_create: invalid arguments");
+  if (mode != VXL_SOLVE_FP64 && mode != VXL_SOLVE_MIXED) return fail("mode must be VXL_SOLVE_FP64 or VXL_SOLVE_MIXED");
+  *out = nullptr;
+  auto* s = new vxl_solver();
+  std::memset(static_cast<void*>(s), 0, sizeof(vxl_solver));
+  s->cell = cell;
+  s->mode = mode;
+  s->R = nrhs;
+  s->pc.lam = lam;
+  s->pc.mu = mu;
+  int rc = solver_create(s);
+  if (rc != VXL_OK) {
+    std::string msg = g_error;
+    solver_release(s);
+    delete s;
+    g_error = msg;
+    return rc;
+  }
+  *out = s;
+  return VXL_OK;
+}
+
+int vxl_solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter, double* ch_host,
+                   int* iterations, int* restarts, double* residuals, double* u_dev_out) {
+  g_error.clear();
+  if (!s || !macro_host || !ch_host) return fail("vxl_solver_run: invalid arguments");
+  if (!(tol > 0 && tol < 1)) return fail("tol must lie in (0, 1)");
+  if (max_iter < 1) return fail("max_iter must be >= 1");
+  return solver_run(s, macro_host, tol, max_iter, ch_host, iterations, restarts, residuals, u_dev_out);
+}
+
+int vxl_solver_timings(vxl_solver* s, double* ms3) {
+  if (!s || !ms3) return fail("vxl_solver_timings: invalid arguments");
+  for (int i = 0; i < 3; ++i) ms3[i] = s->ms[i];
+  return VXL_OK;
+}
+
+int vxl_solver_destroy(vxl_solver* s) {
+  if (!s) return VXL_OK;
+  solver_release(s);
+  delete s;
+  return VXL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,303 @@
+// CG / iterative-refinement vector kernels, gfx950.
+//
+// Vectors are nrhs consecutive fields of `len` entries (len = 3N); blockIdx.y
+// selects the right-hand side and a per-RHS `active` flag lets converged
+// systems skip all work.  Dot products accumulate in FP64 and reduce in two
+// deterministic stages: per-workgroup partials (wavefront shuffles, then LDS),
+// then one workgroup per RHS sums its partials in a fixed order and applies a
+// scalar update (alpha, beta, ...) in the same launch.
+#include "vxl_cg.h"
+
+namespace vxl {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block sum of v; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[kBlock / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w];
+  }
+  __syncthreads();
+  return s;
+}
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(kBlock)
+dot_kernel(const A* a, const B* b, int64_t len, double* partial, const int* active) {
+  const int r = blockIdx.y;
+  if (active && !active[r]) return;
+  const A* ar = a + int64_t(r) * len;
+  const B* br = b + int64_t(r) * len;
+  double acc = 0;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock)
+    acc += double(ar[i]) * double(br[i]);
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[int64_t(r) * gridDim.x + blockIdx.x] = acc;
+}
+
+// x += alpha p; r -= alpha q; partial ||r||^2.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+update_xr_kernel(T* x, T* r, const T* p, const T* q, int64_t len, const double* alpha,
+                 double* partial, const int* active) {
+  const int k = blockIdx.y;
+  if (active && !active[k]) return;
+  const T a = T(alpha[k]);
+  const int64_t off = int64_t(k) * len;
+  double acc = 0;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock) {
+    x[off + i] = fma(a, p[off + i], x[off + i]);
+    const T rn = fma(-a, q[off + i], r[off + i]);
+    r[off + i] = rn;
+    acc += double(rn) * double(rn);
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[int64_t(k) * gridDim.x + blockIdx.x] = acc;
+}
+
+// p = z + beta p (beta = 0 where restart[k]).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+update_p_kernel(T* p, const T* z, int64_t len, const double* beta, const int* active) {
+  const int k = blockIdx.y;
+  if (active && !active[k]) return;
+  const T b = T(beta[k]);
+  const int64_t off = int64_t(k) * len;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock)
+    p[off + i] = fma(b, p[off + i], z[off + i]);
+}
+
+// r64 = f - q64; partial ||r64||^2; optional r32 = float(r64).
+__global__ void __launch_bounds__(kBlock)
+residual_kernel(double* r, const double* f, const double* q, float* r32, int64_t len,
+                double* partial, const int* active) {
+  const int k = blockIdx.y;
+  if (active && !active[k]) return;
+  const int64_t off = int64_t(k) * len;
+  double acc = 0;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock) {
+    const double v = f[off + i] - q[off + i];
+    r[off + i] = v;
+    if (r32) r32[off + i] = float(v);
+    acc += v * v;
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[int64_t(k) * gridDim.x + blockIdx.x] = acc;
+}
+
+// x64 += x32; x32 = 0.
+__global__ void __launch_bounds__(kBlock)
+accumulate_kernel(double* x64, float* x32, int64_t len, const int* active) {
+  const int k = blockIdx.y;
+  if (active && !active[k]) return;
+  const int64_t off = int64_t(k) * len;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock) {
+    x64[off + i] += double(x32[off + i]);
+    x32[off + i] = 0.0f;
+  }
+}
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(kBlock)
+convert_kernel(B* dst, const A* src, int64_t len, const int* active) {
+  const int k = blockIdx.y;
+  if (active && !active[k]) return;
+  const int64_t off = int64_t(k) * len;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock)
+    dst[off + i] = B(src[off + i]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+fill_kernel(T* dst, int64_t len, const int* active) {
+  const int k = blockIdx.y;
+  if (active && !active[k]) return;
+  const int64_t off = int64_t(k) * len;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock)
+    dst[off + i] = T(0);
+}
+
+// Per-(RHS, component) sums of an FP64 field: grid.y = 3 * nrhs rows of n.
+__global__ void __launch_bounds__(kBlock)
+row_sum_kernel(const double* u, int64_t n, double* partial) {
+  const int row = blockIdx.y;
+  const double* ur = u + int64_t(row) * n;
+  double acc = 0;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBlock)
+    acc += ur[i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[int64_t(row) * gridDim.x + blockIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(kBlock)
+subtract_mean_kernel(double* u, int64_t n, const double* partial, int nblk, const int* active,
+                     double inv_n) {
+  const int row = blockIdx.y;
+  if (active && !active[row / 3]) return;
+  __shared__ double mean;
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += partial[int64_t(row) * nblk + b];
+    mean = s * inv_n;
+  }
+  __syncthreads();
+  const double m = mean;
+  double* ur = u + int64_t(row) * n;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBlock)
+    ur[i] -= m;
+}
+
+// One workgroup per RHS: total = sum partial[k][0..nblk) in a fixed order,
+// then the scalar update selected by op.
+__global__ void __launch_bounds__(kBlock)
+finalize_kernel(const double* partial, int nblk, int op, double* scal, int nrhs, const int* active,
+                const int* restart) {
+  const int k = blockIdx.x;
+  if (active && !active[k]) return;
+  double acc = 0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) acc += partial[int64_t(k) * nblk + b];
+  acc = block_sum(acc);
+  if (threadIdx.x != 0) return;
+  double* S = scal;
+  switch (op) {
+    case kOpStore:
+      S[kTmp * nrhs + k] = acc;
+      break;
+    case kOpRho:
+      S[kRho * nrhs + k] = acc;
+      break;
+    case kOpFF:
+      S[kFF * nrhs + k] = acc;
+      break;
+    case kOpRR:
+      S[kRR * nrhs + k] = acc;
+      break;
+    case kOpAlpha: {
+      S[kPQ * nrhs + k] = acc;
+      const bool bad = !isfinite(acc) || acc <= 0.0;
+      S[kAlpha * nrhs + k] = bad ? 0.0 : S[kRho * nrhs + k] / fmax(acc, 1e-300);
+      break;
+    }
+    case kOpBeta: {
+      const double rho_old = S[kRho * nrhs + k];
+      double beta = rho_old > 0.0 ? acc / rho_old : 0.0;
+      if (restart && restart[k]) beta = 0.0;
+      if (!isfinite(beta)) beta = 0.0;
+      S[kBeta * nrhs + k] = beta;
+      S[kRho * nrhs + k] = acc;
+      break;
+    }
+  }
+}
+
+inline dim3 vgrid(int64_t len, int nrhs) {
+  int64_t b = (len + kBlock - 1) / kBlock;
+  if (b > kVecBlocks) b = kVecBlocks;
+  if (b < 1) b = 1;
+  return dim3(unsigned(b), unsigned(nrhs));
+}
+
+}  // namespace
+
+int vec_blocks(int64_t len) {
+  int64_t b = (len + kBlock - 1) / kBlock;
+  return int(b > kVecBlocks ? kVecBlocks : (b < 1 ? 1 : b));
+}
+
+template <typename A, typename B>
+int launch_dot(const A* a, const B* b, int64_t len, int nrhs, double* partial, const int* active,
+               hipStream_t s) {
+  hipLaunchKernelGGL((dot_kernel<A, B>), vgrid(len, nrhs), dim3(kBlock), 0, s, a, b, len, partial, active);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+template int launch_dot<float, float>(const float*, const float*, int64_t, int, double*, const int*, hipStream_t);
+template int launch_dot<double, double>(const double*, const double*, int64_t, int, double*, const int*, hipStream_t);
+
+int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, const int* active,
+                    const int* restart, hipStream_t s) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(nrhs), dim3(kBlock), 0, s, partial, nblk, op, scal, nrhs,
+                     active, restart);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+
+template <typename T>
+int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, const double* alpha,
+                     double* partial, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL((update_xr_kernel<T>), vgrid(len, nrhs), dim3(kBlock), 0, s, x, r, p, q, len,
+                     alpha, partial, active);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+template int launch_update_xr<float>(float*, float*, const float*, const float*, int64_t, int, const double*, double*, const int*, hipStream_t);
+template int launch_update_xr<double>(double*, double*, const double*, const double*, int64_t, int, const double*, double*, const int*, hipStream_t);
+
+template <typename T>
+int launch_update_p(T* p, const T* z, int64_t len, int nrhs, const double* beta, const int* active,
+                    hipStream_t s) {
+  hipLaunchKernelGGL((update_p_kernel<T>), vgrid(len, nrhs), dim3(kBlock), 0, s, p, z, len, beta, active);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+template int launch_update_p<float>(float*, const float*, int64_t, int, const double*, const int*, hipStream_t);
+template int launch_update_p<double>(double*, const double*, int64_t, int, const double*, const int*, hipStream_t);
+
+int launch_residual(double* r, const double* f, const double* q, float* r32, int64_t len, int nrhs,
+                    double* partial, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL(residual_kernel, vgrid(len, nrhs), dim3(kBlock), 0, s, r, f, q, r32, len, partial, active);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+
+int launch_accumulate(double* x64, float* x32, int64_t len, int nrhs, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL(accumulate_kernel, vgrid(len, nrhs), dim3(kBlock), 0, s, x64, x32, len, active);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+
+template <typename A, typename B>
+int launch_convert(B* dst, const A* src, int64_t len, int nrhs, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL((convert_kernel<A, B>), vgrid(len, nrhs), dim3(kBlock), 0, s, dst, src, len, active);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+template int launch_convert<double, float>(float*, const double*, int64_t, int, const int*, hipStream_t);
+template int launch_convert<double, double>(double*, const double*, int64_t, int, const int*, hipStream_t);
+template int launch_convert<float, float>(float*, const float*, int64_t, int, const int*, hipStream_t);
+
+template <typename T>
+int launch_fill_zero(T* dst, int64_t len, int nrhs, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL((fill_kernel<T>), vgrid(len, nrhs), dim3(kBlock), 0, s, dst, len, active);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+template int launch_fill_zero<float>(float*, int64_t, int, const int*, hipStream_t);
+template int launch_fill_zero<double>(double*, int64_t, int, const int*, hipStream_t);
+
+int launch_project_zero_mean(double* u, int64_t n, int nrhs, double* partial, const int* active,
+                             hipStream_t s) {
+  const dim3 g = vgrid(n, 3 * nrhs);
+  hipLaunchKernelGGL(row_sum_kernel, g, dim3(kBlock), 0, s, u, n, partial);
+  VXL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(subtract_mean_kernel, g, dim3(kBlock), 0, s, u, n, partial, int(g.x), active,
+                     1.0 / double(n));
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+
+}  // namespace vxl

@@ -1,0 +1,603 @@
+// FFT Green preconditioner of an isotropic reference medium, gfx950.
+//
+// z = IFFT( K0(k)^-1 FFT(r) ) with K0 the exact Fourier symbol of the periodic
+// fully integrated HEX8 operator of a uniform isotropic medium (lam, mu):
+//   per axis i, theta = 2 pi k_i / n_i:  m = (2+cos)/3, D = (2-2cos)/(h^2 m),
+//   v = sin/(h m), d = D - v^2 = 4 sin^4(theta/2)/(3 h^2 m^2);
+//   K0 = V m_x m_y m_z [diag(mu sum D + (lam+mu) d_i) + (lam+mu) v v^T]
+// (the symbol of repro/bench/solver_comparison/fft_precision.py:65-79 and the
+// diagonal + rank-1 form of _core/reference_medium/formula_kernels.cu:104-137).
+// The inverse is applied per wavevector with Sherman-Morrison from 1-D tables
+// (no Nk-sized coefficient cache); the k = 0 mode is removed and 1/N of the
+// inverse transform is folded in.
+//
+// Transforms are hand-written mixed-radix Stockham FFTs on LDS pencils
+// (radix 8, 4, 2, 3, 5, 7, generic odd primes <= 13, direct DFT otherwise;
+// twiddles from a host table filled with std::cos/std::sin).  Five passes per
+// apply, spectrum stored as S[r][z][y][kx][c] with the three components
+// interleaved so the z pass sees whole 3-vectors:
+//   1. x: real-to-complex (two real rows per complex FFT), half spectrum
+//   2. y: forward complex
+//   3. z: forward complex, 3x3 Green solve, inverse complex (one pass)
+//   4. y: inverse complex
+//   5. x: complex-to-real (two rows per complex FFT)
+#include "vxl_cg.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace vxl {
+namespace {
+
+constexpr int kMaxStages = 12;
+constexpr int kThreads = 256;
+
+struct Plan1D {
+  int n;
+  int stages;  // 0 => direct DFT
+  int radix[kMaxStages];
+};
+
+Plan1D make_plan(int n) {
+  Plan1D p;
+  p.n = n;
+  p.stages = 0;
+  int m = n;
+  const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
+  for (int r : order) {
+    while (m % r == 0 && p.stages < kMaxStages) {
+      if (r == 8 || r == 4) {
+        // keep powers of two in 8/4/2 chunks
+      }
+      p.radix[p.stages++] = r;
+      m /= r;
+    }
+  }
+  if (m != 1) p.stages = 0;  // prime factor > 13: direct DFT
+  return p;
+}
+
+template <typename T> struct CT;
+template <> struct CT<float> { using type = float2; };
+template <> struct CT<double> { using type = double2; };
+
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { return {a.x + b.x, a.y + b.y}; }
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) { return {a.x - b.x, a.y - b.y}; }
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename C>
+__device__ __forceinline__ C cconj(C a) { return {a.x, -a.y}; }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename C>
+__device__ __forceinline__ C rot90(C a) { return INV ? C{-a.y, a.x} : C{a.y, -a.x}; }
+
+// In-register DFTs of size R with kernel exp(-+2 pi i / R).
+template <bool INV, typename T, typename C>
+__device__ __forceinline__ void dft2(C* v) {
+  C a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <bool INV, typename T, typename C>
+__device__ __forceinline__ void dft4(C* v) {
+  C a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+  C a2 = cadd(v[1], v[3]), a3 = rot90<INV>(csub(v[1], v[3]));
+  v[0] = cadd(a0, a2);
+  v[2] = csub(a0, a2);
+  v[1] = cadd(a1, a3);
+  v[3] = csub(a1, a3);
+}
+template <bool INV, typename T, typename C>
+__device__ __forceinline__ void dft8(C* v) {
+  const T s = T(0.70710678118654752440084436210484903928);
+  C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<INV, T>(e);
+  dft4<INV, T>(o);
+  // twiddles w^k, k = 0..3, w = exp(-+ i pi/4)
+  C t1 = INV ? C{s * (o[1].x - o[1].y), s * (o[1].x + o[1].y)}
+             : C{s * (o[1].x + o[1].y), s * (o[1].y - o[1].x)};
+  C t2 = rot90<INV>(o[2]);
+  C t3 = INV ? C{-s * (o[3].x + o[3].y), s * (o[3].x - o[3].y)}
+             : C{s * (o[3].y - o[3].x), -s * (o[3].x + o[3].y)};
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], t1);
+  v[5] = csub(e[1], t1);
+  v[2] = cadd(e[2], t2);
+  v[6] = csub(e[2], t2);
+  v[3] = cadd(e[3], t3);
+  v[7] = csub(e[3], t3);
+}
+template <bool INV, typename T, typename C>
+__device__ __forceinline__ void dft3(C* v) {
+  const T c = T(-0.5), s = T(INV ? 0.86602540378443864676372317075293618347
+                                  : -0.86602540378443864676372317075293618347);
+  C a = cadd(v[1], v[2]), b = csub(v[1], v[2]);
+  C m = {v[0].x + c * a.x, v[0].y + c * a.y};
+  C jb = {-s * b.y, s * b.x};
+  v[0] = cadd(v[0], a);
+  v[1] = cadd(m, jb);
+  v[2] = csub(m, jb);
+}
+// Odd prime radix R via the n-point twiddle table (w[k] = exp(-2 pi i k/n)).
+template <bool INV, int R, typename C>
+__device__ __forceinline__ void dft_prime(C* v, const C* w, int n) {
+  C out[R];
+  const int step = n / R;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    C acc = v[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      C tw = w[((r * k) % R) * step];
+      if (INV) tw = cconj(tw);
+      acc = cadd(acc, cmul(v[r], tw));
+    }
+    out[k] = acc;
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = out[k];
+}
+
+// One Stockham radix-R stage over P pencils: butterfly j reads src[j + r n/R],
+// applies w^(r (j mod Ns) n/(Ns R)), transforms, and writes
+// dst[(j/Ns) Ns R + (j mod Ns) + r Ns].
+template <bool INV, typename T, int R>
+__device__ __forceinline__ void stockham_stage(const typename CT<T>::type* A, typename CT<T>::type* B,
+                                               int P, int n, int Ns, const typename CT<T>::type* w) {
+  using C = typename CT<T>::type;
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  for (int idx = threadIdx.x; idx < P * nb; idx += blockDim.x) {
+    const int p = idx / nb, j = idx - p * nb;
+    const C* src = A + p * n;
+    C* dst = B + p * n;
+    const int jm = j % Ns;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      C x = src[j + r * nb];
+      if (r > 0 && jm > 0) {
+        C tw = w[r * jm * tstep];
+        if (INV) tw = cconj(tw);
+        x = cmul(x, tw);
+      }
+      v[r] = x;
+    }
+    if constexpr (R == 2) dft2<INV, T>(v);
+    else if constexpr (R == 3) dft3<INV, T>(v);
+    else if constexpr (R == 4) dft4<INV, T>(v);
+    else if constexpr (R == 8) dft8<INV, T>(v);
+    else dft_prime<INV, R>(v, w, n);
+    const int base = (j / Ns) * Ns * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+  }
+}
+
+// Batched in-LDS FFT of P contiguous pencils of length n.  A holds the input;
+// returns the buffer (A or B) holding the output.  All threads of the block
+// must call it.
+template <bool INV, typename T>
+__device__ typename CT<T>::type* lds_fft(typename CT<T>::type* A, typename CT<T>::type* B, int P,
+                                         const Plan1D& plan, const typename CT<T>::type* w) {
+  using C = typename CT<T>::type;
+  const int n = plan.n;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (plan.stages == 0) {  // direct DFT
+    for (int idx = tid; idx < P * n; idx += nt) {
+      const int p = idx / n, k = idx % n;
+      const C* src = A + p * n;
+      C acc = {T(0), T(0)};
+      for (int j = 0; j < n; ++j) {
+        C tw = w[(int64_t(j) * k) % n];
+        if (INV) tw = cconj(tw);
+        acc = cadd(acc, cmul(src[j], tw));
+      }
+      B[p * n + k] = acc;
+    }
+    __syncthreads();
+    return B;
+  }
+  int Ns = 1;
+  for (int s = 0; s < plan.stages; ++s) {
+    const int R = plan.radix[s];
+    switch (R) {
+      case 2: stockham_stage<INV, T, 2>(A, B, P, n, Ns, w); break;
+      case 3: stockham_stage<INV, T, 3>(A, B, P, n, Ns, w); break;
+      case 4: stockham_stage<INV, T, 4>(A, B, P, n, Ns, w); break;
+      case 5: stockham_stage<INV, T, 5>(A, B, P, n, Ns, w); break;
+      case 7: stockham_stage<INV, T, 7>(A, B, P, n, Ns, w); break;
+      case 8: stockham_stage<INV, T, 8>(A, B, P, n, Ns, w); break;
+      case 11: stockham_stage<INV, T, 11>(A, B, P, n, Ns, w); break;
+      default: stockham_stage<INV, T, 13>(A, B, P, n, Ns, w); break;
+    }
+    __syncthreads();
+    C* t = A;
+    A = B;
+    B = t;
+    Ns *= R;
+  }
+  return A;
+}
+
+struct Axes {
+  Plan1D px, py, pz;
+};
+
+template <typename T>
+struct Tables {
+  const typename CT<T>::type* wx;
+  const typename CT<T>::type* wy;
+  const typename CT<T>::type* wz;
+  const T* gx;  // 4 x nx: m, D, v, d
+  const T* gy;
+  const T* gz;
+};
+
+// ---- pass 1: x real-to-complex ---------------------------------------------
+// Row pair (rho, rho+1) over the flattened (z, y) row index; three complex
+// pencils per pair: c0+i c1 of row rho, c0+i c1 of row rho+1, c2(rho)+i c2(rho+1).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+fft_x_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int nx, int64_t rows,
+          int nxh, int pairs_per_block, Axes ax, Tables<T> tb, int64_t field_stride,
+          int64_t spec_stride) {
+  using C = typename CT<T>::type;
+  extern __shared__ double2 smem_raw[];
+  C* A = reinterpret_cast<C*>(smem_raw);
+  const int P = 3 * pairs_per_block;
+  C* B = A + P * nx;
+  const int64_t n = rows * nx;
+  const T* f = r + int64_t(blockIdx.y) * field_stride;
+  C* out = S + int64_t(blockIdx.y) * spec_stride;
+  const int64_t pair0 = int64_t(blockIdx.x) * pairs_per_block;
+  for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x) {
+    const int pp = idx / nx, x = idx % nx;
+    const int64_t rho = 2 * (pair0 + pp);
+    const bool has0 = rho < rows, has1 = rho + 1 < rows;
+    T v00 = 0, v01 = 0, v10 = 0, v11 = 0, v20 = 0, v21 = 0;
+    if (has0) {
+      v00 = f[rho * nx + x];
+      v01 = f[n + rho * nx + x];
+      v20 = f[2 * n + rho * nx + x];
+    }
+    if (has1) {
+      v10 = f[(rho + 1) * nx + x];
+      v11 = f[n + (rho + 1) * nx + x];
+      v21 = f[2 * n + (rho + 1) * nx + x];
+    }
+    A[(3 * pp + 0) * nx + x] = C{v00, v01};
+    A[(3 * pp + 1) * nx + x] = C{v10, v11};
+    A[(3 * pp + 2) * nx + x] = C{v20, v21};
+  }
+  __syncthreads();
+  C* F = lds_fft<false, T>(A, B, P, ax.px, tb.wx);
+  for (int idx = threadIdx.x; idx < pairs_per_block * nxh; idx += blockDim.x) {
+    const int pp = idx / nxh, k = idx % nxh;
+    const int64_t rho = 2 * (pair0 + pp);
+    const int km = k == 0 ? 0 : nx - k;
+    C sp[3][2];  // [pencil][real row 0/1]
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const C a = F[(3 * pp + q) * nx + k];
+      const C b = cconj(F[(3 * pp + q) * nx + km]);
+      const C s = cadd(a, b), d = csub(a, b);
+      sp[q][0] = C{T(0.5) * s.x, T(0.5) * s.y};
+      sp[q][1] = C{T(0.5) * d.y, T(-0.5) * d.x};  // d / (2i)
+    }
+    if (rho < rows) {
+      C* o = out + (rho * nxh + k) * 3;
+      o[0] = sp[0][0];
+      o[1] = sp[0][1];
+      o[2] = sp[2][0];
+    }
+    if (rho + 1 < rows) {
+      C* o = out + ((rho + 1) * nxh + k) * 3;
+      o[0] = sp[1][0];
+      o[1] = sp[1][1];
+      o[2] = sp[2][1];
+    }
+  }
+}
+
+// ---- pass 5: x complex-to-real -----------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx, int64_t rows,
+          int nxh, int pairs_per_block, Axes ax, Tables<T> tb, int64_t field_stride,
+          int64_t spec_stride) {
+  using C = typename CT<T>::type;
+  extern __shared__ double2 smem_raw[];
+  C* A = reinterpret_cast<C*>(smem_raw);
+  const int P = 3 * pairs_per_block;
+  C* B = A + P * nx;
+  const int64_t n = rows * nx;
+  T* f = z + int64_t(blockIdx.y) * field_stride;
+  const C* in = S + int64_t(blockIdx.y) * spec_stride;
+  const int64_t pair0 = int64_t(blockIdx.x) * pairs_per_block;
+  const bool even = (nx % 2) == 0;
+  for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x) {
+    const int pp = idx / nx, k = idx % nx;
+    const int64_t rho = 2 * (pair0 + pp);
+    const bool mirror = k >= nxh;
+    const int kk = mirror ? nx - k : k;
+    const bool realonly = kk == 0 || (even && kk == nx / 2);
+    C x[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t row = rho + h;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        C v = {T(0), T(0)};
+        if (row < rows) v = in[(row * nxh + kk) * 3 + c];
+        if (realonly) v.y = T(0);
+        if (mirror) v = cconj(v);
+        x[h][c] = v;
+      }
+    }
+    // pencil = X0 + i X1
+    A[(3 * pp + 0) * nx + k] = C{x[0][0].x - x[0][1].y, x[0][0].y + x[0][1].x};
+    A[(3 * pp + 1) * nx + k] = C{x[1][0].x - x[1][1].y, x[1][0].y + x[1][1].x};
+    A[(3 * pp + 2) * nx + k] = C{x[0][2].x - x[1][2].y, x[0][2].y + x[1][2].x};
+  }
+  __syncthreads();
+  C* F = lds_fft<true, T>(A, B, P, ax.px, tb.wx);
+  for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x) {
+    const int pp = idx / nx, x = idx % nx;
+    const int64_t rho = 2 * (pair0 + pp);
+    const C a = F[(3 * pp + 0) * nx + x], b = F[(3 * pp + 1) * nx + x], c = F[(3 * pp + 2) * nx + x];
+    if (rho < rows) {
+      f[rho * nx + x] = a.x;
+      f[n + rho * nx + x] = a.y;
+      f[2 * n + rho * nx + x] = c.x;
+    }
+    if (rho + 1 < rows) {
+      f[(rho + 1) * nx + x] = b.x;
+      f[n + (rho + 1) * nx + x] = b.y;
+      f[2 * n + (rho + 1) * nx + x] = c.y;
+    }
+  }
+}
+
+// ---- pass 2/4: y complex ------------------------------------------------------
+// Plane (r, z) is ny rows x W columns (W = 3*nxh); a block transforms TJ
+// consecutive columns.
+template <bool INV, typename T>
+__global__ void __launch_bounds__(kThreads)
+fft_y(typename CT<T>::type* __restrict__ S, int ny, int nz, int W, int TJ, Axes ax, Tables<T> tb,
+      int64_t spec_stride) {
+  using C = typename CT<T>::type;
+  extern __shared__ double2 smem_raw[];
+  C* A = reinterpret_cast<C*>(smem_raw);
+  C* B = A + TJ * ny;
+  const int j0 = blockIdx.x * TJ;
+  const int z = blockIdx.y;
+  const int tj = min(TJ, W - j0);
+  C* plane = S + int64_t(blockIdx.z) * spec_stride + int64_t(z) * ny * W;
+  for (int idx = threadIdx.x; idx < ny * TJ; idx += blockDim.x) {
+    const int y = idx / TJ, j = idx % TJ;
+    A[j * ny + y] = j < tj ? plane[int64_t(y) * W + j0 + j] : C{T(0), T(0)};
+  }
+  __syncthreads();
+  C* F = lds_fft<INV, T>(A, B, TJ, ax.py, tb.wy);
+  for (int idx = threadIdx.x; idx < ny * TJ; idx += blockDim.x) {
+    const int y = idx / TJ, j = idx % TJ;
+    if (j < tj) plane[int64_t(y) * W + j0 + j] = F[j * ny + y];
+  }
+}
+
+// ---- pass 3: z forward, Green solve, z inverse ---------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int TK, Axes ax,
+            Tables<T> tb, T lam, T mu, T inv_vn, int64_t spec_stride) {
+  using C = typename CT<T>::type;
+  extern __shared__ double2 smem_raw[];
+  C* A = reinterpret_cast<C*>(smem_raw);
+  const int TJ = 3 * TK;
+  C* B = A + TJ * nz;
+  const int k0 = blockIdx.x * TK;
+  const int y = blockIdx.y;
+  const int tk = min(TK, nxh - k0);
+  const int W = 3 * nxh;
+  const int64_t zstride = int64_t(ny) * W;
+  C* col = S + int64_t(blockIdx.z) * spec_stride + int64_t(y) * W + 3 * k0;
+  for (int idx = threadIdx.x; idx < nz * TJ; idx += blockDim.x) {
+    const int z = idx / TJ, j = idx % TJ;
+    A[j * nz + z] = j < 3 * tk ? col[z * zstride + j] : C{T(0), T(0)};
+  }
+  __syncthreads();
+  C* F = lds_fft<false, T>(A, B, TJ, ax.pz, tb.wz);
+  C* G = F == A ? B : A;
+  // Green solve per (kz, ky, kx), in place in F:
+  //   out = 1/(t N) [q/E - gamma (v/E) (v . q/E)],  E_i = mu sum D + (lam+mu) d_i,
+  //   gamma = (lam+mu)/(1 + (lam+mu) sum v_i^2/E_i),  t N = V N m_x m_y m_z.
+  const T lm = lam + mu;
+  const T my = tb.gy[y], Dy = tb.gy[ny + y], vy = tb.gy[2 * ny + y], dy = tb.gy[3 * ny + y];
+  for (int idx = threadIdx.x; idx < nz * TK; idx += blockDim.x) {
+    const int kz = idx / TK, kk = idx % TK;
+    if (kk >= tk) continue;
+    const int kx = k0 + kk;
+    const T mx = tb.gx[kx], Dx = tb.gx[nxh + kx], vx = tb.gx[2 * nxh + kx], dx = tb.gx[3 * nxh + kx];
+    const T mz = tb.gz[kz], Dz = tb.gz[nz + kz], vz = tb.gz[2 * nz + kz], dz = tb.gz[3 * nz + kz];
+    C* q0 = &F[(3 * kk + 0) * nz + kz];
+    C* q1 = &F[(3 * kk + 1) * nz + kz];
+    C* q2 = &F[(3 * kk + 2) * nz + kz];
+    const T sumD = Dx + Dy + Dz;
+    if (!(sumD > T(0))) {
+      *q0 = *q1 = *q2 = C{T(0), T(0)};
+      continue;
+    }
+    const T i0 = T(1) / (mu * sumD + lm * dx);
+    const T i1 = T(1) / (mu * sumD + lm * dy);
+    const T i2 = T(1) / (mu * sumD + lm * dz);
+    const T gamma = lm / (T(1) + lm * (vx * vx * i0 + vy * vy * i1 + vz * vz * i2));
+    const T scale = inv_vn / (mx * my * mz);
+    const C a0 = {q0->x * i0, q0->y * i0}, a1 = {q1->x * i1, q1->y * i1}, a2 = {q2->x * i2, q2->y * i2};
+    const C sdot = {vx * a0.x + vy * a1.x + vz * a2.x, vx * a0.y + vy * a1.y + vz * a2.y};
+    const T w0 = gamma * vx * i0, w1 = gamma * vy * i1, w2 = gamma * vz * i2;
+    *q0 = C{scale * (a0.x - w0 * sdot.x), scale * (a0.y - w0 * sdot.y)};
+    *q1 = C{scale * (a1.x - w1 * sdot.x), scale * (a1.y - w1 * sdot.y)};
+    *q2 = C{scale * (a2.x - w2 * sdot.x), scale * (a2.y - w2 * sdot.y)};
+  }
+  __syncthreads();
+  C* H = lds_fft<true, T>(F, G, TJ, ax.pz, tb.wz);
+  for (int idx = threadIdx.x; idx < nz * TJ; idx += blockDim.x) {
+    const int z = idx / TJ, j = idx % TJ;
+    if (j < 3 * tk) col[z * zstride + j] = H[j * nz + z];
+  }
+}
+
+
+struct PrecondImpl {
+  int nx, ny, nz, nxh;
+  int64_t rows;
+  Axes axes;
+  void* tw[3];
+  void* tab[3];
+  void* spec;
+  int64_t spec_stride;  // complex entries per RHS
+  int pairs_per_block, tj, tk;
+  size_t lds_x, lds_y, lds_z;
+};
+
+template <typename T>
+int upload_tables(PrecondImpl* im, const double h[3]) {
+  using C = typename CT<T>::type;
+  const int ns[3] = {im->nx, im->ny, im->nz};
+  const int counts[3] = {im->nxh, im->ny, im->nz};
+  for (int a = 0; a < 3; ++a) {
+    const int n = ns[a];
+    std::vector<C> w(n);
+    for (int k = 0; k < n; ++k) {
+      const double ang = 2.0 * M_PI * double(k) / double(n);
+      w[k] = C{T(std::cos(ang)), T(-std::sin(ang))};
+    }
+    VXL_HIP(hipMalloc(&im->tw[a], sizeof(C) * n));
+    VXL_HIP(hipMemcpy(im->tw[a], w.data(), sizeof(C) * n, hipMemcpyHostToDevice));
+    const int cnt = counts[a];
+    std::vector<T> tab(4 * size_t(cnt));
+    for (int k = 0; k < cnt; ++k) {
+      const double th = 2.0 * M_PI * double(k) / double(n);
+      const double sh = std::sin(0.5 * th);
+      const double m = (2.0 + std::cos(th)) / 3.0;
+      const double hh = h[a];
+      tab[k] = T(m);
+      tab[cnt + k] = T(4.0 * sh * sh / (hh * hh * m));
+      tab[2 * cnt + k] = T(std::sin(th) / (hh * m));
+      tab[3 * cnt + k] = T(4.0 * sh * sh * sh * sh / (3.0 * hh * hh * m * m));
+    }
+    VXL_HIP(hipMalloc(&im->tab[a], sizeof(T) * tab.size()));
+    VXL_HIP(hipMemcpy(im->tab[a], tab.data(), sizeof(T) * tab.size(), hipMemcpyHostToDevice));
+  }
+  return VXL_OK;
+}
+
+template <typename T>
+Tables<T> tables_of(const PrecondImpl* im) {
+  using C = typename CT<T>::type;
+  Tables<T> t;
+  t.wx = static_cast<const C*>(im->tw[0]);
+  t.wy = static_cast<const C*>(im->tw[1]);
+  t.wz = static_cast<const C*>(im->tw[2]);
+  t.gx = static_cast<const T*>(im->tab[0]);
+  t.gy = static_cast<const T*>(im->tab[1]);
+  t.gz = static_cast<const T*>(im->tab[2]);
+  return t;
+}
+
+constexpr size_t kMaxLds = 65536;
+
+template <typename T>
+int create_impl(vxl_precond* pc, PrecondImpl* im) {
+  using C = typename CT<T>::type;
+  const vxl_cell* cell = pc->cell;
+  im->nx = cell->grid.nx;
+  im->ny = cell->grid.ny;
+  im->nz = cell->grid.nz;
+  im->nxh = im->nx / 2 + 1;
+  im->rows = int64_t(im->ny) * im->nz;
+  im->axes.px = make_plan(im->nx);
+  im->axes.py = make_plan(im->ny);
+  im->axes.pz = make_plan(im->nz);
+  const int target = 2048;  // complex entries per LDS buffer
+  im->pairs_per_block = std::max(1, target / (3 * im->nx));
+  im->tj = std::max(1, std::min(3 * im->nxh, target / im->ny));
+  im->tk = std::max(1, std::min(im->nxh, target / (3 * im->nz)));
+  im->lds_x = 2 * sizeof(C) * size_t(3 * im->pairs_per_block) * im->nx;
+  im->lds_y = 2 * sizeof(C) * size_t(im->tj) * im->ny;
+  im->lds_z = 2 * sizeof(C) * size_t(3 * im->tk) * im->nz;
+  if (im->lds_x > kMaxLds || im->lds_y > kMaxLds || im->lds_z > kMaxLds)
+    return fail("grid axis too long for the LDS pencil FFT");
+  im->spec_stride = int64_t(im->nz) * im->ny * im->nxh * 3;
+  VXL_TRY(upload_tables<T>(im, cell->h));
+  VXL_HIP(hipMalloc(&im->spec, sizeof(C) * size_t(im->spec_stride) * pc->nrhs_max));
+  return VXL_OK;
+}
+
+template <typename T>
+int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s) {
+  using C = typename CT<T>::type;
+  const Tables<T> tb = tables_of<T>(im);
+  C* S = static_cast<C*>(im->spec);
+  const int64_t fstride = 3 * int64_t(im->nx) * im->rows;
+  const int64_t pairs = (im->rows + 1) / 2;
+  const unsigned bx = unsigned((pairs + im->pairs_per_block - 1) / im->pairs_per_block);
+  const int W = 3 * im->nxh;
+  hipLaunchKernelGGL((fft_x_r2c<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx,
+                     im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride);
+  VXL_HIP(hipGetLastError());
+  hipLaunchKernelGGL((fft_y<false, T>), dim3((W + im->tj - 1) / im->tj, im->nz, nrhs), dim3(kThreads),
+                     im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride);
+  VXL_HIP(hipGetLastError());
+  const double V = pc->cell->volume;
+  const double N = double(pc->cell->grid.n);
+  hipLaunchKernelGGL((fft_z_green<T>), dim3((im->nxh + im->tk - 1) / im->tk, im->ny, nrhs),
+                     dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh, im->tk, im->axes, tb,
+                     T(pc->lam), T(pc->mu), T(1.0 / (V * N)), im->spec_stride);
+  VXL_HIP(hipGetLastError());
+  hipLaunchKernelGGL((fft_y<true, T>), dim3((W + im->tj - 1) / im->tj, im->nz, nrhs), dim3(kThreads),
+                     im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride);
+  VXL_HIP(hipGetLastError());
+  hipLaunchKernelGGL((fft_x_c2r<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
+                     im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+
+}  // namespace
+
+int precond_create(vxl_precond* pc) {
+  auto* im = new PrecondImpl();
+  im->tw[0] = im->tw[1] = im->tw[2] = nullptr;
+  im->tab[0] = im->tab[1] = im->tab[2] = nullptr;
+  im->spec = nullptr;
+  pc->impl = im;
+  if (!(pc->lam >= 0.0) || !(pc->mu > 0.0)) return fail("reference medium requires lam >= 0, mu > 0");
+  return pc->precision == VXL_F64 ? create_impl<double>(pc, im) : create_impl<float>(pc, im);
+}
+
+int precond_apply(vxl_precond* pc, int nrhs, const void* r, void* z, hipStream_t s) {
+  auto* im = static_cast<PrecondImpl*>(pc->impl);
+  if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
+  if (pc->precision == VXL_F64)
+    return apply_impl<double>(pc, im, nrhs, static_cast<const double*>(r), static_cast<double*>(z), s);
+  return apply_impl<float>(pc, im, nrhs, static_cast<const float*>(r), static_cast<float*>(z), s);
+}
+
+void precond_release(vxl_precond* pc) {
+  auto* im = static_cast<PrecondImpl*>(pc->impl);
+  if (!im) return;
+  for (int a = 0; a < 3; ++a) {
+    if (im->tw[a]) (void)hipFree(im->tw[a]);
+    if (im->tab[a]) (void)hipFree(im->tab[a]);
+  }
+  if (im->spec) (void)hipFree(im->spec);
+  delete im;
+  pc->impl = nullptr;
+}
+
+}  // namespace vxl

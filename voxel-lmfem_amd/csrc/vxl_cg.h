@@ -1,0 +1,77 @@
+// CG vector-kernel launchers (cg.hip) and the FFT preconditioner (fft.hip).
+#pragma once
+
+#include "vxl_internal.h"
+
+namespace vxl {
+
+// Device scalar slots, scal[slot * nrhs + k].
+enum ScalarSlot { kRho = 0, kPQ, kAlpha, kBeta, kRR, kFF, kTmp, kNumSlots };
+// finalize_kernel operations.
+enum FinalizeOp { kOpStore = 0, kOpRho, kOpFF, kOpRR, kOpAlpha, kOpBeta };
+
+constexpr int kVecBlocks = 2048;  // max workgroups of a vector kernel (partials per RHS)
+
+int vec_blocks(int64_t len);
+
+template <typename A, typename B>
+int launch_dot(const A* a, const B* b, int64_t len, int nrhs, double* partial, const int* active,
+               hipStream_t s);
+int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, const int* active,
+                    const int* restart, hipStream_t s);
+template <typename T>
+int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, const double* alpha,
+                     double* partial, const int* active, hipStream_t s);
+template <typename T>
+int launch_update_p(T* p, const T* z, int64_t len, int nrhs, const double* beta, const int* active,
+                    hipStream_t s);
+int launch_residual(double* r, const double* f, const double* q, float* r32, int64_t len, int nrhs,
+                    double* partial, const int* active, hipStream_t s);
+int launch_accumulate(double* x64, float* x32, int64_t len, int nrhs, const int* active, hipStream_t s);
+template <typename A, typename B>
+int launch_convert(B* dst, const A* src, int64_t len, int nrhs, const int* active, hipStream_t s);
+template <typename T>
+int launch_fill_zero(T* dst, int64_t len, int nrhs, const int* active, hipStream_t s);
+// partial: >= 3 * nrhs * vec_blocks(n) doubles.
+int launch_project_zero_mean(double* u, int64_t n, int nrhs, double* partial, const int* active,
+                             hipStream_t s);
+
+}  // namespace vxl
+
+// FFT Green preconditioner handle (fft.hip).
+struct vxl_precond {
+  vxl_cell* cell;
+  int precision;
+  int nrhs_max;
+  double lam, mu;
+  void* impl;
+};
+
+namespace vxl {
+int precond_create(vxl_precond* pc);
+int precond_apply(vxl_precond* pc, int nrhs, const void* r, void* z, hipStream_t s);
+void precond_release(vxl_precond* pc);
+}  // namespace vxl
+
+// Solver handle (solver.hip).
+struct vxl_solver {
+  vxl_cell* cell;
+  int mode;
+  int R;
+  vxl_precond pc;
+  bool pc_ready;
+  int64_t len;  // 3N
+  double *f, *x64, *r64, *q64;
+  void *x, *r, *p, *q, *z;  // inner precision (aliases of the FP64 buffers in FP64 mode)
+  double* scal;
+  double* partial;
+  int nblk;
+  int* active_dev;
+  int* restart_dev;
+  double* macro_dev;
+  double* stress_partial;
+  double* ch_dev;
+  hipEvent_t ev[4];
+  double ms[3];
+};
+

@@ -1,0 +1,94 @@
+// Internal declarations shared by the libvxl translation units (gfx950 only).
+//
+// Field layout in HBM (every precision):  field[r][c][z][y][x]
+//   r = right-hand side (load case), c = displacement component x,y,z,
+//   node id (z*ny + y)*nx + x, periodic in all three axes.
+// Material layout: one row index per voxel (uint8/uint16/int32) into a
+// coefficient table of 24-wide padded rows holding the 21 upper-triangle
+// entries of the engineering-Voigt 6x6 stiffness (pairs i<=j row major,
+// 11,22,33,23,13,12 order).  An optional FP32 per-voxel factor multiplies the
+// row (continuous density fields).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/vxl.h"
+
+namespace vxl {
+
+constexpr int kRowWidth = 24;  // padded C21 row
+
+void set_error(const std::string& message);
+int fail(const std::string& message);  // sets the error text, returns VXL_ERR_*
+
+#define VXL_HIP(...)                                                            \
+  do {                                                                          \
+    hipError_t _e = (__VA_ARGS__);                                              \
+    if (_e != hipSuccess)                                                       \
+      return ::vxl::fail(std::string("HIP error in ") + #__VA_ARGS__ + ": " +   \
+                         hipGetErrorString(_e));                                \
+  } while (0)
+
+#define VXL_TRY(...)                                                            \
+  do {                                                                          \
+    int _rc = (__VA_ARGS__);                                                    \
+    if (_rc != VXL_OK) return _rc;                                              \
+  } while (0)
+
+// Geometric factors of the modal FI element action.  For strain class k
+// (0: constant, 1: linear, 2: bilinear) and axis a, g[k][a] = w_k^(1/2) *
+// V^(1/2) / (4 h_a) with w = {1, 1/3, 1/9}: the Gauss weights and the element
+// volume are split evenly between the strain and the back-projection so the
+// element matrix stays exactly symmetric.
+struct ModalFactors {
+  double g[3][3];
+};
+ModalFactors modal_factors(const double h[3]);
+
+struct Grid {
+  int nx, ny, nz;
+  int64_t n;
+};
+
+// Material view used by the kernels (device pointers).
+struct MaterialView {
+  int index_kind;           // VXL_INDEX_U8 / U16 / I32
+  const void* index;        // per-voxel row
+  const double* c64;        // rows x 24
+  const float* c32;         // rows x 24
+  const float* scale;       // optional per-voxel factor (nullptr: 1)
+};
+
+}  // namespace vxl
+
+// Opaque handle types (complete definitions; the C header only forward-declares).
+struct vxl_cell {
+  vxl::Grid grid;
+  double h[3];
+  double volume;
+  int rows;
+  vxl::MaterialView mat;
+  double* c64_owned;
+  float* c32_owned;
+  hipStream_t stream;
+  int device;
+};
+
+namespace vxl {
+
+// ---- stencil.hip -------------------------------------------------------------
+template <typename T>
+int launch_apply(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s);
+int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, hipStream_t s);
+// partial: workspace of stress_partial_size(cell, nload) doubles; out: nload x 6
+int launch_avg_stress(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
+                      double* partial, double* out_dev, hipStream_t s);
+int64_t stress_partial_size(const vxl_cell* cell, int nload);
+int launch_element_energy(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
+                          double* energy, hipStream_t s);
+
+}  // namespace vxl

@@ -1,0 +1,19 @@
+"""MI355X-native periodic HEX8 homogenization: the effective-stiffness solve path.
+
+Drop-in for the GPU backend of Voxel-LMFEM's ``homogenize`` (same signature,
+inputs and result keys); the arithmetic runs in libvxl.so, hand-written gfx950
+HIP kernels reached through a C ABI.
+"""
+
+__version__ = "0.1.3+mi355x"
+
+
+def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="gpu", warmup=False,
+               precision="mixed"):
+    """Solve six unit-strain loads; returns the reference's result dictionary."""
+    from .solver import homogenize as solve
+    return solve(phase_id, materials, domain_size, orientation=orientation, backend=backend,
+                 warmup=warmup, precision=precision)
+
+
+__all__ = ["homogenize", "__version__"]

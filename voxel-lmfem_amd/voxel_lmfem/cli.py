@@ -1,0 +1,67 @@
+"""Command line: ``python -m voxel_lmfem solve ... [--backend gpu|cuda]``.
+
+Mirrors the reference's `solve` subcommand (src/voxel_lmfem/cli.py:166-171,
+213-223, 35-40, 234-235): NPZ + JSON input, refuses an existing output,
+writes JSON with open('x') and allow_nan=False, prints
+{output, backend, max_true_residual}, and exits with status 2 and
+"error: ..." on input or solver errors.  `python -O` is refused.
+"""
+import argparse
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+from . import __version__
+
+
+def _plain(value):
+    if isinstance(value, dict):
+        return {k: _plain(v) for k, v in value.items()}
+    if isinstance(value, (list, tuple)):
+        return [_plain(v) for v in value]
+    if isinstance(value, np.ndarray):
+        return value.tolist()
+    if isinstance(value, np.generic):
+        return value.item()
+    return value
+
+
+def _save(path, value):
+    path.parent.mkdir(parents=True, exist_ok=True)
+    with path.open("x", encoding="utf-8") as stream:
+        json.dump(_plain(value), stream, indent=2, allow_nan=False)
+
+
+def main(argv=None):
+    argv = list(sys.argv[1:] if argv is None else argv)
+    if sys.flags.optimize:
+        raise SystemExit("Run without -O: numerical assertions are required.")
+    parser = argparse.ArgumentParser(prog="voxel-lmfem",
+                                     description="Periodic fully integrated HEX8 elastic homogenization (MI355X).")
+    parser.add_argument("--version", action="version", version=__version__)
+    sub = parser.add_subparsers(dest="command", required=True)
+    solve = sub.add_parser("solve", help="Homogenize an NPZ phase field and JSON material table")
+    solve.add_argument("--input", type=Path, required=True)
+    solve.add_argument("--materials", type=Path, required=True)
+    solve.add_argument("--output", type=Path, required=True)
+    solve.add_argument("--backend", choices=("gpu", "hip", "cuda", "cpu"), default="gpu")
+    solve.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
+    solve.add_argument("--warmup", action="store_true")
+    args = parser.parse_args(argv)
+    try:
+        if args.command == "solve":
+            if args.output.exists():
+                raise FileExistsError(args.output)
+            from .inputs import load_input
+            from .solver import homogenize
+            phase, materials, domain, orientation = load_input(args.input, args.materials)
+            result = homogenize(phase, materials, domain, orientation=orientation, backend=args.backend,
+                                warmup=args.warmup, precision=args.precision)
+            _save(args.output, result)
+            print(json.dumps(dict(output=str(args.output), backend=result["backend"],
+                                  max_true_residual=result["max_true_residual"])))
+    except (ValueError, FileNotFoundError, FileExistsError, RuntimeError) as exc:
+        parser.exit(2, f"error: {exc}\n")
+    return 0

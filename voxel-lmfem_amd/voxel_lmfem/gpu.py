@@ -1,0 +1,226 @@
+"""Device objects over libvxl: cell (operator), FFT Green preconditioner, solver.
+
+PyTorch supplies device memory and the current stream; all arithmetic runs in
+the HIP kernels of libvxl.so.  The operator object exposes the reference's
+operator protocol (src/voxel_lmfem/_core/materials/hybrid_adapter.py:39-125:
+``B``, ``mesh``, ``apply``, ``compute_rhs``, ``compute_avg_stress``,
+``project_zero_mean``, ``apply_precond``) on (1, 3N) FP64 node-major tensors,
+and component-plane batched entry points used by the solver.
+"""
+import ctypes
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import _native
+from ._native import check, darray, dptr
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("No HIP device is available to PyTorch; the GPU backend needs an MI355X")
+    return torch
+
+
+def _stream(torch):
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Cell:
+    """Grid + material field resident on the current device."""
+
+    def __init__(self, field, shape, h, *, scale=None, device=None):
+        torch = _torch()
+        self.lib = _native.load()
+        self.torch = torch
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
+        self.shape = tuple(int(s) for s in shape)
+        self.h = tuple(float(v) for v in h)
+        self.n = int(np.prod(self.shape))
+        self.field = field
+        index = np.ascontiguousarray(field.index)
+        if index.dtype == np.uint16:
+            index = index.view(np.int16)
+        self.index = torch.from_numpy(index).to(self.device)
+        self.scale = None
+        if scale is not None:
+            self.scale = torch.as_tensor(np.ascontiguousarray(scale, dtype=np.float32).reshape(-1),
+                                         device=self.device)
+            if self.scale.numel() != self.n:
+                raise ValueError("scale must have one value per voxel")
+        rows, rows_p = darray(field.rows.reshape(-1, 21))
+        handle = ctypes.c_void_p()
+        nz, ny, nx = self.shape
+        check(self.lib.vxl_cell_create(ctypes.byref(handle), nz, ny, nx, self.h[0], self.h[1], self.h[2],
+                                       field.index_kind, dptr(self.index), len(rows), rows_p,
+                                       dptr(self.scale) if self.scale is not None else None,
+                                       _stream(torch)), "vxl_cell_create")
+        self.handle = handle
+        self.mesh = SimpleNamespace(Nz=nz, Ny=ny, Nx=nx, dx=self.h[0], dy=self.h[1], dz=self.h[2],
+                                    n_elem=self.n, n_nodes=self.n, n_dofs=3 * self.n,
+                                    V_elem=float(np.prod(self.h)))
+        self.B = 1
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.vxl_cell_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- component-plane batched entry points ---------------------------------
+    def apply_planes(self, u, out=None):
+        """u: (R, 3, N) float32/float64 device tensor -> K u (same layout)."""
+        torch = self.torch
+        if u.dim() != 3 or u.shape[1:] != (3, self.n) or u.device != self.device:
+            raise ValueError("expected a (R, 3, N) tensor on the cell's device")
+        precision = {torch.float32: _native.VXL_F32, torch.float64: _native.VXL_F64}.get(u.dtype)
+        if precision is None:
+            raise ValueError("apply supports float32 and float64")
+        u = u.contiguous()
+        out = torch.empty_like(u) if out is None else out
+        check(self.lib.vxl_apply(self.handle, precision, u.shape[0], dptr(u), dptr(out)), "vxl_apply")
+        return out
+
+    def rhs_planes(self, macro):
+        macro, p = darray(np.atleast_2d(macro))
+        out = self.torch.empty((len(macro), 3, self.n), dtype=self.torch.float64, device=self.device)
+        check(self.lib.vxl_rhs(self.handle, len(macro), p, dptr(out)), "vxl_rhs")
+        return out
+
+    def avg_stress_planes(self, u, macro):
+        macro, p = darray(np.atleast_2d(macro))
+        u = u.contiguous()
+        if u.dtype != self.torch.float64 or u.shape != (len(macro), 3, self.n):
+            raise ValueError("expected FP64 (R, 3, N) displacements, one per macro strain")
+        out = np.empty((len(macro), 6))
+        check(self.lib.vxl_avg_stress(self.handle, len(macro), dptr(u), p,
+                                      out.ctypes.data_as(_native._DP)), "vxl_avg_stress")
+        return out
+
+    def project_planes(self, u):
+        check(self.lib.vxl_project_zero_mean(self.handle, u.shape[0], dptr(u)), "vxl_project_zero_mean")
+        return u
+
+    # -- reference operator protocol on (1, 3N) node-major FP64 -----------------
+    def _to_planes(self, v):
+        if v.shape != (1, 3 * self.n) or v.dtype != self.torch.float64 or not v.is_contiguous():
+            raise ValueError("Expected contiguous FP64 displacement (1,3N) on the material device")
+        return v.view(self.n, 3).t().contiguous().view(1, 3, self.n)
+
+    def _from_planes(self, v):
+        return v.view(3, self.n).t().contiguous().view(1, 3 * self.n)
+
+    def apply(self, u):
+        return self._from_planes(self.apply_planes(self._to_planes(u)))
+
+    def compute_rhs(self, macro):
+        if np.shape(macro) != (6,):
+            raise ValueError("Six engineering macro strain components required")
+        return self._from_planes(self.rhs_planes(np.asarray(macro, dtype=float)[None]))
+
+    def compute_avg_stress(self, u, macro):
+        if np.shape(macro) != (6,):
+            raise ValueError("Six engineering macro strain components required")
+        return self.avg_stress_planes(self._to_planes(u), np.asarray(macro, dtype=float)[None])
+
+    def project_zero_mean(self, value):
+        u = value.view(1, self.n, 3)
+        return (u - u.mean(dim=1, keepdim=True)).reshape(1, -1)
+
+
+class Preconditioner:
+    """FFT Green preconditioner of the isotropic reference medium (lam, mu)."""
+
+    def __init__(self, cell, lam, mu, *, precision="float32", nrhs_max=6):
+        self.cell = cell
+        torch = cell.torch
+        self.dtype = torch.float32 if precision in ("float32", "fp32", "mixed") else torch.float64
+        prec = _native.VXL_F32 if self.dtype == torch.float32 else _native.VXL_F64
+        handle = ctypes.c_void_p()
+        check(cell.lib.vxl_precond_create(ctypes.byref(handle), cell.handle, prec, float(lam), float(mu),
+                                          int(nrhs_max)), "vxl_precond_create")
+        self.handle = handle
+        self.nrhs_max = nrhs_max
+
+    def apply_planes(self, r, out=None):
+        torch = self.cell.torch
+        if r.dtype != self.dtype or r.dim() != 3 or r.shape[1:] != (3, self.cell.n):
+            raise ValueError("expected (R, 3, N) in the preconditioner's precision")
+        r = r.contiguous()
+        out = torch.empty_like(r) if out is None else out
+        check(self.cell.lib.vxl_precond_apply(self.handle, r.shape[0], dptr(r), dptr(out)), "vxl_precond_apply")
+        return out
+
+    def apply_precond(self, residual):
+        """Reference protocol: (1,3N) FP64 node-major in and out."""
+        planes = self.cell._to_planes(residual).to(self.dtype)
+        return self.cell._from_planes(self.apply_planes(planes).to(self.cell.torch.float64))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.cell.lib.vxl_precond_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """Block multi-RHS C_eff solver (mode 'mixed' or 'fp64')."""
+
+    def __init__(self, cell, lam, mu, *, mode="mixed", nrhs=6):
+        self.cell = cell
+        self.mode = mode
+        code = {"mixed": _native.VXL_SOLVE_MIXED, "fp64": _native.VXL_SOLVE_FP64}.get(mode)
+        if code is None:
+            raise ValueError("precision must be 'mixed' or 'fp64'")
+        handle = ctypes.c_void_p()
+        check(cell.lib.vxl_solver_create(ctypes.byref(handle), cell.handle, code, int(nrhs), float(lam),
+                                         float(mu)), "vxl_solver_create")
+        self.handle = handle
+        self.nrhs = nrhs
+
+    def run(self, macro=None, *, tol=1e-8, max_iter=10000, return_fields=False):
+        torch = self.cell.torch
+        macro = np.eye(6)[: self.nrhs] if macro is None else np.atleast_2d(macro)
+        if macro.shape != (self.nrhs, 6):
+            raise ValueError("macro strains must be (nrhs, 6)")
+        m, mp = darray(macro)
+        ch = np.empty((6, self.nrhs))
+        its = (ctypes.c_int * self.nrhs)()
+        rst = (ctypes.c_int * self.nrhs)()
+        res = (ctypes.c_double * self.nrhs)()
+        fields = None
+        if return_fields:
+            fields = torch.empty((self.nrhs, 3, self.cell.n), dtype=torch.float64, device=self.cell.device)
+        rc = self.cell.lib.vxl_solver_run(self.handle, mp, float(tol), int(max_iter),
+                                          ch.ctypes.data_as(_native._DP), its, rst, res,
+                                          dptr(fields) if fields is not None else None)
+        ms = (ctypes.c_double * 3)()
+        self.cell.lib.vxl_solver_timings(self.handle, ms)
+        out = dict(CH=ch, iterations=list(its), restarts=list(rst), true_residuals=list(res),
+                   device_ms=dict(setup_rhs=ms[0], iterations=ms[1], certify_stress=ms[2]))
+        if fields is not None:
+            out["fields"] = fields
+        check(rc, "vxl_solver_run")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.cell.lib.vxl_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

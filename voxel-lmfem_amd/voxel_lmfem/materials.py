@@ -1,0 +1,226 @@
+"""Host-side material law and label/orientation bookkeeping.
+
+Mirrors the reference's material interface
+(src/voxel_lmfem/_core/materials/material_field.py:14-114,
+femlib/materials.py:24-49) with the same accepted inputs, validation rules and
+error messages, and turns a labelled cell into the compact device form used by
+libvxl: one small integer row index per voxel plus a table of 21-coefficient
+rows.
+
+Row layout (same order as the reference's hybrid pool,
+src/voxel_lmfem/_core/materials/hybrid_material.py:38-61): rows [0, L) are the
+phase tensors in ascending label order; local-frame voxels get Bond-rotated
+rows after them, label by label, voxels in C order.  Unlike the reference,
+voxels whose frames are bit-identical share one row (an exact de-duplication:
+the same R gives the same rotated row bit for bit), so per-grain or
+per-layer frames cost one row each instead of one row per voxel.
+"""
+from collections.abc import Mapping
+from numbers import Integral
+
+import numpy as np
+
+PAIRS = tuple((i, j) for i in range(6) for j in range(i, 6))
+_PI = np.array([i for i, _ in PAIRS])
+_PJ = np.array([j for _, j in PAIRS])
+ORTHOTROPIC_KEYS = ("E1", "E2", "E3", "nu12", "nu13", "nu23", "G12", "G13", "G23")
+
+
+def isotropic_voigt(E, nu):
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    mu = E / (2.0 * (1.0 + nu))
+    C = np.zeros((6, 6))
+    C[:3, :3] = lam
+    C[np.arange(3), np.arange(3)] += 2.0 * mu
+    C[np.arange(3, 6), np.arange(3, 6)] = mu
+    return C
+
+
+def orthotropic_voigt(E1, E2, E3, nu12, nu13, nu23, G12, G13, G23):
+    S = np.zeros((6, 6))
+    S[0, 0], S[1, 1], S[2, 2] = 1.0 / E1, 1.0 / E2, 1.0 / E3
+    S[0, 1] = S[1, 0] = -nu12 / E1
+    S[0, 2] = S[2, 0] = -nu13 / E1
+    S[1, 2] = S[2, 1] = -nu23 / E2
+    S[3, 3], S[4, 4], S[5, 5] = 1.0 / G23, 1.0 / G13, 1.0 / G12
+    return np.linalg.inv(S)
+
+
+def stiffness(spec):
+    """One phase specification -> validated symmetric positive-definite 6x6."""
+    kind = spec.get("kind")
+    if spec.get("frame", "global") not in ("global", "local"):
+        raise ValueError("Material frame must be 'global' or 'local'")
+    if kind == "isotropic":
+        E, nu = float(spec["E"]), float(spec["nu"])
+        if not (np.isfinite([E, nu]).all() and E > 0 and -1 < nu < .5):
+            raise ValueError("Isotropic elasticity requires E>0 and -1<nu<0.5")
+        C = isotropic_voigt(E, nu)
+    elif kind == "orthotropic":
+        v = np.array([spec[key] for key in ORTHOTROPIC_KEYS], dtype=float)
+        if not np.isfinite(v).all() or (v[[0, 1, 2, 6, 7, 8]] <= 0).any():
+            raise ValueError("Orthotropic moduli must be finite and positive")
+        try:
+            C = orthotropic_voigt(*v)
+        except np.linalg.LinAlgError as exc:
+            raise ValueError("Singular orthotropic compliance") from exc
+    elif kind == "anisotropic":
+        C = np.asarray(spec["C"], dtype=np.float64)
+    else:
+        raise ValueError("kind must be isotropic, orthotropic or anisotropic")
+    if C.shape != (6, 6) or not np.isfinite(C).all():
+        raise ValueError("C must be a finite engineering-Voigt 6x6 matrix")
+    if not np.allclose(C, C.T, rtol=0, atol=1e-12 * np.max(np.abs(C))):
+        raise ValueError("Elastic stiffness must be symmetric")
+    C = (C + C.T) * .5
+    try:
+        np.linalg.cholesky(C)
+    except np.linalg.LinAlgError as exc:
+        raise ValueError("Elastic stiffness must be positive definite") from exc
+    return C
+
+
+def phase_table(materials):
+    """Sorted int64 labels, (L,6,6) tensors, local-frame flags."""
+    if not isinstance(materials, Mapping) or not materials:
+        raise ValueError("materials must be a nonempty mapping of integer labels")
+    if any(not isinstance(k, Integral) or isinstance(k, bool) for k in materials):
+        raise ValueError("Material labels must be integers")
+    labels = np.array(sorted(materials), dtype=np.int64)
+    specs = [materials[int(k)] for k in labels]
+    tensors = np.stack([stiffness(s) for s in specs])
+    local = np.array([s["kind"] != "isotropic" and s.get("frame", "global") == "local" for s in specs])
+    return labels, tensors, local
+
+
+def frames(orientation, shape, flat_indices):
+    """Material frames (k,3,3) at the selected voxels; columns = local axes.
+
+    Accepts a rotation array broadcastable to shape+(3,3) or a mapping with
+    axis1/axis2 broadcastable to shape+(3,).
+    """
+    coordinates = np.unravel_index(flat_indices, shape)
+
+    def pick(values, tail):
+        try:
+            view = np.broadcast_to(np.asarray(values, dtype=float), tuple(shape) + tail)
+        except ValueError as exc:
+            raise ValueError("Orientation shape is not broadcastable to phase_id") from exc
+        out = np.array(view[coordinates], copy=True)
+        if not np.isfinite(out).all():
+            raise ValueError("Active local-frame orientations must be finite")
+        return out
+
+    if isinstance(orientation, Mapping):
+        a, b = pick(orientation["axis1"], (3,)), pick(orientation["axis2"], (3,))
+        na, nb = np.linalg.norm(a, axis=-1), np.linalg.norm(b, axis=-1)
+        if (na <= 0).any() or (nb <= 0).any():
+            raise ValueError("Orientation axes must be nonzero")
+        a, b = a / na[:, None], b / nb[:, None]
+        c = np.cross(a, b)
+        nc = np.linalg.norm(c, axis=-1)
+        if (nc <= 1e-12).any():
+            raise ValueError("Orientation axes must not be collinear")
+        c /= nc[:, None]
+        return np.stack([a, np.cross(c, a), c], axis=-1)
+    R = pick(orientation, (3, 3))
+    if not np.allclose(R.swapaxes(-1, -2) @ R, np.eye(3), rtol=0, atol=1e-10):
+        raise ValueError("Orientation must be an orthonormal rotation matrix")
+    if not np.allclose(np.linalg.det(R), 1, rtol=0, atol=1e-10):
+        raise ValueError("Orientation must be right-handed (det=+1)")
+    return R
+
+
+def bond_rotate(C, R):
+    """Rotated stiffness rows (k,21) = pack(T C T^T), T the Bond stress matrix."""
+    R = np.asarray(R, dtype=np.float64).reshape(-1, 3, 3)
+    i, j = [1, 0, 0], [2, 2, 1]
+    r1, r2 = R[:, i, :], R[:, j, :]
+    T = np.concatenate((np.concatenate((R * R, 2 * R[:, :, i] * R[:, :, j]), -1),
+                        np.concatenate((r1 * r2, r1[:, :, i] * r2[:, :, j] + r1[:, :, j] * r2[:, :, i]), -1)),
+                       -2)
+    TC = T @ C
+    return np.stack([(TC[:, a, :] * T[:, b, :]).sum(-1) for a, b in PAIRS], -1)
+
+
+def pack21(C):
+    return np.asarray(C)[..., _PI, _PJ]
+
+
+def unpack21(rows):
+    rows = np.asarray(rows)
+    out = np.zeros(rows.shape[:-1] + (6, 6))
+    out[..., _PI, _PJ] = rows
+    out[..., _PJ, _PI] = rows
+    return out
+
+
+def _unique_frames(R):
+    """Exact de-duplication of (k,3,3) frames: (unique (u,3,3), inverse (k,))."""
+    flat = np.ascontiguousarray(R.reshape(len(R), 9))
+    keys = flat.view(np.dtype((np.void, flat.dtype.itemsize * 9))).ravel()
+    _, first, inverse = np.unique(keys, return_index=True, return_inverse=True)
+    return R[first], inverse.reshape(-1)
+
+
+class MaterialField:
+    """Device-ready material description of one cell.
+
+    rows   : (rows, 21) FP64 coefficient table
+    index  : (N,) row of every voxel (uint8/uint16/int32), C order over z,y,x
+    labels, tensors, local : the validated phase table
+    """
+
+    def __init__(self, rows, index, labels, tensors, local):
+        self.rows = rows
+        self.index = index
+        self.labels = labels
+        self.tensors = tensors
+        self.local = local
+
+    @property
+    def index_kind(self):
+        return {np.dtype(np.uint8): 1, np.dtype(np.uint16): 2, np.dtype(np.int32): 4}[self.index.dtype]
+
+    def voxel_rows(self, positions):
+        return self.rows[self.index[positions]]
+
+
+def build_field(phase, materials, orientation=None, *, deduplicate=True):
+    """Label/orientation bookkeeping of one cell -> MaterialField."""
+    phase = np.asarray(phase)
+    labels, tensors, local = phase_table(materials)
+    flat = phase.reshape(-1)
+    slot = np.searchsorted(labels, flat)
+    if (slot >= len(labels)).any() or not np.array_equal(labels[np.minimum(slot, len(labels) - 1)], flat):
+        used = np.unique(flat)
+        raise ValueError(f"Undefined material labels: {used[~np.isin(used, labels)].tolist()}")
+    table = [pack21(tensors)]
+    index = slot.astype(np.int64)
+    offset = len(labels)
+    for p in np.flatnonzero(local):
+        selected = np.flatnonzero(slot == p)
+        if not selected.size:
+            continue
+        if orientation is None:
+            raise ValueError("Local-frame anisotropic phases require orientation")
+        R = frames(orientation, phase.shape, selected)
+        if deduplicate:
+            uniq, inverse = _unique_frames(R)
+            table.append(bond_rotate(tensors[p], uniq))
+            index[selected] = offset + inverse
+            offset += len(uniq)
+        else:
+            table.append(bond_rotate(tensors[p], R))
+            index[selected] = offset + np.arange(selected.size)
+            offset += selected.size
+    rows = np.concatenate(table, axis=0)
+    if len(rows) <= 256:
+        dtype = np.uint8
+    elif len(rows) <= 65536:
+        dtype = np.uint16
+    elif len(rows) <= np.iinfo(np.int32).max:
+        dtype = np.int32
+    else:
+        raise ValueError("Coefficient table exceeds int32 indexing")
+    return MaterialField(rows, index.astype(dtype), labels, tensors, local)

@@ -1,0 +1,85 @@
+"""Isotropic reference medium (lam0, mu0) of the FFT Green preconditioner.
+
+Same selection as the reference chain:
+  * Zeman sampling (src/voxel_lmfem/_core/materials/hybrid_adapter.py:101-125):
+    every max(1, n//16)-th voxel per axis; per sample mu_s = (C44+C55+C66)/4
+    and K_s = (sum Cii + 2 sum_{i<j<=3} Cij)/9; mu0 and K0 are the midpoints of
+    the sampled extremes; lam0 = max(0, K0 - 2 mu0/3);
+  * closed-form soft-phase design
+    (src/voxel_lmfem/_core/solver_chain/reference_closed_form.py:45-171):
+    an isotropic phase that is Loewner-below every used phase, scaled to the
+    Zeman shear modulus, replaces the Zeman medium when it strictly lowers the
+    material energy bound max lambda_max / min lambda_min.
+The choice changes iteration counts, never the converged C_eff.
+"""
+import numpy as np
+from scipy.linalg import eigvalsh
+
+
+def _iso(lam, mu):
+    C = np.zeros((6, 6))
+    C[:3, :3] = lam
+    C[np.arange(3), np.arange(3)] += 2 * mu
+    C[np.arange(3, 6), np.arange(3, 6)] = mu
+    return C
+
+
+def zeman(field, shape):
+    nz, ny, nx = shape
+    z = np.arange(0, nz, max(1, nz // 16))
+    y = np.arange(0, ny, max(1, ny // 16))
+    x = np.arange(0, nx, max(1, nx // 16))
+    pos = (z[:, None, None] * (ny * nx) + y[None, :, None] * nx + x[None, None, :]).reshape(-1)
+    c = field.voxel_rows(pos)  # (k, 21) packed: indices of C44,C55,C66 = 15, 18, 20
+    mus = .5 * (c[:, 15] + c[:, 18] + c[:, 20]) / 2.
+    Ks = (c[:, 0] + c[:, 6] + c[:, 11] + 2 * (c[:, 1] + c[:, 2] + c[:, 7])) / 9.
+    mu = .5 * (float(mus.max()) + float(mus.min()))
+    K = .5 * (float(Ks.max()) + float(Ks.min()))
+    return _iso(max(0., K - 2 * mu / 3.), mu)
+
+
+def _bound(phases, reference):
+    spectra = [eigvalsh(c, reference, check_finite=False) for c in phases]
+    return float(max(s[-1] for s in spectra)) / float(min(s[0] for s in spectra))
+
+
+def _is_isotropic(C, rtol=1e-10):
+    bulk = float(np.sum(C[:3, :3]) / 9.0)
+    mu = float(np.trace(C[3:, 3:]) / 3.0)
+    cand = _iso(bulk - 2.0 * mu / 3.0, mu)
+    return float(np.max(np.abs(C - cand)) / np.max(np.abs(C))) <= rtol
+
+
+def design(phases, baseline, *, order_rtol=1e-10, tie_rtol=1e-10):
+    """Return (reference 6x6, status in {'closed_form','baseline','unsupported'})."""
+    phases = [0.5 * (np.asarray(c, float) + np.asarray(c, float).T) for c in phases]
+    base_bound = _bound(phases, baseline)
+    mu = float(baseline[3, 3])
+    best = None
+    for soft in phases:
+        if not _is_isotropic(soft):
+            continue
+        spectra = [eigvalsh(c, soft, check_finite=False) for c in phases]
+        if min(v[0] for v in spectra) < 1.0 - order_rtol:
+            continue
+        ref = (mu / float(soft[3, 3])) * soft
+        b = _bound(phases, ref)
+        if best is None or b < best[1]:
+            best = (ref, b)
+    if best is None:
+        return baseline.copy(), "unsupported"
+    if base_bound <= best[1] * (1.0 + tie_rtol):
+        return baseline.copy(), "baseline"
+    return best[0], "closed_form"
+
+
+def select(field, phase, materials):
+    """(lam0, mu0, metadata) for a cell."""
+    from .materials import stiffness
+    base = zeman(field, phase.shape)
+    used = np.unique(phase)
+    phases = [stiffness(materials[int(k)]) for k in used]
+    ref, status = design(phases, base)
+    lam, mu = float(ref[0, 1]), float(ref[3, 3])
+    return lam, mu, dict(status=status, lam=lam, mu=mu, zeman_lam=float(base[0, 1]),
+                         zeman_mu=float(base[3, 3]))
