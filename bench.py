@@ -1,0 +1,195 @@
+"""Benchmark: 256^3 polycrystal C_eff solves/s on one MI355X.
+
+Workload (the cell the metric is quoted on): 256^3 periodic Voronoi
+polycrystal, 200 grains, cubic C11/C12/C44 = 168.4/121.4/75.4 GPa, one
+uniform random rotation per grain (grain label + per-grain rotated tensor),
+six unit engineering-strain loads solved together.  A step is one full C_eff
+solve on resident inputs: RHS, block PCG (FP32 operator / FFT preconditioner /
+CG with FP64 reliable updates), FP64 true-residual certificate <= 1e-8 and
+stress recovery.  The plain FP64 solve of the same cell is timed beside it and
+the C_eff difference between the two is reported.
+
+roofline: the FP32 six-RHS stencil apply (the dominant kernel), timed with HIP
+events on the stream it runs on; algorithmic bytes per launch =
+N (12 R read + 12 R write + 1 label byte), R = 6.
+cpu_baseline: the FP64 NumPy oracle (operator apply + Green preconditioner)
+timed at 64^3 on the host and extrapolated to a 256^3 six-load solve with the
+GPU's iteration count (labelled as extrapolated).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "voxel-lmfem_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(iterations_total, n_target):
+    """Time the FP64 oracle's per-iteration kernels at 64^3; extrapolate."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import fem, spectral
+    from voxel_lmfem.cells import voronoi_polycrystal
+    from voxel_lmfem import materials as pm
+    n = 64
+    (phase, mats, dom, _), _ = voronoi_polycrystal(n, grains=200, seed=3)
+    f = pm.build_field(phase, mats)
+    full = pm.unpack21(f.rows[f.index.astype(np.int64)])
+    h = dom / n
+    op = fem.MatrixFree(full, phase.shape, h)
+    u = np.random.default_rng(0).normal(size=(n ** 3, 3))
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.apply(u)
+    t_apply = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        spectral.green_apply(u, phase.shape, h, 50.0, 40.0)
+    t_green = (time.perf_counter() - t0) / reps
+    scale = (n_target / n) ** 3
+    per_iter = (t_apply + t_green) * scale
+    seconds = per_iter * iterations_total
+    cores = 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return dict(value=1.0 / seconds, unit="solves/s", cores=1, kind="port",
+                sample=(f"FP64 NumPy oracle: Gauss-B operator apply {t_apply:.3f} s + FFT Green apply "
+                        f"{t_green:.3f} s per RHS-iteration at 64^3 (200-grain polycrystal), x{scale:.0f} to "
+                        f"{n_target}^3, x{iterations_total} RHS-iterations of the GPU run; extrapolated, "
+                        f"single-threaded numpy ({cores} cores visible)"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps(dict(error="this build measures one GPU; multi-GPU sharding is design-only "
+                                    "(see DESIGN.md)", n_gpus=args.gpus)))
+        sys.exit(2)
+    import torch
+    from voxel_lmfem.cells import voronoi_polycrystal
+    from voxel_lmfem import materials as pm
+    from voxel_lmfem.gpu import Cell, Solver
+    from voxel_lmfem.reference_medium import select
+
+    n = args.n
+    t0 = time.perf_counter()
+    (phase, mats, dom, _), meta = voronoi_polycrystal(n, grains=200, seed=3)
+    field = pm.build_field(phase, mats)
+    lam, mu, rmeta = select(field, phase, mats)
+    h = dom / np.array(phase.shape[::-1], dtype=float)
+    print(f"# generated {n}^3 polycrystal in {time.perf_counter() - t0:.1f} s; reference medium "
+          f"{rmeta['status']} lam={lam:.3f} mu={mu:.3f}", file=sys.stderr, flush=True)
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    cell = Cell(field, phase.shape, h)
+    solver = Solver(cell, lam, mu, mode="mixed", nrhs=6)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+
+    for _ in range(args.warmup):
+        solver.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(args.steps):
+        out = solver.run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    its = out["iterations"]
+    print(f"# mixed: {ms_per_step:.1f} ms/solve, iterations {its}, restarts {out['restarts']}, "
+          f"residuals {max(out['true_residuals']):.2e}, device ms {out['device_ms']}", file=sys.stderr, flush=True)
+    CH_mixed = out["CH"]
+
+    # FP64 solve of the same cell.
+    fp64 = None
+    if not args.skip_fp64:
+        s64 = Solver(cell, lam, mu, mode="fp64", nrhs=6)
+        s64.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        o64 = s64.run()
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        err = float(np.linalg.norm(CH_mixed - o64["CH"]) / np.linalg.norm(o64["CH"]))
+        fp64 = dict(ms_per_step=1e3 * t64, iterations=o64["iterations"], speedup=t64 / (elapsed / args.steps),
+                    ceff_rel_err_mixed_vs_fp64=err)
+        print(f"# fp64: {1e3 * t64:.1f} ms/solve, iterations {o64['iterations']}, mixed-vs-fp64 rel err {err:.2e}",
+              file=sys.stderr, flush=True)
+        s64.close()
+
+    # Stencil roofline: FP32 six-RHS apply, HIP events on the current stream.
+    N = cell.n
+    R = 6
+    u = torch.randn(R, 3, N, dtype=torch.float32, device="cuda")
+    o = torch.empty_like(u)
+    for _ in range(3):
+        cell.apply_planes(u, o)
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        cell.apply_planes(u, o)
+    e1.record()
+    torch.cuda.synchronize()
+    t_apply = e0.elapsed_time(e1) / reps * 1e-3
+    alg_bytes = N * (12 * R + 12 * R + 1)
+    achieved = alg_bytes / t_apply / 1e9
+    # FP64 single-RHS apply for reference
+    u64 = torch.randn(1, 3, N, dtype=torch.float64, device="cuda")
+    o64b = torch.empty_like(u64)
+    cell.apply_planes(u64, o64b)
+    e0.record()
+    for _ in range(reps):
+        cell.apply_planes(u64, o64b)
+    e1.record()
+    torch.cuda.synchronize()
+    t_apply64 = e0.elapsed_time(e1) / reps * 1e-3
+    print(f"# stencil fp32 R=6: {t_apply * 1e3:.3f} ms/launch, {achieved:.0f} GB/s algorithmic; "
+          f"fp64 R=1: {t_apply64 * 1e3:.3f} ms/launch, {N * 49 / t_apply64 / 1e9:.0f} GB/s",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(int(sum(its)), n)
+        print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    line = dict(
+        metric="256^3 polycrystal C_eff solves/s (mixed FP32/FP64, 6 loads, FP64 residual <= 1e-8)",
+        value=args.steps / elapsed, unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32 inner + f64 refinement", data="synthetic",
+        config=dict(workload=f"voronoi_polycrystal_{n}^3_200grains_cubic_6loads", grid=[n, n, n], grains=200,
+                    loads=6, precision="mixed", tol=1e-8, iterations=its, setup_s=setup_s,
+                    reference_medium=rmeta),
+        roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=achieved / HBM_PEAK_GBS, traffic=None, kernel="apply_kernel<float,uint8> R=6",
+                      ms_per_launch=t_apply * 1e3, algorithmic_bytes=alg_bytes),
+        fp64=fp64,
+        cpu_baseline=cpu,
+    )
+    print(json.dumps(line), flush=True)
+    solver.close()
+    cell.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -116,6 +116,11 @@ int vxl_solver_run(vxl_solver* s, const double* macro_host, double tol, int max_
 /* Per-phase device time of the last run in milliseconds:
  * [0] setup+rhs, [1] iterations, [2] certification+stress. */
 int vxl_solver_timings(vxl_solver* s, double* ms3);
+/* Residual history of system k from the last run, as in the reference's
+ * per-call `trace` (benchmark.py:62-75): writes up to `capacity` (iteration,
+ * relative residual) pairs, a negative iteration marking an FP64 true
+ * residual; returns the number of entries (>= 0) or an error code. */
+int vxl_solver_trace(vxl_solver* s, int k, int* iterations, double* residuals, int capacity);
 int vxl_solver_destroy(vxl_solver* s);
 
 #ifdef __cplusplus

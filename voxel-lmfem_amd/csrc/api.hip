@@ -251,8 +251,7 @@ int vxl_solver_create(vxl_solver** out, vxl_cell* cell, int mode, int nrhs, doub
   if (!out || !cell || nrhs < 1 || nrhs > 64) return fail("vxl_solver_create: invalid arguments");
   if (mode != VXL_SOLVE_FP64 && mode != VXL_SOLVE_MIXED) return fail("mode must be VXL_SOLVE_FP64 or VXL_SOLVE_MIXED");
   *out = nullptr;
-  auto* s = new vxl_solver();
-  std::memset(static_cast<void*>(s), 0, sizeof(vxl_solver));
+  auto* s = new vxl_solver();  // value-initialised: pointers null, counters zero
   s->cell = cell;
   s->mode = mode;
   s->R = nrhs;
@@ -283,6 +282,18 @@ int vxl_solver_timings(vxl_solver* s, double* ms3) {
   if (!s || !ms3) return fail("vxl_solver_timings: invalid arguments");
   for (int i = 0; i < 3; ++i) ms3[i] = s->ms[i];
   return VXL_OK;
+}
+
+int vxl_solver_trace(vxl_solver* s, int k, int* iterations, double* residuals, int capacity) {
+  g_error.clear();
+  if (!s || k < 0 || k >= s->R || capacity < 0) return fail("vxl_solver_trace: invalid arguments");
+  const auto& t = s->trace[k];
+  const int n = int(t.size());
+  for (int i = 0; i < n && i < capacity; ++i) {
+    if (iterations) iterations[i] = t[i].first;
+    if (residuals) residuals[i] = t[i].second;
+  }
+  return n;
 }
 
 int vxl_solver_destroy(vxl_solver* s) {

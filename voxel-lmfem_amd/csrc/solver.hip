@@ -55,6 +55,9 @@ int free_all(vxl_solver* s) {
   if (s->pc_ready) precond_release(&s->pc);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
+  if (s->flag_ev) (void)hipEventDestroy(s->flag_ev);
+  if (s->pinned_flags) (void)hipHostFree(s->pinned_flags);
+  if (s->pinned_scal) (void)hipHostFree(s->pinned_scal);
   return VXL_OK;
 }
 
@@ -79,14 +82,34 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
 
   std::vector<int> active(R, 1), restart(R, 0), its(R, 0), nrest(R, 0);
   std::vector<double> ff(R), rr(R), rr_ref(R), res_true(R, 0.0);
+  for (auto& t : s->trace) t.clear();
+  // Host<->device flag and scalar traffic goes through pinned staging; every
+  // upload waits for the previous one so the staging area is never rewritten
+  // while a copy is pending.
+  int* pin_flags = s->pinned_flags;
+  double* pin_scal = s->pinned_scal;
   auto upload_flags = [&]() -> int {
-    VXL_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
-    VXL_HIP(hipMemcpyAsync(s->restart_dev, restart.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+    VXL_HIP(hipEventSynchronize(s->flag_ev));
+    for (int k = 0; k < R; ++k) {
+      pin_flags[k] = active[k];
+      pin_flags[R + k] = restart[k];
+    }
+    VXL_HIP(hipMemcpyAsync(act, pin_flags, sizeof(int) * R, hipMemcpyHostToDevice, st));
+    VXL_HIP(hipMemcpyAsync(s->restart_dev, pin_flags + R, sizeof(int) * R, hipMemcpyHostToDevice, st));
+    VXL_HIP(hipEventRecord(s->flag_ev, st));
+    return VXL_OK;
+  };
+  auto upload_mask = [&](const std::vector<int>& mask) -> int {
+    VXL_HIP(hipEventSynchronize(s->flag_ev));
+    for (int k = 0; k < R; ++k) pin_flags[k] = mask[k];
+    VXL_HIP(hipMemcpyAsync(act, pin_flags, sizeof(int) * R, hipMemcpyHostToDevice, st));
+    VXL_HIP(hipEventRecord(s->flag_ev, st));
     return VXL_OK;
   };
   auto read_slot = [&](int slot, std::vector<double>& out) -> int {
-    VXL_HIP(hipMemcpyAsync(out.data(), scal + slot * R, sizeof(double) * R, hipMemcpyDeviceToHost, st));
+    VXL_HIP(hipMemcpyAsync(pin_scal, scal + slot * R, sizeof(double) * R, hipMemcpyDeviceToHost, st));
     VXL_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < R; ++k) out[k] = pin_scal[k];
     return VXL_OK;
   };
   auto any_active = [&]() { return std::any_of(active.begin(), active.end(), [](int a) { return a; }); };
@@ -123,11 +146,12 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   VXL_TRY(launch_finalize(partial, nblk, kOpRho, scal, R, act, nullptr, st));
   VXL_HIP(hipEventRecord(s->ev[1], st));
 
-  // FP64 certification of the given systems; returns count of newly converged.
+  int it = 0;
+  // FP64 certification of the given systems.
   auto certify = [&](const std::vector<int>& which) -> int {
     std::vector<int> mask(R, 0);
     for (int k = 0; k < R; ++k) mask[k] = which[k] && active[k];
-    VXL_HIP(hipMemcpyAsync(act, mask.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+    VXL_TRY(upload_mask(mask));
     if (mixed)
       VXL_TRY(launch_accumulate(s->x64, reinterpret_cast<float*>(x), len, R, act, st));
     VXL_TRY(launch_project_zero_mean(s->x64, n, R, partial, act, st));
@@ -142,6 +166,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
       const double rel = std::sqrt(t[k] / ff[k]);
       if (!std::isfinite(rel)) return fail("Nonfinite true residual");
       nrest[k] += 1;
+      s->trace[k].push_back({-it, rel});  // negative iteration: FP64 true residual
       rr_ref[k] = t[k];
       if (rel <= tol) {
         active[k] = 0;
@@ -153,7 +178,6 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     return VXL_OK;
   };
 
-  int it = 0;
   while (any_active() && it < max_iter) {
     VXL_TRY(launch_apply<T>(cell, R, p, q, st));
     VXL_TRY(launch_dot<T, T>(p, q, len, R, partial, act, st));
@@ -170,6 +194,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
       for (int k = 0; k < R; ++k) {
         if (!active[k]) continue;
         if (!std::isfinite(rr[k])) return fail("Nonfinite residual in CG recursion");
+        s->trace[k].push_back({it, std::sqrt(rr[k] / ff[k])});
         const bool reached = std::sqrt(rr[k] / ff[k]) < tol;
         const bool drop = mixed && rr[k] < kReliable * kReliable * rr_ref[k];
         if (reached || drop) {
@@ -182,8 +207,6 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
           for (int k = 0; k < R; ++k) which[k] = active[k];
         }
         VXL_TRY(certify(which));
-        for (int k = 0; k < R; ++k)
-          if (!active[k]) its[k] = it;
       }
       VXL_TRY(upload_flags());
       if (!any_active()) break;
@@ -268,6 +291,10 @@ int solver_create(vxl_solver* s) {
   VXL_HIP(hipMalloc(&s->stress_partial, sizeof(double) * size_t(stress_partial_size(cell, R))));
   VXL_HIP(hipMalloc(&s->ch_dev, sizeof(double) * 6 * R));
   for (auto& e : s->ev) VXL_HIP(hipEventCreate(&e));
+  VXL_HIP(hipEventCreate(&s->flag_ev));
+  VXL_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_flags), sizeof(int) * 2 * R));
+  VXL_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_scal), sizeof(double) * R));
+  s->trace.assign(R, {});
   s->pc.cell = cell;
   s->pc.precision = s->mode == VXL_SOLVE_MIXED ? VXL_F32 : VXL_F64;
   s->pc.nrhs_max = R;

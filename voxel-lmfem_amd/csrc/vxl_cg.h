@@ -73,5 +73,11 @@ struct vxl_solver {
   double* ch_dev;
   hipEvent_t ev[4];
   double ms[3];
+  hipEvent_t flag_ev;
+  int* pinned_flags;
+  double* pinned_scal;
+  // Per-system residual history of the last run: (iteration, relative
+  // residual); a negative iteration marks an FP64 true-residual evaluation.
+  std::vector<std::vector<std::pair<int, double>>> trace;
 };
 

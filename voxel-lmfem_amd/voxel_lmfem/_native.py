@@ -42,6 +42,7 @@ SIGNATURES = {
     "vxl_solver_create": (_I, [_PP, _P, _I, _I, _D, _D]),
     "vxl_solver_run": (_I, [_P, _DP, _D, _I, _DP, _IP, _IP, _DP, _P]),
     "vxl_solver_timings": (_I, [_P, _DP]),
+    "vxl_solver_trace": (_I, [_P, _I, _IP, _DP, _I]),
     "vxl_solver_destroy": (_I, [_P]),
 }
 

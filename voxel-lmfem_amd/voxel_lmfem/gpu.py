@@ -208,11 +208,24 @@ class Solver:
         ms = (ctypes.c_double * 3)()
         self.cell.lib.vxl_solver_timings(self.handle, ms)
         out = dict(CH=ch, iterations=list(its), restarts=list(rst), true_residuals=list(res),
-                   device_ms=dict(setup_rhs=ms[0], iterations=ms[1], certify_stress=ms[2]))
+                   device_ms=dict(setup_rhs=ms[0], iterations=ms[1], certify_stress=ms[2]),
+                   trace=[self.trace(k) for k in range(self.nrhs)])
         if fields is not None:
             out["fields"] = fields
         check(rc, "vxl_solver_run")
         return out
+
+    def trace(self, k):
+        """[(iteration, relative residual)] of system k; negative iteration =
+        FP64 true residual."""
+        lib = self.cell.lib
+        count = lib.vxl_solver_trace(self.handle, k, None, None, 0)
+        if count < 0:
+            check(count, "vxl_solver_trace")
+        its = (ctypes.c_int * max(count, 1))()
+        res = (ctypes.c_double * max(count, 1))()
+        lib.vxl_solver_trace(self.handle, k, its, res, count)
+        return [(int(its[i]), float(res[i])) for i in range(count)]
 
     def close(self):
         if getattr(self, "handle", None):

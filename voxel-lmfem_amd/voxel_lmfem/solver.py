@@ -60,7 +60,9 @@ def _workload(phase, table, domain, orientation, precision, tol, prepared=None):
     stage("solve")
     seconds = time.perf_counter() - start
     calls = [dict(iterations=int(out["iterations"][j]), restarts=int(out["restarts"][j]),
-                  true_residuals=[float(out["true_residuals"][j])], trace=[]) for j in range(6)]
+                  true_residuals=[float(out["true_residuals"][j])],
+                  trace=[dict(iteration=abs(i), relative_residual=r, true_residual=i < 0)
+                         for i, r in out["trace"][j]]) for j in range(6)]
     result = dict(seconds=seconds, CH=np.asarray(out["CH"]).tolist(), calls=calls, stages=stages,
                   field_statistics={}, peak_allocated_bytes=int(torch.cuda.max_memory_allocated()),
                   peak_reserved_bytes=int(torch.cuda.max_memory_reserved()),
