@@ -164,8 +164,8 @@ subtract_mean_kernel(double* u, int64_t n, const double* partial, int nblk, cons
 // One workgroup per RHS: total = sum partial[k][0..nblk) in a fixed order,
 // then the scalar update selected by op.
 __global__ void __launch_bounds__(kBlock)
-finalize_kernel(const double* partial, int nblk, int op, double* scal, int nrhs, const int* active,
-                const int* restart) {
+finalize_kernel(const double* partial, int nblk, int op, double* scal, int nrhs, int* active,
+                const int* restart, int iteration, double tol, int* stop) {
   const int k = blockIdx.x;
   if (active && !active[k]) return;
   double acc = 0;
@@ -185,6 +185,13 @@ finalize_kernel(const double* partial, int nblk, int op, double* scal, int nrhs,
       break;
     case kOpRR:
       S[kRR * nrhs + k] = acc;
+      // Convergence latch (femlib/batched/pcg.py:676-685 semantics): the first
+      // iteration whose recursive residual is below tol is recorded and the
+      // system is parked (u frozen) until the host certifies it.
+      if (stop && sqrt(acc / S[kFF * nrhs + k]) < tol && stop[k] == 0) {
+        stop[k] = iteration;
+        active[k] = 0;
+      }
       break;
     case kOpAlpha: {
       S[kPQ * nrhs + k] = acc;
@@ -228,10 +235,10 @@ int launch_dot(const A* a, const B* b, int64_t len, int nrhs, double* partial, c
 template int launch_dot<float, float>(const float*, const float*, int64_t, int, double*, const int*, hipStream_t);
 template int launch_dot<double, double>(const double*, const double*, int64_t, int, double*, const int*, hipStream_t);
 
-int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, const int* active,
-                    const int* restart, hipStream_t s) {
+int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, int* active,
+                    const int* restart, hipStream_t s, int iteration, double tol, int* stop) {
   hipLaunchKernelGGL(finalize_kernel, dim3(nrhs), dim3(kBlock), 0, s, partial, nblk, op, scal, nrhs,
-                     active, restart);
+                     active, restart, iteration, tol, stop);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }

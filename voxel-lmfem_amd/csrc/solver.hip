@@ -52,6 +52,7 @@ int free_all(vxl_solver* s) {
   }
   if (s->active_dev) (void)hipFree(s->active_dev);
   if (s->restart_dev) (void)hipFree(s->restart_dev);
+  if (s->stop_dev) (void)hipFree(s->stop_dev);
   if (s->pc_ready) precond_release(&s->pc);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
@@ -87,6 +88,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   // upload waits for the previous one so the staging area is never rewritten
   // while a copy is pending.
   int* pin_flags = s->pinned_flags;
+  int* pin_stop = s->pinned_flags + 2 * R;
   double* pin_scal = s->pinned_scal;
   auto upload_flags = [&]() -> int {
     VXL_HIP(hipEventSynchronize(s->flag_ev));
@@ -117,6 +119,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   VXL_HIP(hipEventRecord(s->ev[0], st));
   VXL_HIP(hipMemcpyAsync(s->macro_dev, macro_host, sizeof(double) * 6 * R, hipMemcpyHostToDevice, st));
   VXL_HIP(hipMemsetAsync(scal, 0, sizeof(double) * kNumSlots * R, st));
+  VXL_HIP(hipMemsetAsync(s->stop_dev, 0, sizeof(int) * R, st));
   VXL_TRY(upload_flags());
   // f = P0 rhs(E); ||f||^2
   VXL_TRY(launch_rhs(cell, R, s->macro_dev, s->f, st));
@@ -183,19 +186,20 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     VXL_TRY(launch_dot<T, T>(p, q, len, R, partial, act, st));
     VXL_TRY(launch_finalize(partial, nblk, kOpAlpha, scal, R, act, nullptr, st));
     VXL_TRY(launch_update_xr<T>(x, r, p, q, len, R, scal + kAlpha * R, partial, act, st));
-    VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st));
     ++it;
-    for (int k = 0; k < R; ++k)
-      if (active[k]) its[k] = it;
+    VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
     if (it == 1 || it % kCheckStride == 0 || it == max_iter) {
       VXL_TRY(read_slot(kRR, rr));
+      VXL_HIP(hipMemcpyAsync(pin_stop, s->stop_dev, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+      VXL_HIP(hipStreamSynchronize(st));
       std::vector<int> which(R, 0);
       bool anyw = false;
       for (int k = 0; k < R; ++k) {
         if (!active[k]) continue;
         if (!std::isfinite(rr[k])) return fail("Nonfinite residual in CG recursion");
-        s->trace[k].push_back({it, std::sqrt(rr[k] / ff[k])});
-        const bool reached = std::sqrt(rr[k] / ff[k]) < tol;
+        s->trace[k].push_back({pin_stop[k] ? pin_stop[k] : it, std::sqrt(rr[k] / ff[k])});
+        its[k] = pin_stop[k] ? pin_stop[k] : it;
+        const bool reached = pin_stop[k] != 0;
         const bool drop = mixed && rr[k] < kReliable * kReliable * rr_ref[k];
         if (reached || drop) {
           which[k] = 1;
@@ -207,6 +211,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
           for (int k = 0; k < R; ++k) which[k] = active[k];
         }
         VXL_TRY(certify(which));
+        VXL_HIP(hipMemsetAsync(s->stop_dev, 0, sizeof(int) * R, st));
       }
       VXL_TRY(upload_flags());
       if (!any_active()) break;
@@ -292,7 +297,8 @@ int solver_create(vxl_solver* s) {
   VXL_HIP(hipMalloc(&s->ch_dev, sizeof(double) * 6 * R));
   for (auto& e : s->ev) VXL_HIP(hipEventCreate(&e));
   VXL_HIP(hipEventCreate(&s->flag_ev));
-  VXL_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_flags), sizeof(int) * 2 * R));
+  VXL_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_flags), sizeof(int) * 3 * R));
+  VXL_HIP(hipMalloc(&s->stop_dev, sizeof(int) * R));
   VXL_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_scal), sizeof(double) * R));
   s->trace.assign(R, {});
   s->pc.cell = cell;

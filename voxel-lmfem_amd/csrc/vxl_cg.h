@@ -17,8 +17,11 @@ int vec_blocks(int64_t len);
 template <typename A, typename B>
 int launch_dot(const A* a, const B* b, int64_t len, int nrhs, double* partial, const int* active,
                hipStream_t s);
-int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, const int* active,
-                    const int* restart, hipStream_t s);
+// kOpRR with `stop` latches the first iteration below tol: stop[k] = iteration,
+// active[k] = 0 (on the device).
+int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, int* active,
+                    const int* restart, hipStream_t s, int iteration = 0, double tol = 0.0,
+                    int* stop = nullptr);
 template <typename T>
 int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, const double* alpha,
                      double* partial, const int* active, hipStream_t s);
@@ -68,6 +71,7 @@ struct vxl_solver {
   int nblk;
   int* active_dev;
   int* restart_dev;
+  int* stop_dev;
   double* macro_dev;
   double* stress_partial;
   double* ch_dev;
