@@ -203,7 +203,7 @@ int vxl_project_zero_mean(vxl_cell* cell, int nrhs, double* u_dev) {
   if (!cell || !u_dev || nrhs < 1) return fail("vxl_project_zero_mean: invalid arguments");
   double* partial = nullptr;
   const int64_t n = cell->grid.n;
-  VXL_HIP(hipMalloc(&partial, sizeof(double) * 3 * nrhs * size_t(vec_blocks(n))));
+  VXL_HIP(hipMalloc(&partial, sizeof(double) * 3 * nrhs * size_t(vec_blocks(n) + 1)));
   int rc = launch_project_zero_mean(u_dev, n, nrhs, partial, nullptr, cell->stream);
   VXL_HIP(hipStreamSynchronize(cell->stream));
   VXL_HIP(hipFree(partial));

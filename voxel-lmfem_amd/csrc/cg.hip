@@ -143,19 +143,21 @@ row_sum_kernel(const double* u, int64_t n, double* partial) {
   if (threadIdx.x == 0) partial[int64_t(row) * gridDim.x + blockIdx.x] = acc;
 }
 
+// One workgroup per row: mean[row] = inv_n * sum of its partials (fixed order).
 __global__ void __launch_bounds__(kBlock)
-subtract_mean_kernel(double* u, int64_t n, const double* partial, int nblk, const int* active,
-                     double inv_n) {
+row_mean_kernel(const double* partial, int nblk, double inv_n, double* mean) {
+  const int row = blockIdx.x;
+  double acc = 0;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) acc += partial[int64_t(row) * nblk + b];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) mean[row] = acc * inv_n;
+}
+
+__global__ void __launch_bounds__(kBlock)
+subtract_mean_kernel(double* u, int64_t n, const double* mean, const int* active) {
   const int row = blockIdx.y;
   if (active && !active[row / 3]) return;
-  __shared__ double mean;
-  if (threadIdx.x == 0) {
-    double s = 0;
-    for (int b = 0; b < nblk; ++b) s += partial[int64_t(row) * nblk + b];
-    mean = s * inv_n;
-  }
-  __syncthreads();
-  const double m = mean;
+  const double m = mean[row];
   double* ur = u + int64_t(row) * n;
   for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBlock)
     ur[i] -= m;
@@ -299,10 +301,13 @@ template int launch_fill_zero<double>(double*, int64_t, int, const int*, hipStre
 int launch_project_zero_mean(double* u, int64_t n, int nrhs, double* partial, const int* active,
                              hipStream_t s) {
   const dim3 g = vgrid(n, 3 * nrhs);
+  double* mean = partial + int64_t(3) * nrhs * g.x;  // 3*nrhs slots after the partials
   hipLaunchKernelGGL(row_sum_kernel, g, dim3(kBlock), 0, s, u, n, partial);
   VXL_HIP(hipGetLastError());
-  hipLaunchKernelGGL(subtract_mean_kernel, g, dim3(kBlock), 0, s, u, n, partial, int(g.x), active,
-                     1.0 / double(n));
+  hipLaunchKernelGGL(row_mean_kernel, dim3(3 * nrhs), dim3(kBlock), 0, s, partial, int(g.x),
+                     1.0 / double(n), mean);
+  VXL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(subtract_mean_kernel, g, dim3(kBlock), 0, s, u, n, mean, active);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }

@@ -141,9 +141,11 @@ __device__ __forceinline__ void dft_prime(C* v, const C* w, int n) {
   for (int k = 0; k < R; ++k) v[k] = out[k];
 }
 
-// One Stockham radix-R stage over P pencils: butterfly j reads src[j + r n/R],
-// applies w^(r (j mod Ns) n/(Ns R)), transforms, and writes
-// dst[(j/Ns) Ns R + (j mod Ns) + r Ns].
+// One Stockham radix-R stage over P pencils stored column-fastest: element e
+// of pencil p lives at X[e * P + p], so consecutive threads (consecutive p)
+// touch consecutive LDS words on every read and write (no bank conflicts) and
+// tiles keep the global row order.  Butterfly j reads e = j + r n/R, applies
+// w^(r (j mod Ns) n/(Ns R)), transforms, and writes e = (j/Ns) Ns R + (j mod Ns) + r Ns.
 template <bool INV, typename T, int R>
 __device__ __forceinline__ void stockham_stage(const typename CT<T>::type* A, typename CT<T>::type* B,
                                                int P, int n, int Ns, const typename CT<T>::type* w) {
@@ -151,14 +153,12 @@ __device__ __forceinline__ void stockham_stage(const typename CT<T>::type* A, ty
   const int nb = n / R;
   const int tstep = n / (Ns * R);
   for (int idx = threadIdx.x; idx < P * nb; idx += blockDim.x) {
-    const int p = idx / nb, j = idx - p * nb;
-    const C* src = A + p * n;
-    C* dst = B + p * n;
+    const int j = idx / P, p = idx - j * P;
     const int jm = j % Ns;
     C v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      C x = src[j + r * nb];
+      C x = A[(j + r * nb) * P + p];
       if (r > 0 && jm > 0) {
         C tw = w[r * jm * tstep];
         if (INV) tw = cconj(tw);
@@ -173,13 +173,13 @@ __device__ __forceinline__ void stockham_stage(const typename CT<T>::type* A, ty
     else dft_prime<INV, R>(v, w, n);
     const int base = (j / Ns) * Ns * R + jm;
 #pragma unroll
-    for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+    for (int r = 0; r < R; ++r) B[(base + r * Ns) * P + p] = v[r];
   }
 }
 
-// Batched in-LDS FFT of P contiguous pencils of length n.  A holds the input;
-// returns the buffer (A or B) holding the output.  All threads of the block
-// must call it.
+// Batched in-LDS FFT of P pencils of length n, column-fastest layout
+// (element e of pencil p at X[e * P + p]).  A holds the input; returns the
+// buffer (A or B) holding the output.  All threads of the block must call it.
 template <bool INV, typename T>
 __device__ typename CT<T>::type* lds_fft(typename CT<T>::type* A, typename CT<T>::type* B, int P,
                                          const Plan1D& plan, const typename CT<T>::type* w) {
@@ -188,15 +188,14 @@ __device__ typename CT<T>::type* lds_fft(typename CT<T>::type* A, typename CT<T>
   const int tid = threadIdx.x, nt = blockDim.x;
   if (plan.stages == 0) {  // direct DFT
     for (int idx = tid; idx < P * n; idx += nt) {
-      const int p = idx / n, k = idx % n;
-      const C* src = A + p * n;
+      const int k = idx / P, p = idx - k * P;
       C acc = {T(0), T(0)};
       for (int j = 0; j < n; ++j) {
         C tw = w[(int64_t(j) * k) % n];
         if (INV) tw = cconj(tw);
-        acc = cadd(acc, cmul(src[j], tw));
+        acc = cadd(acc, cmul(A[j * P + p], tw));
       }
-      B[p * n + k] = acc;
+      B[k * P + p] = acc;
     }
     __syncthreads();
     return B;
@@ -269,9 +268,9 @@ fft_x_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int nx,
       v11 = f[n + (rho + 1) * nx + x];
       v21 = f[2 * n + (rho + 1) * nx + x];
     }
-    A[(3 * pp + 0) * nx + x] = C{v00, v01};
-    A[(3 * pp + 1) * nx + x] = C{v10, v11};
-    A[(3 * pp + 2) * nx + x] = C{v20, v21};
+    A[x * P + 3 * pp + 0] = C{v00, v01};
+    A[x * P + 3 * pp + 1] = C{v10, v11};
+    A[x * P + 3 * pp + 2] = C{v20, v21};
   }
   __syncthreads();
   C* F = lds_fft<false, T>(A, B, P, ax.px, tb.wx);
@@ -282,8 +281,8 @@ fft_x_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int nx,
     C sp[3][2];  // [pencil][real row 0/1]
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const C a = F[(3 * pp + q) * nx + k];
-      const C b = cconj(F[(3 * pp + q) * nx + km]);
+      const C a = F[k * P + 3 * pp + q];
+      const C b = cconj(F[km * P + 3 * pp + q]);
       const C s = cadd(a, b), d = csub(a, b);
       sp[q][0] = C{T(0.5) * s.x, T(0.5) * s.y};
       sp[q][1] = C{T(0.5) * d.y, T(-0.5) * d.x};  // d / (2i)
@@ -339,16 +338,16 @@ fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx,
       }
     }
     // pencil = X0 + i X1
-    A[(3 * pp + 0) * nx + k] = C{x[0][0].x - x[0][1].y, x[0][0].y + x[0][1].x};
-    A[(3 * pp + 1) * nx + k] = C{x[1][0].x - x[1][1].y, x[1][0].y + x[1][1].x};
-    A[(3 * pp + 2) * nx + k] = C{x[0][2].x - x[1][2].y, x[0][2].y + x[1][2].x};
+    A[k * P + 3 * pp + 0] = C{x[0][0].x - x[0][1].y, x[0][0].y + x[0][1].x};
+    A[k * P + 3 * pp + 1] = C{x[1][0].x - x[1][1].y, x[1][0].y + x[1][1].x};
+    A[k * P + 3 * pp + 2] = C{x[0][2].x - x[1][2].y, x[0][2].y + x[1][2].x};
   }
   __syncthreads();
   C* F = lds_fft<true, T>(A, B, P, ax.px, tb.wx);
   for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x) {
     const int pp = idx / nx, x = idx % nx;
     const int64_t rho = 2 * (pair0 + pp);
-    const C a = F[(3 * pp + 0) * nx + x], b = F[(3 * pp + 1) * nx + x], c = F[(3 * pp + 2) * nx + x];
+    const C a = F[x * P + 3 * pp + 0], b = F[x * P + 3 * pp + 1], c = F[x * P + 3 * pp + 2];
     if (rho < rows) {
       f[rho * nx + x] = a.x;
       f[n + rho * nx + x] = a.y;
@@ -379,13 +378,13 @@ fft_y(typename CT<T>::type* __restrict__ S, int ny, int nz, int W, int TJ, Axes 
   C* plane = S + int64_t(blockIdx.z) * spec_stride + int64_t(z) * ny * W;
   for (int idx = threadIdx.x; idx < ny * TJ; idx += blockDim.x) {
     const int y = idx / TJ, j = idx % TJ;
-    A[j * ny + y] = j < tj ? plane[int64_t(y) * W + j0 + j] : C{T(0), T(0)};
+    A[y * TJ + j] = j < tj ? plane[int64_t(y) * W + j0 + j] : C{T(0), T(0)};
   }
   __syncthreads();
   C* F = lds_fft<INV, T>(A, B, TJ, ax.py, tb.wy);
   for (int idx = threadIdx.x; idx < ny * TJ; idx += blockDim.x) {
     const int y = idx / TJ, j = idx % TJ;
-    if (j < tj) plane[int64_t(y) * W + j0 + j] = F[j * ny + y];
+    if (j < tj) plane[int64_t(y) * W + j0 + j] = F[y * TJ + j];
   }
 }
 
@@ -407,7 +406,7 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
   C* col = S + int64_t(blockIdx.z) * spec_stride + int64_t(y) * W + 3 * k0;
   for (int idx = threadIdx.x; idx < nz * TJ; idx += blockDim.x) {
     const int z = idx / TJ, j = idx % TJ;
-    A[j * nz + z] = j < 3 * tk ? col[z * zstride + j] : C{T(0), T(0)};
+    A[z * TJ + j] = j < 3 * tk ? col[z * zstride + j] : C{T(0), T(0)};
   }
   __syncthreads();
   C* F = lds_fft<false, T>(A, B, TJ, ax.pz, tb.wz);
@@ -423,9 +422,9 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
     const int kx = k0 + kk;
     const T mx = tb.gx[kx], Dx = tb.gx[nxh + kx], vx = tb.gx[2 * nxh + kx], dx = tb.gx[3 * nxh + kx];
     const T mz = tb.gz[kz], Dz = tb.gz[nz + kz], vz = tb.gz[2 * nz + kz], dz = tb.gz[3 * nz + kz];
-    C* q0 = &F[(3 * kk + 0) * nz + kz];
-    C* q1 = &F[(3 * kk + 1) * nz + kz];
-    C* q2 = &F[(3 * kk + 2) * nz + kz];
+    C* q0 = &F[kz * TJ + 3 * kk + 0];
+    C* q1 = &F[kz * TJ + 3 * kk + 1];
+    C* q2 = &F[kz * TJ + 3 * kk + 2];
     const T sumD = Dx + Dy + Dz;
     if (!(sumD > T(0))) {
       *q0 = *q1 = *q2 = C{T(0), T(0)};
@@ -447,7 +446,7 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
   C* H = lds_fft<true, T>(F, G, TJ, ax.pz, tb.wz);
   for (int idx = threadIdx.x; idx < nz * TJ; idx += blockDim.x) {
     const int z = idx / TJ, j = idx % TJ;
-    if (j < 3 * tk) col[z * zstride + j] = H[j * nz + z];
+    if (j < 3 * tk) col[z * zstride + j] = H[z * TJ + j];
   }
 }
 
@@ -523,10 +522,14 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   im->axes.px = make_plan(im->nx);
   im->axes.py = make_plan(im->ny);
   im->axes.pz = make_plan(im->nz);
-  const int target = 2048;  // complex entries per LDS buffer
-  im->pairs_per_block = std::max(1, target / (3 * im->nx));
-  im->tj = std::max(1, std::min(3 * im->nxh, target / im->ny));
-  im->tk = std::max(1, std::min(im->nxh, target / (3 * im->nz)));
+  // Row segments of >= 120 B per tile row (16 FP32 / 8 FP64 complex columns)
+  // for coalescing, within kMaxLds for the two ping-pong buffers.
+  const int cols = sizeof(T) == 4 ? 16 : 8;
+  const int cap = int(kMaxLds / (2 * sizeof(C)));  // complex entries per buffer
+  im->pairs_per_block = std::max(1, std::min(cols / 3 + 1, cap / (3 * im->nx)));
+  if (im->pairs_per_block % 2 == 0 && im->pairs_per_block > 1) --im->pairs_per_block;  // odd column count
+  im->tj = std::max(1, std::min({3 * im->nxh, cols, cap / im->ny}));
+  im->tk = std::max(1, std::min({im->nxh, (cols + 2) / 3, cap / (3 * im->nz)}));
   im->lds_x = 2 * sizeof(C) * size_t(3 * im->pairs_per_block) * im->nx;
   im->lds_y = 2 * sizeof(C) * size_t(im->tj) * im->ny;
   im->lds_z = 2 * sizeof(C) * size_t(3 * im->tk) * im->nz;

@@ -287,7 +287,7 @@ int solver_create(vxl_solver* s) {
   }
   s->nblk = vec_blocks(s->len);
   const int nblk_rows = vec_blocks(cell->grid.n);
-  const size_t np = size_t(std::max(s->nblk * R, 3 * R * nblk_rows));
+  const size_t np = size_t(std::max(s->nblk * R, 3 * R * (nblk_rows + 1)));
   VXL_HIP(hipMalloc(&s->partial, sizeof(double) * np));
   VXL_HIP(hipMalloc(&s->scal, sizeof(double) * kNumSlots * R));
   VXL_HIP(hipMalloc(&s->active_dev, sizeof(int) * R));

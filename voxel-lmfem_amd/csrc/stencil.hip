@@ -23,6 +23,8 @@
 // bitwise deterministic.
 #include "vxl_internal.h"
 
+#include <cstdlib>
+
 namespace vxl {
 
 ModalFactors modal_factors(const double h[3]) {
@@ -145,30 +147,56 @@ __device__ __forceinline__ void hadamard_transpose(T (&v)[8]) {
   }
 }
 
-// v[c][a]: nodal displacements in, nodal forces out.  c: 21 packed stiffness.
+// Modal element action: Hadamard coefficients q[c][m] in, modal forces P[c][m]
+// out (nodal forces f = H^T P).  c: 21 packed stiffness.
 template <typename T>
-__device__ __forceinline__ void element_action(T (&v)[3][8], const T* c, const G9<T>& g) {
-#pragma unroll
-  for (int i = 0; i < 3; ++i) hadamard_forward(v[i]);
-  T P[3][8];
+__device__ __forceinline__ void element_modal(const T (&q)[3][8], const T* c, const G9<T>& g,
+                                              T (&P)[3][8]) {
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int m = 0; m < 8; ++m) P[i][m] = T(0);
   // Mode numbering: bit 0 = x, bit 1 = y, bit 2 = z.
-  strain_mode<1, 2, 4, 0>(c, g, v, P);    // constant
-  strain_mode<-1, 3, 5, 1>(c, g, v, P);   // xi
-  strain_mode<3, -1, 6, 1>(c, g, v, P);   // eta
-  strain_mode<5, 6, -1, 1>(c, g, v, P);   // zeta
-  strain_mode<7, -1, -1, 2>(c, g, v, P);  // eta*zeta
-  strain_mode<-1, 7, -1, 2>(c, g, v, P);  // xi*zeta
-  strain_mode<-1, -1, 7, 2>(c, g, v, P);  // xi*eta
+  strain_mode<1, 2, 4, 0>(c, g, q, P);    // constant
+  strain_mode<-1, 3, 5, 1>(c, g, q, P);   // xi
+  strain_mode<3, -1, 6, 1>(c, g, q, P);   // eta
+  strain_mode<5, 6, -1, 1>(c, g, q, P);   // zeta
+  strain_mode<7, -1, -1, 2>(c, g, q, P);  // eta*zeta
+  strain_mode<-1, 7, -1, 2>(c, g, q, P);  // xi*zeta
+  strain_mode<-1, -1, 7, 2>(c, g, q, P);  // xi*eta
+}
+
+// v[c][a]: nodal displacements in, nodal forces out.
+template <typename T>
+__device__ __forceinline__ void element_action(T (&v)[3][8], const T* c, const G9<T>& g) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) hadamard_forward(v[i]);
+  T P[3][8];
+  element_modal(v, c, g, P);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     hadamard_transpose(P[i]);
 #pragma unroll
     for (int a = 0; a < 8; ++a) v[i][a] = P[i][a];
   }
+}
+
+// Two-point Hadamard of one face (4 nodes a2 = ax + 2 ay) and its transpose.
+template <typename T>
+__device__ __forceinline__ void face_forward(T (&v)[4]) {
+  T a = v[1] + v[0], b = v[1] - v[0], c = v[3] + v[2], d = v[3] - v[2];
+  v[0] = c + a;
+  v[2] = c - a;
+  v[1] = d + b;
+  v[3] = d - b;
+}
+template <typename T>
+__device__ __forceinline__ void face_transpose(T (&v)[4]) {
+  T a = v[0] - v[2], c = v[0] + v[2], b = v[1] - v[3], d = v[1] + v[3];
+  v[0] = a - b;
+  v[1] = a + b;
+  v[2] = c - d;
+  v[3] = c + d;
 }
 
 template <typename T>
@@ -318,6 +346,169 @@ apply_kernel(ApplyArgs<T> a) {
   }
 }
 
+// Stencil v2: element columns persist across the z march.  A column keeps the
+// face (xy) Hadamard modes of its bottom node face and the force its previous
+// layer put on that face in registers, so each layer transforms one new face
+// (4 nodes), runs the modal element action, and exchanges only the 4 corner
+// forces of one face through LDS.  One staged node plane; two barriers per
+// plane.  Same arithmetic as the full 8-point transform, regrouped.
+template <typename T, typename Idx, bool kScale, int TX, int TY>
+__global__ void __launch_bounds__((Tile<TX, TY>::NT))
+apply_kernel_v2(ApplyArgs<T> a) {
+  using TL = Tile<TX, TY>;
+  __shared__ T su[3][TL::PLANE];
+  __shared__ T sc[12][TL::EL];
+
+  const int tid = threadIdx.x;
+  const int tile = blockIdx.x;
+  const int x0 = (tile % a.tiles_x) * TX;
+  const int y0 = (tile / a.tiles_x) * TY;
+  const int z0 = blockIdx.y * a.zchunk;
+  const int z1 = min(z0 + a.zchunk, a.nz);
+  if (z0 >= a.nz) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int64_t nxy = int64_t(nx) * ny;
+  const int64_t n = nxy * nz;
+  const T* u = a.u + int64_t(blockIdx.z) * a.rhs_stride;
+  T* out = a.out + int64_t(blockIdx.z) * a.rhs_stride;
+
+  int ld_lds[TL::ITEMS];
+  int64_t ld_off[TL::ITEMS];
+#pragma unroll
+  for (int k = 0; k < TL::ITEMS; ++k) {
+    const int idx = tid + k * TL::NT;
+    ld_lds[k] = idx < TL::PLANE ? idx : -1;
+    const int px = idx % TL::PX, py = idx / TL::PX;
+    ld_off[k] = int64_t(wrapmod(y0 - 1 + py, ny)) * nx + wrapmod(x0 - 1 + px, nx);
+  }
+  auto load_plane = [&](int z) {
+    const int64_t base = int64_t(wrapmod(z, nz)) * nxy;
+#pragma unroll
+    for (int k = 0; k < TL::ITEMS; ++k) {
+      if (ld_lds[k] < 0) continue;
+      const int64_t node = base + ld_off[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) su[c][ld_lds[k]] = u[c * n + node];
+    }
+  };
+
+  const bool is_elem = tid < TL::EL;
+  const int exl = tid % TL::EX, eyl = tid / TL::EX;
+  const int64_t e_xy = int64_t(wrapmod(y0 - 1 + eyl, ny)) * nx + wrapmod(x0 - 1 + exl, nx);
+  auto face_modes = [&](T (&f)[3][4]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int a2 = 0; a2 < 4; ++a2) f[c][a2] = su[c][(eyl + (a2 >> 1)) * TL::PX + exl + (a2 & 1)];
+      face_forward(f[c]);
+    }
+  };
+  T Bf[3][4], Ftp[3][4];
+  // Element of layer z from its bottom (Bf) and top (Tf) face modes; returns the
+  // bottom-face force (Fb) and top-face force (Ft) in face-mode form.
+  auto element = [&](int z, const T (&Tf)[3][4], T (&Fb)[3][4], T (&Ft)[3][4]) {
+    const int64_t voxel = int64_t(wrapmod(z, nz)) * nxy + e_xy;
+    const int row = load_row<Idx>(a.index, voxel);
+    const T* crow = a.table + int64_t(row) * kRowWidth;
+    T c[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) c[k] = crow[k];
+    if constexpr (kScale) {
+      const T sc_ = T(a.scale[voxel]);
+#pragma unroll
+      for (int k = 0; k < 21; ++k) c[k] *= sc_;
+    }
+    T q[3][8], P[3][8];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        q[cc][m] = Tf[cc][m] + Bf[cc][m];
+        q[cc][m + 4] = Tf[cc][m] - Bf[cc][m];
+      }
+    element_modal(q, c, a.g, P);
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        Fb[cc][m] = P[cc][m] - P[cc][m + 4];
+        Ft[cc][m] = P[cc][m] + P[cc][m + 4];
+      }
+  };
+
+  const bool is_node = tid < TL::NODES;
+  const int xl = tid % TX, yl = tid / TX;
+  const bool node_valid = is_node && (x0 + xl) < nx && (y0 + yl) < ny;
+  const int64_t n_xy = int64_t(y0 + yl) * nx + (x0 + xl);
+
+  // Prologue: bottom face of layer z0-1, then layer z0-1 itself.
+  load_plane(z0 - 1);
+  __syncthreads();
+  if (is_elem) face_modes(Bf);
+  __syncthreads();
+  load_plane(z0);
+  __syncthreads();
+  if (is_elem) {
+    T Tf[3][4], Fb[3][4];
+    face_modes(Tf);
+    element(z0 - 1, Tf, Fb, Ftp);
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) Bf[cc][m] = Tf[cc][m];
+  }
+  __syncthreads();
+  for (int k = 0; z0 + k < z1; ++k) {
+    const int z = z0 + k;
+    load_plane(z + 1);
+    __syncthreads();
+    if (is_elem) {
+      T Tf[3][4], Fb[3][4], Ft[3][4];
+      face_modes(Tf);
+      element(z, Tf, Fb, Ft);
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        T F[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          F[m] = Ftp[cc][m] + Fb[cc][m];
+          Ftp[cc][m] = Ft[cc][m];
+          Bf[cc][m] = Tf[cc][m];
+        }
+        face_transpose(F);
+#pragma unroll
+        for (int a2 = 0; a2 < 4; ++a2) sc[a2 * 3 + cc][tid] = F[a2];
+      }
+    }
+    __syncthreads();
+    if (node_valid) {
+      T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+          const int e = (yl + dy) * TL::EX + xl + dx;
+          const int corner = (1 - dx) + 2 * (1 - dy);
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) acc[cc] += sc[corner * 3 + cc][e];
+        }
+      const int64_t node = int64_t(z) * nxy + n_xy;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) out[cc * n + node] = acc[cc];
+    }
+  }
+}
+
+// Stencil schedule: 2 (persistent face columns, default) or 1 (per-layer full
+// 8-node transform); VXL_STENCIL_VARIANT selects it for comparisons.
+int stencil_variant() {
+  static const int v = [] {
+    const char* e = std::getenv("VXL_STENCIL_VARIANT");
+    return (e && e[0] == '1') ? 1 : 2;
+  }();
+  return v;
+}
+
 template <typename T>
 G9<T> cast_g(const ModalFactors& f) {
   G9<T> g;
@@ -350,7 +541,10 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
   chunks = (a.nz + a.zchunk - 1) / a.zchunk;
   a.g = cast_g<T>(modal_factors(cell->h));
   dim3 grid(tiles, chunks, nrhs);
-  hipLaunchKernelGGL((apply_kernel<T, Idx, kScale, TX, TY>), grid, dim3(TL::NT), 0, s, a);
+  if (stencil_variant() == 1)
+    hipLaunchKernelGGL((apply_kernel<T, Idx, kScale, TX, TY>), grid, dim3(TL::NT), 0, s, a);
+  else
+    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY>), grid, dim3(TL::NT), 0, s, a);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }

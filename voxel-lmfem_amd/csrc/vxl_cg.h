@@ -35,7 +35,7 @@ template <typename A, typename B>
 int launch_convert(B* dst, const A* src, int64_t len, int nrhs, const int* active, hipStream_t s);
 template <typename T>
 int launch_fill_zero(T* dst, int64_t len, int nrhs, const int* active, hipStream_t s);
-// partial: >= 3 * nrhs * vec_blocks(n) doubles.
+// partial: >= 3 * nrhs * (vec_blocks(n) + 1) doubles.
 int launch_project_zero_mean(double* u, int64_t n, int nrhs, double* partial, const int* active,
                              hipStream_t s);
 
