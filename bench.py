@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--stencil-only", action="store_true",
+                    help="only run 10 six-RHS FP32 stencil applies and a 1 GiB device copy "
+                         "(the calibration kernel for FETCH_SIZE/WRITE_SIZE counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps(dict(error="this build measures one GPU; multi-GPU sharding is design-only "
@@ -103,6 +106,18 @@ def main():
     solver = Solver(cell, lam, mu, mode="mixed", nrhs=6)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
+
+    if args.stencil_only:
+        u = torch.randn(6, 3, cell.n, dtype=torch.float32, device="cuda")
+        o = torch.empty_like(u)
+        for _ in range(10):
+            cell.apply_planes(u, o)
+        a = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").fill_(1.0)  # 1 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(stencil_only=True, n=n, copy_bytes=a.numel() * 4)), flush=True)
+        return
 
     for _ in range(args.warmup):
         solver.run()

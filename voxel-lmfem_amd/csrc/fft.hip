@@ -239,34 +239,63 @@ struct Tables {
 // ---- pass 1: x real-to-complex ---------------------------------------------
 // Row pair (rho, rho+1) over the flattened (z, y) row index; three complex
 // pencils per pair: c0+i c1 of row rho, c0+i c1 of row rho+1, c2(rho)+i c2(rho+1).
+// Fused CG update (kFuse), per entry of the field before it is transformed:
+// r -= alpha q, x += alpha p (written back), and a per-workgroup FP64 partial
+// of ||r||^2 into partial[rhs * gridDim.x + blockIdx.x].
 template <typename T>
+struct CgUpdate {
+  T* r;  // residual, updated in place (also the transform input)
+  const T* q;
+  T* x;
+  const T* p;
+  const double* alpha;
+  double* partial;
+};
+
+template <typename T, bool kFuse>
 __global__ void __launch_bounds__(kThreads)
 fft_x_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int nx, int64_t rows,
           int nxh, int pairs_per_block, Axes ax, Tables<T> tb, int64_t field_stride,
-          int64_t spec_stride) {
+          int64_t spec_stride, const int* active, CgUpdate<T> cg) {
   using C = typename CT<T>::type;
   extern __shared__ double2 smem_raw[];
+  __shared__ double red[kThreads / 64];
+  if (active && !active[blockIdx.y]) return;
   C* A = reinterpret_cast<C*>(smem_raw);
   const int P = 3 * pairs_per_block;
   C* B = A + P * nx;
   const int64_t n = rows * nx;
-  const T* f = r + int64_t(blockIdx.y) * field_stride;
+  const int64_t off = int64_t(blockIdx.y) * field_stride;
+  const T* f = r + off;
   C* out = S + int64_t(blockIdx.y) * spec_stride;
   const int64_t pair0 = int64_t(blockIdx.x) * pairs_per_block;
+  const T alpha = kFuse ? T(cg.alpha[blockIdx.y]) : T(0);
+  double rr = 0.0;
+  auto fetch = [&](int64_t i) -> T {
+    if constexpr (kFuse) {
+      const T v = fma(-alpha, cg.q[off + i], cg.r[off + i]);
+      cg.r[off + i] = v;
+      cg.x[off + i] = fma(alpha, cg.p[off + i], cg.x[off + i]);
+      rr += double(v) * double(v);
+      return v;
+    } else {
+      return f[i];
+    }
+  };
   for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x) {
     const int pp = idx / nx, x = idx % nx;
     const int64_t rho = 2 * (pair0 + pp);
     const bool has0 = rho < rows, has1 = rho + 1 < rows;
     T v00 = 0, v01 = 0, v10 = 0, v11 = 0, v20 = 0, v21 = 0;
     if (has0) {
-      v00 = f[rho * nx + x];
-      v01 = f[n + rho * nx + x];
-      v20 = f[2 * n + rho * nx + x];
+      v00 = fetch(rho * nx + x);
+      v01 = fetch(n + rho * nx + x);
+      v20 = fetch(2 * n + rho * nx + x);
     }
     if (has1) {
-      v10 = f[(rho + 1) * nx + x];
-      v11 = f[n + (rho + 1) * nx + x];
-      v21 = f[2 * n + (rho + 1) * nx + x];
+      v10 = fetch((rho + 1) * nx + x);
+      v11 = fetch(n + (rho + 1) * nx + x);
+      v21 = fetch(2 * n + (rho + 1) * nx + x);
     }
     A[x * P + 3 * pp + 0] = C{v00, v01};
     A[x * P + 3 * pp + 1] = C{v10, v11};
@@ -300,16 +329,23 @@ fft_x_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int nx,
       o[2] = sp[2][1];
     }
   }
+  if constexpr (kFuse) {
+    const double total = block_reduce_sum<kThreads>(rr, red);
+    if (threadIdx.x == 0) cg.partial[int64_t(blockIdx.y) * gridDim.x + blockIdx.x] = total;
+  }
 }
 
 // ---- pass 5: x complex-to-real -----------------------------------------------
-template <typename T>
+// kDot: per-workgroup FP64 partial of r . z into partial[rhs * gridDim.x + blockIdx.x].
+template <typename T, bool kDot>
 __global__ void __launch_bounds__(kThreads)
 fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx, int64_t rows,
           int nxh, int pairs_per_block, Axes ax, Tables<T> tb, int64_t field_stride,
-          int64_t spec_stride) {
+          int64_t spec_stride, const int* active, const T* rdot, double* partial) {
   using C = typename CT<T>::type;
   extern __shared__ double2 smem_raw[];
+  __shared__ double red[kThreads / 64];
+  if (active && !active[blockIdx.y]) return;
   C* A = reinterpret_cast<C*>(smem_raw);
   const int P = 3 * pairs_per_block;
   C* B = A + P * nx;
@@ -344,6 +380,7 @@ fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx,
   }
   __syncthreads();
   C* F = lds_fft<true, T>(A, B, P, ax.px, tb.wx);
+  double dot = 0.0;
   for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x) {
     const int pp = idx / nx, x = idx % nx;
     const int64_t rho = 2 * (pair0 + pp);
@@ -352,12 +389,26 @@ fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx,
       f[rho * nx + x] = a.x;
       f[n + rho * nx + x] = a.y;
       f[2 * n + rho * nx + x] = c.x;
+      if constexpr (kDot) {
+        const T* rv = rdot + int64_t(blockIdx.y) * field_stride;
+        dot += double(rv[rho * nx + x]) * double(a.x) + double(rv[n + rho * nx + x]) * double(a.y) +
+               double(rv[2 * n + rho * nx + x]) * double(c.x);
+      }
     }
     if (rho + 1 < rows) {
       f[(rho + 1) * nx + x] = b.x;
       f[n + (rho + 1) * nx + x] = b.y;
       f[2 * n + (rho + 1) * nx + x] = c.y;
+      if constexpr (kDot) {
+        const T* rv = rdot + int64_t(blockIdx.y) * field_stride;
+        dot += double(rv[(rho + 1) * nx + x]) * double(b.x) + double(rv[n + (rho + 1) * nx + x]) * double(b.y) +
+               double(rv[2 * n + (rho + 1) * nx + x]) * double(c.y);
+      }
     }
+  }
+  if constexpr (kDot) {
+    const double total = block_reduce_sum<kThreads>(dot, red);
+    if (threadIdx.x == 0) partial[int64_t(blockIdx.y) * gridDim.x + blockIdx.x] = total;
   }
 }
 
@@ -367,9 +418,10 @@ fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx,
 template <bool INV, typename T>
 __global__ void __launch_bounds__(kThreads)
 fft_y(typename CT<T>::type* __restrict__ S, int ny, int nz, int W, int TJ, Axes ax, Tables<T> tb,
-      int64_t spec_stride) {
+      int64_t spec_stride, const int* active) {
   using C = typename CT<T>::type;
   extern __shared__ double2 smem_raw[];
+  if (active && !active[blockIdx.z]) return;
   C* A = reinterpret_cast<C*>(smem_raw);
   C* B = A + TJ * ny;
   const int j0 = blockIdx.x * TJ;
@@ -392,9 +444,10 @@ fft_y(typename CT<T>::type* __restrict__ S, int ny, int nz, int W, int TJ, Axes 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int TK, Axes ax,
-            Tables<T> tb, T lam, T mu, T inv_vn, int64_t spec_stride) {
+            Tables<T> tb, T lam, T mu, T inv_vn, int64_t spec_stride, const int* active) {
   using C = typename CT<T>::type;
   extern __shared__ double2 smem_raw[];
+  if (active && !active[blockIdx.z]) return;
   C* A = reinterpret_cast<C*>(smem_raw);
   const int TJ = 3 * TK;
   C* B = A + TJ * nz;
@@ -522,14 +575,13 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   im->axes.px = make_plan(im->nx);
   im->axes.py = make_plan(im->ny);
   im->axes.pz = make_plan(im->nz);
-  // Row segments of >= 120 B per tile row (16 FP32 / 8 FP64 complex columns)
-  // for coalescing, within kMaxLds for the two ping-pong buffers.
-  const int cols = sizeof(T) == 4 ? 16 : 8;
-  const int cap = int(kMaxLds / (2 * sizeof(C)));  // complex entries per buffer
-  im->pairs_per_block = std::max(1, std::min(cols / 3 + 1, cap / (3 * im->nx)));
+  // ~2048 complex entries per ping-pong buffer (<= 32 KB FP32): several
+  // workgroups per CU so one group's loads overlap another's LDS transforms.
+  const int target = 2048;
+  im->pairs_per_block = std::max(1, target / (3 * im->nx));
   if (im->pairs_per_block % 2 == 0 && im->pairs_per_block > 1) --im->pairs_per_block;  // odd column count
-  im->tj = std::max(1, std::min({3 * im->nxh, cols, cap / im->ny}));
-  im->tk = std::max(1, std::min({im->nxh, (cols + 2) / 3, cap / (3 * im->nz)}));
+  im->tj = std::max(1, std::min(3 * im->nxh, target / im->ny));
+  im->tk = std::max(1, std::min(im->nxh, target / (3 * im->nz)));
   im->lds_x = 2 * sizeof(C) * size_t(3 * im->pairs_per_block) * im->nx;
   im->lds_y = 2 * sizeof(C) * size_t(im->tj) * im->ny;
   im->lds_z = 2 * sizeof(C) * size_t(3 * im->tk) * im->nz;
@@ -541,8 +593,13 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   return VXL_OK;
 }
 
+// Full preconditioner apply z = M r, or the fused CG form: stage 1 also does
+// r -= alpha q, x += alpha p, ||r||^2 partials (cg != nullptr), stage 5 also
+// reduces r.z partials (dot_partial != nullptr).  `skip_x` starts at stage 2
+// with a spectrum already produced by the caller.
 template <typename T>
-int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s) {
+int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s,
+               const int* active, const CgUpdate<T>* cg, double* dot_partial) {
   using C = typename CT<T>::type;
   const Tables<T> tb = tables_of<T>(im);
   C* S = static_cast<C*>(im->spec);
@@ -550,23 +607,35 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
   const int64_t pairs = (im->rows + 1) / 2;
   const unsigned bx = unsigned((pairs + im->pairs_per_block - 1) / im->pairs_per_block);
   const int W = 3 * im->nxh;
-  hipLaunchKernelGGL((fft_x_r2c<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx,
-                     im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride);
+  if (cg)
+    hipLaunchKernelGGL((fft_x_r2c<T, true>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx,
+                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,
+                       active, *cg);
+  else
+    hipLaunchKernelGGL((fft_x_r2c<T, false>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx,
+                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,
+                       active, CgUpdate<T>{});
   VXL_HIP(hipGetLastError());
   hipLaunchKernelGGL((fft_y<false, T>), dim3((W + im->tj - 1) / im->tj, im->nz, nrhs), dim3(kThreads),
-                     im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride);
+                     im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride, active);
   VXL_HIP(hipGetLastError());
   const double V = pc->cell->volume;
   const double N = double(pc->cell->grid.n);
   hipLaunchKernelGGL((fft_z_green<T>), dim3((im->nxh + im->tk - 1) / im->tk, im->ny, nrhs),
                      dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh, im->tk, im->axes, tb,
-                     T(pc->lam), T(pc->mu), T(1.0 / (V * N)), im->spec_stride);
+                     T(pc->lam), T(pc->mu), T(1.0 / (V * N)), im->spec_stride, active);
   VXL_HIP(hipGetLastError());
   hipLaunchKernelGGL((fft_y<true, T>), dim3((W + im->tj - 1) / im->tj, im->nz, nrhs), dim3(kThreads),
-                     im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride);
+                     im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride, active);
   VXL_HIP(hipGetLastError());
-  hipLaunchKernelGGL((fft_x_c2r<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
-                     im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride);
+  if (dot_partial)
+    hipLaunchKernelGGL((fft_x_c2r<T, true>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
+                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,
+                       active, cg ? cg->r : r, dot_partial);
+  else
+    hipLaunchKernelGGL((fft_x_c2r<T, false>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
+                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,
+                       active, nullptr, nullptr);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }
@@ -587,8 +656,41 @@ int precond_apply(vxl_precond* pc, int nrhs, const void* r, void* z, hipStream_t
   auto* im = static_cast<PrecondImpl*>(pc->impl);
   if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
   if (pc->precision == VXL_F64)
-    return apply_impl<double>(pc, im, nrhs, static_cast<const double*>(r), static_cast<double*>(z), s);
-  return apply_impl<float>(pc, im, nrhs, static_cast<const float*>(r), static_cast<float*>(z), s);
+    return apply_impl<double>(pc, im, nrhs, static_cast<const double*>(r), static_cast<double*>(z), s,
+                              nullptr, nullptr, nullptr);
+  return apply_impl<float>(pc, im, nrhs, static_cast<const float*>(r), static_cast<float*>(z), s, nullptr,
+                           nullptr, nullptr);
+}
+
+template <typename T>
+int precond_apply_cg(vxl_precond* pc, int nrhs, T* r, const T* q, T* x, const T* p, const double* alpha,
+                     double* rr_partial, T* z, double* rz_partial, const int* active, hipStream_t s) {
+  auto* im = static_cast<PrecondImpl*>(pc->impl);
+  if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
+  if ((pc->precision == VXL_F64) != std::is_same<T, double>::value) return fail("precond precision mismatch");
+  CgUpdate<T> cg{r, q, x, p, alpha, rr_partial};
+  return apply_impl<T>(pc, im, nrhs, r, z, s, active, q ? &cg : nullptr, rz_partial);
+}
+template int precond_apply_cg<float>(vxl_precond*, int, float*, const float*, float*, const float*,
+                                     const double*, double*, float*, double*, const int*, hipStream_t);
+template int precond_apply_cg<double>(vxl_precond*, int, double*, const double*, double*, const double*,
+                                      const double*, double*, double*, double*, const int*, hipStream_t);
+
+int precond_apply_masked(vxl_precond* pc, int nrhs, const void* r, void* z, const int* active,
+                         hipStream_t s) {
+  auto* im = static_cast<PrecondImpl*>(pc->impl);
+  if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
+  if (pc->precision == VXL_F64)
+    return apply_impl<double>(pc, im, nrhs, static_cast<const double*>(r), static_cast<double*>(z), s,
+                              active, nullptr, nullptr);
+  return apply_impl<float>(pc, im, nrhs, static_cast<const float*>(r), static_cast<float*>(z), s, active,
+                           nullptr, nullptr);
+}
+
+int precond_x_blocks(const vxl_precond* pc) {
+  const auto* im = static_cast<const PrecondImpl*>(pc->impl);
+  const int64_t pairs = (im->rows + 1) / 2;
+  return int((pairs + im->pairs_per_block - 1) / im->pairs_per_block);
 }
 
 void precond_release(vxl_precond* pc) {

@@ -38,15 +38,15 @@ struct Inner {
 
 int free_all(vxl_solver* s) {
   double* bufs[] = {s->f, s->x64, s->r64, s->q64, s->scal, s->partial, s->macro_dev,
-                    s->stress_partial, s->ch_dev};
+                    s->stress_partial, s->ch_dev, s->partial_st, s->partial_rr, s->partial_rz};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (s->mode == VXL_SOLVE_MIXED) {
-    void* inner[] = {s->x, s->r, s->p, s->q, s->z};
+    void* inner[] = {s->x, s->r, s->p, s->p2, s->q, s->z};
     for (void* b : inner)
       if (b) (void)hipFree(b);
   } else {
-    void* inner[] = {s->p, s->z};
+    void* inner[] = {s->p, s->p2, s->z};
     for (void* b : inner)
       if (b) (void)hipFree(b);
   }
@@ -73,7 +73,8 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   const bool mixed = s->mode == VXL_SOLVE_MIXED;
   T* x = static_cast<T*>(s->x);
   T* r = static_cast<T*>(s->r);
-  T* p = static_cast<T*>(s->p);
+  T* pbuf[2] = {static_cast<T*>(s->p), static_cast<T*>(s->p2)};
+  int pi = 0;  // pbuf[pi] holds the current search direction
   T* q = static_cast<T*>(s->q);
   T* z = static_cast<T*>(s->z);
   double* scal = s->scal;
@@ -142,11 +143,12 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   } else {
     VXL_TRY(launch_convert<double, double>(s->r64, s->f, len, R, act, st));
   }
-  // z = M r, p = z, rho = r.z
+  // z = M r, rho = r.z; beta = 0 so the first fused step forms p = z.
   VXL_TRY(precond_apply(&s->pc, R, r, z, st));
-  VXL_TRY(launch_convert<T, T>(p, z, len, R, act, st));
   VXL_TRY(launch_dot<T, T>(r, z, len, R, partial, act, st));
   VXL_TRY(launch_finalize(partial, nblk, kOpRho, scal, R, act, nullptr, st));
+  const int nblk_st = apply_cg_blocks<T>(cell, R);
+  const int nblk_x = precond_x_blocks(&s->pc);
   VXL_HIP(hipEventRecord(s->ev[1], st));
 
   int it = 0;
@@ -181,27 +183,36 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     return VXL_OK;
   };
 
+  // One fused iteration: [stencil: p = z + beta p, q = K p, p.q]
+  // [alpha] [FFT x pass: r -= alpha q, x += alpha p, ||r||^2] [y, z+Green, y]
+  // [FFT x^-1 pass: z, r.z] [latch] [beta].  At host checks that certify a
+  // system (FP64 true residual, reliable update or restart), z and r.z are
+  // recomputed for the new r before beta.
   while (any_active() && it < max_iter) {
-    VXL_TRY(launch_apply<T>(cell, R, p, q, st));
-    VXL_TRY(launch_dot<T, T>(p, q, len, R, partial, act, st));
-    VXL_TRY(launch_finalize(partial, nblk, kOpAlpha, scal, R, act, nullptr, st));
-    VXL_TRY(launch_update_xr<T>(x, r, p, q, len, R, scal + kAlpha * R, partial, act, st));
+    T* p_old = pbuf[pi];
+    T* p_new = pbuf[1 - pi];
+    VXL_TRY(launch_apply_cg<T>(cell, R, p_old, z, scal + kBeta * R, p_new, q, act, s->partial_st, nullptr, st));
+    VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st));
+    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, q, x, p_new, scal + kAlpha * R, s->partial_rr, z,
+                                s->partial_rz, act, st));
     ++it;
-    VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
+    VXL_TRY(launch_finalize(s->partial_rr, nblk_x, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
+    pi = 1 - pi;
+    bool recomputed = false;
     if (it == 1 || it % kCheckStride == 0 || it == max_iter) {
       VXL_TRY(read_slot(kRR, rr));
       VXL_HIP(hipMemcpyAsync(pin_stop, s->stop_dev, sizeof(int) * R, hipMemcpyDeviceToHost, st));
       VXL_HIP(hipStreamSynchronize(st));
-      std::vector<int> which(R, 0);
+      std::vector<int> which(R, 0), latched(R, 0);
       bool anyw = false;
       for (int k = 0; k < R; ++k) {
         if (!active[k]) continue;
         if (!std::isfinite(rr[k])) return fail("Nonfinite residual in CG recursion");
         s->trace[k].push_back({pin_stop[k] ? pin_stop[k] : it, std::sqrt(rr[k] / ff[k])});
         its[k] = pin_stop[k] ? pin_stop[k] : it;
-        const bool reached = pin_stop[k] != 0;
+        latched[k] = pin_stop[k] != 0;
         const bool drop = mixed && rr[k] < kReliable * kReliable * rr_ref[k];
-        if (reached || drop) {
+        if (latched[k] || drop) {
           which[k] = 1;
           anyw = true;
         }
@@ -211,15 +222,32 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
           for (int k = 0; k < R; ++k) which[k] = active[k];
         }
         VXL_TRY(certify(which));
+        // A latched system skipped the iterations since its latch, so its
+        // search direction is stale: restart it (beta = 0) if it continues.
+        for (int k = 0; k < R; ++k)
+          if (active[k] && latched[k]) restart[k] = 1;
         VXL_HIP(hipMemsetAsync(s->stop_dev, 0, sizeof(int) * R, st));
+        std::vector<int> mask(R, 0);
+        bool anym = false;
+        for (int k = 0; k < R; ++k) {
+          mask[k] = which[k] && active[k];
+          anym = anym || mask[k];
+        }
+        if (anym) {
+          VXL_TRY(upload_mask(mask));
+          VXL_TRY(precond_apply_masked(&s->pc, R, r, z, act, st));
+          recomputed = true;
+        }
       }
       VXL_TRY(upload_flags());
       if (!any_active()) break;
     }
-    VXL_TRY(precond_apply(&s->pc, R, r, z, st));
-    VXL_TRY(launch_dot<T, T>(r, z, len, R, partial, act, st));
-    VXL_TRY(launch_finalize(partial, nblk, kOpBeta, scal, R, act, s->restart_dev, st));
-    VXL_TRY(launch_update_p<T>(p, z, len, R, scal + kBeta * R, act, st));
+    if (recomputed) {
+      VXL_TRY(launch_dot<T, T>(r, z, len, R, partial, act, st));
+      VXL_TRY(launch_finalize(partial, nblk, kOpBeta, scal, R, act, s->restart_dev, st));
+    } else {
+      VXL_TRY(launch_finalize(s->partial_rz, nblk_x, kOpBeta, scal, R, act, s->restart_dev, st));
+    }
     if (std::any_of(restart.begin(), restart.end(), [](int v) { return v; })) {
       std::fill(restart.begin(), restart.end(), 0);
       VXL_TRY(upload_flags());
@@ -276,6 +304,7 @@ int solver_create(vxl_solver* s) {
     VXL_HIP(hipMalloc(&s->x, b32));
     VXL_HIP(hipMalloc(&s->r, b32));
     VXL_HIP(hipMalloc(&s->p, b32));
+    VXL_HIP(hipMalloc(&s->p2, b32));
     VXL_HIP(hipMalloc(&s->q, b32));
     VXL_HIP(hipMalloc(&s->z, b32));
   } else {
@@ -283,12 +312,18 @@ int solver_create(vxl_solver* s) {
     s->r = s->r64;
     s->q = s->q64;
     VXL_HIP(hipMalloc(&s->p, b64));
+    VXL_HIP(hipMalloc(&s->p2, b64));
     VXL_HIP(hipMalloc(&s->z, b64));
   }
   s->nblk = vec_blocks(s->len);
   const int nblk_rows = vec_blocks(cell->grid.n);
   const size_t np = size_t(std::max(s->nblk * R, 3 * R * (nblk_rows + 1)));
   VXL_HIP(hipMalloc(&s->partial, sizeof(double) * np));
+  {
+    const int st_blocks = s->mode == VXL_SOLVE_MIXED ? apply_cg_blocks<float>(cell, R)
+                                                     : apply_cg_blocks<double>(cell, R);
+    VXL_HIP(hipMalloc(&s->partial_st, sizeof(double) * size_t(st_blocks) * R));
+  }
   VXL_HIP(hipMalloc(&s->scal, sizeof(double) * kNumSlots * R));
   VXL_HIP(hipMalloc(&s->active_dev, sizeof(int) * R));
   VXL_HIP(hipMalloc(&s->restart_dev, sizeof(int) * R));
@@ -306,6 +341,9 @@ int solver_create(vxl_solver* s) {
   s->pc.nrhs_max = R;
   s->pc_ready = true;
   VXL_TRY(precond_create(&s->pc));
+  const size_t xb = size_t(precond_x_blocks(&s->pc)) * R;
+  VXL_HIP(hipMalloc(&s->partial_rr, sizeof(double) * xb));
+  VXL_HIP(hipMalloc(&s->partial_rz, sizeof(double) * xb));
   return VXL_OK;
 }
 

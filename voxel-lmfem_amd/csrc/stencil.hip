@@ -211,6 +211,14 @@ struct ApplyArgs {
   int zchunk;
   int tiles_x;
   G9<T> g;
+  // Fused CG direction update (kFuse): `u` holds p_old and `z` the
+  // preconditioned residual; p = z + beta[r] p_old is formed while staging,
+  // written to p_new at owned nodes, and p.Kp is reduced into `partial`.
+  const T* z;
+  T* p_new;
+  const double* beta;
+  const int* active;
+  double* partial;
 };
 
 __device__ __forceinline__ int wrapmod(int v, int n) {
@@ -352,12 +360,13 @@ apply_kernel(ApplyArgs<T> a) {
 // (4 nodes), runs the modal element action, and exchanges only the 4 corner
 // forces of one face through LDS.  One staged node plane; two barriers per
 // plane.  Same arithmetic as the full 8-point transform, regrouped.
-template <typename T, typename Idx, bool kScale, int TX, int TY>
+template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse>
 __global__ void __launch_bounds__((Tile<TX, TY>::NT))
 apply_kernel_v2(ApplyArgs<T> a) {
   using TL = Tile<TX, TY>;
   __shared__ T su[3][TL::PLANE];
   __shared__ T sc[12][TL::EL];
+  __shared__ double red[TL::NT / 64];
 
   const int tid = threadIdx.x;
   const int tile = blockIdx.x;
@@ -366,11 +375,19 @@ apply_kernel_v2(ApplyArgs<T> a) {
   const int z0 = blockIdx.y * a.zchunk;
   const int z1 = min(z0 + a.zchunk, a.nz);
   if (z0 >= a.nz) return;
+  if constexpr (kFuse) {
+    if (a.active && !a.active[blockIdx.z]) return;
+  }
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t nxy = int64_t(nx) * ny;
   const int64_t n = nxy * nz;
   const T* u = a.u + int64_t(blockIdx.z) * a.rhs_stride;
   T* out = a.out + int64_t(blockIdx.z) * a.rhs_stride;
+  const T* zf = kFuse ? a.z + int64_t(blockIdx.z) * a.rhs_stride : nullptr;
+  T* pn = kFuse ? a.p_new + int64_t(blockIdx.z) * a.rhs_stride : nullptr;
+  const double beta = kFuse ? a.beta[blockIdx.z] : 0.0;
+  const T bcoef = T(beta);
+  const bool use_p = kFuse && beta != 0.0;
 
   int ld_lds[TL::ITEMS];
   int64_t ld_off[TL::ITEMS];
@@ -388,7 +405,14 @@ apply_kernel_v2(ApplyArgs<T> a) {
       if (ld_lds[k] < 0) continue;
       const int64_t node = base + ld_off[k];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) su[c][ld_lds[k]] = u[c * n + node];
+      for (int c = 0; c < 3; ++c) {
+        if constexpr (kFuse) {
+          const T zv = zf[c * n + node];
+          su[c][ld_lds[k]] = use_p ? fma(bcoef, u[c * n + node], zv) : zv;
+        } else {
+          su[c][ld_lds[k]] = u[c * n + node];
+        }
+      }
     }
   };
 
@@ -440,6 +464,18 @@ apply_kernel_v2(ApplyArgs<T> a) {
   const int xl = tid % TX, yl = tid / TX;
   const bool node_valid = is_node && (x0 + xl) < nx && (y0 + yl) < ny;
   const int64_t n_xy = int64_t(y0 + yl) * nx + (x0 + xl);
+  // Fused: this node's p on the current plane, written once per owned plane.
+  T p_cur[3] = {T(0), T(0), T(0)};
+  double pq = 0.0;
+  auto own_p = [&](int z, T (&pv)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pv[c] = su[c][(yl + 1) * TL::PX + xl + 1];
+    if (z < z1) {
+      const int64_t node = int64_t(z) * nxy + n_xy;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pn[c * n + node] = pv[c];
+    }
+  };
 
   // Prologue: bottom face of layer z0-1, then layer z0-1 itself.
   load_plane(z0 - 1);
@@ -448,6 +484,9 @@ apply_kernel_v2(ApplyArgs<T> a) {
   __syncthreads();
   load_plane(z0);
   __syncthreads();
+  if constexpr (kFuse) {
+    if (node_valid) own_p(z0, p_cur);
+  }
   if (is_elem) {
     T Tf[3][4], Fb[3][4];
     face_modes(Tf);
@@ -462,6 +501,10 @@ apply_kernel_v2(ApplyArgs<T> a) {
     const int z = z0 + k;
     load_plane(z + 1);
     __syncthreads();
+    T p_nxt[3];
+    if constexpr (kFuse) {
+      if (node_valid) own_p(z + 1, p_nxt);
+    }
     if (is_elem) {
       T Tf[3][4], Fb[3][4], Ft[3][4];
       face_modes(Tf);
@@ -495,7 +538,19 @@ apply_kernel_v2(ApplyArgs<T> a) {
       const int64_t node = int64_t(z) * nxy + n_xy;
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) out[cc * n + node] = acc[cc];
+      if constexpr (kFuse) {
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          pq += double(p_cur[cc]) * double(acc[cc]);
+          p_cur[cc] = p_nxt[cc];
+        }
+      }
     }
+  }
+  if constexpr (kFuse) {
+    const double total = block_reduce_sum<TL::NT>(pq, red);
+    if (tid == 0)
+      a.partial[int64_t(blockIdx.z) * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = total;
   }
 }
 
@@ -517,12 +572,32 @@ G9<T> cast_g(const ModalFactors& f) {
   return g;
 }
 
+template <typename T, int TX, int TY>
+void apply_grid(const vxl_cell* cell, int nrhs, int& tiles_x, int& tiles, int& chunks, int& zchunk) {
+  const int nx = cell->grid.nx, ny = cell->grid.ny, nz = cell->grid.nz;
+  tiles_x = (nx + TX - 1) / TX;
+  const int tiles_y = (ny + TY - 1) / TY;
+  tiles = tiles_x * tiles_y;
+  // Enough workgroups to fill 256 CUs several times; chunks of >= 8 planes.
+  chunks = (4096 + tiles * nrhs - 1) / (tiles * nrhs);
+  chunks = std::max(1, std::min(chunks, nz / 8));
+  zchunk = (nz + chunks - 1) / chunks;
+  chunks = (nz + zchunk - 1) / zchunk;
+}
+
 template <typename T, typename Idx, bool kScale, int TX, int TY>
-int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s) {
+int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s,
+                      const T* z = nullptr, const double* beta = nullptr, T* p_new = nullptr,
+                      const int* active = nullptr, double* partial = nullptr, int* blocks = nullptr) {
   using TL = Tile<TX, TY>;
   ApplyArgs<T> a;
   a.u = u;
   a.out = out;
+  a.z = z;
+  a.beta = beta;
+  a.p_new = p_new;
+  a.active = active;
+  a.partial = partial;
   a.rhs_stride = 3 * cell->grid.n;
   a.index = cell->mat.index;
   a.table = std::is_same<T, double>::value ? reinterpret_cast<const T*>(cell->mat.c64)
@@ -531,37 +606,34 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
   a.nx = cell->grid.nx;
   a.ny = cell->grid.ny;
   a.nz = cell->grid.nz;
-  a.tiles_x = (a.nx + TX - 1) / TX;
-  const int tiles_y = (a.ny + TY - 1) / TY;
-  const int tiles = a.tiles_x * tiles_y;
-  // Enough workgroups to fill 256 CUs several times; chunks of >= 8 planes.
-  int chunks = (4096 + tiles * nrhs - 1) / (tiles * nrhs);
-  chunks = std::max(1, std::min(chunks, a.nz / 8));
-  a.zchunk = (a.nz + chunks - 1) / chunks;
-  chunks = (a.nz + a.zchunk - 1) / a.zchunk;
+  int tiles = 0, chunks = 0;
+  apply_grid<T, TX, TY>(cell, nrhs, a.tiles_x, tiles, chunks, a.zchunk);
   a.g = cast_g<T>(modal_factors(cell->h));
   dim3 grid(tiles, chunks, nrhs);
-  if (stencil_variant() == 1)
+  if (blocks) *blocks = tiles * chunks;
+  if (z)
+    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, true>), grid, dim3(TL::NT), 0, s, a);
+  else if (stencil_variant() == 1)
     hipLaunchKernelGGL((apply_kernel<T, Idx, kScale, TX, TY>), grid, dim3(TL::NT), 0, s, a);
   else
-    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY>), grid, dim3(TL::NT), 0, s, a);
+    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, false>), grid, dim3(TL::NT), 0, s, a);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }
 
-template <typename T, int TX, int TY>
-int dispatch_apply(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s) {
+template <typename T, int TX, int TY, typename... Extra>
+int dispatch_apply(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s, Extra... extra) {
   const bool sc = cell->mat.scale != nullptr;
   switch (cell->mat.index_kind) {
     case VXL_INDEX_U8:
-      return sc ? launch_apply_impl<T, uint8_t, true, TX, TY>(cell, nrhs, u, out, s)
-                : launch_apply_impl<T, uint8_t, false, TX, TY>(cell, nrhs, u, out, s);
+      return sc ? launch_apply_impl<T, uint8_t, true, TX, TY>(cell, nrhs, u, out, s, extra...)
+                : launch_apply_impl<T, uint8_t, false, TX, TY>(cell, nrhs, u, out, s, extra...);
     case VXL_INDEX_U16:
-      return sc ? launch_apply_impl<T, uint16_t, true, TX, TY>(cell, nrhs, u, out, s)
-                : launch_apply_impl<T, uint16_t, false, TX, TY>(cell, nrhs, u, out, s);
+      return sc ? launch_apply_impl<T, uint16_t, true, TX, TY>(cell, nrhs, u, out, s, extra...)
+                : launch_apply_impl<T, uint16_t, false, TX, TY>(cell, nrhs, u, out, s, extra...);
     case VXL_INDEX_I32:
-      return sc ? launch_apply_impl<T, int32_t, true, TX, TY>(cell, nrhs, u, out, s)
-                : launch_apply_impl<T, int32_t, false, TX, TY>(cell, nrhs, u, out, s);
+      return sc ? launch_apply_impl<T, int32_t, true, TX, TY>(cell, nrhs, u, out, s, extra...)
+                : launch_apply_impl<T, int32_t, false, TX, TY>(cell, nrhs, u, out, s, extra...);
   }
   return fail("unknown material index kind");
 }
@@ -715,6 +787,31 @@ int launch_apply<float>(const vxl_cell* cell, int nrhs, const float* u, float* o
 template <>
 int launch_apply<double>(const vxl_cell* cell, int nrhs, const double* u, double* out, hipStream_t s) {
   return dispatch_apply<double, 32, 8>(cell, nrhs, u, out, s);
+}
+
+template <>
+int launch_apply_cg<float>(const vxl_cell* cell, int nrhs, const float* p_old, const float* z,
+                           const double* beta, float* p_new, float* q, const int* active,
+                           double* partial, int* blocks, hipStream_t s) {
+  return dispatch_apply<float, 32, 16>(cell, nrhs, p_old, q, s, z, beta, p_new, active, partial, blocks);
+}
+template <>
+int launch_apply_cg<double>(const vxl_cell* cell, int nrhs, const double* p_old, const double* z,
+                            const double* beta, double* p_new, double* q, const int* active,
+                            double* partial, int* blocks, hipStream_t s) {
+  return dispatch_apply<double, 32, 8>(cell, nrhs, p_old, q, s, z, beta, p_new, active, partial, blocks);
+}
+template <>
+int apply_cg_blocks<float>(const vxl_cell* cell, int nrhs) {
+  int tx, tiles, chunks, zc;
+  apply_grid<float, 32, 16>(cell, nrhs, tx, tiles, chunks, zc);
+  return tiles * chunks;
+}
+template <>
+int apply_cg_blocks<double>(const vxl_cell* cell, int nrhs) {
+  int tx, tiles, chunks, zc;
+  apply_grid<double, 32, 8>(cell, nrhs, tx, tiles, chunks, zc);
+  return tiles * chunks;
 }
 
 int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, hipStream_t s) {

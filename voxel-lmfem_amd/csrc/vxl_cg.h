@@ -53,6 +53,17 @@ struct vxl_precond {
 namespace vxl {
 int precond_create(vxl_precond* pc);
 int precond_apply(vxl_precond* pc, int nrhs, const void* r, void* z, hipStream_t s);
+// Fused CG preconditioner step.  With q != nullptr the first pass applies
+// r -= alpha q, x += alpha p (in place) and writes ||r||^2 partials
+// (rr_partial[rhs * precond_x_blocks + b]); the last pass writes z = M r and
+// r.z partials (rz_partial, same layout).  Systems with active == 0 skipped.
+template <typename T>
+int precond_apply_cg(vxl_precond* pc, int nrhs, T* r, const T* q, T* x, const T* p, const double* alpha,
+                     double* rr_partial, T* z, double* rz_partial, const int* active, hipStream_t s);
+int precond_x_blocks(const vxl_precond* pc);
+// z = M r for the systems with active[k] != 0 only.
+int precond_apply_masked(vxl_precond* pc, int nrhs, const void* r, void* z, const int* active,
+                         hipStream_t s);
 void precond_release(vxl_precond* pc);
 }  // namespace vxl
 
@@ -65,9 +76,12 @@ struct vxl_solver {
   bool pc_ready;
   int64_t len;  // 3N
   double *f, *x64, *r64, *q64;
-  void *x, *r, *p, *q, *z;  // inner precision (aliases of the FP64 buffers in FP64 mode)
+  void *x, *r, *p, *p2, *q, *z;  // inner precision (x, r, q alias the FP64 buffers in FP64 mode)
   double* scal;
   double* partial;
+  double* partial_st;  // stencil p.q partials
+  double* partial_rr;  // FFT x-pass ||r||^2 partials
+  double* partial_rz;  // FFT x^-1 pass r.z partials
   int nblk;
   int* active_dev;
   int* restart_dev;

@@ -49,6 +49,24 @@ struct ModalFactors {
 };
 ModalFactors modal_factors(const double h[3]);
 
+// Sum of v over a workgroup of NT threads (NT a multiple of 64); the result
+// is valid in thread 0.  red must hold NT/64 doubles of LDS.  Every thread of
+// the workgroup must call it.
+template <int NT>
+__device__ __forceinline__ double block_reduce_sum(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) s += red[w];
+  }
+  __syncthreads();
+  return s;
+}
+
 struct Grid {
   int nx, ny, nz;
   int64_t n;
@@ -83,6 +101,14 @@ namespace vxl {
 // ---- stencil.hip -------------------------------------------------------------
 template <typename T>
 int launch_apply(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s);
+// Fused CG step: p_new = z + beta[r] p_old (beta = 0: p_new = z), q = K p_new,
+// partial[r * blocks + b] = per-workgroup sums of p_new . q (FP64); systems
+// with active[r] == 0 are skipped.  Returns the workgroups per RHS in *blocks.
+template <typename T>
+int launch_apply_cg(const vxl_cell* cell, int nrhs, const T* p_old, const T* z, const double* beta,
+                    T* p_new, T* q, const int* active, double* partial, int* blocks, hipStream_t s);
+template <typename T>
+int apply_cg_blocks(const vxl_cell* cell, int nrhs);
 int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, hipStream_t s);
 // partial: workspace of stress_partial_size(cell, nload) doubles; out: nload x 6
 int launch_avg_stress(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
