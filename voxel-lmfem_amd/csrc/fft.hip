@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace vxl {
 namespace {
@@ -504,6 +505,186 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
 }
 
 
+// ---- four-step pencil FFTs for power-of-two axes -----------------------------
+// n = N1 * N2.  Step 1: thread t < N2 of a column holds x[t + N2 k1] (k1 < N1),
+// transforms over k1, multiplies by w_n^(j1 t) and writes to LDS; step 2:
+// thread t < N1 reads the N2 values of row t, transforms over k2 and holds
+// X[t + N1 j2].  One LDS buffer of n x TJ complex, column-fastest (element e of
+// column p at L[e * TJ + p]) so every LDS access is conflict-minimal and every
+// global row segment is TJ contiguous complex values.  Twiddles come from the
+// axis table w[k] = exp(-2 pi i k / n).
+
+// In-register DFT of power-of-two size N; w_N^k = w[k * wstride].
+template <int N, bool INV, typename T, typename C>
+__device__ __forceinline__ void dft_reg(C* v, const C* w, int wstride) {
+  if constexpr (N == 1) {
+    return;
+  } else if constexpr (N == 2) {
+    dft2<INV, T>(v);
+  } else if constexpr (N == 4) {
+    dft4<INV, T>(v);
+  } else if constexpr (N == 8) {
+    dft8<INV, T>(v);
+  } else {
+    C e[N / 2], o[N / 2];
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+      e[k] = v[2 * k];
+      o[k] = v[2 * k + 1];
+    }
+    dft_reg<N / 2, INV, T>(e, w, 2 * wstride);
+    dft_reg<N / 2, INV, T>(o, w, 2 * wstride);
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+      C tw = w[k * wstride];
+      if (INV) tw = cconj(tw);
+      const C t = cmul(o[k], tw);
+      v[k] = cadd(e[k], t);
+      v[k + N / 2] = csub(e[k], t);
+    }
+  }
+}
+
+template <int N1, int N2>
+struct FourStep {
+  static constexpr int n = N1 * N2;
+  static constexpr int lanes = N1 > N2 ? N1 : N2;
+};
+
+// Step 1 for column p, lane t: v[] = x[t + N2 k1] -> DFT_N1 -> twiddle -> L.
+template <bool INV, typename T, int N1, int N2, typename C, typename Load>
+__device__ __forceinline__ void four_step_1(C* L, int TJ, int p, int t, const C* w, Load load) {
+  constexpr int n = N1 * N2;
+  C v[N1];
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) v[k1] = load(k1);  // element t + N2 k1
+  dft_reg<N1, INV, T>(v, w, N2);
+#pragma unroll
+  for (int j1 = 0; j1 < N1; ++j1) {
+    if (j1 > 0 && t > 0) {
+      C tw = w[(j1 * t) % n];
+      if (INV) tw = cconj(tw);
+      v[j1] = cmul(v[j1], tw);
+    }
+    L[(j1 * N2 + t) * TJ + p] = v[j1];
+  }
+}
+
+// Step 2 for column p, lane t: u[] = L[t N2 + k2] -> DFT_N2 -> store(t + N1 j2).
+template <bool INV, typename T, int N1, int N2, typename C, typename Store>
+__device__ __forceinline__ void four_step_2(const C* L, int TJ, int p, int t, const C* w, Store store) {
+  C u[N2];
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) u[k2] = L[(t * N2 + k2) * TJ + p];
+  dft_reg<N2, INV, T>(u, w, N1);
+#pragma unroll
+  for (int j2 = 0; j2 < N2; ++j2) store(t + N1 * j2, u[j2]);
+}
+
+template <bool INV, typename T, int N1, int N2>
+__global__ void __launch_bounds__(256)
+fft_y4(typename CT<T>::type* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict__ w,
+       int64_t spec_stride, const int* active) {
+  using C = typename CT<T>::type;
+  constexpr int n = N1 * N2;
+  extern __shared__ double2 smem_raw[];
+  if (active && !active[blockIdx.z]) return;
+  C* L = reinterpret_cast<C*>(smem_raw);
+  const int j0 = blockIdx.x * TJ;
+  const int tj = min(TJ, W - j0);
+  C* plane = S + int64_t(blockIdx.z) * spec_stride + int64_t(blockIdx.y) * n * W + j0;
+  const int p = threadIdx.x % TJ, t = threadIdx.x / TJ;
+  const bool col = p < tj;
+  if (t < N2)
+    four_step_1<INV, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C {
+      return col ? plane[int64_t(t + N2 * k1) * W + p] : C{T(0), T(0)};
+    });
+  __syncthreads();
+  if (t < N1 && col)
+    four_step_2<INV, T, N1, N2>(L, TJ, p, t, w, [&](int e, C v) { plane[int64_t(e) * W + p] = v; });
+}
+
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(256)
+fft_z4_green(typename CT<T>::type* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T mu,
+             T inv_vn, int64_t spec_stride, const int* active) {
+  using C = typename CT<T>::type;
+  constexpr int nz = N1 * N2;
+  extern __shared__ double2 smem_raw[];
+  if (active && !active[blockIdx.z]) return;
+  C* L = reinterpret_cast<C*>(smem_raw);
+  const int TJ = 3 * TK;
+  const int k0 = blockIdx.x * TK;
+  const int y = blockIdx.y;
+  const int tk = min(TK, nxh - k0);
+  const int W = 3 * nxh;
+  const int64_t zstride = int64_t(ny) * W;
+  C* col = S + int64_t(blockIdx.z) * spec_stride + int64_t(y) * W + 3 * k0;
+  const int p = threadIdx.x % TJ, t = threadIdx.x / TJ;
+  const bool live = p < 3 * tk;
+  const C* w = tb.wz;
+  // forward z transform: global -> L (natural kz order)
+  if (t < N2)
+    four_step_1<false, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C {
+      return live ? col[(t + N2 * k1) * zstride + p] : C{T(0), T(0)};
+    });
+  __syncthreads();
+  C u[N2];
+  if (t < N1) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) u[k2] = L[(t * N2 + k2) * TJ + p];
+    dft_reg<N2, false, T>(u, w, N1);
+  }
+  __syncthreads();
+  if (t < N1) {
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) L[(t + N1 * j2) * TJ + p] = u[j2];
+  }
+  __syncthreads();
+  // Green solve per (kz, kx) of this tile (see fft_z_green).
+  const T lm = lam + mu;
+  const T my = tb.gy[y], Dy = tb.gy[ny + y], vy = tb.gy[2 * ny + y], dy = tb.gy[3 * ny + y];
+  for (int idx = threadIdx.x; idx < nz * TK; idx += blockDim.x) {
+    const int kz = idx / TK, kk = idx % TK;
+    if (kk >= tk) continue;
+    const int kx = k0 + kk;
+    const T mx = tb.gx[kx], Dx = tb.gx[nxh + kx], vx = tb.gx[2 * nxh + kx], dx = tb.gx[3 * nxh + kx];
+    const T mz = tb.gz[kz], Dz = tb.gz[nz + kz], vz = tb.gz[2 * nz + kz], dz = tb.gz[3 * nz + kz];
+    C* q0 = &L[kz * TJ + 3 * kk + 0];
+    C* q1 = &L[kz * TJ + 3 * kk + 1];
+    C* q2 = &L[kz * TJ + 3 * kk + 2];
+    const T sumD = Dx + Dy + Dz;
+    if (!(sumD > T(0))) {
+      *q0 = *q1 = *q2 = C{T(0), T(0)};
+      continue;
+    }
+    const T i0 = T(1) / (mu * sumD + lm * dx);
+    const T i1 = T(1) / (mu * sumD + lm * dy);
+    const T i2 = T(1) / (mu * sumD + lm * dz);
+    const T gamma = lm / (T(1) + lm * (vx * vx * i0 + vy * vy * i1 + vz * vz * i2));
+    const T scale = inv_vn / (mx * my * mz);
+    const C a0 = {q0->x * i0, q0->y * i0}, a1 = {q1->x * i1, q1->y * i1}, a2 = {q2->x * i2, q2->y * i2};
+    const C sdot = {vx * a0.x + vy * a1.x + vz * a2.x, vx * a0.y + vy * a1.y + vz * a2.y};
+    const T w0 = gamma * vx * i0, w1 = gamma * vy * i1, w2 = gamma * vz * i2;
+    *q0 = C{scale * (a0.x - w0 * sdot.x), scale * (a0.y - w0 * sdot.y)};
+    *q1 = C{scale * (a1.x - w1 * sdot.x), scale * (a1.y - w1 * sdot.y)};
+    *q2 = C{scale * (a2.x - w2 * sdot.x), scale * (a2.y - w2 * sdot.y)};
+  }
+  __syncthreads();
+  // inverse z transform: L -> L (step 1, read before overwrite) -> global
+  C v[N1];
+  if (t < N2) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) v[k1] = L[(t + N2 * k1) * TJ + p];
+  }
+  __syncthreads();
+  if (t < N2)
+    four_step_1<true, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C { return v[k1]; });
+  __syncthreads();
+  if (t < N1 && live)
+    four_step_2<true, T, N1, N2>(L, TJ, p, t, w, [&](int e, C val) { col[e * zstride + p] = val; });
+}
+
 struct PrecondImpl {
   int nx, ny, nz, nxh;
   int64_t rows;
@@ -514,7 +695,71 @@ struct PrecondImpl {
   int64_t spec_stride;  // complex entries per RHS
   int pairs_per_block, tj, tk;
   size_t lds_x, lds_y, lds_z;
+  // four-step kernels for power-of-two y / z axes (0: use the Stockham pass)
+  int y4, z4;        // axis length handled by fft_y4 / fft_z4_green
+  int tj4, tk4;      // columns / kx per workgroup
+  int thr_y4, thr_z4;
+  size_t lds_y4, lds_z4;
 };
+
+// Power-of-two lengths with a four-step kernel; lanes = max(N1, N2).
+inline int four_step_lanes(int n) {
+  switch (n) {
+    case 2: return 2;
+    case 4: return 2;
+    case 8: return 4;
+    case 16: return 4;
+    case 32: return 8;
+    case 64: return 8;
+    case 128: return 16;
+    case 256: return 16;
+    case 512: return 32;
+  }
+  return 0;
+}
+
+template <bool INV, typename T>
+void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, typename CT<T>::type* S, int W, int TJ,
+               const typename CT<T>::type* w, int64_t stride, const int* active) {
+#define VXL_Y4(N, A, B)                                                                         \
+  case N:                                                                                     \
+    hipLaunchKernelGGL((fft_y4<INV, T, A, B>), grid, dim3(thr), lds, s, S, W, TJ, w, stride, active); \
+    break;
+  switch (n) {
+    VXL_Y4(2, 2, 1)
+    VXL_Y4(4, 2, 2)
+    VXL_Y4(8, 4, 2)
+    VXL_Y4(16, 4, 4)
+    VXL_Y4(32, 8, 4)
+    VXL_Y4(64, 8, 8)
+    VXL_Y4(128, 16, 8)
+    VXL_Y4(256, 16, 16)
+    VXL_Y4(512, 32, 16)
+  }
+#undef VXL_Y4
+}
+
+template <typename T>
+void launch_z4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, typename CT<T>::type* S, int ny, int nxh,
+               int TK, Tables<T> tb, T lam, T mu, T inv_vn, int64_t stride, const int* active) {
+#define VXL_Z4(N, A, B)                                                                         \
+  case N:                                                                                     \
+    hipLaunchKernelGGL((fft_z4_green<T, A, B>), grid, dim3(thr), lds, s, S, ny, nxh, TK, tb, lam, mu, \
+                       inv_vn, stride, active);                                               \
+    break;
+  switch (n) {
+    VXL_Z4(2, 2, 1)
+    VXL_Z4(4, 2, 2)
+    VXL_Z4(8, 4, 2)
+    VXL_Z4(16, 4, 4)
+    VXL_Z4(32, 8, 4)
+    VXL_Z4(64, 8, 8)
+    VXL_Z4(128, 16, 8)
+    VXL_Z4(256, 16, 16)
+    VXL_Z4(512, 32, 16)
+  }
+#undef VXL_Z4
+}
 
 template <typename T>
 int upload_tables(PrecondImpl* im, const double h[3]) {
@@ -587,6 +832,26 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   im->lds_z = 2 * sizeof(C) * size_t(3 * im->tk) * im->nz;
   if (im->lds_x > kMaxLds || im->lds_y > kMaxLds || im->lds_z > kMaxLds)
     return fail("grid axis too long for the LDS pencil FFT");
+  {
+    const int cols = sizeof(T) == 4 ? 16 : 8;  // >= 128 B row segments
+    const int ly = four_step_lanes(im->ny), lz = four_step_lanes(im->nz);
+    const char* env = std::getenv("VXL_FFT_FOURSTEP");
+    const bool enable = !(env && env[0] == '0');
+    im->y4 = (enable && ly) ? im->ny : 0;
+    im->z4 = (enable && lz) ? im->nz : 0;
+    if (im->y4) {
+      im->tj4 = std::max(1, std::min({cols, 256 / ly, 3 * im->nxh}));
+      im->thr_y4 = im->tj4 * ly;
+      im->lds_y4 = sizeof(C) * size_t(im->ny) * im->tj4;
+      if (im->lds_y4 > kMaxLds) im->y4 = 0;
+    }
+    if (im->z4) {
+      im->tk4 = std::max(1, std::min({(cols + 2) / 3, 256 / (3 * lz), im->nxh}));
+      im->thr_z4 = 3 * im->tk4 * lz;
+      im->lds_z4 = sizeof(C) * size_t(im->nz) * 3 * im->tk4;
+      if (im->lds_z4 > kMaxLds || im->thr_z4 > 256 || im->thr_z4 < 1) im->z4 = 0;
+    }
+  }
   im->spec_stride = int64_t(im->nz) * im->ny * im->nxh * 3;
   VXL_TRY(upload_tables<T>(im, cell->h));
   VXL_HIP(hipMalloc(&im->spec, sizeof(C) * size_t(im->spec_stride) * pc->nrhs_max));
@@ -616,18 +881,36 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
                        im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,
                        active, CgUpdate<T>{});
   VXL_HIP(hipGetLastError());
-  hipLaunchKernelGGL((fft_y<false, T>), dim3((W + im->tj - 1) / im->tj, im->nz, nrhs), dim3(kThreads),
-                     im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride, active);
-  VXL_HIP(hipGetLastError());
+  auto y_pass = [&](bool inverse) -> int {
+    if (im->y4) {
+      const dim3 g((W + im->tj4 - 1) / im->tj4, im->nz, nrhs);
+      if (inverse)
+        launch_y4<true, T>(im->y4, g, im->thr_y4, im->lds_y4, s, S, W, im->tj4, tb.wy, im->spec_stride, active);
+      else
+        launch_y4<false, T>(im->y4, g, im->thr_y4, im->lds_y4, s, S, W, im->tj4, tb.wy, im->spec_stride, active);
+    } else if (inverse) {
+      hipLaunchKernelGGL((fft_y<true, T>), dim3((W + im->tj - 1) / im->tj, im->nz, nrhs), dim3(kThreads),
+                         im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride, active);
+    } else {
+      hipLaunchKernelGGL((fft_y<false, T>), dim3((W + im->tj - 1) / im->tj, im->nz, nrhs), dim3(kThreads),
+                         im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride, active);
+    }
+    VXL_HIP(hipGetLastError());
+    return VXL_OK;
+  };
+  VXL_TRY(y_pass(false));
   const double V = pc->cell->volume;
   const double N = double(pc->cell->grid.n);
-  hipLaunchKernelGGL((fft_z_green<T>), dim3((im->nxh + im->tk - 1) / im->tk, im->ny, nrhs),
-                     dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh, im->tk, im->axes, tb,
-                     T(pc->lam), T(pc->mu), T(1.0 / (V * N)), im->spec_stride, active);
+  if (im->z4) {
+    launch_z4<T>(im->z4, dim3((im->nxh + im->tk4 - 1) / im->tk4, im->ny, nrhs), im->thr_z4, im->lds_z4, s, S,
+                 im->ny, im->nxh, im->tk4, tb, T(pc->lam), T(pc->mu), T(1.0 / (V * N)), im->spec_stride, active);
+  } else {
+    hipLaunchKernelGGL((fft_z_green<T>), dim3((im->nxh + im->tk - 1) / im->tk, im->ny, nrhs),
+                       dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh, im->tk, im->axes, tb,
+                       T(pc->lam), T(pc->mu), T(1.0 / (V * N)), im->spec_stride, active);
+  }
   VXL_HIP(hipGetLastError());
-  hipLaunchKernelGGL((fft_y<true, T>), dim3((W + im->tj - 1) / im->tj, im->nz, nrhs), dim3(kThreads),
-                     im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride, active);
-  VXL_HIP(hipGetLastError());
+  VXL_TRY(y_pass(true));
   if (dot_partial)
     hipLaunchKernelGGL((fft_x_c2r<T, true>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
                        im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,

@@ -183,22 +183,18 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     return VXL_OK;
   };
 
-  // One fused iteration: [stencil: p = z + beta p, q = K p, p.q]
-  // [alpha] [FFT x pass: r -= alpha q, x += alpha p, ||r||^2] [y, z+Green, y]
-  // [FFT x^-1 pass: z, r.z] [latch] [beta].  At host checks that certify a
-  // system (FP64 true residual, reliable update or restart), z and r.z are
-  // recomputed for the new r before beta.
+  // One iteration: [stencil: p = z + beta p, q = K p, p.q] [alpha]
+  // [x += alpha p, r -= alpha q, ||r||^2] [latch] (host check / FP64
+  // certification) [z = M r with r.z fused into the last FFT pass] [beta].
   while (any_active() && it < max_iter) {
     T* p_old = pbuf[pi];
     T* p_new = pbuf[1 - pi];
     VXL_TRY(launch_apply_cg<T>(cell, R, p_old, z, scal + kBeta * R, p_new, q, act, s->partial_st, nullptr, st));
     VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st));
-    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, q, x, p_new, scal + kAlpha * R, s->partial_rr, z,
-                                s->partial_rz, act, st));
+    VXL_TRY(launch_update_xr<T>(x, r, p_new, q, len, R, scal + kAlpha * R, partial, act, st));
     ++it;
-    VXL_TRY(launch_finalize(s->partial_rr, nblk_x, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
+    VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
     pi = 1 - pi;
-    bool recomputed = false;
     if (it == 1 || it % kCheckStride == 0 || it == max_iter) {
       VXL_TRY(read_slot(kRR, rr));
       VXL_HIP(hipMemcpyAsync(pin_stop, s->stop_dev, sizeof(int) * R, hipMemcpyDeviceToHost, st));
@@ -227,27 +223,13 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
         for (int k = 0; k < R; ++k)
           if (active[k] && latched[k]) restart[k] = 1;
         VXL_HIP(hipMemsetAsync(s->stop_dev, 0, sizeof(int) * R, st));
-        std::vector<int> mask(R, 0);
-        bool anym = false;
-        for (int k = 0; k < R; ++k) {
-          mask[k] = which[k] && active[k];
-          anym = anym || mask[k];
-        }
-        if (anym) {
-          VXL_TRY(upload_mask(mask));
-          VXL_TRY(precond_apply_masked(&s->pc, R, r, z, act, st));
-          recomputed = true;
-        }
       }
       VXL_TRY(upload_flags());
       if (!any_active()) break;
     }
-    if (recomputed) {
-      VXL_TRY(launch_dot<T, T>(r, z, len, R, partial, act, st));
-      VXL_TRY(launch_finalize(partial, nblk, kOpBeta, scal, R, act, s->restart_dev, st));
-    } else {
-      VXL_TRY(launch_finalize(s->partial_rz, nblk_x, kOpBeta, scal, R, act, s->restart_dev, st));
-    }
+    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, nullptr, nullptr, nullptr, nullptr, nullptr, z, s->partial_rz,
+                                act, st));
+    VXL_TRY(launch_finalize(s->partial_rz, nblk_x, kOpBeta, scal, R, act, s->restart_dev, st));
     if (std::any_of(restart.begin(), restart.end(), [](int v) { return v; })) {
       std::fill(restart.begin(), restart.end(), 0);
       VXL_TRY(upload_flags());

@@ -361,7 +361,7 @@ apply_kernel(ApplyArgs<T> a) {
 // forces of one face through LDS.  One staged node plane; two barriers per
 // plane.  Same arithmetic as the full 8-point transform, regrouped.
 template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse>
-__global__ void __launch_bounds__((Tile<TX, TY>::NT))
+__global__ void __launch_bounds__((Tile<TX, TY>::NT), (sizeof(T) == 4 ? 5 : 0))
 apply_kernel_v2(ApplyArgs<T> a) {
   using TL = Tile<TX, TY>;
   __shared__ T su[3][TL::PLANE];
