@@ -13,8 +13,9 @@ roofline: the FP32 six-RHS stencil apply (the dominant kernel), timed with HIP
 events on the stream it runs on; algorithmic bytes per launch =
 N (12 R read + 12 R write + 1 label byte), R = 6.
 cpu_baseline: the FP64 NumPy oracle (operator apply + Green preconditioner)
-timed at 64^3 on the host and extrapolated to a 256^3 six-load solve with the
-GPU's iteration count (labelled as extrapolated).
+timed at 64^3 on one host thread (child process, BLAS/OpenMP threads = 1,
+started before the GPU is initialised) and extrapolated to a 256^3 six-load
+solve with the GPU's iteration count (labelled as extrapolated).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 """
@@ -33,20 +34,19 @@ sys.path.insert(0, str(ROOT / "voxel-lmfem_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(iterations_total, n_target):
-    """Time the FP64 oracle's per-iteration kernels at 64^3; extrapolate."""
+def cpu_sample(n=64, reps=3):
+    """Seconds per RHS-iteration of the FP64 NumPy oracle at n^3 (operator apply + Green apply)."""
     sys.path.insert(0, str(ROOT))
     from oracle import fem, spectral
     from voxel_lmfem.cells import voronoi_polycrystal
     from voxel_lmfem import materials as pm
-    n = 64
     (phase, mats, dom, _), _ = voronoi_polycrystal(n, grains=200, seed=3)
     f = pm.build_field(phase, mats)
     full = pm.unpack21(f.rows[f.index.astype(np.int64)])
     h = dom / n
     op = fem.MatrixFree(full, phase.shape, h)
     u = np.random.default_rng(0).normal(size=(n ** 3, 3))
-    reps = 3
+    op.apply(u)
     t0 = time.perf_counter()
     for _ in range(reps):
         op.apply(u)
@@ -55,19 +55,33 @@ def cpu_baseline(iterations_total, n_target):
     for _ in range(reps):
         spectral.green_apply(u, phase.shape, h, 50.0, 40.0)
     t_green = (time.perf_counter() - t0) / reps
+    return dict(n=n, t_apply=t_apply, t_green=t_green)
+
+
+def cpu_baseline_child():
+    """Run cpu_sample in a child process pinned to one BLAS/OpenMP thread.
+
+    Called before the GPU is initialised; the child never touches the GPU."""
+    import subprocess
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+        env[k] = "1"
+    out = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-sample"], env=env,
+                         capture_output=True, text=True, timeout=600, check=True)
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(sample, iterations_total, n_target):
+    """Extrapolate the per-iteration oracle sample to a full n_target^3 solve."""
+    n = sample["n"]
     scale = (n_target / n) ** 3
-    per_iter = (t_apply + t_green) * scale
+    per_iter = (sample["t_apply"] + sample["t_green"]) * scale
     seconds = per_iter * iterations_total
-    cores = 1
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
     return dict(value=1.0 / seconds, unit="solves/s", cores=1, kind="port",
-                sample=(f"FP64 NumPy oracle: Gauss-B operator apply {t_apply:.3f} s + FFT Green apply "
-                        f"{t_green:.3f} s per RHS-iteration at 64^3 (200-grain polycrystal), x{scale:.0f} to "
-                        f"{n_target}^3, x{iterations_total} RHS-iterations of the GPU run; extrapolated, "
-                        f"single-threaded numpy ({cores} cores visible)"))
+                sample=(f"FP64 NumPy oracle in a child process with OMP/OpenBLAS/MKL threads = 1: Gauss-B "
+                        f"operator apply {sample['t_apply']:.3f} s + FFT Green apply {sample['t_green']:.3f} s per "
+                        f"RHS-iteration at {n}^3 (200-grain polycrystal, 3 reps each), x{scale:.0f} to "
+                        f"{n_target}^3, x{iterations_total} RHS-iterations of the GPU run; extrapolated"))
 
 
 def main():
@@ -78,14 +92,23 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--cpu-sample", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--stencil-only", action="store_true",
                     help="only run 10 six-RHS FP32 stencil applies and a 1 GiB device copy "
                          "(the calibration kernel for FETCH_SIZE/WRITE_SIZE counter runs)")
     args = ap.parse_args()
+    if args.cpu_sample:
+        print(json.dumps(cpu_sample()), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps(dict(error="this build measures one GPU; multi-GPU sharding is design-only "
                                     "(see DESIGN.md)", n_gpus=args.gpus)))
         sys.exit(2)
+    sample = None
+    if not (args.skip_cpu or args.stencil_only):
+        t0 = time.perf_counter()
+        sample = cpu_baseline_child()
+        print(f"# cpu sample {sample} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     import torch
     from voxel_lmfem.cells import voronoi_polycrystal
     from voxel_lmfem import materials as pm
@@ -183,8 +206,8 @@ def main():
           file=sys.stderr, flush=True)
 
     cpu = None
-    if not args.skip_cpu:
-        cpu = cpu_baseline(int(sum(its)), n)
+    if sample is not None:
+        cpu = cpu_baseline(sample, int(sum(its)), n)
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     line = dict(

@@ -72,6 +72,20 @@ int vxl_cell_destroy(vxl_cell* cell);
  *   hybrid_kernels.cu:117-127 (pool-indexed FP64 K.u). */
 int vxl_apply(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev);
 
+/* Stencil schedule of the cell per precision: 1 = per-layer 8-node modal
+ * transform, 2 = persistent face columns (default), 3 = 2 with the plane
+ * after next prefetched into registers.  All three compute the same sums.
+ * No reference counterpart: the reference's variant choice is recorded in
+ * _core/hybrid_operator/recommended.json:1-6 by hybrid_operator/study.py:64-82;
+ * vxl_tune_stencil is this build's measured equivalent: it times every
+ * schedule on the given buffers (reps launches each, HIP events on the cell's
+ * stream), keeps the fastest, and returns it in *best and the per-launch
+ * milliseconds of schedules 1..3 in ms3 (may be NULL). */
+int vxl_set_stencil_variant(vxl_cell* cell, int precision, int variant);
+int vxl_get_stencil_variant(vxl_cell* cell, int precision);
+int vxl_tune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev, int reps,
+                     int* best, double* ms3);
+
 /* f = -V sum_e B0^T C_e E for nload macro strains (host, nload x 6), FP64.
  * Replaces `rhs`  hybrid_kernels.cu:128-137 (rhs_kernel :49-68). */
 int vxl_rhs(vxl_cell* cell, int nload, const double* macro_host, double* f_dev);

@@ -50,14 +50,37 @@ def test_apply_matches_oracle(torch_gpu, shape):
     U = rng.normal(size=(2, n, 3))
     ref = np.stack([op.apply(u) for u in U])
     u64 = torch.tensor(np.stack([planes(u) for u in U]), device=cell.device)
-    out64 = cell.apply_planes(u64).cpu().numpy()
+    want = np.stack([planes(r) for r in ref])
     scale = np.abs(ref).max()
-    err64 = np.abs(out64 - np.stack([planes(r) for r in ref])).max() / scale
-    out32 = cell.apply_planes(u64.float()).double().cpu().numpy()
-    err32 = np.abs(out32 - np.stack([planes(r) for r in ref])).max() / scale
-    print(f"  fp64 err {err64:.2e} fp32 err {err32:.2e}", flush=True)
-    assert err64 < 1e-13
-    assert err32 < 2e-6
+    for variant in (1, 2, 3):  # every stencil schedule
+        cell.set_variant("float64", variant)
+        cell.set_variant("float32", variant)
+        out64 = cell.apply_planes(u64).cpu().numpy()
+        err64 = np.abs(out64 - want).max() / scale
+        out32 = cell.apply_planes(u64.float()).double().cpu().numpy()
+        err32 = np.abs(out32 - want).max() / scale
+        print(f"  schedule {variant}: fp64 err {err64:.2e} fp32 err {err32:.2e}", flush=True)
+        assert err64 < 1e-13
+        assert err32 < 2e-6
+    cell.close()
+
+
+def test_tune_stencil_keeps_fastest_schedule(torch_gpu):
+    torch = torch_gpu
+    from voxel_lmfem.gpu import Cell
+    shape = (16, 24, 32)
+    print(f"case tune shape={shape}", flush=True)
+    field, full, h = make_cell(shape, seed=5)
+    cell = Cell(field, shape, h)
+    for precision in ("float32", "float64"):
+        best, ms = cell.tune(precision, nrhs=6 if precision == "float32" else 1, reps=3)
+        print(f"  {precision}: best {best} ms {ms}", flush=True)
+        assert best in (1, 2, 3) and all(m > 0 for m in ms)
+        assert ms[best - 1] == min(ms)
+        code = 0 if precision == "float32" else 1
+        assert cell.lib.vxl_get_stencil_variant(cell.handle, code) == best
+    with pytest.raises(ValueError):
+        cell.set_variant("float32", 4)
     cell.close()
 
 

@@ -107,6 +107,7 @@ int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy
   c->mat.scale = scale_dev;
   c->c64_owned = nullptr;
   c->c32_owned = nullptr;
+  c->variant[0] = c->variant[1] = default_stencil_variant();
   (void)hipGetDevice(&c->device);
   std::vector<double> t64(size_t(rows) * kRowWidth, 0.0);
   std::vector<float> t32(size_t(rows) * kRowWidth, 0.0f);
@@ -165,6 +166,57 @@ int vxl_apply(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* 
     return launch_apply<float>(cell, nrhs, static_cast<const float*>(u_dev), static_cast<float*>(out_dev),
                                cell->stream);
   return fail("precision must be VXL_F32 or VXL_F64");
+}
+
+int vxl_set_stencil_variant(vxl_cell* cell, int precision, int variant) {
+  g_error.clear();
+  if (!cell || (precision != VXL_F32 && precision != VXL_F64) || variant < 1 || variant > 3)
+    return fail("vxl_set_stencil_variant: invalid arguments");
+  cell->variant[precision == VXL_F64 ? 1 : 0] = variant;
+  return VXL_OK;
+}
+
+int vxl_get_stencil_variant(vxl_cell* cell, int precision) {
+  if (!cell || (precision != VXL_F32 && precision != VXL_F64)) return fail("vxl_get_stencil_variant: invalid arguments");
+  return cell->variant[precision == VXL_F64 ? 1 : 0];
+}
+
+int vxl_tune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev, int reps,
+                     int* best, double* ms3) {
+  g_error.clear();
+  if (!cell || !u_dev || !out_dev || nrhs < 1 || reps < 1 || (precision != VXL_F32 && precision != VXL_F64))
+    return fail("vxl_tune_stencil: invalid arguments");
+  hipEvent_t e0, e1;
+  VXL_HIP(hipEventCreate(&e0));
+  VXL_HIP(hipEventCreate(&e1));
+  const int slot = precision == VXL_F64 ? 1 : 0;
+  double best_ms = 1e300;
+  int best_v = cell->variant[slot];
+  for (int v = 1; v <= 3; ++v) {
+    cell->variant[slot] = v;
+    int rc = vxl_apply(cell, precision, nrhs, u_dev, out_dev);  // warm-up
+    if (rc != VXL_OK) return rc;
+    VXL_HIP(hipEventRecord(e0, cell->stream));
+    for (int i = 0; i < reps; ++i) {
+      rc = vxl_apply(cell, precision, nrhs, u_dev, out_dev);
+      if (rc != VXL_OK) return rc;
+    }
+    VXL_HIP(hipEventRecord(e1, cell->stream));
+    VXL_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    VXL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    const double per = double(ms) / reps;
+    if (ms3) ms3[v - 1] = per;
+    if (per < best_ms) {
+      best_ms = per;
+      best_v = v;
+    }
+  }
+  cell->variant[slot] = best_v;
+  if (best) *best = best_v;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return VXL_OK;
 }
 
 int vxl_rhs(vxl_cell* cell, int nload, const double* macro_host, double* f_dev) {

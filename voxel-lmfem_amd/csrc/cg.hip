@@ -69,18 +69,6 @@ update_xr_kernel(T* x, T* r, const T* p, const T* q, int64_t len, const double* 
   if (threadIdx.x == 0) partial[int64_t(k) * gridDim.x + blockIdx.x] = acc;
 }
 
-// p = z + beta p (beta = 0 where restart[k]).
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-update_p_kernel(T* p, const T* z, int64_t len, const double* beta, const int* active) {
-  const int k = blockIdx.y;
-  if (active && !active[k]) return;
-  const T b = T(beta[k]);
-  const int64_t off = int64_t(k) * len;
-  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock)
-    p[off + i] = fma(b, p[off + i], z[off + i]);
-}
-
 // r64 = f - q64; partial ||r64||^2; optional r32 = float(r64).
 __global__ void __launch_bounds__(kBlock)
 residual_kernel(double* r, const double* f, const double* q, float* r32, int64_t len,
@@ -256,15 +244,6 @@ int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, 
 template int launch_update_xr<float>(float*, float*, const float*, const float*, int64_t, int, const double*, double*, const int*, hipStream_t);
 template int launch_update_xr<double>(double*, double*, const double*, const double*, int64_t, int, const double*, double*, const int*, hipStream_t);
 
-template <typename T>
-int launch_update_p(T* p, const T* z, int64_t len, int nrhs, const double* beta, const int* active,
-                    hipStream_t s) {
-  hipLaunchKernelGGL((update_p_kernel<T>), vgrid(len, nrhs), dim3(kBlock), 0, s, p, z, len, beta, active);
-  VXL_HIP(hipGetLastError());
-  return VXL_OK;
-}
-template int launch_update_p<float>(float*, const float*, int64_t, int, const double*, const int*, hipStream_t);
-template int launch_update_p<double>(double*, const double*, int64_t, int, const double*, const int*, hipStream_t);
 
 int launch_residual(double* r, const double* f, const double* q, float* r32, int64_t len, int nrhs,
                     double* partial, const int* active, hipStream_t s) {

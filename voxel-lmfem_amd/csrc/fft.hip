@@ -21,6 +21,8 @@
 //   3. z: forward complex, 3x3 Green solve, inverse complex (one pass)
 //   4. y: inverse complex
 //   5. x: complex-to-real (two rows per complex FFT)
+#include <functional>
+
 #include "vxl_cg.h"
 
 #include <algorithm>
@@ -240,49 +242,22 @@ struct Tables {
 // ---- pass 1: x real-to-complex ---------------------------------------------
 // Row pair (rho, rho+1) over the flattened (z, y) row index; three complex
 // pencils per pair: c0+i c1 of row rho, c0+i c1 of row rho+1, c2(rho)+i c2(rho+1).
-// Fused CG update (kFuse), per entry of the field before it is transformed:
-// r -= alpha q, x += alpha p (written back), and a per-workgroup FP64 partial
-// of ||r||^2 into partial[rhs * gridDim.x + blockIdx.x].
 template <typename T>
-struct CgUpdate {
-  T* r;  // residual, updated in place (also the transform input)
-  const T* q;
-  T* x;
-  const T* p;
-  const double* alpha;
-  double* partial;
-};
-
-template <typename T, bool kFuse>
 __global__ void __launch_bounds__(kThreads)
 fft_x_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int nx, int64_t rows,
           int nxh, int pairs_per_block, Axes ax, Tables<T> tb, int64_t field_stride,
-          int64_t spec_stride, const int* active, CgUpdate<T> cg) {
+          int64_t spec_stride, const int* active) {
   using C = typename CT<T>::type;
   extern __shared__ double2 smem_raw[];
-  __shared__ double red[kThreads / 64];
   if (active && !active[blockIdx.y]) return;
   C* A = reinterpret_cast<C*>(smem_raw);
   const int P = 3 * pairs_per_block;
   C* B = A + P * nx;
   const int64_t n = rows * nx;
-  const int64_t off = int64_t(blockIdx.y) * field_stride;
-  const T* f = r + off;
+  const T* f = r + int64_t(blockIdx.y) * field_stride;
   C* out = S + int64_t(blockIdx.y) * spec_stride;
   const int64_t pair0 = int64_t(blockIdx.x) * pairs_per_block;
-  const T alpha = kFuse ? T(cg.alpha[blockIdx.y]) : T(0);
-  double rr = 0.0;
-  auto fetch = [&](int64_t i) -> T {
-    if constexpr (kFuse) {
-      const T v = fma(-alpha, cg.q[off + i], cg.r[off + i]);
-      cg.r[off + i] = v;
-      cg.x[off + i] = fma(alpha, cg.p[off + i], cg.x[off + i]);
-      rr += double(v) * double(v);
-      return v;
-    } else {
-      return f[i];
-    }
-  };
+  auto fetch = [&](int64_t i) -> T { return f[i]; };
   for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x) {
     const int pp = idx / nx, x = idx % nx;
     const int64_t rho = 2 * (pair0 + pp);
@@ -330,22 +305,18 @@ fft_x_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int nx,
       o[2] = sp[2][1];
     }
   }
-  if constexpr (kFuse) {
-    const double total = block_reduce_sum<kThreads>(rr, red);
-    if (threadIdx.x == 0) cg.partial[int64_t(blockIdx.y) * gridDim.x + blockIdx.x] = total;
-  }
 }
 
 // ---- pass 5: x complex-to-real -----------------------------------------------
-// kDot: per-workgroup FP64 partial of r . z into partial[rhs * gridDim.x + blockIdx.x].
-template <typename T, bool kDot>
+// kDir: the output is the CG search direction, updated in place as
+// p = z + beta[rhs] p (beta = 0: p = z, the old p is not read).
+template <typename T, bool kDir>
 __global__ void __launch_bounds__(kThreads)
 fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx, int64_t rows,
           int nxh, int pairs_per_block, Axes ax, Tables<T> tb, int64_t field_stride,
-          int64_t spec_stride, const int* active, const T* rdot, double* partial) {
+          int64_t spec_stride, const int* active, const double* beta) {
   using C = typename CT<T>::type;
   extern __shared__ double2 smem_raw[];
-  __shared__ double red[kThreads / 64];
   if (active && !active[blockIdx.y]) return;
   C* A = reinterpret_cast<C*>(smem_raw);
   const int P = 3 * pairs_per_block;
@@ -381,35 +352,23 @@ fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx,
   }
   __syncthreads();
   C* F = lds_fft<true, T>(A, B, P, ax.px, tb.wx);
-  double dot = 0.0;
+  const T b = kDir ? T(beta[blockIdx.y]) : T(0);
+  const bool keep = kDir && b != T(0);
+  auto put = [&](int64_t i, T v) { f[i] = keep ? fma(b, f[i], v) : v; };
   for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x) {
     const int pp = idx / nx, x = idx % nx;
     const int64_t rho = 2 * (pair0 + pp);
-    const C a = F[x * P + 3 * pp + 0], b = F[x * P + 3 * pp + 1], c = F[x * P + 3 * pp + 2];
+    const C a = F[x * P + 3 * pp + 0], bb = F[x * P + 3 * pp + 1], c = F[x * P + 3 * pp + 2];
     if (rho < rows) {
-      f[rho * nx + x] = a.x;
-      f[n + rho * nx + x] = a.y;
-      f[2 * n + rho * nx + x] = c.x;
-      if constexpr (kDot) {
-        const T* rv = rdot + int64_t(blockIdx.y) * field_stride;
-        dot += double(rv[rho * nx + x]) * double(a.x) + double(rv[n + rho * nx + x]) * double(a.y) +
-               double(rv[2 * n + rho * nx + x]) * double(c.x);
-      }
+      put(rho * nx + x, a.x);
+      put(n + rho * nx + x, a.y);
+      put(2 * n + rho * nx + x, c.x);
     }
     if (rho + 1 < rows) {
-      f[(rho + 1) * nx + x] = b.x;
-      f[n + (rho + 1) * nx + x] = b.y;
-      f[2 * n + (rho + 1) * nx + x] = c.y;
-      if constexpr (kDot) {
-        const T* rv = rdot + int64_t(blockIdx.y) * field_stride;
-        dot += double(rv[(rho + 1) * nx + x]) * double(b.x) + double(rv[n + (rho + 1) * nx + x]) * double(b.y) +
-               double(rv[2 * n + (rho + 1) * nx + x]) * double(c.y);
-      }
+      put((rho + 1) * nx + x, bb.x);
+      put(n + (rho + 1) * nx + x, bb.y);
+      put(2 * n + (rho + 1) * nx + x, c.y);
     }
-  }
-  if constexpr (kDot) {
-    const double total = block_reduce_sum<kThreads>(dot, red);
-    if (threadIdx.x == 0) partial[int64_t(blockIdx.y) * gridDim.x + blockIdx.x] = total;
   }
 }
 
@@ -441,13 +400,54 @@ fft_y(typename CT<T>::type* __restrict__ S, int ny, int nz, int W, int TJ, Axes 
   }
 }
 
+// Green solve at one wavevector (kx, ky, kz), in place:
+//   out = 1/(t N) [q/E - gamma (v/E) (v . q/E)],  E_i = mu sum D + (lam+mu) d_i,
+//   gamma = (lam+mu)/(1 + (lam+mu) sum v_i^2/E_i),  t N = V N m_x m_y m_z;
+// k = 0 (sum D = 0) maps to 0.  Returns Re(conj(q) . out) in FP64.
+template <typename T, typename C>
+__device__ __forceinline__ double green_point(C& q0, C& q1, C& q2, const Tables<T>& tb, int kx, int nxh, T my,
+                                              T Dy, T vy, T dy, int kz, int nz, T lam, T mu, T inv_vn) {
+  const T mx = tb.gx[kx], Dx = tb.gx[nxh + kx], vx = tb.gx[2 * nxh + kx], dx = tb.gx[3 * nxh + kx];
+  const T mz = tb.gz[kz], Dz = tb.gz[nz + kz], vz = tb.gz[2 * nz + kz], dz = tb.gz[3 * nz + kz];
+  const T sumD = Dx + Dy + Dz;
+  if (!(sumD > T(0))) {
+    q0 = q1 = q2 = C{T(0), T(0)};
+    return 0.0;
+  }
+  const T lm = lam + mu;
+  const T i0 = T(1) / (mu * sumD + lm * dx);
+  const T i1 = T(1) / (mu * sumD + lm * dy);
+  const T i2 = T(1) / (mu * sumD + lm * dz);
+  const T gamma = lm / (T(1) + lm * (vx * vx * i0 + vy * vy * i1 + vz * vz * i2));
+  const T scale = inv_vn / (mx * my * mz);
+  const C a0 = {q0.x * i0, q0.y * i0}, a1 = {q1.x * i1, q1.y * i1}, a2 = {q2.x * i2, q2.y * i2};
+  const C sdot = {vx * a0.x + vy * a1.x + vz * a2.x, vx * a0.y + vy * a1.y + vz * a2.y};
+  const T w0 = gamma * vx * i0, w1 = gamma * vy * i1, w2 = gamma * vz * i2;
+  const C o0 = {scale * (a0.x - w0 * sdot.x), scale * (a0.y - w0 * sdot.y)};
+  const C o1 = {scale * (a1.x - w1 * sdot.x), scale * (a1.y - w1 * sdot.y)};
+  const C o2 = {scale * (a2.x - w2 * sdot.x), scale * (a2.y - w2 * sdot.y)};
+  const double d = double(q0.x) * o0.x + double(q0.y) * o0.y + double(q1.x) * o1.x + double(q1.y) * o1.y +
+                   double(q2.x) * o2.x + double(q2.y) * o2.y;
+  q0 = o0;
+  q1 = o1;
+  q2 = o2;
+  return d;
+}
+
 // ---- pass 3: z forward, Green solve, z inverse ---------------------------------
-template <typename T>
+// kDot: r . z by Parseval, sum over the half spectrum of w(kx) Re(conj(r^) z^)
+// with w = 1 at kx = 0 and kx = nyq (nx even), 2 elsewhere; z^ = G r^ already
+// carries the 1/N of the inverse transform, so the sum equals the real-space
+// r . z.  One FP64 partial per workgroup at
+// partial[(rhs * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x].
+template <typename T, bool kDot>
 __global__ void __launch_bounds__(kThreads)
 fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int TK, Axes ax,
-            Tables<T> tb, T lam, T mu, T inv_vn, int64_t spec_stride, const int* active) {
+            Tables<T> tb, T lam, T mu, T inv_vn, int64_t spec_stride, const int* active, int nyq,
+            double* partial) {
   using C = typename CT<T>::type;
   extern __shared__ double2 smem_raw[];
+  __shared__ double red[kThreads / 64];
   if (active && !active[blockIdx.z]) return;
   C* A = reinterpret_cast<C*>(smem_raw);
   const int TJ = 3 * TK;
@@ -465,38 +465,24 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
   __syncthreads();
   C* F = lds_fft<false, T>(A, B, TJ, ax.pz, tb.wz);
   C* G = F == A ? B : A;
-  // Green solve per (kz, ky, kx), in place in F:
-  //   out = 1/(t N) [q/E - gamma (v/E) (v . q/E)],  E_i = mu sum D + (lam+mu) d_i,
-  //   gamma = (lam+mu)/(1 + (lam+mu) sum v_i^2/E_i),  t N = V N m_x m_y m_z.
-  const T lm = lam + mu;
+  // Green solve per (kz, ky, kx), in place in F.
   const T my = tb.gy[y], Dy = tb.gy[ny + y], vy = tb.gy[2 * ny + y], dy = tb.gy[3 * ny + y];
+  double dot = 0.0;
   for (int idx = threadIdx.x; idx < nz * TK; idx += blockDim.x) {
     const int kz = idx / TK, kk = idx % TK;
     if (kk >= tk) continue;
     const int kx = k0 + kk;
-    const T mx = tb.gx[kx], Dx = tb.gx[nxh + kx], vx = tb.gx[2 * nxh + kx], dx = tb.gx[3 * nxh + kx];
-    const T mz = tb.gz[kz], Dz = tb.gz[nz + kz], vz = tb.gz[2 * nz + kz], dz = tb.gz[3 * nz + kz];
-    C* q0 = &F[kz * TJ + 3 * kk + 0];
-    C* q1 = &F[kz * TJ + 3 * kk + 1];
-    C* q2 = &F[kz * TJ + 3 * kk + 2];
-    const T sumD = Dx + Dy + Dz;
-    if (!(sumD > T(0))) {
-      *q0 = *q1 = *q2 = C{T(0), T(0)};
-      continue;
-    }
-    const T i0 = T(1) / (mu * sumD + lm * dx);
-    const T i1 = T(1) / (mu * sumD + lm * dy);
-    const T i2 = T(1) / (mu * sumD + lm * dz);
-    const T gamma = lm / (T(1) + lm * (vx * vx * i0 + vy * vy * i1 + vz * vz * i2));
-    const T scale = inv_vn / (mx * my * mz);
-    const C a0 = {q0->x * i0, q0->y * i0}, a1 = {q1->x * i1, q1->y * i1}, a2 = {q2->x * i2, q2->y * i2};
-    const C sdot = {vx * a0.x + vy * a1.x + vz * a2.x, vx * a0.y + vy * a1.y + vz * a2.y};
-    const T w0 = gamma * vx * i0, w1 = gamma * vy * i1, w2 = gamma * vz * i2;
-    *q0 = C{scale * (a0.x - w0 * sdot.x), scale * (a0.y - w0 * sdot.y)};
-    *q1 = C{scale * (a1.x - w1 * sdot.x), scale * (a1.y - w1 * sdot.y)};
-    *q2 = C{scale * (a2.x - w2 * sdot.x), scale * (a2.y - w2 * sdot.y)};
+    const double d = green_point<T>(F[kz * TJ + 3 * kk + 0], F[kz * TJ + 3 * kk + 1], F[kz * TJ + 3 * kk + 2],
+                                    tb, kx, nxh, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
+    if constexpr (kDot) dot += (kx == 0 || kx == nyq) ? d : 2.0 * d;
   }
-  __syncthreads();
+  if constexpr (kDot) {
+    const double total = block_reduce_sum<kThreads>(dot, red);
+    if (threadIdx.x == 0)
+      partial[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = total;
+  } else {
+    __syncthreads();
+  }
   C* H = lds_fft<true, T>(F, G, TJ, ax.pz, tb.wz);
   for (int idx = threadIdx.x; idx < nz * TJ; idx += blockDim.x) {
     const int z = idx / TJ, j = idx % TJ;
@@ -604,13 +590,16 @@ fft_y4(typename CT<T>::type* __restrict__ S, int W, int TJ, const typename CT<T>
     four_step_2<INV, T, N1, N2>(L, TJ, p, t, w, [&](int e, C v) { plane[int64_t(e) * W + p] = v; });
 }
 
-template <typename T, int N1, int N2>
+// Launched with a multiple of 64 threads; threads past 3 TK lanes-per-column
+// only take part in the Green loop and the reduction.  kDot as in fft_z_green.
+template <typename T, int N1, int N2, bool kDot>
 __global__ void __launch_bounds__(256)
 fft_z4_green(typename CT<T>::type* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T mu,
-             T inv_vn, int64_t spec_stride, const int* active) {
+             T inv_vn, int64_t spec_stride, const int* active, int nyq, double* partial) {
   using C = typename CT<T>::type;
   constexpr int nz = N1 * N2;
   extern __shared__ double2 smem_raw[];
+  __shared__ double red[4];
   if (active && !active[blockIdx.z]) return;
   C* L = reinterpret_cast<C*>(smem_raw);
   const int TJ = 3 * TK;
@@ -641,36 +630,24 @@ fft_z4_green(typename CT<T>::type* __restrict__ S, int ny, int nxh, int TK, Tabl
     for (int j2 = 0; j2 < N2; ++j2) L[(t + N1 * j2) * TJ + p] = u[j2];
   }
   __syncthreads();
-  // Green solve per (kz, kx) of this tile (see fft_z_green).
-  const T lm = lam + mu;
+  // Green solve per (kz, kx) of this tile.
   const T my = tb.gy[y], Dy = tb.gy[ny + y], vy = tb.gy[2 * ny + y], dy = tb.gy[3 * ny + y];
+  double dot = 0.0;
   for (int idx = threadIdx.x; idx < nz * TK; idx += blockDim.x) {
     const int kz = idx / TK, kk = idx % TK;
     if (kk >= tk) continue;
     const int kx = k0 + kk;
-    const T mx = tb.gx[kx], Dx = tb.gx[nxh + kx], vx = tb.gx[2 * nxh + kx], dx = tb.gx[3 * nxh + kx];
-    const T mz = tb.gz[kz], Dz = tb.gz[nz + kz], vz = tb.gz[2 * nz + kz], dz = tb.gz[3 * nz + kz];
-    C* q0 = &L[kz * TJ + 3 * kk + 0];
-    C* q1 = &L[kz * TJ + 3 * kk + 1];
-    C* q2 = &L[kz * TJ + 3 * kk + 2];
-    const T sumD = Dx + Dy + Dz;
-    if (!(sumD > T(0))) {
-      *q0 = *q1 = *q2 = C{T(0), T(0)};
-      continue;
-    }
-    const T i0 = T(1) / (mu * sumD + lm * dx);
-    const T i1 = T(1) / (mu * sumD + lm * dy);
-    const T i2 = T(1) / (mu * sumD + lm * dz);
-    const T gamma = lm / (T(1) + lm * (vx * vx * i0 + vy * vy * i1 + vz * vz * i2));
-    const T scale = inv_vn / (mx * my * mz);
-    const C a0 = {q0->x * i0, q0->y * i0}, a1 = {q1->x * i1, q1->y * i1}, a2 = {q2->x * i2, q2->y * i2};
-    const C sdot = {vx * a0.x + vy * a1.x + vz * a2.x, vx * a0.y + vy * a1.y + vz * a2.y};
-    const T w0 = gamma * vx * i0, w1 = gamma * vy * i1, w2 = gamma * vz * i2;
-    *q0 = C{scale * (a0.x - w0 * sdot.x), scale * (a0.y - w0 * sdot.y)};
-    *q1 = C{scale * (a1.x - w1 * sdot.x), scale * (a1.y - w1 * sdot.y)};
-    *q2 = C{scale * (a2.x - w2 * sdot.x), scale * (a2.y - w2 * sdot.y)};
+    const double d = green_point<T>(L[kz * TJ + 3 * kk + 0], L[kz * TJ + 3 * kk + 1], L[kz * TJ + 3 * kk + 2],
+                                    tb, kx, nxh, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
+    if constexpr (kDot) dot += (kx == 0 || kx == nyq) ? d : 2.0 * d;
   }
-  __syncthreads();
+  if constexpr (kDot) {
+    const double total = block_reduce_dyn(dot, red);
+    if (threadIdx.x == 0)
+      partial[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = total;
+  } else {
+    __syncthreads();
+  }
   // inverse z transform: L -> L (step 1, read before overwrite) -> global
   C v[N1];
   if (t < N2) {
@@ -739,13 +716,14 @@ void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, typename CT
 #undef VXL_Y4
 }
 
-template <typename T>
+template <typename T, bool kDot>
 void launch_z4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, typename CT<T>::type* S, int ny, int nxh,
-               int TK, Tables<T> tb, T lam, T mu, T inv_vn, int64_t stride, const int* active) {
-#define VXL_Z4(N, A, B)                                                                         \
-  case N:                                                                                     \
-    hipLaunchKernelGGL((fft_z4_green<T, A, B>), grid, dim3(thr), lds, s, S, ny, nxh, TK, tb, lam, mu, \
-                       inv_vn, stride, active);                                               \
+               int TK, Tables<T> tb, T lam, T mu, T inv_vn, int64_t stride, const int* active, int nyq,
+               double* partial) {
+#define VXL_Z4(N, A, B)                                                                              \
+  case N:                                                                                          \
+    hipLaunchKernelGGL((fft_z4_green<T, A, B, kDot>), grid, dim3(thr), lds, s, S, ny, nxh, TK, tb, lam, mu, \
+                       inv_vn, stride, active, nyq, partial);                                      \
     break;
   switch (n) {
     VXL_Z4(2, 2, 1)
@@ -847,7 +825,7 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
     }
     if (im->z4) {
       im->tk4 = std::max(1, std::min({(cols + 2) / 3, 256 / (3 * lz), im->nxh}));
-      im->thr_z4 = 3 * im->tk4 * lz;
+      im->thr_z4 = (3 * im->tk4 * lz + 63) / 64 * 64;  // whole waves for the block reduction
       im->lds_z4 = sizeof(C) * size_t(im->nz) * 3 * im->tk4;
       if (im->lds_z4 > kMaxLds || im->thr_z4 > 256 || im->thr_z4 < 1) im->z4 = 0;
     }
@@ -858,13 +836,20 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   return VXL_OK;
 }
 
-// Full preconditioner apply z = M r, or the fused CG form: stage 1 also does
-// r -= alpha q, x += alpha p, ||r||^2 partials (cg != nullptr), stage 5 also
-// reduces r.z partials (dot_partial != nullptr).  `skip_x` starts at stage 2
-// with a spectrum already produced by the caller.
+// Preconditioner apply z = M r.  CG form (cg != nullptr): the z pass also
+// reduces r . z partials (Parseval) into cg->rz_partial, cg->mid() runs
+// between the z pass and the inverse y pass (it turns the partials into beta
+// on the same stream), and the last pass writes the search direction
+// p = z + beta p in place of z.
+struct CgHooks {
+  double* rz_partial;
+  const double* beta;
+  std::function<int()> mid;
+};
+
 template <typename T>
 int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s,
-               const int* active, const CgUpdate<T>* cg, double* dot_partial) {
+               const int* active, const CgHooks* cg) {
   using C = typename CT<T>::type;
   const Tables<T> tb = tables_of<T>(im);
   C* S = static_cast<C*>(im->spec);
@@ -872,14 +857,8 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
   const int64_t pairs = (im->rows + 1) / 2;
   const unsigned bx = unsigned((pairs + im->pairs_per_block - 1) / im->pairs_per_block);
   const int W = 3 * im->nxh;
-  if (cg)
-    hipLaunchKernelGGL((fft_x_r2c<T, true>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx,
-                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,
-                       active, *cg);
-  else
-    hipLaunchKernelGGL((fft_x_r2c<T, false>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx,
-                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,
-                       active, CgUpdate<T>{});
+  hipLaunchKernelGGL((fft_x_r2c<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx, im->rows,
+                     im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active);
   VXL_HIP(hipGetLastError());
   auto y_pass = [&](bool inverse) -> int {
     if (im->y4) {
@@ -901,24 +880,37 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
   VXL_TRY(y_pass(false));
   const double V = pc->cell->volume;
   const double N = double(pc->cell->grid.n);
+  const T lam = T(pc->lam), mu = T(pc->mu), inv_vn = T(1.0 / (V * N));
+  const int nyq = im->nx % 2 == 0 ? im->nx / 2 : -1;
+  double* part = cg ? cg->rz_partial : nullptr;
   if (im->z4) {
-    launch_z4<T>(im->z4, dim3((im->nxh + im->tk4 - 1) / im->tk4, im->ny, nrhs), im->thr_z4, im->lds_z4, s, S,
-                 im->ny, im->nxh, im->tk4, tb, T(pc->lam), T(pc->mu), T(1.0 / (V * N)), im->spec_stride, active);
+    const dim3 g((im->nxh + im->tk4 - 1) / im->tk4, im->ny, nrhs);
+    if (cg)
+      launch_z4<T, true>(im->z4, g, im->thr_z4, im->lds_z4, s, S, im->ny, im->nxh, im->tk4, tb, lam, mu, inv_vn,
+                         im->spec_stride, active, nyq, part);
+    else
+      launch_z4<T, false>(im->z4, g, im->thr_z4, im->lds_z4, s, S, im->ny, im->nxh, im->tk4, tb, lam, mu, inv_vn,
+                          im->spec_stride, active, nyq, part);
   } else {
-    hipLaunchKernelGGL((fft_z_green<T>), dim3((im->nxh + im->tk - 1) / im->tk, im->ny, nrhs),
-                       dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh, im->tk, im->axes, tb,
-                       T(pc->lam), T(pc->mu), T(1.0 / (V * N)), im->spec_stride, active);
+    const dim3 g((im->nxh + im->tk - 1) / im->tk, im->ny, nrhs);
+    if (cg)
+      hipLaunchKernelGGL((fft_z_green<T, true>), g, dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh,
+                         im->tk, im->axes, tb, lam, mu, inv_vn, im->spec_stride, active, nyq, part);
+    else
+      hipLaunchKernelGGL((fft_z_green<T, false>), g, dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh,
+                         im->tk, im->axes, tb, lam, mu, inv_vn, im->spec_stride, active, nyq, part);
   }
   VXL_HIP(hipGetLastError());
+  if (cg) VXL_TRY(cg->mid());
   VXL_TRY(y_pass(true));
-  if (dot_partial)
+  if (cg)
     hipLaunchKernelGGL((fft_x_c2r<T, true>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
-                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,
-                       active, cg ? cg->r : r, dot_partial);
+                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active,
+                       cg->beta);
   else
     hipLaunchKernelGGL((fft_x_c2r<T, false>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
-                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride,
-                       active, nullptr, nullptr);
+                       im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active,
+                       nullptr);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }
@@ -936,28 +928,22 @@ int precond_create(vxl_precond* pc) {
 }
 
 int precond_apply(vxl_precond* pc, int nrhs, const void* r, void* z, hipStream_t s) {
-  auto* im = static_cast<PrecondImpl*>(pc->impl);
-  if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
-  if (pc->precision == VXL_F64)
-    return apply_impl<double>(pc, im, nrhs, static_cast<const double*>(r), static_cast<double*>(z), s,
-                              nullptr, nullptr, nullptr);
-  return apply_impl<float>(pc, im, nrhs, static_cast<const float*>(r), static_cast<float*>(z), s, nullptr,
-                           nullptr, nullptr);
+  return precond_apply_masked(pc, nrhs, r, z, nullptr, s);
 }
 
 template <typename T>
-int precond_apply_cg(vxl_precond* pc, int nrhs, T* r, const T* q, T* x, const T* p, const double* alpha,
-                     double* rr_partial, T* z, double* rz_partial, const int* active, hipStream_t s) {
+int precond_apply_cg(vxl_precond* pc, int nrhs, const T* r, T* p, double* rz_partial, const double* beta,
+                     const std::function<int()>& mid, const int* active, hipStream_t s) {
   auto* im = static_cast<PrecondImpl*>(pc->impl);
   if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
   if ((pc->precision == VXL_F64) != std::is_same<T, double>::value) return fail("precond precision mismatch");
-  CgUpdate<T> cg{r, q, x, p, alpha, rr_partial};
-  return apply_impl<T>(pc, im, nrhs, r, z, s, active, q ? &cg : nullptr, rz_partial);
+  const CgHooks cg{rz_partial, beta, mid};
+  return apply_impl<T>(pc, im, nrhs, r, p, s, active, &cg);
 }
-template int precond_apply_cg<float>(vxl_precond*, int, float*, const float*, float*, const float*,
-                                     const double*, double*, float*, double*, const int*, hipStream_t);
-template int precond_apply_cg<double>(vxl_precond*, int, double*, const double*, double*, const double*,
-                                      const double*, double*, double*, double*, const int*, hipStream_t);
+template int precond_apply_cg<float>(vxl_precond*, int, const float*, float*, double*, const double*,
+                                     const std::function<int()>&, const int*, hipStream_t);
+template int precond_apply_cg<double>(vxl_precond*, int, const double*, double*, double*, const double*,
+                                      const std::function<int()>&, const int*, hipStream_t);
 
 int precond_apply_masked(vxl_precond* pc, int nrhs, const void* r, void* z, const int* active,
                          hipStream_t s) {
@@ -965,15 +951,15 @@ int precond_apply_masked(vxl_precond* pc, int nrhs, const void* r, void* z, cons
   if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
   if (pc->precision == VXL_F64)
     return apply_impl<double>(pc, im, nrhs, static_cast<const double*>(r), static_cast<double*>(z), s,
-                              active, nullptr, nullptr);
+                              active, nullptr);
   return apply_impl<float>(pc, im, nrhs, static_cast<const float*>(r), static_cast<float*>(z), s, active,
-                           nullptr, nullptr);
+                           nullptr);
 }
 
-int precond_x_blocks(const vxl_precond* pc) {
+int precond_rz_blocks(const vxl_precond* pc) {
   const auto* im = static_cast<const PrecondImpl*>(pc->impl);
-  const int64_t pairs = (im->rows + 1) / 2;
-  return int((pairs + im->pairs_per_block - 1) / im->pairs_per_block);
+  const int gx = im->z4 ? (im->nxh + im->tk4 - 1) / im->tk4 : (im->nxh + im->tk - 1) / im->tk;
+  return gx * im->ny;
 }
 
 void precond_release(vxl_precond* pc) {

@@ -38,17 +38,15 @@ struct Inner {
 
 int free_all(vxl_solver* s) {
   double* bufs[] = {s->f, s->x64, s->r64, s->q64, s->scal, s->partial, s->macro_dev,
-                    s->stress_partial, s->ch_dev, s->partial_st, s->partial_rr, s->partial_rz};
+                    s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (s->mode == VXL_SOLVE_MIXED) {
-    void* inner[] = {s->x, s->r, s->p, s->p2, s->q, s->z};
+    void* inner[] = {s->x, s->r, s->p, s->q};
     for (void* b : inner)
       if (b) (void)hipFree(b);
-  } else {
-    void* inner[] = {s->p, s->p2, s->z};
-    for (void* b : inner)
-      if (b) (void)hipFree(b);
+  } else if (s->p) {
+    (void)hipFree(s->p);
   }
   if (s->active_dev) (void)hipFree(s->active_dev);
   if (s->restart_dev) (void)hipFree(s->restart_dev);
@@ -73,10 +71,8 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   const bool mixed = s->mode == VXL_SOLVE_MIXED;
   T* x = static_cast<T*>(s->x);
   T* r = static_cast<T*>(s->r);
-  T* pbuf[2] = {static_cast<T*>(s->p), static_cast<T*>(s->p2)};
-  int pi = 0;  // pbuf[pi] holds the current search direction
+  T* p = static_cast<T*>(s->p);
   T* q = static_cast<T*>(s->q);
-  T* z = static_cast<T*>(s->z);
   double* scal = s->scal;
   double* partial = s->partial;
   const int nblk = s->nblk;
@@ -143,12 +139,13 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   } else {
     VXL_TRY(launch_convert<double, double>(s->r64, s->f, len, R, act, st));
   }
-  // z = M r, rho = r.z; beta = 0 so the first fused step forms p = z.
-  VXL_TRY(precond_apply(&s->pc, R, r, z, st));
-  VXL_TRY(launch_dot<T, T>(r, z, len, R, partial, act, st));
-  VXL_TRY(launch_finalize(partial, nblk, kOpRho, scal, R, act, nullptr, st));
+  // p = z = M r with rho = r.z: beta = 0 because rho_old = 0 (slots cleared).
   const int nblk_st = apply_cg_blocks<T>(cell, R);
-  const int nblk_x = precond_x_blocks(&s->pc);
+  const int nblk_rz = precond_rz_blocks(&s->pc);
+  const std::function<int()> form_beta = [&]() -> int {
+    return launch_finalize(s->partial_rz, nblk_rz, kOpBeta, scal, R, act, s->restart_dev, st);
+  };
+  VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
   VXL_HIP(hipEventRecord(s->ev[1], st));
 
   int it = 0;
@@ -183,18 +180,16 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     return VXL_OK;
   };
 
-  // One iteration: [stencil: p = z + beta p, q = K p, p.q] [alpha]
-  // [x += alpha p, r -= alpha q, ||r||^2] [latch] (host check / FP64
-  // certification) [z = M r with r.z fused into the last FFT pass] [beta].
+  // One iteration: [stencil: q = K p, p.q] [alpha] [x += alpha p,
+  // r -= alpha q, ||r||^2] [latch] (host check / FP64 certification)
+  // [FFT passes of M r; the z pass reduces r.z] [beta] [last pass:
+  // p = M r + beta p].
   while (any_active() && it < max_iter) {
-    T* p_old = pbuf[pi];
-    T* p_new = pbuf[1 - pi];
-    VXL_TRY(launch_apply_cg<T>(cell, R, p_old, z, scal + kBeta * R, p_new, q, act, s->partial_st, nullptr, st));
+    VXL_TRY(launch_apply_cg<T>(cell, R, p, q, act, s->partial_st, nullptr, st));
     VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st));
-    VXL_TRY(launch_update_xr<T>(x, r, p_new, q, len, R, scal + kAlpha * R, partial, act, st));
+    VXL_TRY(launch_update_xr<T>(x, r, p, q, len, R, scal + kAlpha * R, partial, act, st));
     ++it;
     VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
-    pi = 1 - pi;
     if (it == 1 || it % kCheckStride == 0 || it == max_iter) {
       VXL_TRY(read_slot(kRR, rr));
       VXL_HIP(hipMemcpyAsync(pin_stop, s->stop_dev, sizeof(int) * R, hipMemcpyDeviceToHost, st));
@@ -227,9 +222,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
       VXL_TRY(upload_flags());
       if (!any_active()) break;
     }
-    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, nullptr, nullptr, nullptr, nullptr, nullptr, z, s->partial_rz,
-                                act, st));
-    VXL_TRY(launch_finalize(s->partial_rz, nblk_x, kOpBeta, scal, R, act, s->restart_dev, st));
+    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
     if (std::any_of(restart.begin(), restart.end(), [](int v) { return v; })) {
       std::fill(restart.begin(), restart.end(), 0);
       VXL_TRY(upload_flags());
@@ -286,16 +279,12 @@ int solver_create(vxl_solver* s) {
     VXL_HIP(hipMalloc(&s->x, b32));
     VXL_HIP(hipMalloc(&s->r, b32));
     VXL_HIP(hipMalloc(&s->p, b32));
-    VXL_HIP(hipMalloc(&s->p2, b32));
     VXL_HIP(hipMalloc(&s->q, b32));
-    VXL_HIP(hipMalloc(&s->z, b32));
   } else {
     s->x = s->x64;
     s->r = s->r64;
     s->q = s->q64;
     VXL_HIP(hipMalloc(&s->p, b64));
-    VXL_HIP(hipMalloc(&s->p2, b64));
-    VXL_HIP(hipMalloc(&s->z, b64));
   }
   s->nblk = vec_blocks(s->len);
   const int nblk_rows = vec_blocks(cell->grid.n);
@@ -323,9 +312,7 @@ int solver_create(vxl_solver* s) {
   s->pc.nrhs_max = R;
   s->pc_ready = true;
   VXL_TRY(precond_create(&s->pc));
-  const size_t xb = size_t(precond_x_blocks(&s->pc)) * R;
-  VXL_HIP(hipMalloc(&s->partial_rr, sizeof(double) * xb));
-  VXL_HIP(hipMalloc(&s->partial_rz, sizeof(double) * xb));
+  VXL_HIP(hipMalloc(&s->partial_rz, sizeof(double) * size_t(precond_rz_blocks(&s->pc)) * R));
   return VXL_OK;
 }
 

@@ -211,12 +211,8 @@ struct ApplyArgs {
   int zchunk;
   int tiles_x;
   G9<T> g;
-  // Fused CG direction update (kFuse): `u` holds p_old and `z` the
-  // preconditioned residual; p = z + beta[r] p_old is formed while staging,
-  // written to p_new at owned nodes, and p.Kp is reduced into `partial`.
-  const T* z;
-  T* p_new;
-  const double* beta;
+  // kFuse: systems with active[r] == 0 are skipped and u . (K u) of each
+  // workgroup is written to partial[(r * chunks + chunk) * tiles + tile].
   const int* active;
   double* partial;
 };
@@ -360,7 +356,7 @@ apply_kernel(ApplyArgs<T> a) {
 // (4 nodes), runs the modal element action, and exchanges only the 4 corner
 // forces of one face through LDS.  One staged node plane; two barriers per
 // plane.  Same arithmetic as the full 8-point transform, regrouped.
-template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse>
+template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse, bool kPF>
 __global__ void __launch_bounds__((Tile<TX, TY>::NT), (sizeof(T) == 4 ? 5 : 0))
 apply_kernel_v2(ApplyArgs<T> a) {
   using TL = Tile<TX, TY>;
@@ -383,11 +379,6 @@ apply_kernel_v2(ApplyArgs<T> a) {
   const int64_t n = nxy * nz;
   const T* u = a.u + int64_t(blockIdx.z) * a.rhs_stride;
   T* out = a.out + int64_t(blockIdx.z) * a.rhs_stride;
-  const T* zf = kFuse ? a.z + int64_t(blockIdx.z) * a.rhs_stride : nullptr;
-  T* pn = kFuse ? a.p_new + int64_t(blockIdx.z) * a.rhs_stride : nullptr;
-  const double beta = kFuse ? a.beta[blockIdx.z] : 0.0;
-  const T bcoef = T(beta);
-  const bool use_p = kFuse && beta != 0.0;
 
   int ld_lds[TL::ITEMS];
   int64_t ld_off[TL::ITEMS];
@@ -405,14 +396,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
       if (ld_lds[k] < 0) continue;
       const int64_t node = base + ld_off[k];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if constexpr (kFuse) {
-          const T zv = zf[c * n + node];
-          su[c][ld_lds[k]] = use_p ? fma(bcoef, u[c * n + node], zv) : zv;
-        } else {
-          su[c][ld_lds[k]] = u[c * n + node];
-        }
-      }
+      for (int c = 0; c < 3; ++c) su[c][ld_lds[k]] = u[c * n + node];
     }
   };
 
@@ -464,20 +448,38 @@ apply_kernel_v2(ApplyArgs<T> a) {
   const int xl = tid % TX, yl = tid / TX;
   const bool node_valid = is_node && (x0 + xl) < nx && (y0 + yl) < ny;
   const int64_t n_xy = int64_t(y0 + yl) * nx + (x0 + xl);
-  // Fused: this node's p on the current plane, written once per owned plane.
+  // kFuse: this node's u on the current plane, for the u . (K u) partial.
   T p_cur[3] = {T(0), T(0), T(0)};
   double pq = 0.0;
-  auto own_p = [&](int z, T (&pv)[3]) {
+  auto own_p = [&](T (&pv)[3]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) pv[c] = su[c][(yl + 1) * TL::PX + xl + 1];
-    if (z < z1) {
-      const int64_t node = int64_t(z) * nxy + n_xy;
+  };
+
+  // Next-plane prefetch: raw values of the plane after next are loaded into
+  // registers before the element computation and stored to LDS after it, so
+  // their HBM latency overlaps the arithmetic.
+  T pfz[TL::ITEMS][3];
+  auto prefetch = [&](int z) {
+    const int64_t base = int64_t(wrapmod(z, nz)) * nxy;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) pn[c * n + node] = pv[c];
+    for (int k = 0; k < TL::ITEMS; ++k) {
+      if (ld_lds[k] < 0) continue;
+      const int64_t node = base + ld_off[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pfz[k][c] = u[c * n + node];
+    }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int k = 0; k < TL::ITEMS; ++k) {
+      if (ld_lds[k] < 0) continue;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) su[c][ld_lds[k]] = pfz[k][c];
     }
   };
 
-  // Prologue: bottom face of layer z0-1, then layer z0-1 itself.
+  // Prologue: bottom face of layer z0-1, layer z0-1 itself, then plane z0+1.
   load_plane(z0 - 1);
   __syncthreads();
   if (is_elem) face_modes(Bf);
@@ -485,7 +487,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
   load_plane(z0);
   __syncthreads();
   if constexpr (kFuse) {
-    if (node_valid) own_p(z0, p_cur);
+    if (node_valid) own_p(p_cur);
   }
   if (is_elem) {
     T Tf[3][4], Fb[3][4];
@@ -497,17 +499,27 @@ apply_kernel_v2(ApplyArgs<T> a) {
       for (int m = 0; m < 4; ++m) Bf[cc][m] = Tf[cc][m];
   }
   __syncthreads();
+  if constexpr (kPF) {
+    load_plane(z0 + 1);
+    __syncthreads();
+  }
   for (int k = 0; z0 + k < z1; ++k) {
     const int z = z0 + k;
-    load_plane(z + 1);
-    __syncthreads();
+    const bool more = kPF && z + 1 < z1;  // another step follows: it needs plane z+2
+    if constexpr (!kPF) {
+      load_plane(z + 1);
+      __syncthreads();
+    }
+    // LDS holds plane z+1: read everything this step needs from it first.
     T p_nxt[3];
     if constexpr (kFuse) {
-      if (node_valid) own_p(z + 1, p_nxt);
+      if (node_valid) own_p(p_nxt);
     }
+    T Tf[3][4];
+    if (is_elem) face_modes(Tf);
+    if (more) prefetch(z + 2);
     if (is_elem) {
-      T Tf[3][4], Fb[3][4], Ft[3][4];
-      face_modes(Tf);
+      T Fb[3][4], Ft[3][4];
       element(z, Tf, Fb, Ft);
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
@@ -524,6 +536,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
       }
     }
     __syncthreads();
+    if (more) commit();
     if (node_valid) {
       T acc[3] = {T(0), T(0), T(0)};
 #pragma unroll
@@ -546,6 +559,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
         }
       }
     }
+    if constexpr (kPF) __syncthreads();
   }
   if constexpr (kFuse) {
     const double total = block_reduce_sum<TL::NT>(pq, red);
@@ -554,12 +568,14 @@ apply_kernel_v2(ApplyArgs<T> a) {
   }
 }
 
-// Stencil schedule: 2 (persistent face columns, default) or 1 (per-layer full
-// 8-node transform); VXL_STENCIL_VARIANT selects it for comparisons.
-int stencil_variant() {
+// Stencil schedule: 2 (persistent face columns, default), 3 (the same with the
+// plane after next prefetched into registers) or 1 (per-layer full 8-node
+// transform); VXL_STENCIL_VARIANT selects one for comparisons.
+int env_stencil_variant() {
   static const int v = [] {
     const char* e = std::getenv("VXL_STENCIL_VARIANT");
-    return (e && e[0] == '1') ? 1 : 2;
+    if (e && (e[0] == '1' || e[0] == '3')) return e[0] - '0';
+    return 2;
   }();
   return v;
 }
@@ -587,15 +603,11 @@ void apply_grid(const vxl_cell* cell, int nrhs, int& tiles_x, int& tiles, int& c
 
 template <typename T, typename Idx, bool kScale, int TX, int TY>
 int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s,
-                      const T* z = nullptr, const double* beta = nullptr, T* p_new = nullptr,
                       const int* active = nullptr, double* partial = nullptr, int* blocks = nullptr) {
   using TL = Tile<TX, TY>;
   ApplyArgs<T> a;
   a.u = u;
   a.out = out;
-  a.z = z;
-  a.beta = beta;
-  a.p_new = p_new;
   a.active = active;
   a.partial = partial;
   a.rhs_stride = 3 * cell->grid.n;
@@ -611,12 +623,19 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
   a.g = cast_g<T>(modal_factors(cell->h));
   dim3 grid(tiles, chunks, nrhs);
   if (blocks) *blocks = tiles * chunks;
-  if (z)
-    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, true>), grid, dim3(TL::NT), 0, s, a);
-  else if (stencil_variant() == 1)
+  const int v = cell->variant[std::is_same<T, double>::value ? 1 : 0];
+  if (partial) {
+    if (v == 3)
+      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, true, true>), grid, dim3(TL::NT), 0, s, a);
+    else
+      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, true, false>), grid, dim3(TL::NT), 0, s, a);
+  } else if (v == 1) {
     hipLaunchKernelGGL((apply_kernel<T, Idx, kScale, TX, TY>), grid, dim3(TL::NT), 0, s, a);
-  else
-    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, false>), grid, dim3(TL::NT), 0, s, a);
+  } else if (v == 3) {
+    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, false, true>), grid, dim3(TL::NT), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, false, false>), grid, dim3(TL::NT), 0, s, a);
+  }
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }
@@ -780,6 +799,8 @@ int by_index(const vxl_cell* cell, F&& f) {
 
 }  // namespace
 
+int default_stencil_variant() { return env_stencil_variant(); }
+
 template <>
 int launch_apply<float>(const vxl_cell* cell, int nrhs, const float* u, float* out, hipStream_t s) {
   return dispatch_apply<float, 32, 16>(cell, nrhs, u, out, s);
@@ -790,16 +811,14 @@ int launch_apply<double>(const vxl_cell* cell, int nrhs, const double* u, double
 }
 
 template <>
-int launch_apply_cg<float>(const vxl_cell* cell, int nrhs, const float* p_old, const float* z,
-                           const double* beta, float* p_new, float* q, const int* active,
+int launch_apply_cg<float>(const vxl_cell* cell, int nrhs, const float* p, float* q, const int* active,
                            double* partial, int* blocks, hipStream_t s) {
-  return dispatch_apply<float, 32, 16>(cell, nrhs, p_old, q, s, z, beta, p_new, active, partial, blocks);
+  return dispatch_apply<float, 32, 16>(cell, nrhs, p, q, s, active, partial, blocks);
 }
 template <>
-int launch_apply_cg<double>(const vxl_cell* cell, int nrhs, const double* p_old, const double* z,
-                            const double* beta, double* p_new, double* q, const int* active,
+int launch_apply_cg<double>(const vxl_cell* cell, int nrhs, const double* p, double* q, const int* active,
                             double* partial, int* blocks, hipStream_t s) {
-  return dispatch_apply<double, 32, 8>(cell, nrhs, p_old, q, s, z, beta, p_new, active, partial, blocks);
+  return dispatch_apply<double, 32, 8>(cell, nrhs, p, q, s, active, partial, blocks);
 }
 template <>
 int apply_cg_blocks<float>(const vxl_cell* cell, int nrhs) {

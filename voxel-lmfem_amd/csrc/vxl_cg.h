@@ -25,9 +25,6 @@ int launch_finalize(const double* partial, int nblk, int op, double* scal, int n
 template <typename T>
 int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, const double* alpha,
                      double* partial, const int* active, hipStream_t s);
-template <typename T>
-int launch_update_p(T* p, const T* z, int64_t len, int nrhs, const double* beta, const int* active,
-                    hipStream_t s);
 int launch_residual(double* r, const double* f, const double* q, float* r32, int64_t len, int nrhs,
                     double* partial, const int* active, hipStream_t s);
 int launch_accumulate(double* x64, float* x32, int64_t len, int nrhs, const int* active, hipStream_t s);
@@ -53,14 +50,16 @@ struct vxl_precond {
 namespace vxl {
 int precond_create(vxl_precond* pc);
 int precond_apply(vxl_precond* pc, int nrhs, const void* r, void* z, hipStream_t s);
-// Fused CG preconditioner step.  With q != nullptr the first pass applies
-// r -= alpha q, x += alpha p (in place) and writes ||r||^2 partials
-// (rr_partial[rhs * precond_x_blocks + b]); the last pass writes z = M r and
-// r.z partials (rz_partial, same layout).  Systems with active == 0 skipped.
+// CG preconditioner step: z = M r with r . z reduced by Parseval in the z
+// pass (rz_partial[rhs * precond_rz_blocks + b], FP64), mid() run between the
+// z pass and the inverse passes (it forms beta from the partials), and the
+// last pass writing the search direction p = z + beta[rhs] p in place
+// (beta = 0: p = z; the old p is then not read).  Systems with active == 0
+// are skipped.
 template <typename T>
-int precond_apply_cg(vxl_precond* pc, int nrhs, T* r, const T* q, T* x, const T* p, const double* alpha,
-                     double* rr_partial, T* z, double* rz_partial, const int* active, hipStream_t s);
-int precond_x_blocks(const vxl_precond* pc);
+int precond_apply_cg(vxl_precond* pc, int nrhs, const T* r, T* p, double* rz_partial, const double* beta,
+                     const std::function<int()>& mid, const int* active, hipStream_t s);
+int precond_rz_blocks(const vxl_precond* pc);
 // z = M r for the systems with active[k] != 0 only.
 int precond_apply_masked(vxl_precond* pc, int nrhs, const void* r, void* z, const int* active,
                          hipStream_t s);
@@ -76,12 +75,11 @@ struct vxl_solver {
   bool pc_ready;
   int64_t len;  // 3N
   double *f, *x64, *r64, *q64;
-  void *x, *r, *p, *p2, *q, *z;  // inner precision (x, r, q alias the FP64 buffers in FP64 mode)
+  void *x, *r, *p, *q;  // inner precision (x, r, q alias the FP64 buffers in FP64 mode)
   double* scal;
   double* partial;
   double* partial_st;  // stencil p.q partials
-  double* partial_rr;  // FFT x-pass ||r||^2 partials
-  double* partial_rz;  // FFT x^-1 pass r.z partials
+  double* partial_rz;  // FFT z-pass r.z partials (Parseval)
   int nblk;
   int* active_dev;
   int* restart_dev;

@@ -67,6 +67,20 @@ __device__ __forceinline__ double block_reduce_sum(double v, double* red) {
   return s;
 }
 
+// Sum of v over a workgroup whose size is a multiple of 64 (at most 1024);
+// red holds blockDim.x / 64 doubles.  Result valid in thread 0.
+__device__ __forceinline__ double block_reduce_dyn(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
 struct Grid {
   int nx, ny, nz;
   int64_t n;
@@ -94,6 +108,7 @@ struct vxl_cell {
   float* c32_owned;
   hipStream_t stream;
   int device;
+  int variant[2];  // stencil schedule per precision (VXL_F32, VXL_F64): 1, 2 or 3
 };
 
 namespace vxl {
@@ -101,12 +116,12 @@ namespace vxl {
 // ---- stencil.hip -------------------------------------------------------------
 template <typename T>
 int launch_apply(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s);
-// Fused CG step: p_new = z + beta[r] p_old (beta = 0: p_new = z), q = K p_new,
-// partial[r * blocks + b] = per-workgroup sums of p_new . q (FP64); systems
-// with active[r] == 0 are skipped.  Returns the workgroups per RHS in *blocks.
+// CG stencil step: q = K p and partial[r * blocks + b] = per-workgroup sums of
+// p . q (FP64); systems with active[r] == 0 are skipped.  Returns the
+// workgroups per RHS in *blocks.
 template <typename T>
-int launch_apply_cg(const vxl_cell* cell, int nrhs, const T* p_old, const T* z, const double* beta,
-                    T* p_new, T* q, const int* active, double* partial, int* blocks, hipStream_t s);
+int launch_apply_cg(const vxl_cell* cell, int nrhs, const T* p, T* q, const int* active, double* partial,
+                    int* blocks, hipStream_t s);
 template <typename T>
 int apply_cg_blocks(const vxl_cell* cell, int nrhs);
 int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, hipStream_t s);
@@ -114,6 +129,7 @@ int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double*
 int launch_avg_stress(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
                       double* partial, double* out_dev, hipStream_t s);
 int64_t stress_partial_size(const vxl_cell* cell, int nload);
+int default_stencil_variant();
 int launch_element_energy(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
                           double* energy, hipStream_t s);
 

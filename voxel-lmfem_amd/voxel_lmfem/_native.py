@@ -33,6 +33,9 @@ SIGNATURES = {
     "vxl_cell_create": (_I, [_PP, _I, _I, _I, _D, _D, _D, _I, _P, _I, _DP, _P, _P]),
     "vxl_cell_destroy": (_I, [_P]),
     "vxl_apply": (_I, [_P, _I, _I, _P, _P]),
+    "vxl_set_stencil_variant": (_I, [_P, _I, _I]),
+    "vxl_get_stencil_variant": (_I, [_P, _I]),
+    "vxl_tune_stencil": (_I, [_P, _I, _I, _P, _P, _I, _IP, _DP]),
     "vxl_rhs": (_I, [_P, _I, _DP, _P]),
     "vxl_avg_stress": (_I, [_P, _I, _P, _DP, _DP]),
     "vxl_project_zero_mean": (_I, [_P, _I, _P]),

@@ -87,6 +87,25 @@ class Cell:
         check(self.lib.vxl_apply(self.handle, precision, u.shape[0], dptr(u), dptr(out)), "vxl_apply")
         return out
 
+    def tune(self, precision="float32", nrhs=6, reps=5):
+        """Time every stencil schedule on this cell and keep the fastest.
+
+        Returns (best schedule, [ms per launch of schedules 1, 2, 3])."""
+        torch = self.torch
+        dtype = torch.float32 if precision in ("float32", "fp32", "mixed") else torch.float64
+        code = _native.VXL_F32 if dtype == torch.float32 else _native.VXL_F64
+        u = torch.randn(nrhs, 3, self.n, dtype=dtype, device=self.device)
+        out = torch.empty_like(u)
+        best = ctypes.c_int()
+        ms = (ctypes.c_double * 3)()
+        check(self.lib.vxl_tune_stencil(self.handle, code, nrhs, dptr(u), dptr(out), int(reps),
+                                        ctypes.byref(best), ms), "vxl_tune_stencil")
+        return best.value, [ms[0], ms[1], ms[2]]
+
+    def set_variant(self, precision, variant):
+        code = _native.VXL_F64 if precision in ("float64", "fp64") else _native.VXL_F32
+        check(self.lib.vxl_set_stencil_variant(self.handle, code, int(variant)), "vxl_set_stencil_variant")
+
     def rhs_planes(self, macro):
         macro, p = darray(np.atleast_2d(macro))
         out = self.torch.empty((len(macro), 3, self.n), dtype=self.torch.float64, device=self.device)
