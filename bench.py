@@ -34,8 +34,9 @@ sys.path.insert(0, str(ROOT / "voxel-lmfem_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_sample(n=64, reps=3):
-    """Seconds per RHS-iteration of the FP64 NumPy oracle at n^3 (operator apply + Green apply)."""
+def cpu_sample(n=64, budget_s=12.0):
+    """Seconds per RHS-iteration of the FP64 NumPy oracle at n^3 (operator apply + Green apply),
+    repeated until about budget_s seconds of CPU work have been timed."""
     sys.path.insert(0, str(ROOT))
     from oracle import fem, spectral
     from voxel_lmfem.cells import voronoi_polycrystal
@@ -46,7 +47,10 @@ def cpu_sample(n=64, reps=3):
     h = dom / n
     op = fem.MatrixFree(full, phase.shape, h)
     u = np.random.default_rng(0).normal(size=(n ** 3, 3))
+    t0 = time.perf_counter()
     op.apply(u)
+    spectral.green_apply(u, phase.shape, h, 50.0, 40.0)
+    reps = max(3, int(budget_s / max(time.perf_counter() - t0, 1e-3)))
     t0 = time.perf_counter()
     for _ in range(reps):
         op.apply(u)
@@ -55,7 +59,7 @@ def cpu_sample(n=64, reps=3):
     for _ in range(reps):
         spectral.green_apply(u, phase.shape, h, 50.0, 40.0)
     t_green = (time.perf_counter() - t0) / reps
-    return dict(n=n, t_apply=t_apply, t_green=t_green)
+    return dict(n=n, reps=reps, t_apply=t_apply, t_green=t_green)
 
 
 def cpu_baseline_child():
@@ -80,7 +84,7 @@ def cpu_baseline(sample, iterations_total, n_target):
     return dict(value=1.0 / seconds, unit="solves/s", cores=1, kind="port",
                 sample=(f"FP64 NumPy oracle in a child process with OMP/OpenBLAS/MKL threads = 1: Gauss-B "
                         f"operator apply {sample['t_apply']:.3f} s + FFT Green apply {sample['t_green']:.3f} s per "
-                        f"RHS-iteration at {n}^3 (200-grain polycrystal, 3 reps each), x{scale:.0f} to "
+                        f"RHS-iteration at {n}^3 (200-grain polycrystal, {sample['reps']} reps each), x{scale:.0f} to "
                         f"{n_target}^3, x{iterations_total} RHS-iterations of the GPU run; extrapolated"))
 
 

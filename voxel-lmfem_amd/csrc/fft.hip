@@ -786,6 +786,14 @@ Tables<T> tables_of(const PrecondImpl* im) {
 
 constexpr size_t kMaxLds = 65536;
 
+// Positive integer from the environment (tile-size overrides for sweeps), else 0.
+int env_int(const char* name) {
+  const char* e = std::getenv(name);
+  if (!e) return 0;
+  const int v = std::atoi(e);
+  return v > 0 ? v : 0;
+}
+
 template <typename T>
 int create_impl(vxl_precond* pc, PrecondImpl* im) {
   using C = typename CT<T>::type;
@@ -803,6 +811,7 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   const int target = 2048;
   im->pairs_per_block = std::max(1, target / (3 * im->nx));
   if (im->pairs_per_block % 2 == 0 && im->pairs_per_block > 1) --im->pairs_per_block;  // odd column count
+  if (const int v = env_int("VXL_FFT_XPAIRS")) im->pairs_per_block = v;
   im->tj = std::max(1, std::min(3 * im->nxh, target / im->ny));
   im->tk = std::max(1, std::min(im->nxh, target / (3 * im->nz)));
   im->lds_x = 2 * sizeof(C) * size_t(3 * im->pairs_per_block) * im->nx;
@@ -819,12 +828,14 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
     im->z4 = (enable && lz) ? im->nz : 0;
     if (im->y4) {
       im->tj4 = std::max(1, std::min({cols, 256 / ly, 3 * im->nxh}));
+      if (const int v = env_int("VXL_FFT_YCOLS")) im->tj4 = std::min({v, 256 / ly, 3 * im->nxh});
       im->thr_y4 = im->tj4 * ly;
       im->lds_y4 = sizeof(C) * size_t(im->ny) * im->tj4;
       if (im->lds_y4 > kMaxLds) im->y4 = 0;
     }
     if (im->z4) {
       im->tk4 = std::max(1, std::min({(cols + 2) / 3, 256 / (3 * lz), im->nxh}));
+      if (const int v = env_int("VXL_FFT_ZCOLS")) im->tk4 = std::min({v, 256 / (3 * lz), im->nxh});
       im->thr_z4 = (3 * im->tk4 * lz + 63) / 64 * 64;  // whole waves for the block reduction
       im->lds_z4 = sizeof(C) * size_t(im->nz) * 3 * im->tk4;
       if (im->lds_z4 > kMaxLds || im->thr_z4 > 256 || im->thr_z4 < 1) im->z4 = 0;

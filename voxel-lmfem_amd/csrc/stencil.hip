@@ -448,13 +448,12 @@ apply_kernel_v2(ApplyArgs<T> a) {
   const int xl = tid % TX, yl = tid / TX;
   const bool node_valid = is_node && (x0 + xl) < nx && (y0 + yl) < ny;
   const int64_t n_xy = int64_t(y0 + yl) * nx + (x0 + xl);
-  // kFuse: this node's u on the current plane, for the u . (K u) partial.
-  T p_cur[3] = {T(0), T(0), T(0)};
-  double pq = 0.0;
-  auto own_p = [&](T (&pv)[3]) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) pv[c] = su[c][(yl + 1) * TL::PX + xl + 1];
-  };
+  // kFuse: u . (K u) = sum over elements of u_e . f_e, and in face-mode form
+  // u_e . f_e = Tf . Ft + Bf . Fb (the face and z transforms are applied with
+  // their transposes to the forces).  Each element is counted by the tile and
+  // z chunk that own it: local columns 1..TX, rows 1..TY, inside the grid.
+  const bool own_elem = is_elem && exl >= 1 && eyl >= 1 && x0 - 1 + exl < nx && y0 - 1 + eyl < ny;
+  T pq = T(0);  // at most zchunk elements per thread; FP64 from the block sum on
 
   // Next-plane prefetch: raw values of the plane after next are loaded into
   // registers before the element computation and stored to LDS after it, so
@@ -486,9 +485,6 @@ apply_kernel_v2(ApplyArgs<T> a) {
   __syncthreads();
   load_plane(z0);
   __syncthreads();
-  if constexpr (kFuse) {
-    if (node_valid) own_p(p_cur);
-  }
   if (is_elem) {
     T Tf[3][4], Fb[3][4];
     face_modes(Tf);
@@ -511,21 +507,19 @@ apply_kernel_v2(ApplyArgs<T> a) {
       __syncthreads();
     }
     // LDS holds plane z+1: read everything this step needs from it first.
-    T p_nxt[3];
-    if constexpr (kFuse) {
-      if (node_valid) own_p(p_nxt);
-    }
     T Tf[3][4];
     if (is_elem) face_modes(Tf);
     if (more) prefetch(z + 2);
     if (is_elem) {
       T Fb[3][4], Ft[3][4];
       element(z, Tf, Fb, Ft);
+      T w = T(0);  // kFuse: u_e . f_e of this element
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
         T F[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
+          if constexpr (kFuse) w = fma(Tf[cc][m], Ft[cc][m], fma(Bf[cc][m], Fb[cc][m], w));
           F[m] = Ftp[cc][m] + Fb[cc][m];
           Ftp[cc][m] = Ft[cc][m];
           Bf[cc][m] = Tf[cc][m];
@@ -534,6 +528,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
 #pragma unroll
         for (int a2 = 0; a2 < 4; ++a2) sc[a2 * 3 + cc][tid] = F[a2];
       }
+      if constexpr (kFuse) pq += own_elem ? w : T(0);
     }
     __syncthreads();
     if (more) commit();
@@ -551,18 +546,11 @@ apply_kernel_v2(ApplyArgs<T> a) {
       const int64_t node = int64_t(z) * nxy + n_xy;
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) out[cc * n + node] = acc[cc];
-      if constexpr (kFuse) {
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-          pq += double(p_cur[cc]) * double(acc[cc]);
-          p_cur[cc] = p_nxt[cc];
-        }
-      }
     }
     if constexpr (kPF) __syncthreads();
   }
   if constexpr (kFuse) {
-    const double total = block_reduce_sum<TL::NT>(pq, red);
+    const double total = block_reduce_sum<TL::NT>(double(pq), red);
     if (tid == 0)
       a.partial[int64_t(blockIdx.z) * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = total;
   }
