@@ -48,6 +48,31 @@ const char* vxl_last_error(void);
 /* Library ABI version (major*10000 + minor*100 + patch). */
 int vxl_version(void);
 
+/* Device allocation cache.  Handles allocate their workspaces through a
+ * per-(device, size) cache: a destroyed handle's blocks are kept and reused by
+ * the next handle that asks for the same sizes, so repeated set-up of one grid
+ * does not return to the driver (the reference gets the same from the PyTorch
+ * caching allocator; torch.cuda.empty_cache is the counterpart of
+ * vxl_release_cached_memory).  Running out of device memory releases the
+ * cache of that device and retries once. */
+int vxl_cached_bytes(uint64_t* bytes);
+int vxl_release_cached_memory(void);
+
+/* Material front end on the device: index_dev[e] = lut_host[labels[e] - label_lo]
+ * over a resident label array (label_bytes 4 = int32, 8 = int64; C order
+ * z, y, x), where lut_host (span entries) holds each label's coefficient row
+ * or -1 for labels without a material.  Returns present_host[span] (1 where
+ * the label occurs), *undefined_host (1 if a voxel's label has no row; its
+ * index is then 0) and samples_host, the rows at the Zeman sampling lattice
+ * (every max(1, n/16)-th voxel per axis, C order,
+ * ceil(nz/sz) * ceil(ny/sy) * ceil(nx/sx) entries).  Synchronises `stream`.
+ * Replaces the label half of build_hybrid_material (hybrid_material.py:38-48,
+ * rows in phase_table order, material_field.py:57-68) and the sampling of
+ * _compute_reference_medium (hybrid_adapter.py:101-125). */
+int vxl_label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,
+                   const int* lut_host, int index_kind, void* index_dev, int* present_host, int* undefined_host,
+                   int* samples_host, void* stream);
+
 /* Cell: grid, voxel size and material field.
  * Replaces the operator construction of the reference's hybrid FI storage:
  *   HybridFIOperator.__init__  src/voxel_lmfem/_core/materials/hybrid_adapter.py:45-68
@@ -94,6 +119,15 @@ int vxl_rhs(vxl_cell* cell, int nload, const double* macro_host, double* f_dev);
  * Replaces `stress`  hybrid_kernels.cu:138-147 (stress_kernel :69-106). */
 int vxl_avg_stress(vxl_cell* cell, int nload, const double* u_dev, const double* macro_host,
                    double* out_host);
+
+/* Element-centre fields of nload FP64 solutions (macro_host: nload x 6):
+ * strain = E + B0 u_e (engineering Voigt), stress = C_e strain, both written
+ * as planes [r][i][e] (nload x 6 x N), and energy = strain . stress / 2 as
+ * [r][e] (nload x N); any output may be NULL.  Replaces recover_center_fields
+ * (femlib/batched/c3d8r_hourglass_operator.py:752-758, called by
+ * benchmark.workload for its field statistics, benchmark.py:99-107). */
+int vxl_center_fields(vxl_cell* cell, int nload, const double* u_dev, const double* macro_host,
+                      double* strain_dev, double* stress_dev, double* energy_dev);
 
 /* Subtract the per-component mean of nrhs FP64 fields in place.
  * Replaces HybridFIOperator.project_zero_mean  hybrid_adapter.py:78-80. */

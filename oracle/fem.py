@@ -9,7 +9,11 @@ Restates
   * ``src/voxel_lmfem/_core/materials/hybrid_kernels.cu:49-106`` semantics of
     the node-owned RHS (centroid B0) and the mean stress (centroid strain);
   * ``src/voxel_lmfem/_core/hybrid_operator/background.py:10-43`` (uniform
-    27-point stencil from tensor-product 1-D integrals).
+    27-point stencil from tensor-product 1-D integrals);
+  * ``src/voxel_lmfem/_core/runtime/fi/source/femlib/batched/
+    c3d8r_hourglass_operator.py:752-758`` (``recover_center_fields``: centroid
+    strain E + B0 u_e and stress C_e strain per element) and the energy
+    0.5 strain . stress of ``_core/runtime/benchmark.py:101``.
 
 Node numbering inside an element is a = ax + 2*ay + 4*az with node offset
 (ax, ay, az); global node id (z*Ny + y)*Nx + x, periodic.  Fields are stored
@@ -23,7 +27,12 @@ Pinning
   * ``uniform_stencil`` equals the assembled K rows of a uniform cell;
   * RHS and stress recovery: a uniform cell has f = 0 and <sigma> = C E;
     the laminate Schur formula and homogeneous/rotated KATs of
-    tests/test_solver.py:17-56 fix the full chain through ``assemble``.
+    tests/test_solver.py:17-56 fix the full chain through ``assemble``;
+  * ``center_fields``: a periodic affine-free field (u = 0) gives strain = E
+    and stress = C_e E in every element; a rigid translation leaves both
+    unchanged; a node-local displacement enters the 8 adjacent elements with
+    the centroid derivatives +-1/(4h); the element mean of the stress equals
+    ``avg_stress``.
 """
 import itertools
 
@@ -164,6 +173,14 @@ class MatrixFree:
         local = u[self.nodes].reshape(self.n, 24)
         strain = local @ self.B0.T + np.asarray(macro, dtype=np.float64)
         return np.einsum("eij,ej->ei", self.full, strain).mean(axis=0)
+
+    def center_fields(self, u, macro):
+        """Element-centre strain E + B0 u_e, stress C_e strain, energy strain.stress/2."""
+        u = np.asarray(u, dtype=np.float64).reshape(self.n, 3)
+        local = u[self.nodes].reshape(self.n, 24)
+        strain = local @ self.B0.T + np.asarray(macro, dtype=np.float64)
+        stress = np.einsum("eij,ej->ei", self.full, strain)
+        return strain, stress, 0.5 * (strain * stress).sum(axis=1)
 
     @staticmethod
     def project_zero_mean(u):

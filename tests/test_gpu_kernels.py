@@ -149,6 +149,34 @@ def test_rhs_and_stress_match_oracle(torch_gpu, shape):
     cell.close()
 
 
+@pytest.mark.parametrize("shape", [(4, 4, 4), (5, 3, 7)])
+def test_center_fields_match_oracle(torch_gpu, shape):
+    torch = torch_gpu
+    from voxel_lmfem.gpu import Cell
+    print(f"case centre fields shape={shape}", flush=True)
+    field, full, h = make_cell(shape, seed=11)
+    cell = Cell(field, shape, h)
+    op = fem.MatrixFree(full, shape, h)
+    rng = np.random.default_rng(12)
+    n = int(np.prod(shape))
+    macro = rng.normal(size=(2, 6))
+    U = rng.normal(size=(2, n, 3))
+    u = torch.tensor(np.stack([planes(x) for x in U]), device="cuda")
+    eps, sig, w = (t.cpu().numpy() for t in cell.center_fields_planes(u, macro))
+    for j in range(2):
+        e_ref, s_ref, w_ref = op.center_fields(U[j], macro[j])
+        assert np.abs(eps[j] - e_ref.T).max() < 1e-13 * np.abs(e_ref).max()
+        assert np.abs(sig[j] - s_ref.T).max() < 1e-13 * np.abs(s_ref).max()
+        assert np.abs(w[j] - w_ref).max() < 1e-13 * np.abs(w_ref).max()
+    # reference protocol: (1, N, 6) per load, node-major (1, 3N) input
+    u_ref = torch.tensor(U[0].reshape(1, -1), device="cuda")
+    strain, stress = cell.recover_center_fields(u_ref, macro[0])
+    assert tuple(strain.shape) == (1, n, 6) and tuple(stress.shape) == (1, n, 6)
+    np.testing.assert_allclose(stress[0].cpu().numpy(), op.center_fields(U[0], macro[0])[1], rtol=1e-12,
+                               atol=1e-13 * np.abs(sig).max())
+    cell.close()
+
+
 PC_SHAPES = [(4, 4, 4), (8, 8, 8), (6, 5, 4), (3, 7, 5), (16, 12, 10), (17, 2, 9), (32, 48, 20)]
 
 

@@ -110,6 +110,37 @@ def test_uniform_cell_rhs_zero_and_stress_is_C():
         np.testing.assert_allclose(op.avg_stress(np.zeros((n, 3)), E), C @ E, atol=1e-13)
 
 
+def test_center_fields_pins():
+    rng = np.random.default_rng(4)
+    shape, h = (3, 4, 5), np.array([.7, 1.1, 1.3])
+    n = int(np.prod(shape))
+    full = random_spd(rng, n)
+    op = fem.MatrixFree(full, shape, h)
+    E = rng.normal(size=6)
+    eps, sig, w = op.center_fields(np.zeros((n, 3)), E)
+    np.testing.assert_allclose(eps, np.broadcast_to(E, (n, 6)), atol=0)
+    np.testing.assert_allclose(sig, full @ E, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(w, 0.5 * np.einsum("ei,ei->e", sig, eps), rtol=1e-14)
+    u = rng.normal(size=(n, 3))
+    a = op.center_fields(u, E)
+    b = op.center_fields(u + rng.normal(size=3), E)  # rigid translation
+    np.testing.assert_allclose(a[0], b[0], atol=1e-12)
+    np.testing.assert_allclose(a[1].mean(axis=0), op.avg_stress(u, E), rtol=1e-12)
+    # one node displaced along x: the 8 elements that share it see du/dx = +-1/(4 hx)
+    nz, ny, nx = shape
+    node = (1 * ny + 2) * nx + 3
+    v = np.zeros((n, 3))
+    v[node, 0] = 1.0
+    eps1 = op.center_fields(v, np.zeros(6))[0]
+    touched = np.nonzero(np.abs(eps1[:, 0]) > 0)[0]
+    assert len(touched) == 8
+    got = {(int(e) // (ny * nx), int(e) // nx % ny, int(e) % nx): eps1[e, 0] for e in touched}
+    for (ez, ey, ex), val in got.items():
+        sx = 1 if ex == 3 - 1 else -1  # node is the +x corner of element ex = 2
+        assert val == pytest.approx(sx / (4 * h[0]))
+        assert ez in (0, 1) and ey in (1, 2)
+
+
 # ---- KATs of the reference CPU backend (tests/test_solver.py) -------------------
 def test_homogeneous_cell_kat():
     r = H.homogenize_direct(np.full((2, 3, 4), 17, dtype=np.int32), {17: dict(kind="isotropic", E=3500., nu=.35)},

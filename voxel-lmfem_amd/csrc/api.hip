@@ -36,9 +36,26 @@ __global__ void index_max_kernel(const Idx* index, int64_t n, unsigned int* out)
   atomicMax(out, m);
 }
 
+// Device buffer returned to the allocation cache on scope exit, after the
+// stream that used it has drained (temporaries of the non-solver entry points).
+struct DevBuf {
+  void* p = nullptr;
+  hipStream_t stream = nullptr;
+  explicit DevBuf(hipStream_t s) : stream(s) {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) {
+      (void)hipStreamSynchronize(stream);
+      dev_free(p);
+    }
+  }
+};
+
 int check_index(vxl_cell* c) {
-  unsigned int* d = nullptr;
-  VXL_HIP(hipMalloc(&d, sizeof(unsigned int)));
+  DevBuf buf(c->stream);
+  VXL_HIP(dev_malloc(&buf.p, sizeof(unsigned int)));
+  unsigned int* d = static_cast<unsigned int*>(buf.p);
   unsigned int h = 0;
   VXL_HIP(hipMemcpyAsync(d, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
   const int64_t n = c->grid.n;
@@ -60,7 +77,6 @@ int check_index(vxl_cell* c) {
   VXL_HIP(hipGetLastError());
   VXL_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   VXL_HIP(hipStreamSynchronize(c->stream));
-  VXL_HIP(hipFree(d));
   if (h >= static_cast<unsigned int>(c->rows)) return fail("material row index out of range");
   return VXL_OK;
 }
@@ -75,6 +91,33 @@ extern "C" {
 const char* vxl_last_error(void) { return g_error.c_str(); }
 
 int vxl_version(void) { return 100; }
+
+int vxl_cached_bytes(uint64_t* bytes) {
+  if (!bytes) return fail("vxl_cached_bytes: invalid arguments");
+  *bytes = cached_bytes();
+  return VXL_OK;
+}
+
+int vxl_release_cached_memory(void) {
+  release_cached();
+  return VXL_OK;
+}
+
+int vxl_label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,
+                   const int* lut_host, int index_kind, void* index_dev, int* present_host, int* undefined_host,
+                   int* samples_host, void* stream) {
+  g_error.clear();
+  if (nz < 2 || ny < 2 || nx < 2 || (label_bytes != 4 && label_bytes != 8) || !labels_dev || span < 1 ||
+      span > (1 << 24) || !lut_host || !index_dev || !present_host || !undefined_host || !samples_host ||
+      (index_kind != VXL_INDEX_U8 && index_kind != VXL_INDEX_U16 && index_kind != VXL_INDEX_I32))
+    return fail("vxl_label_rows: invalid arguments");
+  int max_row = -1;
+  for (int i = 0; i < span; ++i) max_row = std::max(max_row, lut_host[i]);
+  const int limit = index_kind == VXL_INDEX_U8 ? 255 : index_kind == VXL_INDEX_U16 ? 65535 : INT32_MAX;
+  if (max_row > limit) return fail("vxl_label_rows: row index does not fit the index kind");
+  return label_rows(nz, ny, nx, label_bytes, labels_dev, label_lo, span, lut_host, index_kind, index_dev,
+                    present_host, undefined_host, samples_host, static_cast<hipStream_t>(stream));
+}
 
 int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy, double hz,
                     int index_kind, const void* index_dev, int rows, const double* c21_host,
@@ -118,12 +161,12 @@ int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy
     }
   int rc = VXL_OK;
   auto cleanup = [&]() {
-    if (c->c64_owned) (void)hipFree(c->c64_owned);
-    if (c->c32_owned) (void)hipFree(c->c32_owned);
+    if (c->c64_owned) dev_free(c->c64_owned);
+    if (c->c32_owned) dev_free(c->c32_owned);
     delete c;
   };
-  if (hipMalloc(&c->c64_owned, t64.size() * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->c32_owned, t32.size() * sizeof(float)) != hipSuccess) {
+  if (dev_malloc(&c->c64_owned, t64.size() * sizeof(double)) != hipSuccess ||
+      dev_malloc(&c->c32_owned, t32.size() * sizeof(float)) != hipSuccess) {
     cleanup();
     g_error = "hipMalloc failed for the coefficient table";
     return VXL_ERR_HIP;
@@ -149,8 +192,9 @@ int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy
 
 int vxl_cell_destroy(vxl_cell* cell) {
   if (!cell) return VXL_OK;
-  if (cell->c64_owned) (void)hipFree(cell->c64_owned);
-  if (cell->c32_owned) (void)hipFree(cell->c32_owned);
+  (void)hipStreamSynchronize(cell->stream);
+  if (cell->c64_owned) dev_free(cell->c64_owned);
+  if (cell->c32_owned) dev_free(cell->c32_owned);
   delete cell;
   return VXL_OK;
 }
@@ -222,44 +266,52 @@ int vxl_tune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev,
 int vxl_rhs(vxl_cell* cell, int nload, const double* macro_host, double* f_dev) {
   g_error.clear();
   if (!cell || !macro_host || !f_dev || nload < 1) return fail("vxl_rhs: invalid arguments");
-  double* m = nullptr;
-  VXL_HIP(hipMalloc(&m, sizeof(double) * 6 * nload));
-  VXL_HIP(hipMemcpyAsync(m, macro_host, sizeof(double) * 6 * nload, hipMemcpyHostToDevice, cell->stream));
-  int rc = launch_rhs(cell, nload, m, f_dev, cell->stream);
+  DevBuf m(cell->stream);
+  VXL_HIP(dev_malloc(&m.p, sizeof(double) * 6 * nload));
+  VXL_HIP(hipMemcpyAsync(m.p, macro_host, sizeof(double) * 6 * nload, hipMemcpyHostToDevice, cell->stream));
+  VXL_TRY(launch_rhs(cell, nload, static_cast<double*>(m.p), f_dev, cell->stream));
   VXL_HIP(hipStreamSynchronize(cell->stream));
-  VXL_HIP(hipFree(m));
-  return rc;
+  return VXL_OK;
 }
 
 int vxl_avg_stress(vxl_cell* cell, int nload, const double* u_dev, const double* macro_host,
                    double* out_host) {
   g_error.clear();
   if (!cell || !macro_host || !u_dev || !out_host || nload < 1) return fail("vxl_avg_stress: invalid arguments");
-  double *m = nullptr, *partial = nullptr, *out = nullptr;
-  VXL_HIP(hipMalloc(&m, sizeof(double) * 6 * nload));
-  VXL_HIP(hipMalloc(&partial, sizeof(double) * size_t(stress_partial_size(cell, nload))));
-  VXL_HIP(hipMalloc(&out, sizeof(double) * 6 * nload));
-  VXL_HIP(hipMemcpyAsync(m, macro_host, sizeof(double) * 6 * nload, hipMemcpyHostToDevice, cell->stream));
-  int rc = launch_avg_stress(cell, nload, u_dev, m, partial, out, cell->stream);
-  if (rc == VXL_OK)
-    VXL_HIP(hipMemcpyAsync(out_host, out, sizeof(double) * 6 * nload, hipMemcpyDeviceToHost, cell->stream));
+  DevBuf m(cell->stream), partial(cell->stream), out(cell->stream);
+  VXL_HIP(dev_malloc(&m.p, sizeof(double) * 6 * nload));
+  VXL_HIP(dev_malloc(&partial.p, sizeof(double) * size_t(stress_partial_size(cell, nload))));
+  VXL_HIP(dev_malloc(&out.p, sizeof(double) * 6 * nload));
+  VXL_HIP(hipMemcpyAsync(m.p, macro_host, sizeof(double) * 6 * nload, hipMemcpyHostToDevice, cell->stream));
+  VXL_TRY(launch_avg_stress(cell, nload, u_dev, static_cast<double*>(m.p), static_cast<double*>(partial.p),
+                            static_cast<double*>(out.p), cell->stream));
+  VXL_HIP(hipMemcpyAsync(out_host, out.p, sizeof(double) * 6 * nload, hipMemcpyDeviceToHost, cell->stream));
   VXL_HIP(hipStreamSynchronize(cell->stream));
-  VXL_HIP(hipFree(m));
-  VXL_HIP(hipFree(partial));
-  VXL_HIP(hipFree(out));
-  return rc;
+  return VXL_OK;
+}
+
+int vxl_center_fields(vxl_cell* cell, int nload, const double* u_dev, const double* macro_host,
+                      double* strain_dev, double* stress_dev, double* energy_dev) {
+  g_error.clear();
+  if (!cell || !macro_host || !u_dev || nload < 1) return fail("vxl_center_fields: invalid arguments");
+  DevBuf m(cell->stream);
+  VXL_HIP(dev_malloc(&m.p, sizeof(double) * 6 * nload));
+  VXL_HIP(hipMemcpyAsync(m.p, macro_host, sizeof(double) * 6 * nload, hipMemcpyHostToDevice, cell->stream));
+  VXL_TRY(launch_center_fields(cell, nload, u_dev, static_cast<double*>(m.p), strain_dev, stress_dev, energy_dev,
+                               cell->stream));
+  VXL_HIP(hipStreamSynchronize(cell->stream));
+  return VXL_OK;
 }
 
 int vxl_project_zero_mean(vxl_cell* cell, int nrhs, double* u_dev) {
   g_error.clear();
   if (!cell || !u_dev || nrhs < 1) return fail("vxl_project_zero_mean: invalid arguments");
-  double* partial = nullptr;
+  DevBuf partial(cell->stream);
   const int64_t n = cell->grid.n;
-  VXL_HIP(hipMalloc(&partial, sizeof(double) * 3 * nrhs * size_t(vec_blocks(n) + 1)));
-  int rc = launch_project_zero_mean(u_dev, n, nrhs, partial, nullptr, cell->stream);
+  VXL_HIP(dev_malloc(&partial.p, sizeof(double) * 3 * nrhs * size_t(vec_blocks(n) + 1)));
+  VXL_TRY(launch_project_zero_mean(u_dev, n, nrhs, static_cast<double*>(partial.p), nullptr, cell->stream));
   VXL_HIP(hipStreamSynchronize(cell->stream));
-  VXL_HIP(hipFree(partial));
-  return rc;
+  return VXL_OK;
 }
 
 int vxl_precond_create(vxl_precond** out, vxl_cell* cell, int precision, double lam, double mu,

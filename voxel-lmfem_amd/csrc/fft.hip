@@ -751,7 +751,7 @@ int upload_tables(PrecondImpl* im, const double h[3]) {
       const double ang = 2.0 * M_PI * double(k) / double(n);
       w[k] = C{T(std::cos(ang)), T(-std::sin(ang))};
     }
-    VXL_HIP(hipMalloc(&im->tw[a], sizeof(C) * n));
+    VXL_HIP(dev_malloc(&im->tw[a], sizeof(C) * n));
     VXL_HIP(hipMemcpy(im->tw[a], w.data(), sizeof(C) * n, hipMemcpyHostToDevice));
     const int cnt = counts[a];
     std::vector<T> tab(4 * size_t(cnt));
@@ -765,7 +765,7 @@ int upload_tables(PrecondImpl* im, const double h[3]) {
       tab[2 * cnt + k] = T(std::sin(th) / (hh * m));
       tab[3 * cnt + k] = T(4.0 * sh * sh * sh * sh / (3.0 * hh * hh * m * m));
     }
-    VXL_HIP(hipMalloc(&im->tab[a], sizeof(T) * tab.size()));
+    VXL_HIP(dev_malloc(&im->tab[a], sizeof(T) * tab.size()));
     VXL_HIP(hipMemcpy(im->tab[a], tab.data(), sizeof(T) * tab.size(), hipMemcpyHostToDevice));
   }
   return VXL_OK;
@@ -843,7 +843,7 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   }
   im->spec_stride = int64_t(im->nz) * im->ny * im->nxh * 3;
   VXL_TRY(upload_tables<T>(im, cell->h));
-  VXL_HIP(hipMalloc(&im->spec, sizeof(C) * size_t(im->spec_stride) * pc->nrhs_max));
+  VXL_HIP(dev_malloc(&im->spec, sizeof(C) * size_t(im->spec_stride) * pc->nrhs_max));
   return VXL_OK;
 }
 
@@ -976,11 +976,12 @@ int precond_rz_blocks(const vxl_precond* pc) {
 void precond_release(vxl_precond* pc) {
   auto* im = static_cast<PrecondImpl*>(pc->impl);
   if (!im) return;
+  (void)hipStreamSynchronize(pc->cell->stream);
   for (int a = 0; a < 3; ++a) {
-    if (im->tw[a]) (void)hipFree(im->tw[a]);
-    if (im->tab[a]) (void)hipFree(im->tab[a]);
+    if (im->tw[a]) dev_free(im->tw[a]);
+    if (im->tab[a]) dev_free(im->tab[a]);
   }
-  if (im->spec) (void)hipFree(im->spec);
+  if (im->spec) dev_free(im->spec);
   delete im;
   pc->impl = nullptr;
 }

@@ -37,20 +37,21 @@ struct Inner {
 };
 
 int free_all(vxl_solver* s) {
+  if (s->cell) (void)hipStreamSynchronize(s->cell->stream);
   double* bufs[] = {s->f, s->x64, s->r64, s->q64, s->scal, s->partial, s->macro_dev,
                     s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz};
   for (double* b : bufs)
-    if (b) (void)hipFree(b);
+    if (b) dev_free(b);
   if (s->mode == VXL_SOLVE_MIXED) {
     void* inner[] = {s->x, s->r, s->p, s->q};
     for (void* b : inner)
-      if (b) (void)hipFree(b);
+      if (b) dev_free(b);
   } else if (s->p) {
-    (void)hipFree(s->p);
+    dev_free(s->p);
   }
-  if (s->active_dev) (void)hipFree(s->active_dev);
-  if (s->restart_dev) (void)hipFree(s->restart_dev);
-  if (s->stop_dev) (void)hipFree(s->stop_dev);
+  if (s->active_dev) dev_free(s->active_dev);
+  if (s->restart_dev) dev_free(s->restart_dev);
+  if (s->stop_dev) dev_free(s->stop_dev);
   if (s->pc_ready) precond_release(&s->pc);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
@@ -270,41 +271,41 @@ int solver_create(vxl_solver* s) {
   const int R = s->R;
   s->len = 3 * cell->grid.n;
   const size_t b64 = sizeof(double) * size_t(s->len) * R;
-  VXL_HIP(hipMalloc(&s->f, b64));
-  VXL_HIP(hipMalloc(&s->x64, b64));
-  VXL_HIP(hipMalloc(&s->r64, b64));
-  VXL_HIP(hipMalloc(&s->q64, b64));
+  VXL_HIP(dev_malloc(&s->f, b64));
+  VXL_HIP(dev_malloc(&s->x64, b64));
+  VXL_HIP(dev_malloc(&s->r64, b64));
+  VXL_HIP(dev_malloc(&s->q64, b64));
   if (s->mode == VXL_SOLVE_MIXED) {
     const size_t b32 = sizeof(float) * size_t(s->len) * R;
-    VXL_HIP(hipMalloc(&s->x, b32));
-    VXL_HIP(hipMalloc(&s->r, b32));
-    VXL_HIP(hipMalloc(&s->p, b32));
-    VXL_HIP(hipMalloc(&s->q, b32));
+    VXL_HIP(dev_malloc(&s->x, b32));
+    VXL_HIP(dev_malloc(&s->r, b32));
+    VXL_HIP(dev_malloc(&s->p, b32));
+    VXL_HIP(dev_malloc(&s->q, b32));
   } else {
     s->x = s->x64;
     s->r = s->r64;
     s->q = s->q64;
-    VXL_HIP(hipMalloc(&s->p, b64));
+    VXL_HIP(dev_malloc(&s->p, b64));
   }
   s->nblk = vec_blocks(s->len);
   const int nblk_rows = vec_blocks(cell->grid.n);
   const size_t np = size_t(std::max(s->nblk * R, 3 * R * (nblk_rows + 1)));
-  VXL_HIP(hipMalloc(&s->partial, sizeof(double) * np));
+  VXL_HIP(dev_malloc(&s->partial, sizeof(double) * np));
   {
     const int st_blocks = s->mode == VXL_SOLVE_MIXED ? apply_cg_blocks<float>(cell, R)
                                                      : apply_cg_blocks<double>(cell, R);
-    VXL_HIP(hipMalloc(&s->partial_st, sizeof(double) * size_t(st_blocks) * R));
+    VXL_HIP(dev_malloc(&s->partial_st, sizeof(double) * size_t(st_blocks) * R));
   }
-  VXL_HIP(hipMalloc(&s->scal, sizeof(double) * kNumSlots * R));
-  VXL_HIP(hipMalloc(&s->active_dev, sizeof(int) * R));
-  VXL_HIP(hipMalloc(&s->restart_dev, sizeof(int) * R));
-  VXL_HIP(hipMalloc(&s->macro_dev, sizeof(double) * 6 * R));
-  VXL_HIP(hipMalloc(&s->stress_partial, sizeof(double) * size_t(stress_partial_size(cell, R))));
-  VXL_HIP(hipMalloc(&s->ch_dev, sizeof(double) * 6 * R));
+  VXL_HIP(dev_malloc(&s->scal, sizeof(double) * kNumSlots * R));
+  VXL_HIP(dev_malloc(&s->active_dev, sizeof(int) * R));
+  VXL_HIP(dev_malloc(&s->restart_dev, sizeof(int) * R));
+  VXL_HIP(dev_malloc(&s->macro_dev, sizeof(double) * 6 * R));
+  VXL_HIP(dev_malloc(&s->stress_partial, sizeof(double) * size_t(stress_partial_size(cell, R))));
+  VXL_HIP(dev_malloc(&s->ch_dev, sizeof(double) * 6 * R));
   for (auto& e : s->ev) VXL_HIP(hipEventCreate(&e));
   VXL_HIP(hipEventCreate(&s->flag_ev));
   VXL_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_flags), sizeof(int) * 3 * R));
-  VXL_HIP(hipMalloc(&s->stop_dev, sizeof(int) * R));
+  VXL_HIP(dev_malloc(&s->stop_dev, sizeof(int) * R));
   VXL_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_scal), sizeof(double) * R));
   s->trace.assign(R, {});
   s->pc.cell = cell;
@@ -312,7 +313,7 @@ int solver_create(vxl_solver* s) {
   s->pc.nrhs_max = R;
   s->pc_ready = true;
   VXL_TRY(precond_create(&s->pc));
-  VXL_HIP(hipMalloc(&s->partial_rz, sizeof(double) * size_t(precond_rz_blocks(&s->pc)) * R));
+  VXL_HIP(dev_malloc(&s->partial_rz, sizeof(double) * size_t(precond_rz_blocks(&s->pc)) * R));
   return VXL_OK;
 }
 

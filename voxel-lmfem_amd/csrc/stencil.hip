@@ -692,11 +692,14 @@ rhs_kernel(const void* index, const double* table, const float* scale, const dou
 // ---- mean stress <C (E + B0 u_e)>: per-block partial sums + final reduction --
 constexpr int kStressBlock = 256;
 
-template <typename Idx, bool kScale, bool kEnergy>
+// kFields: also write the element-centre fields as planes [r][i][e] (strain,
+// stress) and [r][e] (energy = strain . stress / 2); null outputs are skipped.
+// partial == nullptr skips the block sums.
+template <typename Idx, bool kScale, bool kFields>
 __global__ void __launch_bounds__(kStressBlock)
 stress_kernel(const void* index, const double* table, const float* scale, const double* u,
-              const double* macro, int nload, double* partial, double* energy, int nx, int ny,
-              int nz, double gx, double gy, double gz) {
+              const double* macro, int nload, double* partial, double* strain, double* stress,
+              double* energy, int nx, int ny, int nz, double gx, double gy, double gz) {
   __shared__ double red[kStressBlock];
   const int64_t nxy = int64_t(nx) * ny, n = nxy * nz;
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -739,13 +742,20 @@ stress_kernel(const void* index, const double* table, const float* scale, const 
         for (int j = 0; j < 6; ++j) acc = fma(c[pidx(i, j)], eps[j], acc);
         s[i] = acc * sc;
       }
-      if constexpr (kEnergy) {
-        double w = 0;
+      if constexpr (kFields) {
+        if (strain)
+          for (int i = 0; i < 6; ++i) strain[(int64_t(r) * 6 + i) * n + e] = eps[i];
+        if (stress)
+          for (int i = 0; i < 6; ++i) stress[(int64_t(r) * 6 + i) * n + e] = s[i];
+        if (energy) {
+          double w = 0;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) w += s[i] * eps[i];
-        energy[int64_t(r) * n + e] = 0.5 * w;
+          for (int i = 0; i < 6; ++i) w += s[i] * eps[i];
+          energy[int64_t(r) * n + e] = 0.5 * w;
+        }
       }
     }
+    if (!partial) continue;
     for (int i = 0; i < 6; ++i) {
       red[threadIdx.x] = s[i];
       __syncthreads();
@@ -846,37 +856,36 @@ int64_t stress_partial_size(const vxl_cell* cell, int nload) {
 }
 
 static int stress_common(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
-                         double* partial, double* out_dev, double* energy, hipStream_t s) {
+                         double* partial, double* out_dev, double* strain, double* stress, double* energy,
+                         hipStream_t s) {
   const auto& g = cell->grid;
   const int64_t blocks = (g.n + kStressBlock - 1) / kStressBlock;
   const double gx = 0.25 / cell->h[0], gy = 0.25 / cell->h[1], gz = 0.25 / cell->h[2];
+  const bool fields = strain || stress || energy;
   int rc = by_index(cell, [&](auto tag) -> int {
     using Idx = decltype(tag);
     const bool sc = cell->mat.scale != nullptr;
-    if (energy) {
+#define VXL_STRESS(SC, F)                                                                                \
+  hipLaunchKernelGGL((stress_kernel<Idx, SC, F>), dim3(unsigned(blocks)), dim3(kStressBlock), 0, s,        \
+                     cell->mat.index, cell->mat.c64, cell->mat.scale, u, macro_dev, nload, partial, strain, \
+                     stress, energy, g.nx, g.ny, g.nz, gx, gy, gz)
+    if (fields) {
       if (sc)
-        hipLaunchKernelGGL((stress_kernel<Idx, true, true>), dim3(blocks), dim3(kStressBlock), 0, s,
-                           cell->mat.index, cell->mat.c64, cell->mat.scale, u, macro_dev, nload,
-                           partial, energy, g.nx, g.ny, g.nz, gx, gy, gz);
+        VXL_STRESS(true, true);
       else
-        hipLaunchKernelGGL((stress_kernel<Idx, false, true>), dim3(blocks), dim3(kStressBlock), 0, s,
-                           cell->mat.index, cell->mat.c64, cell->mat.scale, u, macro_dev, nload,
-                           partial, energy, g.nx, g.ny, g.nz, gx, gy, gz);
+        VXL_STRESS(false, true);
     } else {
       if (sc)
-        hipLaunchKernelGGL((stress_kernel<Idx, true, false>), dim3(blocks), dim3(kStressBlock), 0,
-                           s, cell->mat.index, cell->mat.c64, cell->mat.scale, u, macro_dev, nload,
-                           partial, energy, g.nx, g.ny, g.nz, gx, gy, gz);
+        VXL_STRESS(true, false);
       else
-        hipLaunchKernelGGL((stress_kernel<Idx, false, false>), dim3(blocks), dim3(kStressBlock), 0,
-                           s, cell->mat.index, cell->mat.c64, cell->mat.scale, u, macro_dev, nload,
-                           partial, energy, g.nx, g.ny, g.nz, gx, gy, gz);
+        VXL_STRESS(false, false);
     }
+#undef VXL_STRESS
     VXL_HIP(hipGetLastError());
     return VXL_OK;
   });
   if (rc != VXL_OK) return rc;
-  if (out_dev) {
+  if (out_dev && partial) {
     hipLaunchKernelGGL(rows_sum_kernel, dim3(6 * nload), dim3(256), 0, s, partial, blocks,
                        1.0 / double(g.n), out_dev);
     VXL_HIP(hipGetLastError());
@@ -886,16 +895,12 @@ static int stress_common(const vxl_cell* cell, int nload, const double* u, const
 
 int launch_avg_stress(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
                       double* partial, double* out_dev, hipStream_t s) {
-  return stress_common(cell, nload, u, macro_dev, partial, out_dev, nullptr, s);
+  return stress_common(cell, nload, u, macro_dev, partial, out_dev, nullptr, nullptr, nullptr, s);
 }
 
-int launch_element_energy(const vxl_cell* cell, int nload, const double* u,
-                          const double* macro_dev, double* energy, hipStream_t s) {
-  // Energy only needs the per-element values; partial sums go to a scratch
-  // area inside the caller's energy buffer tail (callers size it with
-  // stress_partial_size).
-  double* partial = energy + int64_t(nload) * cell->grid.n;
-  return stress_common(cell, nload, u, macro_dev, partial, nullptr, energy, s);
+int launch_center_fields(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
+                         double* strain, double* stress, double* energy, hipStream_t s) {
+  return stress_common(cell, nload, u, macro_dev, nullptr, nullptr, strain, stress, energy, s);
 }
 
 }  // namespace vxl

@@ -67,6 +67,18 @@ __device__ __forceinline__ double block_reduce_sum(double v, double* red) {
   return s;
 }
 
+// Cached device allocations (devmem.hip): freed blocks are kept per
+// (device, size) and reused; the owner synchronises the stream that used a
+// block before dev_free.
+hipError_t dev_malloc(void** p, size_t bytes);
+template <typename T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+  return dev_malloc(reinterpret_cast<void**>(p), bytes);
+}
+void dev_free(void* p);
+size_t cached_bytes();
+void release_cached();
+
 // Sum of v over a workgroup whose size is a multiple of 64 (at most 1024);
 // red holds blockDim.x / 64 doubles.  Result valid in thread 0.
 __device__ __forceinline__ double block_reduce_dyn(double v, double* red) {
@@ -130,7 +142,14 @@ int launch_avg_stress(const vxl_cell* cell, int nload, const double* u, const do
                       double* partial, double* out_dev, hipStream_t s);
 int64_t stress_partial_size(const vxl_cell* cell, int nload);
 int default_stencil_variant();
-int launch_element_energy(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
-                          double* energy, hipStream_t s);
+// Label -> row index on the device with label presence and the Zeman lattice
+// samples (material.hip).
+int label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,
+               const int* lut_host, int index_kind, void* index_dev, int* present_host, int* undefined_host,
+               int* samples_host, hipStream_t s);
+// Element-centre fields (any output may be null): strain = E + B0 u_e and
+// stress = C_e strain as planes [r][6][N], energy = strain . stress / 2 as [r][N].
+int launch_center_fields(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
+                         double* strain, double* stress, double* energy, hipStream_t s);
 
 }  // namespace vxl

@@ -30,6 +30,9 @@ _IP = ctypes.POINTER(ctypes.c_int)
 SIGNATURES = {
     "vxl_last_error": (ctypes.c_char_p, []),
     "vxl_version": (_I, []),
+    "vxl_cached_bytes": (_I, [ctypes.POINTER(ctypes.c_uint64)]),
+    "vxl_release_cached_memory": (_I, []),
+    "vxl_label_rows": (_I, [_I, _I, _I, _I, _P, ctypes.c_longlong, _I, _IP, _I, _P, _IP, _IP, _IP, _P]),
     "vxl_cell_create": (_I, [_PP, _I, _I, _I, _D, _D, _D, _I, _P, _I, _DP, _P, _P]),
     "vxl_cell_destroy": (_I, [_P]),
     "vxl_apply": (_I, [_P, _I, _I, _P, _P]),
@@ -38,6 +41,7 @@ SIGNATURES = {
     "vxl_tune_stencil": (_I, [_P, _I, _I, _P, _P, _I, _IP, _DP]),
     "vxl_rhs": (_I, [_P, _I, _DP, _P]),
     "vxl_avg_stress": (_I, [_P, _I, _P, _DP, _DP]),
+    "vxl_center_fields": (_I, [_P, _I, _P, _DP, _P, _P, _P]),
     "vxl_project_zero_mean": (_I, [_P, _I, _P]),
     "vxl_precond_create": (_I, [_PP, _P, _I, _D, _D, _I]),
     "vxl_precond_apply": (_I, [_P, _I, _P, _P]),

@@ -122,6 +122,23 @@ class Cell:
                                       out.ctypes.data_as(_native._DP)), "vxl_avg_stress")
         return out
 
+    def center_fields_planes(self, u, macro, *, strain=True, stress=True, energy=True):
+        """Element-centre strain/stress planes (R, 6, N) and energy (R, N), FP64."""
+        torch = self.torch
+        macro, p = darray(np.atleast_2d(macro))
+        u = u.contiguous()
+        if u.dtype != torch.float64 or u.shape != (len(macro), 3, self.n):
+            raise ValueError("expected FP64 (R, 3, N) displacements, one per macro strain")
+        R = len(macro)
+        eps = torch.empty(R, 6, self.n, dtype=torch.float64, device=self.device) if strain else None
+        sig = torch.empty(R, 6, self.n, dtype=torch.float64, device=self.device) if stress else None
+        w = torch.empty(R, self.n, dtype=torch.float64, device=self.device) if energy else None
+        null = ctypes.c_void_p()
+        check(self.lib.vxl_center_fields(self.handle, R, dptr(u), p, dptr(eps) if strain else null,
+                                         dptr(sig) if stress else null, dptr(w) if energy else null),
+              "vxl_center_fields")
+        return eps, sig, w
+
     def project_planes(self, u):
         check(self.lib.vxl_project_zero_mean(self.handle, u.shape[0], dptr(u)), "vxl_project_zero_mean")
         return u
@@ -149,8 +166,15 @@ class Cell:
         return self.avg_stress_planes(self._to_planes(u), np.asarray(macro, dtype=float)[None])
 
     def project_zero_mean(self, value):
-        u = value.view(1, self.n, 3)
-        return (u - u.mean(dim=1, keepdim=True)).reshape(1, -1)
+        return self._from_planes(self.project_planes(self._to_planes(value)))
+
+    def recover_center_fields(self, u, macro):
+        """Reference protocol: element-centre (strain, stress), each (1, N, 6) FP64."""
+        if np.shape(macro) != (6,):
+            raise ValueError("Six engineering macro strain components required")
+        eps, sig, _ = self.center_fields_planes(self._to_planes(u), np.asarray(macro, dtype=float)[None],
+                                                energy=False)
+        return eps.transpose(1, 2), sig.transpose(1, 2)
 
 
 class Preconditioner:
