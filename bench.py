@@ -4,14 +4,19 @@ Workload (the cell the metric is quoted on): 256^3 periodic Voronoi
 polycrystal, 200 grains, cubic C11/C12/C44 = 168.4/121.4/75.4 GPa, one
 uniform random rotation per grain (grain label + per-grain rotated tensor),
 six unit engineering-strain loads solved together.  A step is one full C_eff
-solve on resident inputs: RHS, block PCG (FP32 operator / FFT preconditioner /
-CG with FP64 reliable updates), FP64 true-residual certificate <= 1e-8 and
-stress recovery.  The plain FP64 solve of the same cell is timed beside it and
-the C_eff difference between the two is reported.
+solve with the int32 labels already resident in HBM, timed like the
+reference's workload (set-up included, geometry generation excluded): device
+label -> row front end, reference-medium selection, operator and FFT
+preconditioner set-up, RHS, block PCG (FP32 operator / preconditioner / CG
+with FP64 reliable updates), FP64 true-residual certificate <= 1e-8, stress
+recovery and teardown.  The same solve on a resident operator is reported
+beside it, and the plain FP64 solve of the same cell (same step definition)
+is timed with the C_eff difference between the two.
 
 roofline: the FP32 six-RHS stencil apply (the dominant kernel), timed with HIP
 events on the stream it runs on; algorithmic bytes per launch =
-N (12 R read + 12 R write + 1 label byte), R = 6.
+N (12 R read + 12 R write + 1 label byte), R = 6.  `traffic` is read from the
+committed FETCH_SIZE/WRITE_SIZE summary of the same launch (profiles/).
 cpu_baseline: the FP64 NumPy oracle (operator apply + Green preconditioner)
 timed at 64^3 on one host thread (child process, BLAS/OpenMP threads = 1,
 started before the GPU is initialised) and extrapolated to a 256^3 six-load
@@ -88,6 +93,19 @@ def cpu_baseline(sample, iterations_total, n_target):
                         f"{n_target}^3, x{iterations_total} RHS-iterations of the GPU run; extrapolated"))
 
 
+TRAFFIC_SUMMARY = ROOT / "profiles" / "apply_kernel_v2_fp32_r6_polycrystal256_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP32 six-RHS stencil from the committed
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or (None, None)."""
+    import re
+    if not TRAFFIC_SUMMARY.is_file():
+        return None, None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_SUMMARY.read_text(), flags=re.M)
+    return (float(m.group(1)), str(TRAFFIC_SUMMARY.relative_to(ROOT))) if m else (None, None)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,19 +140,31 @@ def main():
     n = args.n
     t0 = time.perf_counter()
     (phase, mats, dom, _), meta = voronoi_polycrystal(n, grains=200, seed=3)
-    field = pm.build_field(phase, mats)
-    lam, mu, rmeta = select(field, phase, mats)
     h = dom / np.array(phase.shape[::-1], dtype=float)
-    print(f"# generated {n}^3 polycrystal in {time.perf_counter() - t0:.1f} s; reference medium "
-          f"{rmeta['status']} lam={lam:.3f} mu={mu:.3f}", file=sys.stderr, flush=True)
+    print(f"# generated {n}^3 polycrystal in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.set_device(0)
-    t0 = time.perf_counter()
-    cell = Cell(field, phase.shape, h)
-    solver = Solver(cell, lam, mu, mode="mixed", nrhs=6)
+    # Inputs resident in HBM before any timing: the int32 grain labels (the
+    # per-grain tensors are a 200-entry host table).
+    labels = torch.from_numpy(phase.astype(np.int32)).to("cuda")
     torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t0
+
+    def setup(mode):
+        field = pm.build_field_device(labels, mats)
+        lam, mu, rmeta = select(field, phase, mats)
+        cell = Cell(field, phase.shape, h)
+        return cell, Solver(cell, lam, mu, mode=mode, nrhs=6), rmeta
+
+    def step(mode):
+        """One full C_eff: device label front end, reference medium, operator and
+        preconditioner set-up, RHS, block PCG, FP64 certificate, stress, teardown."""
+        cell, solver, rmeta = setup(mode)
+        out = solver.run()
+        solver.close()
+        cell.close()
+        return out, rmeta
 
     if args.stencil_only:
+        cell, solver, _ = setup("mixed")
         u = torch.randn(6, 3, cell.n, dtype=torch.float32, device="cuda")
         o = torch.empty_like(u)
         for _ in range(10):
@@ -147,28 +177,39 @@ def main():
         return
 
     for _ in range(args.warmup):
-        solver.run()
+        step("mixed")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    out = None
+    out = rmeta = None
     for _ in range(args.steps):
-        out = solver.run()
+        out, rmeta = step("mixed")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     its = out["iterations"]
-    print(f"# mixed: {ms_per_step:.1f} ms/solve, iterations {its}, restarts {out['restarts']}, "
-          f"residuals {max(out['true_residuals']):.2e}, device ms {out['device_ms']}", file=sys.stderr, flush=True)
+    print(f"# mixed: {ms_per_step:.1f} ms/solve (set-up included), iterations {its}, restarts {out['restarts']}, "
+          f"residuals {max(out['true_residuals']):.2e}, device ms {out['device_ms']}, reference medium "
+          f"{rmeta['status']} lam={rmeta['lam']:.3f} mu={rmeta['mu']:.3f}", file=sys.stderr, flush=True)
     CH_mixed = out["CH"]
 
-    # FP64 solve of the same cell.
+    # The same solve on a resident operator (set-up excluded), for the breakdown.
+    cell, solver, _ = setup("mixed")
+    solver.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.run()
+    torch.cuda.synchronize()
+    resident_ms = 1e3 * (time.perf_counter() - t0)
+    solver.close()
+    print(f"# mixed, resident operator: {resident_ms:.1f} ms/solve", file=sys.stderr, flush=True)
+
+    # Plain FP64 GPU solve of the same cell, same step definition.
     fp64 = None
     if not args.skip_fp64:
-        s64 = Solver(cell, lam, mu, mode="fp64", nrhs=6)
-        s64.run()
+        step("fp64")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        o64 = s64.run()
+        o64, _ = step("fp64")
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t0
         err = float(np.linalg.norm(CH_mixed - o64["CH"]) / np.linalg.norm(o64["CH"]))
@@ -176,61 +217,64 @@ def main():
                     ceff_rel_err_mixed_vs_fp64=err)
         print(f"# fp64: {1e3 * t64:.1f} ms/solve, iterations {o64['iterations']}, mixed-vs-fp64 rel err {err:.2e}",
               file=sys.stderr, flush=True)
-        s64.close()
 
-    # Stencil roofline: FP32 six-RHS apply, HIP events on the current stream.
+    # Stencil roofline: FP32 six-RHS apply timed with HIP events recorded on
+    # the stream the kernel is launched on (the cell's stream).
     N = cell.n
     R = 6
+    stream = torch.cuda.current_stream()
     u = torch.randn(R, 3, N, dtype=torch.float32, device="cuda")
     o = torch.empty_like(u)
     for _ in range(3):
         cell.apply_planes(u, o)
     reps = 20
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
+    e0.record(stream)
     for _ in range(reps):
         cell.apply_planes(u, o)
-    e1.record()
+    e1.record(stream)
     torch.cuda.synchronize()
     t_apply = e0.elapsed_time(e1) / reps * 1e-3
     alg_bytes = N * (12 * R + 12 * R + 1)
     achieved = alg_bytes / t_apply / 1e9
-    # FP64 single-RHS apply for reference
     u64 = torch.randn(1, 3, N, dtype=torch.float64, device="cuda")
     o64b = torch.empty_like(u64)
     cell.apply_planes(u64, o64b)
-    e0.record()
+    e0.record(stream)
     for _ in range(reps):
         cell.apply_planes(u64, o64b)
-    e1.record()
+    e1.record(stream)
     torch.cuda.synchronize()
     t_apply64 = e0.elapsed_time(e1) / reps * 1e-3
     print(f"# stencil fp32 R=6: {t_apply * 1e3:.3f} ms/launch, {achieved:.0f} GB/s algorithmic; "
           f"fp64 R=1: {t_apply64 * 1e3:.3f} ms/launch, {N * 49 / t_apply64 / 1e9:.0f} GB/s",
           file=sys.stderr, flush=True)
+    cell.close()
 
     cpu = None
     if sample is not None:
         cpu = cpu_baseline(sample, int(sum(its)), n)
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
+    traffic, traffic_src = measured_traffic()
     line = dict(
         metric="256^3 polycrystal C_eff solves/s (mixed FP32/FP64, 6 loads, FP64 residual <= 1e-8)",
         value=args.steps / elapsed, unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f32 inner + f64 refinement", data="synthetic",
         config=dict(workload=f"voronoi_polycrystal_{n}^3_200grains_cubic_6loads", grid=[n, n, n], grains=200,
-                    loads=6, precision="mixed", tol=1e-8, iterations=its, setup_s=setup_s,
-                    reference_medium=rmeta),
+                    loads=6, precision="mixed", tol=1e-8, iterations=its,
+                    step=("device label front end + reference medium + operator/preconditioner set-up + "
+                          "RHS + block PCG + FP64 certificate + stress + teardown; labels resident in HBM"),
+                    resident_operator_ms_per_solve=resident_ms, reference_medium=rmeta),
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=achieved / HBM_PEAK_GBS, traffic=None, kernel="apply_kernel<float,uint8> R=6",
+                      frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_src,
+                      kernel="apply_kernel_v2<float,uint8> R=6 (schedule 2)",
                       ms_per_launch=t_apply * 1e3, algorithmic_bytes=alg_bytes),
         fp64=fp64,
         cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
-    solver.close()
-    cell.close()
 
 
 if __name__ == "__main__":

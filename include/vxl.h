@@ -98,8 +98,10 @@ int vxl_cell_destroy(vxl_cell* cell);
 int vxl_apply(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev);
 
 /* Stencil schedule of the cell per precision: 1 = per-layer 8-node modal
- * transform, 2 = persistent face columns (default), 3 = 2 with the plane
- * after next prefetched into registers.  All three compute the same sums.
+ * transform, 2 = persistent face columns, 3 = 2 with the plane after next
+ * prefetched into registers (defaults: 2 for VXL_F32, 3 for VXL_F64; the
+ * CG form of the stencil used inside vxl_solver_run has schedules 2 and 3
+ * only and runs 2 when 1 is set).  All schedules compute the same sums.
  * No reference counterpart: the reference's variant choice is recorded in
  * _core/hybrid_operator/recommended.json:1-6 by hybrid_operator/study.py:64-82;
  * vxl_tune_stencil is this build's measured equivalent: it times every

@@ -1,0 +1,112 @@
+"""Device material front end (labels -> rows on the GPU), element-centre
+fields in the solve result, and the device allocation cache."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from voxel_lmfem import materials as pm
+
+pytestmark = pytest.mark.gpu
+
+
+def labelled_cell(shape, seed, labels):
+    rng = np.random.default_rng(seed)
+    phase = rng.choice(labels, size=shape).astype(np.int64)
+    table = {}
+    for k in labels:
+        X = rng.normal(size=(6, 6))
+        table[int(k)] = dict(kind="anisotropic", C=(X @ X.T + 6 * np.eye(6)).tolist())
+    return phase, table
+
+
+@pytest.mark.parametrize("shape,labels,dtype", [
+    ((4, 5, 6), [3, 7, 11], "int32"),
+    ((17, 9, 33), [-5, 0, 2, 900], "int64"),
+    ((8, 8, 8), list(range(300)), "int32"),      # uint16 index
+])
+def test_device_label_rows_equal_host_bookkeeping(torch_gpu, shape, labels, dtype):
+    torch = torch_gpu
+    print(f"case label rows shape={shape} labels={len(labels)} {dtype}", flush=True)
+    phase, table = labelled_cell(shape, seed=len(labels), labels=labels)
+    phase[0, 0, :] = labels[0]  # a long x-run of one label
+    host = pm.build_field(phase, table)
+    dev = pm.build_field_device(torch.tensor(phase.astype(dtype), device="cuda"), table)
+    assert dev.index_kind == host.index_kind
+    idx = dev.index_dev.cpu().numpy()
+    if host.index.dtype == np.uint16:
+        idx = idx.view(np.uint16)
+    np.testing.assert_array_equal(idx, host.index)                     # bit-exact bookkeeping
+    np.testing.assert_array_equal(dev.rows, host.rows)
+    np.testing.assert_array_equal(dev.used_labels(), np.unique(phase))
+    np.testing.assert_array_equal(dev.lattice_rows(shape), host.lattice_rows(shape))
+
+
+def test_device_label_rows_reports_undefined_labels(torch_gpu):
+    torch = torch_gpu
+    phase, table = labelled_cell((4, 4, 4), seed=1, labels=[1, 2])
+    phase[2, 3, 1] = 9
+    with pytest.raises(ValueError, match=r"Undefined material labels: \[9\]"):
+        pm.build_field_device(torch.tensor(phase, device="cuda"), table)
+
+
+def test_solve_from_device_field_matches_host_field(torch_gpu):
+    torch = torch_gpu
+    from voxel_lmfem.cells import periodic_spheres
+    from voxel_lmfem.gpu import Cell, Solver
+    from voxel_lmfem.reference_medium import select
+    (phase, mats, dom, _), _ = periodic_spheres(24, seed=2)
+    out = []
+    for field in (pm.build_field(phase, mats),
+                  pm.build_field_device(torch.tensor(phase.astype(np.int32), device="cuda"), mats)):
+        lam, mu, _ = select(field, phase, mats)
+        h = dom / np.array(phase.shape[::-1], dtype=float)
+        cell = Cell(field, phase.shape, h)
+        solver = Solver(cell, lam, mu, mode="mixed", nrhs=6)
+        out.append((lam, mu, solver.run()["CH"]))
+        solver.close()
+        cell.close()
+    assert out[0][:2] == out[1][:2]
+    np.testing.assert_array_equal(out[0][2], out[1][2])   # same device work, same bits
+
+
+def test_allocation_cache_reuses_and_releases(torch_gpu):
+    torch = torch_gpu
+    from voxel_lmfem import _native
+    from voxel_lmfem.cells import periodic_spheres
+    from voxel_lmfem.gpu import Cell, Solver
+    lib = _native.load()
+    (phase, mats, dom, _), _ = periodic_spheres(16, seed=3)
+    field = pm.build_field(phase, mats)
+    h = dom / np.array(phase.shape[::-1], dtype=float)
+    cached = ctypes.c_uint64()
+    results = []
+    for _ in range(2):
+        cell = Cell(field, phase.shape, h)
+        solver = Solver(cell, 1.0, 1.0, mode="mixed", nrhs=6)
+        results.append(solver.run()["CH"])
+        solver.close()
+        cell.close()
+        assert lib.vxl_cached_bytes(ctypes.byref(cached)) == 0
+        assert cached.value > 0
+    np.testing.assert_array_equal(results[0], results[1])
+    assert lib.vxl_release_cached_memory() == 0
+    lib.vxl_cached_bytes(ctypes.byref(cached))
+    assert cached.value == 0
+
+
+def test_homogenize_field_statistics(torch_gpu):
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import periodic_spheres
+    (phase, mats, dom, _), _ = periodic_spheres(16, seed=4)
+    out = homogenize(phase, mats, dom, fields=True)
+    assert out["storage"]["front_end"] == "device"
+    stats = out["field_statistics"]
+    assert set(stats) == {"0", "5"}
+    frac = {str(k): float(np.mean(phase == k)) for k in np.unique(phase)}
+    for j in ("0", "5"):
+        s = stats[j]
+        assert s["stress_l2"] > 0 and s["mean_energy"] > 0
+        mix = sum(frac[k] * v for k, v in s["phase_mean_energy"].items())
+        assert mix == pytest.approx(s["mean_energy"], rel=1e-12)
+    assert out["slices"]["energy_load0"].shape == phase.shape[1:]

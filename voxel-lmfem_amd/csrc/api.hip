@@ -150,7 +150,8 @@ int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy
   c->mat.scale = scale_dev;
   c->c64_owned = nullptr;
   c->c32_owned = nullptr;
-  c->variant[0] = c->variant[1] = default_stencil_variant();
+  c->variant[0] = default_stencil_variant(VXL_F32);
+  c->variant[1] = default_stencil_variant(VXL_F64);
   (void)hipGetDevice(&c->device);
   std::vector<double> t64(size_t(rows) * kRowWidth, 0.0);
   std::vector<float> t32(size_t(rows) * kRowWidth, 0.0f);
@@ -269,7 +270,10 @@ int vxl_rhs(vxl_cell* cell, int nload, const double* macro_host, double* f_dev) 
   DevBuf m(cell->stream);
   VXL_HIP(dev_malloc(&m.p, sizeof(double) * 6 * nload));
   VXL_HIP(hipMemcpyAsync(m.p, macro_host, sizeof(double) * 6 * nload, hipMemcpyHostToDevice, cell->stream));
-  VXL_TRY(launch_rhs(cell, nload, static_cast<double*>(m.p), f_dev, cell->stream));
+  DevBuf sig(cell->stream);
+  const int64_t ns = rhs_scratch_size(cell, nload);
+  if (ns > 0) VXL_HIP(dev_malloc(&sig.p, sizeof(double) * size_t(ns)));
+  VXL_TRY(launch_rhs(cell, nload, static_cast<double*>(m.p), f_dev, static_cast<double*>(sig.p), cell->stream));
   VXL_HIP(hipStreamSynchronize(cell->stream));
   return VXL_OK;
 }

@@ -79,7 +79,7 @@ residual_kernel(double* r, const double* f, const double* q, float* r32, int64_t
   double acc = 0;
   for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock) {
     const double v = f[off + i] - q[off + i];
-    r[off + i] = v;
+    if (r) r[off + i] = v;
     if (r32) r32[off + i] = float(v);
     acc += v * v;
   }

@@ -39,7 +39,7 @@ struct Inner {
 int free_all(vxl_solver* s) {
   if (s->cell) (void)hipStreamSynchronize(s->cell->stream);
   double* bufs[] = {s->f, s->x64, s->r64, s->q64, s->scal, s->partial, s->macro_dev,
-                    s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz};
+                    s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz, s->rhs_sig};
   for (double* b : bufs)
     if (b) dev_free(b);
   if (s->mode == VXL_SOLVE_MIXED) {
@@ -120,7 +120,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   VXL_HIP(hipMemsetAsync(s->stop_dev, 0, sizeof(int) * R, st));
   VXL_TRY(upload_flags());
   // f = P0 rhs(E); ||f||^2
-  VXL_TRY(launch_rhs(cell, R, s->macro_dev, s->f, st));
+  VXL_TRY(launch_rhs(cell, R, s->macro_dev, s->f, s->rhs_sig, st));
   VXL_TRY(launch_project_zero_mean(s->f, n, R, partial, nullptr, st));
   VXL_TRY(launch_dot<double, double>(s->f, s->f, len, R, partial, nullptr, st));
   VXL_TRY(launch_finalize(partial, nblk, kOpFF, scal, R, nullptr, nullptr, st));
@@ -157,9 +157,10 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     VXL_TRY(upload_mask(mask));
     if (mixed)
       VXL_TRY(launch_accumulate(s->x64, reinterpret_cast<float*>(x), len, R, act, st));
-    VXL_TRY(launch_project_zero_mean(s->x64, n, R, partial, act, st));
+    // K annihilates the mean, so the zero-mean projection of x64 is left to
+    // the end of the run; mixed mode needs the residual only in FP32.
     VXL_TRY(launch_apply<double>(cell, R, s->x64, s->q64, st));
-    VXL_TRY(launch_residual(s->r64, s->f, s->q64, mixed ? reinterpret_cast<float*>(r) : nullptr, len,
+    VXL_TRY(launch_residual(mixed ? nullptr : s->r64, s->f, s->q64, mixed ? reinterpret_cast<float*>(r) : nullptr, len,
                             R, partial, act, st));
     VXL_TRY(launch_finalize(partial, nblk, kOpStore, scal, R, act, nullptr, st));
     std::vector<double> t(R);
@@ -273,7 +274,7 @@ int solver_create(vxl_solver* s) {
   const size_t b64 = sizeof(double) * size_t(s->len) * R;
   VXL_HIP(dev_malloc(&s->f, b64));
   VXL_HIP(dev_malloc(&s->x64, b64));
-  VXL_HIP(dev_malloc(&s->r64, b64));
+  if (s->mode != VXL_SOLVE_MIXED) VXL_HIP(dev_malloc(&s->r64, b64));
   VXL_HIP(dev_malloc(&s->q64, b64));
   if (s->mode == VXL_SOLVE_MIXED) {
     const size_t b32 = sizeof(float) * size_t(s->len) * R;
@@ -300,6 +301,7 @@ int solver_create(vxl_solver* s) {
   VXL_HIP(dev_malloc(&s->active_dev, sizeof(int) * R));
   VXL_HIP(dev_malloc(&s->restart_dev, sizeof(int) * R));
   VXL_HIP(dev_malloc(&s->macro_dev, sizeof(double) * 6 * R));
+  if (const int64_t ns = rhs_scratch_size(cell, R)) VXL_HIP(dev_malloc(&s->rhs_sig, sizeof(double) * size_t(ns)));
   VXL_HIP(dev_malloc(&s->stress_partial, sizeof(double) * size_t(stress_partial_size(cell, R))));
   VXL_HIP(dev_malloc(&s->ch_dev, sizeof(double) * 6 * R));
   for (auto& e : s->ev) VXL_HIP(hipEventCreate(&e));

@@ -222,6 +222,26 @@ __device__ __forceinline__ int wrapmod(int v, int n) {
   return v < 0 ? v + n : v;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by linear id, and each XCD
+// has its own L2.  Remap ids so every XCD owns a contiguous run of (tile,
+// chunk, rhs) in tile-fastest order: x-neighbouring tiles, whose halo rows
+// share cache lines, then run on the same XCD at about the same time.
+struct BlockId {
+  int tile, chunk, rhs;
+};
+__device__ __forceinline__ BlockId xcd_block() {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned total = gx * gy * gridDim.z;
+  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned full = total & ~7u;
+  if (lin < full) lin = (lin & 7u) * (full >> 3) + (lin >> 3);
+  BlockId b;
+  b.tile = int(lin % gx);
+  b.chunk = int((lin / gx) % gy);
+  b.rhs = int(lin / (gx * gy));
+  return b;
+}
+
 template <int TX, int TY>
 struct Tile {
   static constexpr int PX = TX + 2, PY = TY + 2, PLANE = PX * PY;
@@ -240,17 +260,18 @@ apply_kernel(ApplyArgs<T> a) {
   __shared__ T sf[24][TL::EL];
 
   const int tid = threadIdx.x;
-  const int tile = blockIdx.x;
+  const BlockId bid = xcd_block();
+  const int tile = bid.tile;
   const int x0 = (tile % a.tiles_x) * TX;
   const int y0 = (tile / a.tiles_x) * TY;
-  const int z0 = blockIdx.y * a.zchunk;
+  const int z0 = bid.chunk * a.zchunk;
   const int z1 = min(z0 + a.zchunk, a.nz);
   if (z0 >= a.nz) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t nxy = int64_t(nx) * ny;
   const int64_t n = nxy * nz;
-  const T* u = a.u + int64_t(blockIdx.z) * a.rhs_stride;
-  T* out = a.out + int64_t(blockIdx.z) * a.rhs_stride;
+  const T* u = a.u + int64_t(bid.rhs) * a.rhs_stride;
+  T* out = a.out + int64_t(bid.rhs) * a.rhs_stride;
 
   // Plane staging: each thread copies up to ITEMS halo-tile nodes.
   int ld_lds[TL::ITEMS];
@@ -365,20 +386,21 @@ apply_kernel_v2(ApplyArgs<T> a) {
   __shared__ double red[TL::NT / 64];
 
   const int tid = threadIdx.x;
-  const int tile = blockIdx.x;
+  const BlockId bid = xcd_block();
+  const int tile = bid.tile;
   const int x0 = (tile % a.tiles_x) * TX;
   const int y0 = (tile / a.tiles_x) * TY;
-  const int z0 = blockIdx.y * a.zchunk;
+  const int z0 = bid.chunk * a.zchunk;
   const int z1 = min(z0 + a.zchunk, a.nz);
   if (z0 >= a.nz) return;
   if constexpr (kFuse) {
-    if (a.active && !a.active[blockIdx.z]) return;
+    if (a.active && !a.active[bid.rhs]) return;
   }
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int64_t nxy = int64_t(nx) * ny;
   const int64_t n = nxy * nz;
-  const T* u = a.u + int64_t(blockIdx.z) * a.rhs_stride;
-  T* out = a.out + int64_t(blockIdx.z) * a.rhs_stride;
+  const T* u = a.u + int64_t(bid.rhs) * a.rhs_stride;
+  T* out = a.out + int64_t(bid.rhs) * a.rhs_stride;
 
   int ld_lds[TL::ITEMS];
   int64_t ld_off[TL::ITEMS];
@@ -552,18 +574,18 @@ apply_kernel_v2(ApplyArgs<T> a) {
   if constexpr (kFuse) {
     const double total = block_reduce_sum<TL::NT>(double(pq), red);
     if (tid == 0)
-      a.partial[int64_t(blockIdx.z) * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = total;
+      a.partial[(int64_t(bid.rhs) * gridDim.y + bid.chunk) * gridDim.x + bid.tile] = total;
   }
 }
 
-// Stencil schedule: 2 (persistent face columns, default), 3 (the same with the
-// plane after next prefetched into registers) or 1 (per-layer full 8-node
-// transform); VXL_STENCIL_VARIANT selects one for comparisons.
+// Stencil schedule: 2 (persistent face columns), 3 (the same with the plane
+// after next prefetched into registers) or 1 (per-layer full 8-node
+// transform); VXL_STENCIL_VARIANT overrides the per-precision default.
 int env_stencil_variant() {
   static const int v = [] {
     const char* e = std::getenv("VXL_STENCIL_VARIANT");
-    if (e && (e[0] == '1' || e[0] == '3')) return e[0] - '0';
-    return 2;
+    if (e && e[0] >= '1' && e[0] <= '3') return e[0] - '0';
+    return 0;
   }();
   return v;
 }
@@ -646,11 +668,29 @@ int dispatch_apply(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream
 }
 
 // ---- RHS: f = -V sum_e B0^T C_e E, node-owned gather (no atomics) ----------
+// sig[(row * nload + r) * 6 + i] = (C_row E_r)_i, once per coefficient row.
+__global__ void __launch_bounds__(256)
+row_stress_kernel(const double* table, int rows, const double* macro, int nload, double* sig) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(rows) * nload) return;
+  const int64_t row = t / nload;
+  const int r = int(t % nload);
+  const double* c = table + row * kRowWidth;
+  const double* E = macro + 6 * r;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double acc = 0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc = fma(c[pidx(i, j)], E[j], acc);
+    sig[t * 6 + i] = acc;
+  }
+}
+
+// Direct form for large row tables (one row per voxel): C_e E per element.
 template <typename Idx, bool kScale>
 __global__ void __launch_bounds__(256)
-rhs_kernel(const void* index, const double* table, const float* scale, const double* macro,
-           int nload, double* f, int nx, int ny, int nz, double gx, double gy, double gz,
-           double volume) {
+rhs_direct_kernel(const void* index, const double* table, const float* scale, const double* macro, int nload,
+                  double* f, int nx, int ny, int nz, double gx, double gy, double gz, double volume) {
   const int64_t nxy = int64_t(nx) * ny, n = nxy * nz;
   const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (node >= n) return;
@@ -664,19 +704,54 @@ rhs_kernel(const void* index, const double* table, const float* scale, const dou
       const int64_t e = int64_t(wrapmod(z - az, nz)) * nxy + int64_t(wrapmod(y - ay, ny)) * nx +
                         wrapmod(x - ax, nx);
       const double* c = table + int64_t(load_row<Idx>(index, e)) * kRowWidth;
+      const double sc = kScale ? double(scale[e]) : 1.0;
       double s[6];
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
         double acc = 0;
 #pragma unroll
         for (int j = 0; j < 6; ++j) acc = fma(c[pidx(i, j)], E[j], acc);
-        s[i] = acc;
+        s[i] = kScale ? acc * sc : acc;
       }
-      if constexpr (kScale) {
-        const double sc = double(scale[e]);
+      const double dx = ax ? gx : -gx, dy = ay ? gy : -gy, dz = az ? gz : -gz;
+      fx -= volume * (dx * s[0] + dy * s[5] + dz * s[4]);
+      fy -= volume * (dx * s[5] + dy * s[1] + dz * s[3]);
+      fz -= volume * (dx * s[4] + dy * s[3] + dz * s[2]);
+    }
+    double* fr = f + int64_t(r) * 3 * n;
+    fr[node] = fx;
+    fr[n + node] = fy;
+    fr[2 * n + node] = fz;
+  }
+}
+
+// Node-owned gather of the 8 precomputed element stresses around each node.
+template <typename Idx, bool kScale>
+__global__ void __launch_bounds__(256)
+rhs_kernel(const void* index, const double* sig, const float* scale, int nload, double* f, int nx, int ny,
+           int nz, double gx, double gy, double gz, double volume) {
+  const int64_t nxy = int64_t(nx) * ny, n = nxy * nz;
+  const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= n) return;
+  const int x = int(node % nx), y = int((node / nx) % ny), z = int(node / nxy);
+  int64_t base[8];
+  double sc[8];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) s[i] *= sc;
-      }
+  for (int q = 0; q < 8; ++q) {
+    const int ax = q & 1, ay = (q >> 1) & 1, az = q >> 2;
+    const int64_t e = int64_t(wrapmod(z - az, nz)) * nxy + int64_t(wrapmod(y - ay, ny)) * nx + wrapmod(x - ax, nx);
+    base[q] = int64_t(load_row<Idx>(index, e)) * nload * 6;
+    sc[q] = kScale ? double(scale[e]) : 1.0;
+  }
+  for (int r = 0; r < nload; ++r) {
+    double fx = 0, fy = 0, fz = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int ax = q & 1, ay = (q >> 1) & 1, az = q >> 2;
+      const double* sp = sig + base[q] + 6 * r;
+      double s[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) s[i] = kScale ? sp[i] * sc[q] : sp[i];
       const double dx = ax ? gx : -gx, dy = ay ? gy : -gy, dz = az ? gz : -gz;
       fx -= volume * (dx * s[0] + dy * s[5] + dz * s[4]);
       fy -= volume * (dx * s[5] + dy * s[1] + dz * s[3]);
@@ -797,7 +872,12 @@ int by_index(const vxl_cell* cell, F&& f) {
 
 }  // namespace
 
-int default_stencil_variant() { return env_stencil_variant(); }
+// Measured at 256^3 (six RHS): FP32 is fastest without the prefetch (2),
+// FP64 with it (3).
+int default_stencil_variant(int precision) {
+  if (const int v = env_stencil_variant()) return v;
+  return precision == VXL_F64 ? 3 : 2;
+}
 
 template <>
 int launch_apply<float>(const vxl_cell* cell, int nrhs, const float* u, float* out, hipStream_t s) {
@@ -831,23 +911,43 @@ int apply_cg_blocks<double>(const vxl_cell* cell, int nrhs) {
   return tiles * chunks;
 }
 
-int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, hipStream_t s) {
+int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, double* sig, hipStream_t s) {
   const auto& g = cell->grid;
   const int blocks = int((g.n + 255) / 256);
   const double gx = 0.25 / cell->h[0], gy = 0.25 / cell->h[1], gz = 0.25 / cell->h[2];
+  if (sig) {
+    const int64_t work = int64_t(cell->rows) * nload;
+    hipLaunchKernelGGL(row_stress_kernel, dim3(unsigned((work + 255) / 256)), dim3(256), 0, s, cell->mat.c64,
+                       cell->rows, macro_dev, nload, sig);
+    VXL_HIP(hipGetLastError());
+  }
   return by_index(cell, [&](auto tag) -> int {
     using Idx = decltype(tag);
-    if (cell->mat.scale)
-      hipLaunchKernelGGL((rhs_kernel<Idx, true>), dim3(blocks), dim3(256), 0, s, cell->mat.index,
-                         cell->mat.c64, cell->mat.scale, macro_dev, nload, f, g.nx, g.ny, g.nz, gx,
-                         gy, gz, cell->volume);
+    if (!sig) {
+      if (cell->mat.scale)
+        hipLaunchKernelGGL((rhs_direct_kernel<Idx, true>), dim3(blocks), dim3(256), 0, s, cell->mat.index,
+                           cell->mat.c64, cell->mat.scale, macro_dev, nload, f, g.nx, g.ny, g.nz, gx, gy, gz,
+                           cell->volume);
+      else
+        hipLaunchKernelGGL((rhs_direct_kernel<Idx, false>), dim3(blocks), dim3(256), 0, s, cell->mat.index,
+                           cell->mat.c64, cell->mat.scale, macro_dev, nload, f, g.nx, g.ny, g.nz, gx, gy, gz,
+                           cell->volume);
+    } else if (cell->mat.scale)
+      hipLaunchKernelGGL((rhs_kernel<Idx, true>), dim3(blocks), dim3(256), 0, s, cell->mat.index, sig,
+                         cell->mat.scale, nload, f, g.nx, g.ny, g.nz, gx, gy, gz, cell->volume);
     else
-      hipLaunchKernelGGL((rhs_kernel<Idx, false>), dim3(blocks), dim3(256), 0, s, cell->mat.index,
-                         cell->mat.c64, cell->mat.scale, macro_dev, nload, f, g.nx, g.ny, g.nz, gx,
-                         gy, gz, cell->volume);
+      hipLaunchKernelGGL((rhs_kernel<Idx, false>), dim3(blocks), dim3(256), 0, s, cell->mat.index, sig,
+                         cell->mat.scale, nload, f, g.nx, g.ny, g.nz, gx, gy, gz, cell->volume);
     VXL_HIP(hipGetLastError());
     return VXL_OK;
   });
+}
+
+// Per-row stresses pay off while the row table is small next to the grid;
+// one-row-per-voxel tables use the direct form (scratch size 0).
+int64_t rhs_scratch_size(const vxl_cell* cell, int nload) {
+  if (int64_t(cell->rows) * 64 > cell->grid.n && cell->rows > 4096) return 0;
+  return int64_t(cell->rows) * nload * 6;
 }
 
 int64_t stress_partial_size(const vxl_cell* cell, int nload) {

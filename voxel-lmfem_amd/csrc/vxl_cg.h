@@ -85,6 +85,7 @@ struct vxl_solver {
   int* restart_dev;
   int* stop_dev;
   double* macro_dev;
+  double* rhs_sig;  // per-row element stresses of the RHS (nullptr: direct form)
   double* stress_partial;
   double* ch_dev;
   hipEvent_t ev[4];

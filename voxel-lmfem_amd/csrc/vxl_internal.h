@@ -136,12 +136,15 @@ int launch_apply_cg(const vxl_cell* cell, int nrhs, const T* p, T* q, const int*
                     int* blocks, hipStream_t s);
 template <typename T>
 int apply_cg_blocks(const vxl_cell* cell, int nrhs);
-int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, hipStream_t s);
+int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, double* sig, hipStream_t s);
+// Doubles of the `sig` scratch of launch_rhs (rows x nload x 6 element
+// stresses); 0 means pass sig = nullptr (direct per-element form).
+int64_t rhs_scratch_size(const vxl_cell* cell, int nload);
 // partial: workspace of stress_partial_size(cell, nload) doubles; out: nload x 6
 int launch_avg_stress(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
                       double* partial, double* out_dev, hipStream_t s);
 int64_t stress_partial_size(const vxl_cell* cell, int nload);
-int default_stencil_variant();
+int default_stencil_variant(int precision);
 // Label -> row index on the device with label presence and the Zeman lattice
 // samples (material.hip).
 int label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,

@@ -39,10 +39,13 @@ class Cell:
         self.h = tuple(float(v) for v in h)
         self.n = int(np.prod(self.shape))
         self.field = field
-        index = np.ascontiguousarray(field.index)
-        if index.dtype == np.uint16:
-            index = index.view(np.int16)
-        self.index = torch.from_numpy(index).to(self.device)
+        if getattr(field, "index_dev", None) is not None:  # built on the device
+            self.index = field.index_dev
+        else:
+            index = np.ascontiguousarray(field.index)
+            if index.dtype == np.uint16:
+                index = index.view(np.int16)
+            self.index = torch.from_numpy(index).to(self.device)
         self.scale = None
         if scale is not None:
             self.scale = torch.as_tensor(np.ascontiguousarray(scale, dtype=np.float32).reshape(-1),

@@ -185,6 +185,98 @@ class MaterialField:
     def voxel_rows(self, positions):
         return self.rows[self.index[positions]]
 
+    def lattice_rows(self, shape):
+        """Rows at the Zeman sampling lattice (every max(1, n//16)-th voxel per axis, C order)."""
+        nz, ny, nx = shape
+        z = np.arange(0, nz, max(1, nz // 16))
+        y = np.arange(0, ny, max(1, ny // 16))
+        x = np.arange(0, nx, max(1, nx // 16))
+        pos = (z[:, None, None] * (ny * nx) + y[None, :, None] * nx + x[None, None, :]).reshape(-1)
+        return self.voxel_rows(pos)
+
+    def used_labels(self, phase):
+        return np.unique(np.asarray(phase))
+
+
+class DeviceField(MaterialField):
+    """MaterialField whose row index lives on the GPU (built by libvxl from a
+    resident label array); the host keeps the row table, the labels present in
+    the cell and the Zeman lattice rows."""
+
+    def __init__(self, rows, index_dev, kind, labels, tensors, local, present, samples):
+        super().__init__(rows, None, labels, tensors, local)
+        self.index_dev = index_dev
+        self.kind = kind
+        self.present = present
+        self.samples = samples
+
+    @property
+    def index_kind(self):
+        return self.kind
+
+    def lattice_rows(self, shape):
+        return self.rows[self.samples]
+
+    def used_labels(self, phase=None):
+        return self.present
+
+
+def index_dtype(rows):
+    """Narrowest row-index type for a table of `rows` rows (libvxl index kinds)."""
+    if rows <= 256:
+        return np.uint8
+    if rows <= 65536:
+        return np.uint16
+    if rows <= np.iinfo(np.int32).max:
+        return np.int32
+    raise ValueError("Coefficient table exceeds int32 indexing")
+
+
+def build_field_device(phase_dev, materials):
+    """Label bookkeeping of a cell whose int32/int64 labels are resident on the GPU.
+
+    Same rows and per-voxel row index as ``build_field`` for tables without
+    local-frame phases (rows = the phase tensors in ascending label order); the
+    index is computed on the device by ``vxl_label_rows``.  Raises the
+    reference's ValueError for labels without a material.
+    """
+    import ctypes
+    import torch
+    from . import _native
+    from ._native import check, dptr
+    labels, tensors, local = phase_table(materials)
+    if local.any():
+        raise ValueError("build_field_device handles global-frame phases; use build_field for local frames")
+    if phase_dev.dim() != 3 or phase_dev.dtype not in (torch.int32, torch.int64) or not phase_dev.is_cuda:
+        raise ValueError("phase_dev must be a 3-D int32/int64 GPU tensor")
+    phase_dev = phase_dev.contiguous()
+    nz, ny, nx = (int(v) for v in phase_dev.shape)
+    lo = int(labels[0])
+    span = int(labels[-1]) - lo + 1
+    if span > (1 << 24):
+        raise ValueError("label range too wide for the device lookup table; use build_field")
+    lut = np.full(span, -1, dtype=np.int32)
+    lut[labels - lo] = np.arange(len(labels), dtype=np.int32)
+    dtype = index_dtype(len(labels))
+    kind = {np.uint8: 1, np.uint16: 2, np.int32: 4}[dtype]
+    tdtype = {np.uint8: torch.uint8, np.uint16: torch.int16, np.int32: torch.int32}[dtype]
+    index = torch.empty(nz * ny * nx, dtype=tdtype, device=phase_dev.device)
+    present = np.zeros(span, dtype=np.int32)
+    undefined = np.zeros(1, dtype=np.int32)
+    cz, cy, cx = (-(-n // max(1, n // 16)) for n in (nz, ny, nx))
+    samples = np.zeros(cz * cy * cx, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib = _native.load()
+    check(lib.vxl_label_rows(nz, ny, nx, phase_dev.element_size(), dptr(phase_dev), lo, span,
+                             lut.ctypes.data_as(ip), kind, dptr(index), present.ctypes.data_as(ip),
+                             undefined.ctypes.data_as(ip), samples.ctypes.data_as(ip),
+                             ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "vxl_label_rows")
+    if undefined[0]:
+        used = np.unique(phase_dev.cpu().numpy())
+        raise ValueError(f"Undefined material labels: {used[~np.isin(used, labels)].tolist()}")
+    used = labels[present[labels - lo] != 0]
+    return DeviceField(pack21(tensors), index, kind, labels, tensors, local, used, samples)
+
 
 def build_field(phase, materials, orientation=None, *, deduplicate=True):
     """Label/orientation bookkeeping of one cell -> MaterialField."""
@@ -215,12 +307,4 @@ def build_field(phase, materials, orientation=None, *, deduplicate=True):
             index[selected] = offset + np.arange(selected.size)
             offset += selected.size
     rows = np.concatenate(table, axis=0)
-    if len(rows) <= 256:
-        dtype = np.uint8
-    elif len(rows) <= 65536:
-        dtype = np.uint16
-    elif len(rows) <= np.iinfo(np.int32).max:
-        dtype = np.int32
-    else:
-        raise ValueError("Coefficient table exceeds int32 indexing")
-    return MaterialField(rows, index.astype(dtype), labels, tensors, local)
+    return MaterialField(rows, index.astype(index_dtype(len(rows))), labels, tensors, local)

@@ -25,12 +25,7 @@ def _iso(lam, mu):
 
 
 def zeman(field, shape):
-    nz, ny, nx = shape
-    z = np.arange(0, nz, max(1, nz // 16))
-    y = np.arange(0, ny, max(1, ny // 16))
-    x = np.arange(0, nx, max(1, nx // 16))
-    pos = (z[:, None, None] * (ny * nx) + y[None, :, None] * nx + x[None, None, :]).reshape(-1)
-    c = field.voxel_rows(pos)  # (k, 21) packed: indices of C44,C55,C66 = 15, 18, 20
+    c = field.lattice_rows(shape)  # (k, 21) packed: indices of C44,C55,C66 = 15, 18, 20
     mus = .5 * (c[:, 15] + c[:, 18] + c[:, 20]) / 2.
     Ks = (c[:, 0] + c[:, 6] + c[:, 11] + 2 * (c[:, 1] + c[:, 2] + c[:, 7])) / 9.
     mu = .5 * (float(mus.max()) + float(mus.min()))
@@ -77,7 +72,7 @@ def select(field, phase, materials):
     """(lam0, mu0, metadata) for a cell."""
     from .materials import stiffness
     base = zeman(field, phase.shape)
-    used = np.unique(phase)
+    used = field.used_labels(phase)
     phases = [stiffness(materials[int(k)]) for k in used]
     ref, status = design(phases, base)
     lam, mu = float(ref[0, 1]), float(ref[3, 3])

@@ -25,13 +25,44 @@ TOL = 1e-8
 
 
 def _prepare(phase, table, domain, orientation):
+    """Host bookkeeping (local-frame cells): labels, frames and Bond-rotated rows."""
     field = mat.build_field(phase, table, orientation)
     h = domain / np.array(phase.shape[::-1], dtype=float)
     lam, mu, meta = select(field, phase, table)
     return field, h, lam, mu, meta
 
 
-def _workload(phase, table, domain, orientation, precision, tol, prepared=None):
+def _prepare_device(phase, table, domain):
+    """Device bookkeeping (global-frame cells): labels uploaded once, rows on the GPU."""
+    import torch
+    small = phase.min() >= np.iinfo(np.int32).min and phase.max() <= np.iinfo(np.int32).max
+    labels = torch.from_numpy(np.ascontiguousarray(phase, dtype=np.int32 if small else np.int64)).to("cuda")
+    field = mat.build_field_device(labels, table)
+    h = domain / np.array(phase.shape[::-1], dtype=float)
+    lam, mu, meta = select(field, phase, table)
+    return field, h, lam, mu, meta
+
+
+def _field_statistics(cell, u, phase, loads=(0, 5)):
+    """Element-centre energy/stress diagnostics of the reference workload
+    (benchmark.py:99-107), with per-phase mean energies in place of its
+    woven-specific yarn/matrix split."""
+    flat = np.asarray(phase).reshape(-1)
+    used = np.unique(flat)
+    stats, slices = {}, {}
+    nz = cell.shape[0]
+    for j in loads:
+        _, sig, w = cell.center_fields_planes(u[j:j + 1].contiguous(), np.eye(6)[j][None], strain=False)
+        w = w[0].cpu().numpy()
+        sig = sig[0].cpu().numpy()
+        stats[str(j)] = dict(mean_energy=float(w.mean()), stress_l2=float(np.linalg.norm(sig)),
+                             phase_mean_energy={str(int(k)): float(w[flat == k].mean()) for k in used})
+        slices[f"energy_load{j}"] = w.reshape(cell.shape)[nz // 2]
+        slices[f"stress11_load{j}"] = sig[0].reshape(cell.shape)[nz // 2]
+    return stats, slices
+
+
+def _workload(phase, table, domain, orientation, precision, tol, local, fields=False):
     from .gpu import Cell, Solver
     import torch
     torch.cuda.synchronize()
@@ -45,39 +76,48 @@ def _workload(phase, table, domain, orientation, precision, tol, prepared=None):
         stage.t = now
     stage.t = start
     torch.cuda.reset_peak_memory_stats()
-    if prepared is None:
-        prepared = _prepare(phase, table, domain, orientation)
-    else:  # host bookkeeping already done (and timed) by the caller
-        stages.append(dict(name="material", seconds=prepared[-1], peak=0))
-        start -= prepared[-1]
-    field, h, lam, mu, meta = prepared[:5]
-    if len(stages) == 0:
-        stage("material")
+    if local.any():
+        field, h, lam, mu, meta = _prepare(phase, table, domain, orientation)
+    else:
+        field, h, lam, mu, meta = _prepare_device(phase, table, domain)
+    stage("material")
     cell = Cell(field, phase.shape, h)
     solver = Solver(cell, lam, mu, mode=precision, nrhs=6)
     stage("operator")
-    out = solver.run(np.eye(6), tol=tol)
+    out = solver.run(np.eye(6), tol=tol, return_fields=fields)
     stage("solve")
+    field_stats, slices = {}, {}
+    if fields:
+        field_stats, slices = _field_statistics(cell, out["fields"], phase)
+        stage("fields")
     seconds = time.perf_counter() - start
     calls = [dict(iterations=int(out["iterations"][j]), restarts=int(out["restarts"][j]),
                   true_residuals=[float(out["true_residuals"][j])],
                   trace=[dict(iteration=abs(i), relative_residual=r, true_residual=i < 0)
                          for i, r in out["trace"][j]]) for j in range(6)]
     result = dict(seconds=seconds, CH=np.asarray(out["CH"]).tolist(), calls=calls, stages=stages,
-                  field_statistics={}, peak_allocated_bytes=int(torch.cuda.max_memory_allocated()),
+                  field_statistics=field_stats, peak_allocated_bytes=int(torch.cuda.max_memory_allocated()),
                   peak_reserved_bytes=int(torch.cuda.max_memory_reserved()),
                   device_ms=out["device_ms"])
     storage = dict(mode="label_rows", n_elements=int(phase.size), pool_rows=int(len(field.rows)),
-                   index_bytes=int(field.index.nbytes), coefficient_bytes=int(field.rows.size * 8))
+                   index_bytes=int(cell.index.numel() * cell.index.element_size()),
+                   coefficient_bytes=int(field.rows.size * 8),
+                   front_end="device" if getattr(field, "index_dev", None) is not None else "host")
     operator = dict(variant="modal_fi_lds_stencil", precision=precision, reference_medium=meta)
     solver.close()
     cell.close()
+    if fields:
+        result["slices"] = slices
     return result, storage, operator
 
 
 def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="gpu", warmup=False,
-               precision="mixed", tol=TOL):
-    """Return effective stiffness in engineering-Voigt order 11,22,33,23,13,12."""
+               precision="mixed", tol=TOL, fields=False):
+    """Return effective stiffness in engineering-Voigt order 11,22,33,23,13,12.
+
+    fields=True adds the reference workload's field diagnostics for loads 0
+    and 5 (``field_statistics`` and mid-plane ``slices`` of the element-centre
+    energy and sigma_11)."""
     if backend not in GPU_BACKENDS + ("cpu",):
         raise ValueError("backend must be 'gpu', 'hip', 'cuda' or 'cpu'")
     if precision not in ("mixed", "fp64"):
@@ -87,22 +127,21 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="g
     if backend == "cpu":
         raise ValueError("backend='cpu' is the reference's sparse-direct oracle and is not part of this "
                          "GPU build; use backend='gpu'")
-    t0 = time.perf_counter()
-    prepared = _prepare(phase, table, domain, orientation)  # validates every local frame
-    prepared = prepared + (time.perf_counter() - t0,)
+    if local.any():  # every local frame is validated before any GPU work
+        for k in np.flatnonzero(local):
+            selected = np.flatnonzero(phase.reshape(-1) == labels[k])
+            for first in range(0, len(selected), 65536):
+                mat.frames(orientation, phase.shape, selected[first:first + 65536])
     try:
         import torch
     except ImportError as exc:
         raise RuntimeError("Install a ROCm-enabled PyTorch build for device memory") from exc
     if not torch.cuda.is_available():
         raise RuntimeError("No GPU is available. This backend runs only on an MI355X (gfx950).")
-    start = time.perf_counter() - prepared[-1]
     if warmup:
-        _workload(phase, table, domain, orientation, precision, tol)
-        start = time.perf_counter()
-        result, storage, operator = _workload(phase, table, domain, orientation, precision, tol)
-    else:
-        result, storage, operator = _workload(phase, table, domain, orientation, precision, tol, prepared)
+        _workload(phase, table, domain, orientation, precision, tol, local, fields)
+    start = time.perf_counter()
+    result, storage, operator = _workload(phase, table, domain, orientation, precision, tol, local, fields)
     result.update(backend="gpu", storage=storage, operator=operator,
                   scope=("FI HEX8 matrix-free stiffness; FFT Green preconditioner; "
                          + ("FP32 PCG with FP64 reliable updates" if precision == "mixed" else "FP64 PCG")
