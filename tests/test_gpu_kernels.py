@@ -177,11 +177,14 @@ def test_center_fields_match_oracle(torch_gpu, shape):
     cell.close()
 
 
-PC_SHAPES = [(4, 4, 4), (8, 8, 8), (6, 5, 4), (3, 7, 5), (16, 12, 10), (17, 2, 9), (32, 48, 20)]
+PC_SHAPES = [(4, 4, 4), (8, 8, 8), (6, 5, 4), (3, 7, 5), (16, 12, 10), (17, 2, 9), (32, 48, 20),
+             (5, 3, 16), (3, 6, 64), (2, 5, 128), (3, 3, 256), (2, 3, 512)]
 
 
 @pytest.mark.parametrize("shape", PC_SHAPES)
-def test_precond_matches_oracle(torch_gpu, shape):
+def test_precond_matches_oracle(torch_gpu, shape, monkeypatch):
+    """Power-of-two nx >= 8 runs the four-step x passes; they are also checked
+    against the Stockham x passes (VXL_FFT_XFOURSTEP=0) on the same input."""
     torch = torch_gpu
     from voxel_lmfem.gpu import Cell, Preconditioner
     print(f"case precond shape={shape}", flush=True)
@@ -193,11 +196,14 @@ def test_precond_matches_oracle(torch_gpu, shape):
     R -= R.mean(axis=1, keepdims=True)
     ref = np.stack([planes(spectral.green_apply(r, shape, h, lam, mu)) for r in R])
     r = torch.tensor(np.stack([planes(x) for x in R]), device="cuda")
-    for precision, tol in (("float64", 1e-11), ("float32", 5e-6)):
-        pc = Preconditioner(cell, lam, mu, precision=precision, nrhs_max=3)
-        z = pc.apply_planes(r.to(pc.dtype)).double().cpu().numpy()
-        err = np.abs(z - ref).max() / np.abs(ref).max()
-        print(f"  {precision} err {err:.2e}", flush=True)
-        assert err < tol
-        pc.close()
+    pow2 = shape[2] >= 8 and shape[2] & (shape[2] - 1) == 0
+    for xfour in (("1", "0") if pow2 else ("1",)):
+        monkeypatch.setenv("VXL_FFT_XFOURSTEP", xfour)
+        for precision, tol in (("float64", 1e-11), ("float32", 5e-6)):
+            pc = Preconditioner(cell, lam, mu, precision=precision, nrhs_max=3)
+            z = pc.apply_planes(r.to(pc.dtype)).double().cpu().numpy()
+            err = np.abs(z - ref).max() / np.abs(ref).max()
+            print(f"  x four-step {xfour} {precision} err {err:.2e}", flush=True)
+            assert err < tol
+            pc.close()
     cell.close()

@@ -126,7 +126,7 @@ int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy
   if (!out) return fail("out handle is NULL");
   *out = nullptr;
   if (nz < 2 || ny < 2 || nx < 2) return fail("grid dimensions must be >= 2");
-  if (int64_t(nz) * ny * nx > (int64_t(1) << 40)) return fail("grid too large");
+  if (int64_t(nz) * ny * nx > (int64_t(1) << 40) || int64_t(ny) * nx > INT32_MAX) return fail("grid too large");
   if (!(hx > 0 && hy > 0 && hz > 0) || !std::isfinite(hx * hy * hz))
     return fail("voxel sizes must be finite and positive");
   if (index_kind != VXL_INDEX_U8 && index_kind != VXL_INDEX_U16 && index_kind != VXL_INDEX_I32)

@@ -662,6 +662,183 @@ fft_z4_green(typename CT<T>::type* __restrict__ S, int ny, int nxh, int TK, Tabl
     four_step_2<true, T, N1, N2>(L, TJ, p, t, w, [&](int e, C val) { col[e * zstride + p] = val; });
 }
 
+// ---- four-step x passes for power-of-two nx -------------------------------
+// The x pencils are contiguous in memory, so lanes run along x: pencil p of a
+// workgroup is worked by `lanes` consecutive threads (t fastest), every global
+// access is a run of consecutive floats, and no integer division by runtime
+// values remains.  Pencils 3 pp + {0, 1, 2} of row pair pp are c0 + i c1 of
+// row 2pp, of row 2pp + 1, and c2(2pp) + i c2(2pp + 1), as in fft_x_r2c.
+// LDS: the N1 x N2 step-1 tile of pencil p at p * TS + j1 * (N2 + 1) + t
+// (row padding: step-2 reads run down a column), and the natural-order
+// spectrum at p * (n + 1) + e.
+template <int N1, int N2>
+struct XTile {
+  static constexpr int n = N1 * N2;
+  static constexpr int TS = N1 * (N2 + 1) + 1;  // step-1 tile stride per pencil
+  static constexpr int NS = n + 1;              // natural-order stride per pencil
+  static constexpr int stride = TS > NS ? TS : NS;
+};
+
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(256)
+fft_x4_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int64_t rows, int PP,
+           const typename CT<T>::type* __restrict__ w, int64_t field_stride, int64_t spec_stride,
+           const int* active) {
+  using C = typename CT<T>::type;
+  using XT = XTile<N1, N2>;
+  constexpr int n = XT::n, nxh = n / 2 + 1, lanes = FourStep<N1, N2>::lanes;
+  extern __shared__ double2 smem_raw[];
+  if (active && !active[blockIdx.y]) return;
+  C* L = reinterpret_cast<C*>(smem_raw);
+  const int P = 3 * PP;
+  const int t = threadIdx.x % lanes, p = threadIdx.x / lanes;
+  const int pp = p / 3, q = p - 3 * pp;
+  const int64_t plane = rows * n;
+  const T* f = r + int64_t(blockIdx.y) * field_stride;
+  C* out = S + int64_t(blockIdx.y) * spec_stride;
+  const int64_t rho = 2 * (int64_t(blockIdx.x) * PP + pp);
+  const bool live = p < P;
+  // component/row of the real and imaginary part of this pencil
+  const int64_t re_off = q == 2 ? 2 * plane + rho * n : rho * n + (q == 1 ? n : 0);
+  const int64_t im_off = q == 2 ? 2 * plane + (rho + 1) * n : plane + rho * n + (q == 1 ? n : 0);
+  const bool re_ok = live && (q == 1 ? rho + 1 < rows : rho < rows);
+  const bool im_ok = live && rho + (q == 0 ? 0 : 1) < rows;
+  C* T1 = L + p * XT::stride;
+  if (live && t < N2) {
+    C v[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int e = t + N2 * k1;
+      v[k1] = C{re_ok ? f[re_off + e] : T(0), im_ok ? f[im_off + e] : T(0)};
+    }
+    dft_reg<N1, false, T>(v, w, N2);
+#pragma unroll
+    for (int j1 = 0; j1 < N1; ++j1) {
+      if (j1 > 0 && t > 0) v[j1] = cmul(v[j1], w[(j1 * t) % n]);
+      T1[j1 * (N2 + 1) + t] = v[j1];
+    }
+  }
+  __syncthreads();
+  C u[N2];
+  if (live && t < N1) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) u[k2] = T1[t * (N2 + 1) + k2];
+    dft_reg<N2, false, T>(u, w, N1);
+  }
+  __syncthreads();
+  if (live && t < N1) {
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) T1[t + N1 * j2] = u[j2];  // natural order X[e]
+  }
+  __syncthreads();
+  // Two-for-one split, as in fft_x_r2c.
+  for (int idx = threadIdx.x; idx < PP * nxh; idx += blockDim.x) {
+    const int a = idx / nxh, k = idx - a * nxh;
+    const int64_t rr = 2 * (int64_t(blockIdx.x) * PP + a);
+    const int km = k == 0 ? 0 : n - k;
+    C sp[3][2];
+#pragma unroll
+    for (int qq = 0; qq < 3; ++qq) {
+      const C* X = L + (3 * a + qq) * XT::stride;
+      const C x1 = X[k], x2 = cconj(X[km]);
+      const C sm = cadd(x1, x2), df = csub(x1, x2);
+      sp[qq][0] = C{T(0.5) * sm.x, T(0.5) * sm.y};
+      sp[qq][1] = C{T(0.5) * df.y, T(-0.5) * df.x};
+    }
+    if (rr < rows) {
+      C* o = out + (rr * nxh + k) * 3;
+      o[0] = sp[0][0];
+      o[1] = sp[0][1];
+      o[2] = sp[2][0];
+    }
+    if (rr + 1 < rows) {
+      C* o = out + ((rr + 1) * nxh + k) * 3;
+      o[0] = sp[1][0];
+      o[1] = sp[1][1];
+      o[2] = sp[2][1];
+    }
+  }
+}
+
+// kDir as in fft_x_c2r: the output is p = z + beta[rhs] p, in place.
+template <typename T, int N1, int N2, bool kDir>
+__global__ void __launch_bounds__(256)
+fft_x4_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int64_t rows, int PP,
+           const typename CT<T>::type* __restrict__ w, int64_t field_stride, int64_t spec_stride,
+           const int* active, const double* beta) {
+  using C = typename CT<T>::type;
+  using XT = XTile<N1, N2>;
+  constexpr int n = XT::n, nxh = n / 2 + 1, lanes = FourStep<N1, N2>::lanes;
+  extern __shared__ double2 smem_raw[];
+  if (active && !active[blockIdx.y]) return;
+  C* L = reinterpret_cast<C*>(smem_raw);
+  const int P = 3 * PP;
+  const C* in = S + int64_t(blockIdx.y) * spec_stride;
+  // Hermitian extension and packing of the three pencils per row pair.
+  for (int idx = threadIdx.x; idx < PP * n; idx += blockDim.x) {
+    const int a = idx / n, k = idx - a * n;
+    const int64_t rr = 2 * (int64_t(blockIdx.x) * PP + a);
+    const bool mirror = k >= nxh;
+    const int kk = mirror ? n - k : k;
+    const bool realonly = kk == 0 || kk == n / 2;
+    C x[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        C v = {T(0), T(0)};
+        if (rr + h < rows) v = in[((rr + h) * nxh + kk) * 3 + c];
+        if (realonly) v.y = T(0);
+        if (mirror) v = cconj(v);
+        x[h][c] = v;
+      }
+    }
+    L[(3 * a + 0) * XT::stride + k] = C{x[0][0].x - x[0][1].y, x[0][0].y + x[0][1].x};
+    L[(3 * a + 1) * XT::stride + k] = C{x[1][0].x - x[1][1].y, x[1][0].y + x[1][1].x};
+    L[(3 * a + 2) * XT::stride + k] = C{x[0][2].x - x[1][2].y, x[0][2].y + x[1][2].x};
+  }
+  __syncthreads();
+  const int t = threadIdx.x % lanes, p = threadIdx.x / lanes;
+  const int pp = p / 3, q = p - 3 * pp;
+  const bool live = p < P;
+  C* X = L + p * XT::stride;
+  C v[N1];
+  if (live && t < N2) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) v[k1] = X[t + N2 * k1];
+  }
+  __syncthreads();
+  if (live && t < N2) {
+    dft_reg<N1, true, T>(v, w, N2);
+#pragma unroll
+    for (int j1 = 0; j1 < N1; ++j1) {
+      if (j1 > 0 && t > 0) v[j1] = cmul(v[j1], cconj(w[(j1 * t) % n]));
+      X[j1 * (N2 + 1) + t] = v[j1];
+    }
+  }
+  __syncthreads();
+  if (!(live && t < N1)) return;
+  C u[N2];
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) u[k2] = X[t * (N2 + 1) + k2];
+  dft_reg<N2, true, T>(u, w, N1);
+  const int64_t plane = rows * n;
+  T* f = z + int64_t(blockIdx.y) * field_stride;
+  const int64_t rho = 2 * (int64_t(blockIdx.x) * PP + pp);
+  const int64_t re_off = q == 2 ? 2 * plane + rho * n : rho * n + (q == 1 ? n : 0);
+  const int64_t im_off = q == 2 ? 2 * plane + (rho + 1) * n : plane + rho * n + (q == 1 ? n : 0);
+  const bool re_ok = q == 1 ? rho + 1 < rows : rho < rows;
+  const bool im_ok = rho + (q == 0 ? 0 : 1) < rows;
+  const T b = kDir ? T(beta[blockIdx.y]) : T(0);
+  const bool keep = kDir && b != T(0);
+#pragma unroll
+  for (int j2 = 0; j2 < N2; ++j2) {
+    const int e = t + N1 * j2;
+    if (re_ok) f[re_off + e] = keep ? fma(b, f[re_off + e], u[j2].x) : u[j2].x;
+    if (im_ok) f[im_off + e] = keep ? fma(b, f[im_off + e], u[j2].y) : u[j2].y;
+  }
+}
+
 struct PrecondImpl {
   int nx, ny, nz, nxh;
   int64_t rows;
@@ -673,6 +850,9 @@ struct PrecondImpl {
   int pairs_per_block, tj, tk;
   size_t lds_x, lds_y, lds_z;
   // four-step kernels for power-of-two y / z axes (0: use the Stockham pass)
+  int x4;            // nx handled by fft_x4_r2c / fft_x4_c2r (0: Stockham x passes)
+  int pp4, thr_x4;   // row pairs and threads per x4 workgroup
+  size_t lds_x4;
   int y4, z4;        // axis length handled by fft_y4 / fft_z4_green
   int tj4, tk4;      // columns / kx per workgroup
   int thr_y4, thr_z4;
@@ -737,6 +917,44 @@ void launch_z4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, typename CT
     VXL_Z4(512, 32, 16)
   }
 #undef VXL_Z4
+}
+
+#define VXL_X4_CASES(M) M(8, 4, 2) M(16, 4, 4) M(32, 8, 4) M(64, 8, 8) M(128, 16, 8) M(256, 16, 16) M(512, 32, 16)
+
+template <typename T>
+size_t x4_stride(int n) {
+#define VXL_X4S(N, A, B) \
+  case N:                \
+    return XTile<A, B>::stride;
+  switch (n) { VXL_X4_CASES(VXL_X4S) }
+#undef VXL_X4S
+  return 0;
+}
+
+template <typename T>
+void launch_x4_r2c(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const T* r, typename CT<T>::type* S,
+                   int64_t rows, int PP, const typename CT<T>::type* w, int64_t fstride, int64_t sstride,
+                   const int* active) {
+#define VXL_X4R(N, A, B)                                                                                   \
+  case N:                                                                                                \
+    hipLaunchKernelGGL((fft_x4_r2c<T, A, B>), grid, dim3(thr), lds, s, r, S, rows, PP, w, fstride, sstride, \
+                       active);                                                                          \
+    break;
+  switch (n) { VXL_X4_CASES(VXL_X4R) }
+#undef VXL_X4R
+}
+
+template <typename T, bool kDir>
+void launch_x4_c2r(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const typename CT<T>::type* S, T* z,
+                   int64_t rows, int PP, const typename CT<T>::type* w, int64_t fstride, int64_t sstride,
+                   const int* active, const double* beta) {
+#define VXL_X4C(N, A, B)                                                                                      \
+  case N:                                                                                                   \
+    hipLaunchKernelGGL((fft_x4_c2r<T, A, B, kDir>), grid, dim3(thr), lds, s, S, z, rows, PP, w, fstride,     \
+                       sstride, active, beta);                                                              \
+    break;
+  switch (n) { VXL_X4_CASES(VXL_X4C) }
+#undef VXL_X4C
 }
 
 template <typename T>
@@ -841,6 +1059,20 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
       if (im->lds_z4 > kMaxLds || im->thr_z4 > 256 || im->thr_z4 < 1) im->z4 = 0;
     }
   }
+  {
+    // Four-step x passes for power-of-two nx >= 8: 64 / lanes row pairs per
+    // workgroup (3 * 64 threads).
+    const int lx = four_step_lanes(im->nx);
+    const char* env = std::getenv("VXL_FFT_XFOURSTEP");
+    im->x4 = (lx && im->nx >= 8 && !(env && env[0] == '0')) ? im->nx : 0;
+    if (im->x4) {
+      im->pp4 = 64 / lx;  // 3 * 64 = 192 threads, whole waves
+      if (const int v = env_int("VXL_FFT_X4PAIRS")) im->pp4 = v;
+      im->thr_x4 = 3 * im->pp4 * lx;
+      im->lds_x4 = sizeof(C) * x4_stride<T>(im->nx) * 3 * im->pp4;
+      if (im->thr_x4 > 256 || im->thr_x4 % 64 != 0 || im->lds_x4 > kMaxLds) im->x4 = 0;
+    }
+  }
   im->spec_stride = int64_t(im->nz) * im->ny * im->nxh * 3;
   VXL_TRY(upload_tables<T>(im, cell->h));
   VXL_HIP(dev_malloc(&im->spec, sizeof(C) * size_t(im->spec_stride) * pc->nrhs_max));
@@ -868,8 +1100,13 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
   const int64_t pairs = (im->rows + 1) / 2;
   const unsigned bx = unsigned((pairs + im->pairs_per_block - 1) / im->pairs_per_block);
   const int W = 3 * im->nxh;
-  hipLaunchKernelGGL((fft_x_r2c<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx, im->rows,
-                     im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active);
+  const unsigned bx4 = im->x4 ? unsigned((pairs + im->pp4 - 1) / im->pp4) : 0u;
+  if (im->x4)
+    launch_x4_r2c<T>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, S, im->rows, im->pp4, tb.wx, fstride,
+                     im->spec_stride, active);
+  else
+    hipLaunchKernelGGL((fft_x_r2c<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx, im->rows,
+                       im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active);
   VXL_HIP(hipGetLastError());
   auto y_pass = [&](bool inverse) -> int {
     if (im->y4) {
@@ -914,7 +1151,13 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
   VXL_HIP(hipGetLastError());
   if (cg) VXL_TRY(cg->mid());
   VXL_TRY(y_pass(true));
-  if (cg)
+  if (im->x4 && cg)
+    launch_x4_c2r<T, true>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, S, z, im->rows, im->pp4, tb.wx,
+                           fstride, im->spec_stride, active, cg->beta);
+  else if (im->x4)
+    launch_x4_c2r<T, false>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, S, z, im->rows, im->pp4, tb.wx,
+                            fstride, im->spec_stride, active, nullptr);
+  else if (cg)
     hipLaunchKernelGGL((fft_x_c2r<T, true>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
                        im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active,
                        cg->beta);

@@ -275,13 +275,13 @@ apply_kernel(ApplyArgs<T> a) {
 
   // Plane staging: each thread copies up to ITEMS halo-tile nodes.
   int ld_lds[TL::ITEMS];
-  int64_t ld_off[TL::ITEMS];
+  int ld_off[TL::ITEMS];  // offsets inside one z plane (nx * ny < 2^31)
 #pragma unroll
   for (int k = 0; k < TL::ITEMS; ++k) {
     const int idx = tid + k * TL::NT;
     ld_lds[k] = idx < TL::PLANE ? idx : -1;
     const int px = idx % TL::PX, py = idx / TL::PX;
-    ld_off[k] = int64_t(wrapmod(y0 - 1 + py, ny)) * nx + wrapmod(x0 - 1 + px, nx);
+    ld_off[k] = wrapmod(y0 - 1 + py, ny) * nx + wrapmod(x0 - 1 + px, nx);
   }
   auto load_plane = [&](int z, int buf) {
     const int64_t base = int64_t(wrapmod(z, nz)) * nxy;
@@ -297,7 +297,7 @@ apply_kernel(ApplyArgs<T> a) {
   // Element ownership.
   const bool is_elem = tid < TL::EL;
   const int exl = tid % TL::EX, eyl = tid / TL::EX;
-  const int64_t e_xy = int64_t(wrapmod(y0 - 1 + eyl, ny)) * nx + wrapmod(x0 - 1 + exl, nx);
+  const int e_xy = wrapmod(y0 - 1 + eyl, ny) * nx + wrapmod(x0 - 1 + exl, nx);
   auto compute_layer = [&](int z, int bottom, int top) {
     if (!is_elem) return;
     const int64_t voxel = int64_t(wrapmod(z, nz)) * nxy + e_xy;
@@ -330,7 +330,7 @@ apply_kernel(ApplyArgs<T> a) {
   const bool is_node = tid < TL::NODES;
   const int xl = tid % TX, yl = tid / TX;
   const bool node_valid = is_node && (x0 + xl) < nx && (y0 + yl) < ny;
-  const int64_t n_xy = int64_t(y0 + yl) * nx + (x0 + xl);
+  const int n_xy = (y0 + yl) * nx + (x0 + xl);
   // The four elements (ex, ey) in {xl, xl+1} x {yl, yl+1}; corner index
   // (ax, ay) = (xl+1-ex, yl+1-ey).
   auto gather = [&](int zbit, T (&acc)[3]) {
@@ -403,13 +403,13 @@ apply_kernel_v2(ApplyArgs<T> a) {
   T* out = a.out + int64_t(bid.rhs) * a.rhs_stride;
 
   int ld_lds[TL::ITEMS];
-  int64_t ld_off[TL::ITEMS];
+  int ld_off[TL::ITEMS];  // offsets inside one z plane (nx * ny < 2^31)
 #pragma unroll
   for (int k = 0; k < TL::ITEMS; ++k) {
     const int idx = tid + k * TL::NT;
     ld_lds[k] = idx < TL::PLANE ? idx : -1;
     const int px = idx % TL::PX, py = idx / TL::PX;
-    ld_off[k] = int64_t(wrapmod(y0 - 1 + py, ny)) * nx + wrapmod(x0 - 1 + px, nx);
+    ld_off[k] = wrapmod(y0 - 1 + py, ny) * nx + wrapmod(x0 - 1 + px, nx);
   }
   auto load_plane = [&](int z) {
     const int64_t base = int64_t(wrapmod(z, nz)) * nxy;
@@ -424,7 +424,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
 
   const bool is_elem = tid < TL::EL;
   const int exl = tid % TL::EX, eyl = tid / TL::EX;
-  const int64_t e_xy = int64_t(wrapmod(y0 - 1 + eyl, ny)) * nx + wrapmod(x0 - 1 + exl, nx);
+  const int e_xy = wrapmod(y0 - 1 + eyl, ny) * nx + wrapmod(x0 - 1 + exl, nx);
   auto face_modes = [&](T (&f)[3][4]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -469,7 +469,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
   const bool is_node = tid < TL::NODES;
   const int xl = tid % TX, yl = tid / TX;
   const bool node_valid = is_node && (x0 + xl) < nx && (y0 + yl) < ny;
-  const int64_t n_xy = int64_t(y0 + yl) * nx + (x0 + xl);
+  const int n_xy = (y0 + yl) * nx + (x0 + xl);
   // kFuse: u . (K u) = sum over elements of u_e . f_e, and in face-mode form
   // u_e . f_e = Tf . Ft + Bf . Fb (the face and z transforms are applied with
   // their transposes to the forces).  Each element is counted by the tile and
