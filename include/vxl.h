@@ -131,6 +131,22 @@ int vxl_avg_stress(vxl_cell* cell, int nload, const double* u_dev, const double*
 int vxl_center_fields(vxl_cell* cell, int nload, const double* u_dev, const double* macro_host,
                       double* strain_dev, double* stress_dev, double* energy_dev);
 
+/* Adjoint of C_eff.  For nload converged FP64 fluctuation fields u_dev
+ * (nload x 3 x N planes, e.g. the u_dev_out of vxl_solver_run) of macro
+ * strains E (macro_host, nload x 6) and a symmetric weight matrix W (w_host,
+ * nload x nload, row major):
+ *   out_dev[e] = sum_ij W_ij int_e (E_i + B u_i)^T D_e (E_j + B u_j) dV
+ * with the operator's 2x2x2 Gauss rule, where D_e is the cell's FP64
+ * coefficient row of e (without the per-voxel scale) or, if d21_dev is not
+ * NULL, the per-element tensor d21_dev[e] (N x 21, PAIRS order).  Because the
+ * problem is self-adjoint (no second solve), with C_ij = E_i . CH E_j:
+ *   d(sum_ij W_ij C_ij)/d s_e = out_dev[e] / |Omega|   for C_e = s_e C_row(e),
+ *   d(sum_ij W_ij C_ij)/d t_e = out_dev[e] / |Omega|   for d21 = dC_e/dt_e.
+ * No counterpart in the reference, which has no gradient; checked against
+ * central finite differences of the C_eff solve. */
+int vxl_energy_gradient(vxl_cell* cell, int nload, const double* u_dev, const double* macro_host,
+                        const double* w_host, const double* d21_dev, double* out_dev);
+
 /* Subtract the per-component mean of nrhs FP64 fields in place.
  * Replaces HybridFIOperator.project_zero_mean  hybrid_adapter.py:78-80. */
 int vxl_project_zero_mean(vxl_cell* cell, int nrhs, double* u_dev);
@@ -163,6 +179,17 @@ int vxl_solver_create(vxl_solver** out, vxl_cell* cell, int mode, int nrhs, doub
 int vxl_solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter,
                    double* ch_host, int* iterations, int* restarts, double* residuals,
                    double* u_dev_out);
+/* Runs count independent solvers (each bound to its own cell and stream)
+ * concurrently from `workers` host threads, largest cells first, so small
+ * cells overlap on the GPU instead of waiting on each other's host checks.
+ * macro_host packs each solver's R x 6 strains one after the other; ch_host
+ * packs each solver's 6 x R CH block, and iterations/restarts/residuals R
+ * entries per solver, in the same order.  Returns the first failure (the
+ * message names the cell).  The reference runs cells one process at a time
+ * (solver.py:13-15); this is the batched-cell driver of the uneven-cell
+ * workload. */
+int vxl_solver_run_batch(vxl_solver* const* solvers, int count, const double* macro_host, double tol, int max_iter,
+                         double* ch_host, int* iterations, int* restarts, double* residuals, int workers);
 /* Per-phase device time of the last run in milliseconds:
  * [0] setup+rhs, [1] iterations, [2] certification+stress. */
 int vxl_solver_timings(vxl_solver* s, double* ms3);

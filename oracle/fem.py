@@ -28,6 +28,8 @@ Pinning
   * RHS and stress recovery: a uniform cell has f = 0 and <sigma> = C E;
     the laminate Schur formula and homogeneous/rotated KATs of
     tests/test_solver.py:17-56 fix the full chain through ``assemble``;
+  * ``energy_contraction``: central differences of ``assemble``'s C_eff
+    with respect to one element's stiffness scale (tests/test_oracle.py);
   * ``center_fields``: a periodic affine-free field (u = 0) gives strain = E
     and stress = C_e E in every element; a rigid translation leaves both
     unchanged; a node-local displacement enters the 8 adjacent elements with
@@ -186,6 +188,27 @@ class MatrixFree:
     def project_zero_mean(u):
         u = np.asarray(u, dtype=np.float64)
         return u - u.mean(axis=0, keepdims=True)
+
+
+def energy_contraction(Chat, shape, h, u, macro, W):
+    """out[e] = sum_ij W_ij sum_g (V/8) (E_i + B_g u_i)^T Chat_e (E_j + B_g u_j).
+
+    Chat (N,6,6) per element, u (N,3,R) fluctuations of the macro strains
+    macro (R,6), W (R,R).  With C_e = s_e Chat_e this is |Omega| dC_W/ds_e at
+    the discrete solution (self-adjoint sensitivity; the reference has no
+    gradient, so this is pinned by central differences of ``assemble``).
+    """
+    n = int(np.prod(shape))
+    nodes = element_nodes(shape)
+    R = u.shape[-1]
+    local = u[nodes].transpose(0, 3, 1, 2).reshape(n, R, 24)  # (N, R, 24)
+    V = float(np.prod(h))
+    out = np.zeros(n)
+    for B in gauss_B(h):
+        eps = np.einsum("ia,era->eri", B, local) + np.asarray(macro)[None]      # (N, R, 6)
+        sig = np.einsum("eij,erj->eri", Chat, eps)                              # (N, R, 6)
+        out += (V / 8) * np.einsum("ij,eik,ejk->e", np.asarray(W), eps, sig)
+    return out
 
 
 STENCIL_OFFSETS = np.array([d for d in itertools.product((-1, 0, 1), repeat=3)

@@ -81,3 +81,34 @@ def test_backend_names():
     with pytest.raises(ValueError, match="backend"):
         homogenize(np.zeros((2, 2, 2), dtype=int), {0: dict(kind="isotropic", E=1., nu=.2)}, [1, 1, 1],
                    backend="tpu")
+
+
+def test_angle_rows_and_their_derivative():
+    from voxel_lmfem import adjoint
+    from voxel_lmfem.cells import random_rotations
+    rng = np.random.default_rng(8)
+    R = random_rotations(5, rng)
+    C = pm.orthotropic_voigt(135., 9., 9., .3, .3, .35, 5., 5., 3.5)
+    np.testing.assert_allclose(pm.pack21(adjoint._bond(R) @ C @ adjoint._bond(R).transpose(0, 2, 1)),
+                               pm.bond_rotate(C, R), rtol=1e-13, atol=1e-12)
+    n = 7
+    local = np.array([True, False, True, True, False, True, True])
+    iso = np.tile(pm.pack21(pm.isotropic_voigt(3.5, .35)), (n, 1))
+    theta = rng.uniform(-.5, .5, n)
+    rows, drows = adjoint.angle_rows(theta, C, R, iso, local)
+    assert np.array_equal(rows[~local], iso[~local]) and not drows[~local].any()
+    step = 1e-6
+    plus, _ = adjoint.angle_rows(theta + step, C, R, iso, local)
+    minus, _ = adjoint.angle_rows(theta - step, C, R, iso, local)
+    np.testing.assert_allclose(drows, (plus - minus) / (2 * step), rtol=1e-6, atol=1e-6)
+    # theta = 0 reproduces the frames' Bond-rotated rows
+    rows0, _ = adjoint.angle_rows(np.zeros(n), C, R, iso, local)
+    np.testing.assert_allclose(rows0[local], pm.bond_rotate(C, R), rtol=1e-13, atol=1e-12)
+
+
+def test_objective_weights():
+    from voxel_lmfem import adjoint
+    C = np.arange(36.).reshape(6, 6)
+    C = C + C.T
+    KV = (C[0, 0] + C[1, 1] + C[2, 2] + 2 * (C[0, 1] + C[0, 2] + C[1, 2])) / 9
+    assert float((adjoint.objective_weights() * C).sum()) == pytest.approx(C[0, 0] + KV)

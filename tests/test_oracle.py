@@ -141,6 +141,45 @@ def test_center_fields_pins():
         assert ez in (0, 1) and ey in (1, 2)
 
 
+def simp_objective_weights():
+    """J = C_1111 + K_V, K_V = (sum_i<3 C_ii + 2 sum_i<j<3 C_ij) / 9, as sum_ij W_ij C_ij."""
+    W = np.zeros((6, 6))
+    W[0, 0] = 1.0
+    W[:3, :3] += 1.0 / 9.0
+    return W
+
+
+def test_energy_contraction_is_the_ceff_gradient():
+    rng = np.random.default_rng(21)
+    shape, h = (3, 4, 4), np.array([.8, 1.1, 1.3])
+    n = int(np.prod(shape))
+    Chat = random_spd(rng, n)
+    s = rng.uniform(.5, 2., n)
+    W = simp_objective_weights()
+
+    def J(scale):
+        CH = fem.assemble(scale[:, None, None] * Chat, shape, h)[3]
+        return float((W * CH).sum()), CH
+
+    J0, CH = J(s)
+    u = fem.assemble(s[:, None, None] * Chat, shape, h)[2]
+    volume = n * np.prod(h)
+    g = fem.energy_contraction(Chat, shape, h, u, np.eye(6), W) / volume
+    # Euler: C is homogeneous of degree one in the scales
+    assert float(g @ s) == pytest.approx(J0, rel=1e-10)
+    for e in (0, 17, 40):
+        d = np.zeros(n)
+        d[e] = 1e-4 * s[e]
+        fd = (J(s + d)[0] - J(s - d)[0]) / (2 * d[e])
+        assert fd == pytest.approx(g[e], rel=1e-6)
+    # the pair form gives every C_ij
+    for i, j in ((0, 0), (1, 2), (3, 5)):
+        Wij = np.zeros((6, 6))
+        Wij[i, j] = Wij[j, i] = 0.5
+        gij = fem.energy_contraction(Chat, shape, h, u, np.eye(6), Wij) / volume
+        assert float(gij @ s) == pytest.approx(float((Wij * CH).sum()), rel=1e-10)
+
+
 # ---- KATs of the reference CPU backend (tests/test_solver.py) -------------------
 def test_homogeneous_cell_kat():
     r = H.homogenize_direct(np.full((2, 3, 4), 17, dtype=np.int32), {17: dict(kind="isotropic", E=3500., nu=.35)},

@@ -3,6 +3,9 @@
 
 #include <cmath>
 #include <cstring>
+#include <atomic>
+#include <algorithm>
+#include <thread>
 
 namespace vxl {
 
@@ -307,6 +310,18 @@ int vxl_center_fields(vxl_cell* cell, int nload, const double* u_dev, const doub
   return VXL_OK;
 }
 
+int vxl_energy_gradient(vxl_cell* cell, int nload, const double* u_dev, const double* macro_host,
+                        const double* w_host, const double* d21_dev, double* out_dev) {
+  g_error.clear();
+  if (!cell || nload < 1 || nload > 64 || !u_dev || !macro_host || !w_host || !out_dev)
+    return fail("vxl_energy_gradient: invalid arguments");
+  for (int i = 0; i < nload; ++i)
+    for (int j = 0; j < nload; ++j)
+      if (!std::isfinite(w_host[i * nload + j]) || w_host[i * nload + j] != w_host[j * nload + i])
+        return fail("vxl_energy_gradient: weights must be a finite symmetric nload x nload matrix");
+  return energy_gradient(cell, nload, u_dev, macro_host, w_host, d21_dev, out_dev, cell->stream);
+}
+
 int vxl_project_zero_mean(vxl_cell* cell, int nrhs, double* u_dev) {
   g_error.clear();
   if (!cell || !u_dev || nrhs < 1) return fail("vxl_project_zero_mean: invalid arguments");
@@ -384,6 +399,53 @@ int vxl_solver_run(vxl_solver* s, const double* macro_host, double tol, int max_
   if (!(tol > 0 && tol < 1)) return fail("tol must lie in (0, 1)");
   if (max_iter < 1) return fail("max_iter must be >= 1");
   return solver_run(s, macro_host, tol, max_iter, ch_host, iterations, restarts, residuals, u_dev_out);
+}
+
+int vxl_solver_run_batch(vxl_solver* const* solvers, int count, const double* macro_host, double tol, int max_iter,
+                         double* ch_host, int* iterations, int* restarts, double* residuals, int workers) {
+  g_error.clear();
+  if (!solvers || count < 1 || !macro_host || !ch_host || workers < 1)
+    return fail("vxl_solver_run_batch: invalid arguments");
+  if (!(tol > 0 && tol < 1)) return fail("tol must lie in (0, 1)");
+  if (max_iter < 1) return fail("max_iter must be >= 1");
+  std::vector<int64_t> first(count + 1, 0);  // first load of each solver in the packed arrays
+  for (int i = 0; i < count; ++i) {
+    if (!solvers[i]) return fail("vxl_solver_run_batch: null solver");
+    first[i + 1] = first[i] + solvers[i]->R;
+  }
+  // Largest cells first (LPT): a worker that takes a small cell late finishes soon.
+  std::vector<int> order(count);
+  for (int i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return solvers[a]->cell->grid.n * solvers[a]->R > solvers[b]->cell->grid.n * solvers[b]->R;
+  });
+  std::atomic<int> next{0};
+  std::vector<int> rc(count, VXL_OK);
+  std::vector<std::string> err(count);
+  auto worker = [&]() {
+    for (;;) {
+      const int k = next.fetch_add(1);
+      if (k >= count) return;
+      const int i = order[k];
+      vxl_solver* sv = solvers[i];
+      (void)hipSetDevice(sv->cell->device);
+      const int64_t o = first[i];
+      rc[i] = solver_run(sv, macro_host + 6 * o, tol, max_iter, ch_host + 6 * o, iterations ? iterations + o : nullptr,
+                         restarts ? restarts + o : nullptr, residuals ? residuals + o : nullptr, nullptr);
+      if (rc[i] != VXL_OK) err[i] = vxl_last_error();
+    }
+  };
+  const int nt = std::min(workers, count);
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& t : pool) t.join();
+  for (int i = 0; i < count; ++i)
+    if (rc[i] != VXL_OK) {
+      g_error = "cell " + std::to_string(i) + ": " + err[i];
+      return rc[i];
+    }
+  return VXL_OK;
 }
 
 int vxl_solver_timings(vxl_solver* s, double* ms3) {

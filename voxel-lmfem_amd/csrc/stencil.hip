@@ -38,19 +38,10 @@ ModalFactors modal_factors(const double h[3]) {
 
 namespace {
 
-__host__ __device__ constexpr int pidx(int i, int j) {
-  return i <= j ? i * 6 - (i * (i - 1)) / 2 + (j - i) : j * 6 - (j * (j - 1)) / 2 + (i - j);
-}
-
 template <typename T>
 struct G9 {
   T g[3][3];  // [class][axis]
 };
-
-template <typename Idx>
-__device__ __forceinline__ int load_row(const void* index, int64_t v) {
-  return int(static_cast<const Idx*>(index)[v]);
-}
 
 // s = C e restricted to the strain components in MASK (bit k = Voigt k).
 template <int MASK, typename T>

@@ -22,6 +22,16 @@ namespace vxl {
 
 constexpr int kRowWidth = 24;  // padded C21 row
 
+// Position of C(i, j) in a 21-coefficient upper-triangle row (i <= j, row major).
+__host__ __device__ constexpr int pidx(int i, int j) {
+  return i <= j ? i * 6 - (i * (i - 1)) / 2 + (j - i) : j * 6 - (j * (j - 1)) / 2 + (i - j);
+}
+
+template <typename Idx>
+__device__ __forceinline__ int load_row(const void* index, int64_t v) {
+  return int(static_cast<const Idx*>(index)[v]);
+}
+
 void set_error(const std::string& message);
 int fail(const std::string& message);  // sets the error text, returns VXL_ERR_*
 
@@ -145,6 +155,10 @@ int launch_avg_stress(const vxl_cell* cell, int nload, const double* u, const do
                       double* partial, double* out_dev, hipStream_t s);
 int64_t stress_partial_size(const vxl_cell* cell, int nload);
 int default_stencil_variant(int precision);
+// out[e] = sum_ij W_ij int_e (E_i + B u_i)^T D_e (E_j + B u_j) dV with D_e the
+// cell's coefficient row of e, or d21[e] when d21 != nullptr (adjoint.hip).
+int energy_gradient(const vxl_cell* cell, int nload, const double* u, const double* macro_host,
+                    const double* w_host, const double* d21, double* out, hipStream_t s);
 // Label -> row index on the device with label presence and the Zeman lattice
 // samples (material.hip).
 int label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,

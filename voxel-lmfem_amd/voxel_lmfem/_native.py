@@ -42,12 +42,14 @@ SIGNATURES = {
     "vxl_rhs": (_I, [_P, _I, _DP, _P]),
     "vxl_avg_stress": (_I, [_P, _I, _P, _DP, _DP]),
     "vxl_center_fields": (_I, [_P, _I, _P, _DP, _P, _P, _P]),
+    "vxl_energy_gradient": (_I, [_P, _I, _P, _DP, _DP, _P, _P]),
     "vxl_project_zero_mean": (_I, [_P, _I, _P]),
     "vxl_precond_create": (_I, [_PP, _P, _I, _D, _D, _I]),
     "vxl_precond_apply": (_I, [_P, _I, _P, _P]),
     "vxl_precond_destroy": (_I, [_P]),
     "vxl_solver_create": (_I, [_PP, _P, _I, _I, _D, _D]),
     "vxl_solver_run": (_I, [_P, _DP, _D, _I, _DP, _IP, _IP, _DP, _P]),
+    "vxl_solver_run_batch": (_I, [_PP, _I, _DP, _D, _I, _DP, _IP, _IP, _DP, _I]),
     "vxl_solver_timings": (_I, [_P, _DP]),
     "vxl_solver_trace": (_I, [_P, _I, _IP, _DP, _I]),
     "vxl_solver_destroy": (_I, [_P]),

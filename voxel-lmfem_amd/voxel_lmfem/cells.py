@@ -190,3 +190,43 @@ def woven_like(n=128, seed=2, tows=4, section=(4, 12), amplitude=6.0):
     materials = {0: dict(kind="isotropic", E=3.5, nu=.35), 1: dict(tow), 2: dict(tow)}
     return (phase, materials, np.ones(3), dict(axis1=axis1, axis2=axis2)), dict(
         tow_fraction=float((phase > 0).mean()))
+
+
+def simp_density(n=96, sigma=4.0, seed=5, E_min=1e-3, E_max=70.0, nu=0.3, p=3.0):
+    """SIMP density cell: rho in [0, 1] from seeded white noise N(0,1)
+    convolved periodically with a Gaussian (sigma voxels), min-max scaled; the
+    stiffness is E(rho) = E_min + rho^p (E_max - E_min), nu fixed.
+
+    Returns (rho (n,n,n), unit-modulus isotropic 6x6 C1 with E = 1, E(rho),
+    dE/drho); C_e = E(rho_e) C1.
+    """
+    from .materials import isotropic_voigt
+    rng = np.random.default_rng(seed)
+    noise = rng.normal(size=(n, n, n))
+    k = np.fft.fftfreq(n)
+    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2
+    field = np.fft.ifftn(np.fft.fftn(noise) * np.exp(-2.0 * (np.pi * sigma) ** 2 * k2)).real
+    rho = (field - field.min()) / (field.max() - field.min())
+    E = E_min + rho ** p * (E_max - E_min)
+    dE = p * rho ** (p - 1) * (E_max - E_min)
+    return rho, isotropic_voigt(1.0, nu), E, dE
+
+
+def random_inclusion_batch(count=64, seed=7, sizes=(32, 48, 64, 80, 96), fractions=(0.1, 0.4), radii=(2, 5),
+                           contrasts=(10.0, 1000.0), nu=0.3, E_matrix=1.0):
+    """Uneven batch of two-phase random-inclusion cells: per cell n ~ U(sizes),
+    volume fraction ~ U[fractions], sphere radius ~ U{radii}, stiffness
+    contrast E_inc/E_mat ~ log-uniform in `contrasts`, the same nu for both
+    phases.  Returns a list of ((phase, materials, domain, None), meta)."""
+    rng = np.random.default_rng(seed)
+    cells = []
+    for k in range(count):
+        n = int(rng.choice(sizes))
+        vf = float(rng.uniform(*fractions))
+        radius = float(rng.integers(radii[0], radii[1] + 1))
+        contrast = float(np.exp(rng.uniform(np.log(contrasts[0]), np.log(contrasts[1]))))
+        (phase, mats, dom, _), meta = periodic_spheres(n, radius=radius, fraction=vf, seed=seed * 1000 + k,
+                                                       E=(E_matrix, E_matrix * contrast), nu=(nu, nu))
+        meta.update(n=n, radius=radius, contrast=contrast, target_fraction=vf)
+        cells.append(((phase, mats, dom, None), meta))
+    return cells

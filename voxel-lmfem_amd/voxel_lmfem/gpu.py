@@ -23,15 +23,14 @@ def _torch():
     return torch
 
 
-def _stream(torch):
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-
-
 class Cell:
     """Grid + material field resident on the current device."""
 
-    def __init__(self, field, shape, h, *, scale=None, device=None):
+    def __init__(self, field, shape, h, *, scale=None, device=None, stream=None):
+        """stream: torch.cuda.Stream the cell (and everything built on it) runs
+        on; default the current stream."""
         torch = _torch()
+        self.stream = torch.cuda.current_stream() if stream is None else stream
         self.lib = _native.load()
         self.torch = torch
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
@@ -52,13 +51,15 @@ class Cell:
                                          device=self.device)
             if self.scale.numel() != self.n:
                 raise ValueError("scale must have one value per voxel")
+        if stream is not None:  # uploads above ran on the current stream
+            torch.cuda.current_stream().synchronize()
         rows, rows_p = darray(field.rows.reshape(-1, 21))
         handle = ctypes.c_void_p()
         nz, ny, nx = self.shape
         check(self.lib.vxl_cell_create(ctypes.byref(handle), nz, ny, nx, self.h[0], self.h[1], self.h[2],
                                        field.index_kind, dptr(self.index), len(rows), rows_p,
                                        dptr(self.scale) if self.scale is not None else None,
-                                       _stream(torch)), "vxl_cell_create")
+                                       ctypes.c_void_p(self.stream.cuda_stream)), "vxl_cell_create")
         self.handle = handle
         self.mesh = SimpleNamespace(Nz=nz, Ny=ny, Nx=nx, dx=self.h[0], dy=self.h[1], dz=self.h[2],
                                     n_elem=self.n, n_nodes=self.n, n_dofs=3 * self.n,
@@ -283,3 +284,35 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def run_batch(solvers, macro=None, *, tol=1e-8, max_iter=10000, workers=4):
+    """Run independent solvers (each on its own cell and stream) concurrently
+    from `workers` host threads, largest cells first; returns one dict per
+    solver with CH (6 x R), iterations, restarts and true_residuals."""
+    if not solvers:
+        return []
+    lib = solvers[0].cell.lib
+    Rs = [s.nrhs for s in solvers]
+    total = sum(Rs)
+    if macro is None:
+        macro = np.concatenate([np.eye(6)[:R] for R in Rs])
+    macro, mp = darray(macro)
+    if macro.shape != (total, 6):
+        raise ValueError("macro strains must stack (R_i, 6) blocks of every solver")
+    handles = (ctypes.c_void_p * len(solvers))(*[s.handle.value for s in solvers])
+    ch = np.empty(6 * total)
+    its = (ctypes.c_int * total)()
+    rst = (ctypes.c_int * total)()
+    res = (ctypes.c_double * total)()
+    for s in solvers:
+        s.cell.stream.synchronize()
+    check(lib.vxl_solver_run_batch(handles, len(solvers), mp, float(tol), int(max_iter),
+                                   ch.ctypes.data_as(_native._DP), its, rst, res, int(workers)),
+          "vxl_solver_run_batch")
+    out, o = [], 0
+    for R in Rs:
+        out.append(dict(CH=ch[6 * o:6 * (o + R)].reshape(6, R).copy(), iterations=list(its[o:o + R]),
+                        restarts=list(rst[o:o + R]), true_residuals=list(res[o:o + R])))
+        o += R
+    return out

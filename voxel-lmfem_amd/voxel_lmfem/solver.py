@@ -155,3 +155,41 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="g
     result["elapsed_seconds"] = time.perf_counter() - start
     result["voigt_order"] = ["11", "22", "33", "23", "13", "12"]
     return result
+
+
+def homogenize_batch(cells, *, precision="mixed", tol=TOL, streams=4, workers=4):
+    """C_eff of many global-frame cells of uneven size, solved concurrently.
+
+    cells: iterable of (phase_id, materials, domain_size).  Each cell gets the
+    device label front end, its own reference medium, and a solver on one of
+    `streams` HIP streams; ``vxl_solver_run_batch`` then drives them from
+    `workers` host threads, largest first, so small cells overlap on the GPU.
+    Returns one dict per cell (CH, calls, max_true_residual, grid_shape).
+    """
+    import torch
+    from .gpu import Cell, Solver, run_batch
+    pool = [torch.cuda.Stream() for _ in range(max(1, int(streams)))]
+    built = []
+    try:
+        for k, (phase_id, materials, domain_size) in enumerate(cells):
+            phase, table, domain, orientation, labels, tensors, local = validate(phase_id, materials, domain_size)
+            if local.any():
+                raise ValueError("homogenize_batch takes global-frame cells; use homogenize for local frames")
+            field, h, lam, mu, meta = _prepare_device(phase, table, domain)
+            cell = Cell(field, phase.shape, h, stream=pool[k % len(pool)])
+            built.append((cell, Solver(cell, lam, mu, mode=precision, nrhs=6), phase.shape, meta))
+        outs = run_batch([b[1] for b in built], tol=tol, workers=workers)
+    finally:
+        for cell, solver, _, _ in built:
+            solver.close()
+            cell.close()
+    results = []
+    for (cell, solver, shape, meta), out in zip(built, outs):
+        residuals = [float(r) for r in out["true_residuals"]]
+        if not np.isfinite(residuals).all() or max(residuals) > tol:
+            raise RuntimeError(f"True-residual check failed: {residuals}")
+        results.append(dict(CH=out["CH"].tolist(), grid_shape=list(shape), max_true_residual=max(residuals),
+                            calls=[dict(iterations=int(i), restarts=int(r), true_residuals=[float(t)])
+                                   for i, r, t in zip(out["iterations"], out["restarts"], out["true_residuals"])],
+                            reference_medium=meta, voigt_order=["11", "22", "33", "23", "13", "12"]))
+    return results
