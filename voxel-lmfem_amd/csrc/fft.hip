@@ -234,7 +234,7 @@ struct Tables {
   const typename CT<T>::type* wx;
   const typename CT<T>::type* wy;
   const typename CT<T>::type* wz;
-  const T* gx;  // 4 x nx: m, D, v, d
+  const T* gx;  // 4 x (nx/2+1): 1/m, D, v, d
   const T* gy;
   const T* gz;
 };
@@ -400,9 +400,15 @@ fft_y(typename CT<T>::type* __restrict__ S, int ny, int nz, int W, int TJ, Axes 
   }
 }
 
+// Reciprocal for the Green solve: the hardware estimate (<= 1 ulp) in FP32,
+// exact division in FP64.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+
 // Green solve at one wavevector (kx, ky, kz), in place:
 //   out = 1/(t N) [q/E - gamma (v/E) (v . q/E)],  E_i = mu sum D + (lam+mu) d_i,
-//   gamma = (lam+mu)/(1 + (lam+mu) sum v_i^2/E_i),  t N = V N m_x m_y m_z;
+//   gamma = (lam+mu)/(1 + (lam+mu) sum v_i^2/E_i),  t N = V N m_x m_y m_z
+// (the axis tables hold 1/m, D, v, d);
 // k = 0 (sum D = 0) maps to 0.  Returns Re(conj(q) . out) in FP64.
 template <typename T, typename C>
 __device__ __forceinline__ double green_point(C& q0, C& q1, C& q2, const Tables<T>& tb, int kx, int nxh, T my,
@@ -415,11 +421,11 @@ __device__ __forceinline__ double green_point(C& q0, C& q1, C& q2, const Tables<
     return 0.0;
   }
   const T lm = lam + mu;
-  const T i0 = T(1) / (mu * sumD + lm * dx);
-  const T i1 = T(1) / (mu * sumD + lm * dy);
-  const T i2 = T(1) / (mu * sumD + lm * dz);
-  const T gamma = lm / (T(1) + lm * (vx * vx * i0 + vy * vy * i1 + vz * vz * i2));
-  const T scale = inv_vn / (mx * my * mz);
+  const T i0 = rcp(mu * sumD + lm * dx);
+  const T i1 = rcp(mu * sumD + lm * dy);
+  const T i2 = rcp(mu * sumD + lm * dz);
+  const T gamma = lm * rcp(T(1) + lm * (vx * vx * i0 + vy * vy * i1 + vz * vz * i2));
+  const T scale = inv_vn * mx * my * mz;  // tables hold 1/m
   const C a0 = {q0.x * i0, q0.y * i0}, a1 = {q1.x * i1, q1.y * i1}, a2 = {q2.x * i2, q2.y * i2};
   const C sdot = {vx * a0.x + vy * a1.x + vz * a2.x, vx * a0.y + vy * a1.y + vz * a2.y};
   const T w0 = gamma * vx * i0, w1 = gamma * vy * i1, w2 = gamma * vz * i2;
@@ -671,6 +677,16 @@ fft_z4_green(typename CT<T>::type* __restrict__ S, int ny, int nxh, int TK, Tabl
 // LDS: the N1 x N2 step-1 tile of pencil p at p * TS + j1 * (N2 + 1) + t
 // (row padding: step-2 reads run down a column), and the natural-order
 // spectrum at p * (n + 1) + e.
+template <typename T>
+struct XUpdate {
+  T* r;
+  T* x;
+  const T* p;
+  const T* q;
+  const double* alpha;
+  double* partial;
+};
+
 template <int N1, int N2>
 struct XTile {
   static constexpr int n = N1 * N2;
@@ -679,15 +695,19 @@ struct XTile {
   static constexpr int stride = TS > NS ? TS : NS;
 };
 
-template <typename T, int N1, int N2>
+// kUpd: the CG update r -= alpha q, x += alpha p is applied to every entry as it
+// is loaded (r and x written back), the updated r is transformed, and the
+// FP64 ||r||^2 of the workgroup goes to upd.partial[rhs * gridDim.x + blockIdx.x].
+template <typename T, int N1, int N2, bool kUpd>
 __global__ void __launch_bounds__(256)
 fft_x4_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int64_t rows, int PP,
            const typename CT<T>::type* __restrict__ w, int64_t field_stride, int64_t spec_stride,
-           const int* active) {
+           const int* active, XUpdate<T> upd) {
   using C = typename CT<T>::type;
   using XT = XTile<N1, N2>;
   constexpr int n = XT::n, nxh = n / 2 + 1, lanes = FourStep<N1, N2>::lanes;
   extern __shared__ double2 smem_raw[];
+  __shared__ double red[4];
   if (active && !active[blockIdx.y]) return;
   C* L = reinterpret_cast<C*>(smem_raw);
   const int P = 3 * PP;
@@ -704,12 +724,26 @@ fft_x4_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int64_
   const bool re_ok = live && (q == 1 ? rho + 1 < rows : rho < rows);
   const bool im_ok = live && rho + (q == 0 ? 0 : 1) < rows;
   C* T1 = L + p * XT::stride;
+  double rr = 0.0;
+  const int64_t base = int64_t(blockIdx.y) * field_stride;
+  const T alpha = kUpd ? T(upd.alpha[blockIdx.y]) : T(0);
+  auto fetch = [&](int64_t i) -> T {
+    if constexpr (kUpd) {
+      const T v = fma(-alpha, upd.q[base + i], upd.r[base + i]);
+      upd.r[base + i] = v;
+      upd.x[base + i] = fma(alpha, upd.p[base + i], upd.x[base + i]);
+      rr += double(v) * double(v);
+      return v;
+    } else {
+      return f[i];
+    }
+  };
   if (live && t < N2) {
     C v[N1];
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {
       const int e = t + N2 * k1;
-      v[k1] = C{re_ok ? f[re_off + e] : T(0), im_ok ? f[im_off + e] : T(0)};
+      v[k1] = C{re_ok ? fetch(re_off + e) : T(0), im_ok ? fetch(im_off + e) : T(0)};
     }
     dft_reg<N1, false, T>(v, w, N2);
 #pragma unroll
@@ -757,6 +791,10 @@ fft_x4_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int64_
       o[1] = sp[1][1];
       o[2] = sp[2][1];
     }
+  }
+  if constexpr (kUpd) {
+    const double total = block_reduce_dyn(rr, red);
+    if (threadIdx.x == 0) upd.partial[int64_t(blockIdx.y) * gridDim.x + blockIdx.x] = total;
   }
 }
 
@@ -931,14 +969,14 @@ size_t x4_stride(int n) {
   return 0;
 }
 
-template <typename T>
+template <typename T, bool kUpd>
 void launch_x4_r2c(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const T* r, typename CT<T>::type* S,
                    int64_t rows, int PP, const typename CT<T>::type* w, int64_t fstride, int64_t sstride,
-                   const int* active) {
-#define VXL_X4R(N, A, B)                                                                                   \
-  case N:                                                                                                \
-    hipLaunchKernelGGL((fft_x4_r2c<T, A, B>), grid, dim3(thr), lds, s, r, S, rows, PP, w, fstride, sstride, \
-                       active);                                                                          \
+                   const int* active, const XUpdate<T>& upd) {
+#define VXL_X4R(N, A, B)                                                                                     \
+  case N:                                                                                                  \
+    hipLaunchKernelGGL((fft_x4_r2c<T, A, B, kUpd>), grid, dim3(thr), lds, s, r, S, rows, PP, w, fstride,     \
+                       sstride, active, upd);                                                              \
     break;
   switch (n) { VXL_X4_CASES(VXL_X4R) }
 #undef VXL_X4R
@@ -978,7 +1016,7 @@ int upload_tables(PrecondImpl* im, const double h[3]) {
       const double sh = std::sin(0.5 * th);
       const double m = (2.0 + std::cos(th)) / 3.0;
       const double hh = h[a];
-      tab[k] = T(m);
+      tab[k] = T(1.0 / m);  // 1/m: the Green scale is a product, no division
       tab[cnt + k] = T(4.0 * sh * sh / (hh * hh * m));
       tab[2 * cnt + k] = T(std::sin(th) / (hh * m));
       tab[3 * cnt + k] = T(4.0 * sh * sh * sh * sh / (3.0 * hh * hh * m * m));
@@ -1090,9 +1128,13 @@ struct CgHooks {
   std::function<int()> mid;
 };
 
+enum Stages { kForwardX = 1, kRest = 2, kAllStages = 3 };
+
+// Runs the x forward pass (kForwardX, optionally fused with the CG update upd,
+// four-step x passes only) and/or the remaining passes (kRest).
 template <typename T>
 int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s,
-               const int* active, const CgHooks* cg) {
+               const int* active, const CgHooks* cg, int stages = kAllStages, const XUpdate<T>* upd = nullptr) {
   using C = typename CT<T>::type;
   const Tables<T> tb = tables_of<T>(im);
   C* S = static_cast<C*>(im->spec);
@@ -1101,13 +1143,20 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
   const unsigned bx = unsigned((pairs + im->pairs_per_block - 1) / im->pairs_per_block);
   const int W = 3 * im->nxh;
   const unsigned bx4 = im->x4 ? unsigned((pairs + im->pp4 - 1) / im->pp4) : 0u;
-  if (im->x4)
-    launch_x4_r2c<T>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, S, im->rows, im->pp4, tb.wx, fstride,
-                     im->spec_stride, active);
-  else
-    hipLaunchKernelGGL((fft_x_r2c<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx, im->rows,
-                       im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active);
-  VXL_HIP(hipGetLastError());
+  if (stages & kForwardX) {
+    if (upd && !im->x4) return fail("fused CG update needs the four-step x pass");
+    if (upd)
+      launch_x4_r2c<T, true>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, S, im->rows, im->pp4, tb.wx,
+                             fstride, im->spec_stride, active, *upd);
+    else if (im->x4)
+      launch_x4_r2c<T, false>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, S, im->rows, im->pp4, tb.wx,
+                              fstride, im->spec_stride, active, XUpdate<T>{});
+    else
+      hipLaunchKernelGGL((fft_x_r2c<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx, im->rows,
+                         im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active);
+    VXL_HIP(hipGetLastError());
+  }
+  if (!(stages & kRest)) return VXL_OK;
   auto y_pass = [&](bool inverse) -> int {
     if (im->y4) {
       const dim3 g((W + im->tj4 - 1) / im->tj4, im->nz, nrhs);
@@ -1198,6 +1247,42 @@ template int precond_apply_cg<float>(vxl_precond*, int, const float*, float*, do
                                      const std::function<int()>&, const int*, hipStream_t);
 template int precond_apply_cg<double>(vxl_precond*, int, const double*, double*, double*, const double*,
                                       const std::function<int()>&, const int*, hipStream_t);
+
+template <typename T>
+int precond_cg_forward(vxl_precond* pc, int nrhs, T* r, T* x, const T* p, const T* q, const double* alpha,
+                       double* rr_partial, const int* active, hipStream_t s) {
+  auto* im = static_cast<PrecondImpl*>(pc->impl);
+  if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
+  if ((pc->precision == VXL_F64) != std::is_same<T, double>::value) return fail("precond precision mismatch");
+  if (!x) return apply_impl<T>(pc, im, nrhs, r, nullptr, s, active, nullptr, kForwardX);
+  const XUpdate<T> upd{r, x, p, q, alpha, rr_partial};
+  return apply_impl<T>(pc, im, nrhs, r, nullptr, s, active, nullptr, kForwardX, &upd);
+}
+template int precond_cg_forward<float>(vxl_precond*, int, float*, float*, const float*, const float*, const double*,
+                                       double*, const int*, hipStream_t);
+template int precond_cg_forward<double>(vxl_precond*, int, double*, double*, const double*, const double*,
+                                        const double*, double*, const int*, hipStream_t);
+
+template <typename T>
+int precond_cg_rest(vxl_precond* pc, int nrhs, T* p, double* rz_partial, const double* beta,
+                    const std::function<int()>& mid, const int* active, hipStream_t s) {
+  auto* im = static_cast<PrecondImpl*>(pc->impl);
+  if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
+  if ((pc->precision == VXL_F64) != std::is_same<T, double>::value) return fail("precond precision mismatch");
+  const CgHooks cg{rz_partial, beta, mid};
+  return apply_impl<T>(pc, im, nrhs, static_cast<const T*>(nullptr), p, s, active, &cg, kRest);
+}
+template int precond_cg_rest<float>(vxl_precond*, int, float*, double*, const double*, const std::function<int()>&,
+                                    const int*, hipStream_t);
+template int precond_cg_rest<double>(vxl_precond*, int, double*, double*, const double*,
+                                     const std::function<int()>&, const int*, hipStream_t);
+
+int precond_fused_update_blocks(const vxl_precond* pc) {
+  const auto* im = static_cast<const PrecondImpl*>(pc->impl);
+  if (!im->x4) return 0;
+  const int64_t pairs = (im->rows + 1) / 2;
+  return int((pairs + im->pp4 - 1) / im->pp4);
+}
 
 int precond_apply_masked(vxl_precond* pc, int nrhs, const void* r, void* z, const int* active,
                          hipStream_t s) {

@@ -39,7 +39,7 @@ struct Inner {
 int free_all(vxl_solver* s) {
   if (s->cell) (void)hipStreamSynchronize(s->cell->stream);
   double* bufs[] = {s->f, s->x64, s->r64, s->q64, s->scal, s->partial, s->macro_dev,
-                    s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz, s->rhs_sig};
+                    s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz, s->rhs_sig, s->partial_fx};
   for (double* b : bufs)
     if (b) dev_free(b);
   if (s->mode == VXL_SOLVE_MIXED) {
@@ -186,12 +186,24 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   // r -= alpha q, ||r||^2] [latch] (host check / FP64 certification)
   // [FFT passes of M r; the z pass reduces r.z] [beta] [last pass:
   // p = M r + beta p].
+  // With four-step x passes the x/r update and ||r||^2 ride on the forward x
+  // pass of the next preconditioner apply (its input is the updated r); a
+  // certification that replaces r repeats that pass on the new r.
+  const int nblk_fx = precond_fused_update_blocks(&s->pc);
   while (any_active() && it < max_iter) {
     VXL_TRY(launch_apply_cg<T>(cell, R, p, q, act, s->partial_st, nullptr, st));
     VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st));
-    VXL_TRY(launch_update_xr<T>(x, r, p, q, len, R, scal + kAlpha * R, partial, act, st));
-    ++it;
-    VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
+    bool forward_done = false;
+    if (nblk_fx > 0) {
+      VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, x, p, q, scal + kAlpha * R, s->partial_fx, act, st));
+      ++it;
+      VXL_TRY(launch_finalize(s->partial_fx, nblk_fx, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
+      forward_done = true;
+    } else {
+      VXL_TRY(launch_update_xr<T>(x, r, p, q, len, R, scal + kAlpha * R, partial, act, st));
+      ++it;
+      VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
+    }
     if (it == 1 || it % kCheckStride == 0 || it == max_iter) {
       VXL_TRY(read_slot(kRR, rr));
       VXL_HIP(hipMemcpyAsync(pin_stop, s->stop_dev, sizeof(int) * R, hipMemcpyDeviceToHost, st));
@@ -215,6 +227,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
           for (int k = 0; k < R; ++k) which[k] = active[k];
         }
         VXL_TRY(certify(which));
+        forward_done = false;  // r was replaced
         // A latched system skipped the iterations since its latch, so its
         // search direction is stale: restart it (beta = 0) if it continues.
         for (int k = 0; k < R; ++k)
@@ -224,7 +237,10 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
       VXL_TRY(upload_flags());
       if (!any_active()) break;
     }
-    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
+    if (!forward_done)
+      VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, nullptr, nullptr, nullptr, act,
+                                    st));
+    VXL_TRY(precond_cg_rest<T>(&s->pc, R, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
     if (std::any_of(restart.begin(), restart.end(), [](int v) { return v; })) {
       std::fill(restart.begin(), restart.end(), 0);
       VXL_TRY(upload_flags());
@@ -316,6 +332,8 @@ int solver_create(vxl_solver* s) {
   s->pc_ready = true;
   VXL_TRY(precond_create(&s->pc));
   VXL_HIP(dev_malloc(&s->partial_rz, sizeof(double) * size_t(precond_rz_blocks(&s->pc)) * R));
+  if (const int fx = precond_fused_update_blocks(&s->pc))
+    VXL_HIP(dev_malloc(&s->partial_fx, sizeof(double) * size_t(fx) * R));
   return VXL_OK;
 }
 

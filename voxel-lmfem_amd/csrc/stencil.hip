@@ -427,9 +427,9 @@ apply_kernel_v2(ApplyArgs<T> a) {
   T Bf[3][4], Ftp[3][4];
   // Element of layer z from its bottom (Bf) and top (Tf) face modes; returns the
   // bottom-face force (Fb) and top-face force (Ft) in face-mode form.
-  auto element = [&](int z, const T (&Tf)[3][4], T (&Fb)[3][4], T (&Ft)[3][4]) {
+  auto row_of = [&](int z) { return load_row<Idx>(a.index, int64_t(wrapmod(z, nz)) * nxy + e_xy); };
+  auto element = [&](int z, int row, const T (&Tf)[3][4], T (&Fb)[3][4], T (&Ft)[3][4]) {
     const int64_t voxel = int64_t(wrapmod(z, nz)) * nxy + e_xy;
-    const int row = load_row<Idx>(a.index, voxel);
     const T* crow = a.table + int64_t(row) * kRowWidth;
     T c[21];
 #pragma unroll
@@ -501,7 +501,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
   if (is_elem) {
     T Tf[3][4], Fb[3][4];
     face_modes(Tf);
-    element(z0 - 1, Tf, Fb, Ftp);
+    element(z0 - 1, row_of(z0 - 1), Tf, Fb, Ftp);
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
@@ -512,8 +512,13 @@ apply_kernel_v2(ApplyArgs<T> a) {
     load_plane(z0 + 1);
     __syncthreads();
   }
+  // The material row of the next layer is fetched one step ahead, so the
+  // label -> row -> coefficients chain never stalls a step on two latencies.
+  int row_next = is_elem ? row_of(z0) : 0;
   for (int k = 0; z0 + k < z1; ++k) {
     const int z = z0 + k;
+    const int row_cur = row_next;
+    if (is_elem && z + 1 < z1) row_next = row_of(z + 1);
     const bool more = kPF && z + 1 < z1;  // another step follows: it needs plane z+2
     if constexpr (!kPF) {
       load_plane(z + 1);
@@ -525,7 +530,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
     if (more) prefetch(z + 2);
     if (is_elem) {
       T Fb[3][4], Ft[3][4];
-      element(z, Tf, Fb, Ft);
+      element(z, row_cur, Tf, Fb, Ft);
       T w = T(0);  // kFuse: u_e . f_e of this element
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {

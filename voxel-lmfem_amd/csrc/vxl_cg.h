@@ -60,6 +60,20 @@ template <typename T>
 int precond_apply_cg(vxl_precond* pc, int nrhs, const T* r, T* p, double* rz_partial, const double* beta,
                      const std::function<int()>& mid, const int* active, hipStream_t s);
 int precond_rz_blocks(const vxl_precond* pc);
+// The two halves of precond_apply_cg, for a caller that acts between them.
+// Forward x pass of r; with x != nullptr it first applies r -= alpha q,
+// x += alpha p to every entry it loads and writes FP64 ||r||^2 partials
+// (rr_partial[rhs * precond_fused_update_blocks + b]); that form needs the
+// four-step x pass (precond_fused_update_blocks > 0).
+template <typename T>
+int precond_cg_forward(vxl_precond* pc, int nrhs, T* r, T* x, const T* p, const T* q, const double* alpha,
+                       double* rr_partial, const int* active, hipStream_t s);
+// y, z (+ Green, r.z), mid(), y^-1 and the x inverse pass writing p = z + beta p.
+template <typename T>
+int precond_cg_rest(vxl_precond* pc, int nrhs, T* p, double* rz_partial, const double* beta,
+                    const std::function<int()>& mid, const int* active, hipStream_t s);
+// Workgroups per RHS of the fused forward pass, 0 if it is not available.
+int precond_fused_update_blocks(const vxl_precond* pc);
 // z = M r for the systems with active[k] != 0 only.
 int precond_apply_masked(vxl_precond* pc, int nrhs, const void* r, void* z, const int* active,
                          hipStream_t s);
@@ -80,6 +94,7 @@ struct vxl_solver {
   double* partial;
   double* partial_st;  // stencil p.q partials
   double* partial_rz;  // FFT z-pass r.z partials (Parseval)
+  double* partial_fx;  // ||r||^2 partials of the fused forward x pass (four-step x only)
   int nblk;
   int* active_dev;
   int* restart_dev;
