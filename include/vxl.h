@@ -61,7 +61,9 @@ int vxl_release_cached_memory(void);
 /* Material front end on the device: index_dev[e] = lut_host[labels[e] - label_lo]
  * over a resident label array (label_bytes 4 = int32, 8 = int64; C order
  * z, y, x), where lut_host (span entries) holds each label's coefficient row
- * or -1 for labels without a material.  Returns present_host[span] (1 where
+ * or -1 for labels without a material, or -2 for a local-frame phase whose
+ * voxel e gets its own row local_base + e (see vxl_local_rows; int32 index).
+ * Returns present_host[span] (1 where
  * the label occurs), *undefined_host (1 if a voxel's label has no row; its
  * index is then 0) and samples_host, the rows at the Zeman sampling lattice
  * (every max(1, n/16)-th voxel per axis, C order,
@@ -70,8 +72,26 @@ int vxl_release_cached_memory(void);
  * rows in phase_table order, material_field.py:57-68) and the sampling of
  * _compute_reference_medium (hybrid_adapter.py:101-125). */
 int vxl_label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,
-                   const int* lut_host, int index_kind, void* index_dev, int* present_host, int* undefined_host,
-                   int* samples_host, void* stream);
+                   const int* lut_host, int index_kind, void* index_dev, long long local_base, int* present_host,
+                   int* undefined_host, int* samples_host, void* stream);
+
+/* Local-frame phases on the device (the per-voxel half of
+ * build_hybrid_material, hybrid_material.py:49-61, with the frame rules of
+ * _orientation_values, material_field.py:71-102, and the Bond rotation of
+ * _rotate_packed, material_field.py:105-114).  A lookup entry of -2 in
+ * vxl_label_rows gives voxel e the row local_base + e (int32 index); this call
+ * fills those rows: for every voxel whose label has slot_host[label - lo] =
+ * p >= 0, rows_dev[(local_base + e) * 21 ...] = Bond(R_e) C_p Bond(R_e)^T with
+ * C_p = c21_host[p * 21 ...].  mode 1: R_e from the axis pair a_dev, b_dev
+ * (columns a, c x a, c with c = a x b normalised); mode 2: a_dev holds R_e
+ * row major.  stride is 0 (one frame for every voxel) or 3 (mode 1) / 9
+ * (mode 2) doubles per voxel.  bad_host[5] flags, as the host checks raise:
+ * 0 nonfinite, 1 zero axis, 2 collinear axes, 3 not orthonormal (1e-10),
+ * 4 det != +1 (1e-10).  Synchronises `stream`. */
+int vxl_local_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,
+                   const int* slot_host, int phases, const double* c21_host, int mode, const double* a_dev,
+                   const double* b_dev, long long stride, double* rows_dev, long long local_base, int* bad_host,
+                   void* stream);
 
 /* Cell: grid, voxel size and material field.
  * Replaces the operator construction of the reference's hybrid FI storage:
@@ -89,6 +109,11 @@ int vxl_label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_d
 int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy, double hz,
                     int index_kind, const void* index_dev, int rows, const double* c21_host,
                     const float* scale_dev, void* stream);
+/* vxl_cell_create with the rows x 21 FP64 coefficient table already on the
+ * device (c21_dev, caller-owned, read once). */
+int vxl_cell_create_device_rows(vxl_cell** out, int nz, int ny, int nx, double hx, double hy, double hz,
+                                int index_kind, const void* index_dev, int rows, const double* c21_dev,
+                                const float* scale_dev, void* stream);
 int vxl_cell_destroy(vxl_cell* cell);
 
 /* out = K u for nrhs fields (precision VXL_F32/VXL_F64), component-plane

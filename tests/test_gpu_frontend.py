@@ -110,3 +110,65 @@ def test_homogenize_field_statistics(torch_gpu):
         mix = sum(frac[k] * v for k, v in s["phase_mean_energy"].items())
         assert mix == pytest.approx(s["mean_energy"], rel=1e-12)
     assert out["slices"]["energy_load0"].shape == phase.shape[1:]
+
+
+def _voxel_tensors_from_device(field):
+    rows = field.rows_dev.cpu().numpy()
+    idx = field.index_dev.cpu().numpy().astype(np.int64)
+    return pm.unpack21(rows[idx])
+
+
+@pytest.mark.parametrize("kind", ["woven_axis_pairs", "fixture_matrices", "single_matrix"])
+def test_device_local_frames_equal_host_rows(torch_gpu, kind):
+    torch = torch_gpu
+    from test_oracle import gpu_fixture_cell
+    from voxel_lmfem.cells import woven_like
+    print(f"case local frames {kind}", flush=True)
+    if kind == "woven_axis_pairs":
+        (phase, mats, dom, ori), _ = woven_like(32, section=(4, 8), amplitude=3.0)
+    elif kind == "fixture_matrices":
+        phase, mats, dom, ori = gpu_fixture_cell()
+    else:
+        phase, mats, dom, _ = gpu_fixture_cell()
+        ori = np.array([[0., -1., 0.], [1., 0., 0.], [0., 0., 1.]])
+    host = pm.build_field(phase, mats, ori)
+    want = pm.unpack21(host.rows[host.index.astype(np.int64)])
+    dev = pm.build_field_device(torch.tensor(np.asarray(phase).astype(np.int32), device="cuda"), mats, ori)
+    got = _voxel_tensors_from_device(dev)
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() <= 1e-14 * scale
+    np.testing.assert_array_equal(dev.used_labels(), np.unique(phase))
+
+
+@pytest.mark.parametrize("orientation,message", [
+    (dict(axis1=[1., 0., 0.], axis2=[2., 0., 0.]), "collinear"),
+    (dict(axis1=[0., 0., 0.], axis2=[0., 1., 0.]), "nonzero"),
+    (np.ones((3, 3)), "orthonormal"),
+    (np.diag([1., 1., -1.]), "right-handed"),
+])
+def test_device_local_frames_reject_bad_frames(torch_gpu, orientation, message):
+    torch = torch_gpu
+    phase = np.zeros((2, 2, 3), dtype=np.int32)
+    mats = {0: dict(kind="anisotropic", C=np.diag([10., 20., 30., 4., 5., 6.]).tolist(), frame="local")}
+    with pytest.raises(ValueError, match=message):
+        pm.build_field_device(torch.tensor(phase, device="cuda"), mats, orientation)
+
+
+def test_woven_solve_device_front_end_matches_host_field(torch_gpu):
+    torch = torch_gpu
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import woven_like
+    from voxel_lmfem.gpu import Cell, Solver
+    from voxel_lmfem.reference_medium import select
+    (phase, mats, dom, ori), _ = woven_like(32, section=(4, 8), amplitude=3.0)
+    out = homogenize(phase, mats, dom, orientation=ori, precision="fp64")
+    assert out["storage"]["front_end"] == "device"
+    field = pm.build_field(phase, mats, ori)
+    lam, mu, _ = select(field, phase, mats)
+    h = dom / np.array(phase.shape[::-1], dtype=float)
+    cell = Cell(field, phase.shape, h)
+    solver = Solver(cell, lam, mu, mode="fp64", nrhs=6)
+    CH = solver.run()["CH"]
+    solver.close()
+    cell.close()
+    assert np.linalg.norm(np.asarray(out["CH"]) - CH) <= 1e-11 * np.linalg.norm(CH)

@@ -84,6 +84,83 @@ int check_index(vxl_cell* c) {
   return VXL_OK;
 }
 
+// Shared part of the two cell constructors: argument checks, the handle, and
+// the padded FP64/FP32 tables filled by `fill` (host upload or device pad).
+template <typename Fill>
+int cell_create_common(vxl_cell** out, int nz, int ny, int nx, double hx, double hy, double hz, int index_kind,
+                       const void* index_dev, int rows, const float* scale_dev, void* stream, Fill fill) {
+  g_error.clear();
+  if (!out) return fail("out handle is NULL");
+  *out = nullptr;
+  if (nz < 2 || ny < 2 || nx < 2) return fail("grid dimensions must be >= 2");
+  if (int64_t(nz) * ny * nx > (int64_t(1) << 40) || int64_t(ny) * nx > INT32_MAX) return fail("grid too large");
+  if (!(hx > 0 && hy > 0 && hz > 0) || !std::isfinite(hx * hy * hz))
+    return fail("voxel sizes must be finite and positive");
+  if (index_kind != VXL_INDEX_U8 && index_kind != VXL_INDEX_U16 && index_kind != VXL_INDEX_I32)
+    return fail("index_kind must be VXL_INDEX_U8, VXL_INDEX_U16 or VXL_INDEX_I32");
+  if (!index_dev || rows < 1) return fail("index, coefficient table and rows are required");
+  auto* c = new vxl_cell();
+  c->grid.nx = nx;
+  c->grid.ny = ny;
+  c->grid.nz = nz;
+  c->grid.n = int64_t(nx) * ny * nz;
+  c->h[0] = hx;
+  c->h[1] = hy;
+  c->h[2] = hz;
+  c->volume = hx * hy * hz;
+  c->rows = rows;
+  c->stream = static_cast<hipStream_t>(stream);
+  c->mat.index_kind = index_kind;
+  c->mat.index = index_dev;
+  c->mat.scale = scale_dev;
+  c->c64_owned = nullptr;
+  c->c32_owned = nullptr;
+  c->variant[0] = default_stencil_variant(VXL_F32);
+  c->variant[1] = default_stencil_variant(VXL_F64);
+  (void)hipGetDevice(&c->device);
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(c->stream);
+    if (c->c64_owned) dev_free(c->c64_owned);
+    if (c->c32_owned) dev_free(c->c32_owned);
+    delete c;
+  };
+  if (dev_malloc(&c->c64_owned, size_t(rows) * kRowWidth * sizeof(double)) != hipSuccess ||
+      dev_malloc(&c->c32_owned, size_t(rows) * kRowWidth * sizeof(float)) != hipSuccess) {
+    cleanup();
+    g_error = "hipMalloc failed for the coefficient table";
+    return VXL_ERR_HIP;
+  }
+  int rc = fill(c);
+  if (rc == VXL_OK) {
+    c->mat.c64 = c->c64_owned;
+    c->mat.c32 = c->c32_owned;
+    rc = check_index(c);
+  }
+  if (rc != VXL_OK) {
+    std::string msg = g_error;
+    cleanup();
+    g_error = msg;
+    return rc;
+  }
+  *out = c;
+  return VXL_OK;
+}
+
+// 21-coefficient FP64 rows -> padded FP64 and FP32 tables; bad[0] = 1 on a
+// nonfinite coefficient.
+__global__ void __launch_bounds__(256)
+pad_rows_kernel(const double* c21, int64_t rows, double* t64, float* t32, int* bad) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < rows * kRowWidth;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = i / kRowWidth;
+    const int k = int(i - r * kRowWidth);
+    const double v = k < 21 ? c21[r * 21 + k] : 0.0;
+    if (!isfinite(v)) bad[0] = 1;
+    t64[i] = v;
+    t32[i] = float(v);
+  }
+}
+
 }  // namespace
 }  // namespace vxl
 
@@ -107,91 +184,92 @@ int vxl_release_cached_memory(void) {
 }
 
 int vxl_label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,
-                   const int* lut_host, int index_kind, void* index_dev, int* present_host, int* undefined_host,
-                   int* samples_host, void* stream) {
+                   const int* lut_host, int index_kind, void* index_dev, long long local_base, int* present_host,
+                   int* undefined_host, int* samples_host, void* stream) {
   g_error.clear();
   if (nz < 2 || ny < 2 || nx < 2 || (label_bytes != 4 && label_bytes != 8) || !labels_dev || span < 1 ||
       span > (1 << 24) || !lut_host || !index_dev || !present_host || !undefined_host || !samples_host ||
       (index_kind != VXL_INDEX_U8 && index_kind != VXL_INDEX_U16 && index_kind != VXL_INDEX_I32))
     return fail("vxl_label_rows: invalid arguments");
   int max_row = -1;
-  for (int i = 0; i < span; ++i) max_row = std::max(max_row, lut_host[i]);
+  bool local = false;
+  for (int i = 0; i < span; ++i) {
+    if (lut_host[i] < -2) return fail("vxl_label_rows: lookup entries must be rows, -1 or -2");
+    max_row = std::max(max_row, lut_host[i]);
+    local = local || lut_host[i] == -2;
+  }
+  const int64_t n = int64_t(nz) * ny * nx;
+  if (local) {
+    if (index_kind != VXL_INDEX_I32 || local_base <= max_row || local_base + n > INT32_MAX)
+      return fail("vxl_label_rows: local-frame rows need an int32 index and local_base above the phase rows");
+  }
   const int limit = index_kind == VXL_INDEX_U8 ? 255 : index_kind == VXL_INDEX_U16 ? 65535 : INT32_MAX;
   if (max_row > limit) return fail("vxl_label_rows: row index does not fit the index kind");
   return label_rows(nz, ny, nx, label_bytes, labels_dev, label_lo, span, lut_host, index_kind, index_dev,
-                    present_host, undefined_host, samples_host, static_cast<hipStream_t>(stream));
+                    present_host, undefined_host, samples_host, local_base, static_cast<hipStream_t>(stream));
+}
+
+int vxl_local_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,
+                   const int* slot_host, int phases, const double* c21_host, int mode, const double* a_dev,
+                   const double* b_dev, long long stride, double* rows_dev, long long local_base, int* bad_host,
+                   void* stream) {
+  g_error.clear();
+  if (nz < 2 || ny < 2 || nx < 2 || (label_bytes != 4 && label_bytes != 8) || !labels_dev || span < 1 ||
+      span > (1 << 24) || !slot_host || phases < 1 || !c21_host || (mode != 1 && mode != 2) || !a_dev ||
+      (mode == 1 && !b_dev) || (stride != 0 && stride != (mode == 1 ? 3 : 9)) || !rows_dev || local_base < 0 ||
+      !bad_host)
+    return fail("vxl_local_rows: invalid arguments");
+  for (int i = 0; i < span; ++i)
+    if (slot_host[i] < -1 || slot_host[i] >= phases) return fail("vxl_local_rows: phase slot out of range");
+  return local_rows(int64_t(nz) * ny * nx, label_bytes, labels_dev, label_lo, span, slot_host, phases, c21_host,
+                    mode, a_dev, b_dev, stride, rows_dev, local_base, bad_host, static_cast<hipStream_t>(stream));
 }
 
 int vxl_cell_create(vxl_cell** out, int nz, int ny, int nx, double hx, double hy, double hz,
                     int index_kind, const void* index_dev, int rows, const double* c21_host,
                     const float* scale_dev, void* stream) {
-  g_error.clear();
-  if (!out) return fail("out handle is NULL");
-  *out = nullptr;
-  if (nz < 2 || ny < 2 || nx < 2) return fail("grid dimensions must be >= 2");
-  if (int64_t(nz) * ny * nx > (int64_t(1) << 40) || int64_t(ny) * nx > INT32_MAX) return fail("grid too large");
-  if (!(hx > 0 && hy > 0 && hz > 0) || !std::isfinite(hx * hy * hz))
-    return fail("voxel sizes must be finite and positive");
-  if (index_kind != VXL_INDEX_U8 && index_kind != VXL_INDEX_U16 && index_kind != VXL_INDEX_I32)
-    return fail("index_kind must be VXL_INDEX_U8, VXL_INDEX_U16 or VXL_INDEX_I32");
-  if (!index_dev || !c21_host || rows < 1) return fail("index, coefficient table and rows are required");
-  for (int64_t i = 0; i < int64_t(rows) * 21; ++i)
-    if (!std::isfinite(c21_host[i])) return fail("coefficient table contains nonfinite values");
-  auto* c = new vxl_cell();
-  c->grid.nx = nx;
-  c->grid.ny = ny;
-  c->grid.nz = nz;
-  c->grid.n = int64_t(nx) * ny * nz;
-  c->h[0] = hx;
-  c->h[1] = hy;
-  c->h[2] = hz;
-  c->volume = hx * hy * hz;
-  c->rows = rows;
-  c->stream = static_cast<hipStream_t>(stream);
-  c->mat.index_kind = index_kind;
-  c->mat.index = index_dev;
-  c->mat.scale = scale_dev;
-  c->c64_owned = nullptr;
-  c->c32_owned = nullptr;
-  c->variant[0] = default_stencil_variant(VXL_F32);
-  c->variant[1] = default_stencil_variant(VXL_F64);
-  (void)hipGetDevice(&c->device);
-  std::vector<double> t64(size_t(rows) * kRowWidth, 0.0);
-  std::vector<float> t32(size_t(rows) * kRowWidth, 0.0f);
-  for (int r = 0; r < rows; ++r)
-    for (int k = 0; k < 21; ++k) {
-      t64[size_t(r) * kRowWidth + k] = c21_host[size_t(r) * 21 + k];
-      t32[size_t(r) * kRowWidth + k] = float(c21_host[size_t(r) * 21 + k]);
-    }
-  int rc = VXL_OK;
-  auto cleanup = [&]() {
-    if (c->c64_owned) dev_free(c->c64_owned);
-    if (c->c32_owned) dev_free(c->c32_owned);
-    delete c;
-  };
-  if (dev_malloc(&c->c64_owned, t64.size() * sizeof(double)) != hipSuccess ||
-      dev_malloc(&c->c32_owned, t32.size() * sizeof(float)) != hipSuccess) {
-    cleanup();
-    g_error = "hipMalloc failed for the coefficient table";
-    return VXL_ERR_HIP;
-  }
-  if (hipMemcpy(c->c64_owned, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->c32_owned, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
-    cleanup();
-    g_error = "hipMemcpy failed for the coefficient table";
-    return VXL_ERR_HIP;
-  }
-  c->mat.c64 = c->c64_owned;
-  c->mat.c32 = c->c32_owned;
-  rc = check_index(c);
-  if (rc != VXL_OK) {
-    std::string msg = g_error;
-    cleanup();
-    g_error = msg;
-    return rc;
-  }
-  *out = c;
-  return VXL_OK;
+  return cell_create_common(out, nz, ny, nx, hx, hy, hz, index_kind, index_dev, rows, scale_dev, stream,
+                            [&](vxl_cell* c) -> int {
+                              if (!c21_host) return fail("index, coefficient table and rows are required");
+                              for (int64_t i = 0; i < int64_t(rows) * 21; ++i)
+                                if (!std::isfinite(c21_host[i]))
+                                  return fail("coefficient table contains nonfinite values");
+                              std::vector<double> t64(size_t(rows) * kRowWidth, 0.0);
+                              std::vector<float> t32(size_t(rows) * kRowWidth, 0.0f);
+                              for (int r = 0; r < rows; ++r)
+                                for (int k = 0; k < 21; ++k) {
+                                  t64[size_t(r) * kRowWidth + k] = c21_host[size_t(r) * 21 + k];
+                                  t32[size_t(r) * kRowWidth + k] = float(c21_host[size_t(r) * 21 + k]);
+                                }
+                              VXL_HIP(hipMemcpy(c->c64_owned, t64.data(), t64.size() * sizeof(double),
+                                                hipMemcpyHostToDevice));
+                              VXL_HIP(hipMemcpy(c->c32_owned, t32.data(), t32.size() * sizeof(float),
+                                                hipMemcpyHostToDevice));
+                              return VXL_OK;
+                            });
+}
+
+int vxl_cell_create_device_rows(vxl_cell** out, int nz, int ny, int nx, double hx, double hy, double hz,
+                                int index_kind, const void* index_dev, int rows, const double* c21_dev,
+                                const float* scale_dev, void* stream) {
+  return cell_create_common(out, nz, ny, nx, hx, hy, hz, index_kind, index_dev, rows, scale_dev, stream,
+                            [&](vxl_cell* c) -> int {
+                              if (!c21_dev) return fail("index, coefficient table and rows are required");
+                              DevBuf flag(c->stream);
+                              VXL_HIP(dev_malloc(&flag.p, sizeof(int)));
+                              VXL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), c->stream));
+                              const int64_t total = int64_t(rows) * kRowWidth;
+                              const unsigned blocks = unsigned(std::min<int64_t>((total + 255) / 256, 65536));
+                              hipLaunchKernelGGL(pad_rows_kernel, dim3(blocks), dim3(256), 0, c->stream, c21_dev,
+                                                 int64_t(rows), static_cast<double*>(c->c64_owned),
+                                                 static_cast<float*>(c->c32_owned), static_cast<int*>(flag.p));
+                              VXL_HIP(hipGetLastError());
+                              int bad = 0;
+                              VXL_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+                              VXL_HIP(hipStreamSynchronize(c->stream));
+                              if (bad) return fail("coefficient table contains nonfinite values");
+                              return VXL_OK;
+                            });
 }
 
 int vxl_cell_destroy(vxl_cell* cell) {

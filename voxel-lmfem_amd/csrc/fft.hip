@@ -727,11 +727,17 @@ fft_x4_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int64_
   double rr = 0.0;
   const int64_t base = int64_t(blockIdx.y) * field_stride;
   const T alpha = kUpd ? T(upd.alpha[blockIdx.y]) : T(0);
+  // Distinct fields: restrict lets the loads of later entries run ahead of
+  // the stores of earlier ones.
+  T* __restrict__ ur = kUpd ? upd.r + base : nullptr;
+  T* __restrict__ ux = kUpd ? upd.x + base : nullptr;
+  const T* __restrict__ up = kUpd ? upd.p + base : nullptr;
+  const T* __restrict__ uq = kUpd ? upd.q + base : nullptr;
   auto fetch = [&](int64_t i) -> T {
     if constexpr (kUpd) {
-      const T v = fma(-alpha, upd.q[base + i], upd.r[base + i]);
-      upd.r[base + i] = v;
-      upd.x[base + i] = fma(alpha, upd.p[base + i], upd.x[base + i]);
+      const T v = fma(-alpha, uq[i], ur[i]);
+      ur[i] = v;
+      ux[i] = fma(alpha, up[i], ux[i]);
       rr += double(v) * double(v);
       return v;
     } else {
@@ -1279,7 +1285,8 @@ template int precond_cg_rest<double>(vxl_precond*, int, double*, double*, const 
 
 int precond_fused_update_blocks(const vxl_precond* pc) {
   const auto* im = static_cast<const PrecondImpl*>(pc->impl);
-  if (!im->x4) return 0;
+  const char* env = std::getenv("VXL_FUSED_UPDATE");
+  if (!im->x4 || (env && env[0] == '0')) return 0;
   const int64_t pairs = (im->rows + 1) / 2;
   return int((pairs + im->pp4 - 1) / im->pp4);
 }

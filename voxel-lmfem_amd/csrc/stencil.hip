@@ -514,11 +514,16 @@ apply_kernel_v2(ApplyArgs<T> a) {
   }
   // The material row of the next layer is fetched one step ahead, so the
   // label -> row -> coefficients chain never stalls a step on two latencies.
-  int row_next = is_elem ? row_of(z0) : 0;
+  // (Plain form only: the CG form is at its register limit.)
+  int row_next = (!kFuse && is_elem) ? row_of(z0) : 0;
   for (int k = 0; z0 + k < z1; ++k) {
     const int z = z0 + k;
-    const int row_cur = row_next;
-    if (is_elem && z + 1 < z1) row_next = row_of(z + 1);
+    int row_cur = row_next;
+    if constexpr (kFuse) {
+      if (is_elem) row_cur = row_of(z);
+    } else {
+      if (is_elem && z + 1 < z1) row_next = row_of(z + 1);
+    }
     const bool more = kPF && z + 1 < z1;  // another step follows: it needs plane z+2
     if constexpr (!kPF) {
       load_plane(z + 1);

@@ -163,7 +163,11 @@ int energy_gradient(const vxl_cell* cell, int nload, const double* u, const doub
 // samples (material.hip).
 int label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,
                const int* lut_host, int index_kind, void* index_dev, int* present_host, int* undefined_host,
-               int* samples_host, hipStream_t s);
+               int* samples_host, int64_t local_base, hipStream_t s);
+// Bond-rotated rows of local-frame voxels at rows_dev[(local_base + e) * 21] (material.hip).
+int local_rows(int64_t n, int label_bytes, const void* labels_dev, long long label_lo, int span, const int* slot_host,
+               int phases, const double* c21_host, int mode, const double* a_dev, const double* b_dev, int64_t stride,
+               double* rows_dev, int64_t local_base, int* bad_host, hipStream_t s);
 // Element-centre fields (any output may be null): strain = E + B0 u_e and
 // stress = C_e strain as planes [r][6][N], energy = strain . stress / 2 as [r][N].
 int launch_center_fields(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,

@@ -32,7 +32,11 @@ SIGNATURES = {
     "vxl_version": (_I, []),
     "vxl_cached_bytes": (_I, [ctypes.POINTER(ctypes.c_uint64)]),
     "vxl_release_cached_memory": (_I, []),
-    "vxl_label_rows": (_I, [_I, _I, _I, _I, _P, ctypes.c_longlong, _I, _IP, _I, _P, _IP, _IP, _IP, _P]),
+    "vxl_label_rows": (_I, [_I, _I, _I, _I, _P, ctypes.c_longlong, _I, _IP, _I, _P, ctypes.c_longlong, _IP, _IP,
+                            _IP, _P]),
+    "vxl_local_rows": (_I, [_I, _I, _I, _I, _P, ctypes.c_longlong, _I, _IP, _I, _DP, _I, _P, _P, ctypes.c_longlong,
+                            _P, ctypes.c_longlong, _IP, _P]),
+    "vxl_cell_create_device_rows": (_I, [_PP, _I, _I, _I, _D, _D, _D, _I, _P, _I, _P, _P, _P]),
     "vxl_cell_create": (_I, [_PP, _I, _I, _I, _D, _D, _D, _I, _P, _I, _DP, _P, _P]),
     "vxl_cell_destroy": (_I, [_P]),
     "vxl_apply": (_I, [_P, _I, _I, _P, _P]),

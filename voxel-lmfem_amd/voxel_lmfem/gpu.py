@@ -53,13 +53,20 @@ class Cell:
                 raise ValueError("scale must have one value per voxel")
         if stream is not None:  # uploads above ran on the current stream
             torch.cuda.current_stream().synchronize()
-        rows, rows_p = darray(field.rows.reshape(-1, 21))
         handle = ctypes.c_void_p()
         nz, ny, nx = self.shape
-        check(self.lib.vxl_cell_create(ctypes.byref(handle), nz, ny, nx, self.h[0], self.h[1], self.h[2],
-                                       field.index_kind, dptr(self.index), len(rows), rows_p,
-                                       dptr(self.scale) if self.scale is not None else None,
-                                       ctypes.c_void_p(self.stream.cuda_stream)), "vxl_cell_create")
+        scale_p = dptr(self.scale) if self.scale is not None else None
+        rows_dev = getattr(field, "rows_dev", None)
+        if rows_dev is not None:   # row table built on the device
+            check(self.lib.vxl_cell_create_device_rows(
+                ctypes.byref(handle), nz, ny, nx, self.h[0], self.h[1], self.h[2], field.index_kind,
+                dptr(self.index), int(rows_dev.shape[0]), dptr(rows_dev), scale_p,
+                ctypes.c_void_p(self.stream.cuda_stream)), "vxl_cell_create_device_rows")
+        else:
+            rows, rows_p = darray(field.rows.reshape(-1, 21))
+            check(self.lib.vxl_cell_create(ctypes.byref(handle), nz, ny, nx, self.h[0], self.h[1], self.h[2],
+                                           field.index_kind, dptr(self.index), len(rows), rows_p, scale_p,
+                                           ctypes.c_void_p(self.stream.cuda_stream)), "vxl_cell_create")
         self.handle = handle
         self.mesh = SimpleNamespace(Nz=nz, Ny=ny, Nx=nx, dx=self.h[0], dy=self.h[1], dz=self.h[2],
                                     n_elem=self.n, n_nodes=self.n, n_dofs=3 * self.n,

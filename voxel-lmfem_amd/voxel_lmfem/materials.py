@@ -200,22 +200,36 @@ class MaterialField:
 
 class DeviceField(MaterialField):
     """MaterialField whose row index lives on the GPU (built by libvxl from a
-    resident label array); the host keeps the row table, the labels present in
-    the cell and the Zeman lattice rows."""
+    resident label array); the host keeps the phase rows, the labels present
+    in the cell and the Zeman lattice rows.  With local-frame phases the full
+    row table (phase rows, then one Bond-rotated row per voxel) is `rows_dev`
+    on the device."""
 
-    def __init__(self, rows, index_dev, kind, labels, tensors, local, present, samples):
+    def __init__(self, rows, index_dev, kind, labels, tensors, local, present, samples, rows_dev=None):
         super().__init__(rows, None, labels, tensors, local)
         self.index_dev = index_dev
         self.kind = kind
         self.present = present
         self.samples = samples
+        self.rows_dev = rows_dev
+        self._lattice = None
 
     @property
     def index_kind(self):
         return self.kind
 
+    @property
+    def row_count(self):
+        return len(self.rows) if self.rows_dev is None else int(self.rows_dev.shape[0])
+
     def lattice_rows(self, shape):
-        return self.rows[self.samples]
+        if self.rows_dev is None:
+            return self.rows[self.samples]
+        if self._lattice is None:   # gather the sampled rows from the device table
+            import torch
+            pick = torch.as_tensor(self.samples.astype(np.int64), device=self.rows_dev.device)
+            self._lattice = self.rows_dev.index_select(0, pick).cpu().numpy()
+        return self._lattice
 
     def used_labels(self, phase=None):
         return self.present
@@ -232,50 +246,107 @@ def index_dtype(rows):
     raise ValueError("Coefficient table exceeds int32 indexing")
 
 
-def build_field_device(phase_dev, materials):
-    """Label bookkeeping of a cell whose int32/int64 labels are resident on the GPU.
+_FRAME_ERRORS = ("Active local-frame orientations must be finite", "Orientation axes must be nonzero",
+                 "Orientation axes must not be collinear", "Orientation must be an orthonormal rotation matrix",
+                 "Orientation must be right-handed (det=+1)")
 
-    Same rows and per-voxel row index as ``build_field`` for tables without
-    local-frame phases (rows = the phase tensors in ascending label order); the
-    index is computed on the device by ``vxl_label_rows``.  Raises the
-    reference's ValueError for labels without a material.
+
+def _device_frames(orientation, shape, torch, device):
+    """(mode, a_dev, b_dev, stride) for vxl_local_rows: one frame for all
+    voxels (stride 0) or one per voxel, from a rotation array or an axis pair
+    broadcastable to the grid (the host path's accepted forms)."""
+    def grid_array(values, tail):
+        a = np.asarray(values, dtype=np.float64)
+        if a.shape == tail:
+            return torch.as_tensor(np.ascontiguousarray(a), device=device), 0
+        try:
+            full = np.broadcast_to(a, tuple(shape) + tail)
+        except ValueError as exc:
+            raise ValueError("Orientation shape is not broadcastable to phase_id") from exc
+        return torch.as_tensor(np.ascontiguousarray(full.reshape(-1, *tail)), device=device), int(np.prod(tail))
+
+    if isinstance(orientation, Mapping):
+        a, sa = grid_array(orientation["axis1"], (3,))
+        b, sb = grid_array(orientation["axis2"], (3,))
+        if sa != sb:   # one of the two varies per voxel: expand the other too
+            a, sa = grid_array(np.broadcast_to(np.asarray(orientation["axis1"], float), tuple(shape) + (3,)), (3,))
+            b, sb = grid_array(np.broadcast_to(np.asarray(orientation["axis2"], float), tuple(shape) + (3,)), (3,))
+        return 1, a, b, sa
+    R, sr = grid_array(orientation, (3, 3))
+    return 2, R, None, sr
+
+
+def build_field_device(phase_dev, materials, orientation=None):
+    """Label and orientation bookkeeping of a cell whose int32/int64 labels are
+    resident on the GPU.
+
+    Global-frame phases: the same rows and per-voxel row index as
+    ``build_field`` (rows = phase tensors in ascending label order), index
+    computed by ``vxl_label_rows``.  Local-frame phases: every voxel of such a
+    phase gets its own Bond-rotated row, built on the device by
+    ``vxl_local_rows`` after the phase rows (int32 index), with the host
+    path's frame checks and messages.  Raises the reference's ValueError for
+    labels without a material.
     """
     import ctypes
     import torch
     from . import _native
     from ._native import check, dptr
     labels, tensors, local = phase_table(materials)
-    if local.any():
-        raise ValueError("build_field_device handles global-frame phases; use build_field for local frames")
     if phase_dev.dim() != 3 or phase_dev.dtype not in (torch.int32, torch.int64) or not phase_dev.is_cuda:
         raise ValueError("phase_dev must be a 3-D int32/int64 GPU tensor")
     phase_dev = phase_dev.contiguous()
     nz, ny, nx = (int(v) for v in phase_dev.shape)
+    n = nz * ny * nx
     lo = int(labels[0])
     span = int(labels[-1]) - lo + 1
     if span > (1 << 24):
         raise ValueError("label range too wide for the device lookup table; use build_field")
+    L = len(labels)
     lut = np.full(span, -1, dtype=np.int32)
-    lut[labels - lo] = np.arange(len(labels), dtype=np.int32)
-    dtype = index_dtype(len(labels))
+    lut[labels - lo] = np.arange(L, dtype=np.int32)
+    lut[labels[local] - lo] = -2
+    with_local = bool(local.any())
+    dtype = np.int32 if with_local else index_dtype(L)
     kind = {np.uint8: 1, np.uint16: 2, np.int32: 4}[dtype]
     tdtype = {np.uint8: torch.uint8, np.uint16: torch.int16, np.int32: torch.int32}[dtype]
-    index = torch.empty(nz * ny * nx, dtype=tdtype, device=phase_dev.device)
+    index = torch.empty(n, dtype=tdtype, device=phase_dev.device)
     present = np.zeros(span, dtype=np.int32)
     undefined = np.zeros(1, dtype=np.int32)
-    cz, cy, cx = (-(-n // max(1, n // 16)) for n in (nz, ny, nx))
+    cz, cy, cx = (-(-v // max(1, v // 16)) for v in (nz, ny, nx))
     samples = np.zeros(cz * cy * cx, dtype=np.int32)
     ip = ctypes.POINTER(ctypes.c_int)
     lib = _native.load()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     check(lib.vxl_label_rows(nz, ny, nx, phase_dev.element_size(), dptr(phase_dev), lo, span,
-                             lut.ctypes.data_as(ip), kind, dptr(index), present.ctypes.data_as(ip),
-                             undefined.ctypes.data_as(ip), samples.ctypes.data_as(ip),
-                             ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "vxl_label_rows")
+                             lut.ctypes.data_as(ip), kind, dptr(index), L, present.ctypes.data_as(ip),
+                             undefined.ctypes.data_as(ip), samples.ctypes.data_as(ip), stream), "vxl_label_rows")
     if undefined[0]:
         used = np.unique(phase_dev.cpu().numpy())
         raise ValueError(f"Undefined material labels: {used[~np.isin(used, labels)].tolist()}")
     used = labels[present[labels - lo] != 0]
-    return DeviceField(pack21(tensors), index, kind, labels, tensors, local, used, samples)
+    phase_rows = pack21(tensors)
+    rows_dev = None
+    if with_local:
+        if orientation is None and np.isin(labels[local], used).any():
+            raise ValueError("Local-frame anisotropic phases require orientation")
+        rows_dev = torch.zeros((L + n, 21), dtype=torch.float64, device=phase_dev.device)
+        rows_dev[:L] = torch.as_tensor(phase_rows, device=phase_dev.device)
+        if orientation is not None and np.isin(labels[local], used).any():
+            slot = np.full(span, -1, dtype=np.int32)
+            slot[labels[local] - lo] = np.arange(int(local.sum()), dtype=np.int32)
+            cloc = np.ascontiguousarray(phase_rows[local])
+            mode, a, b, stride = _device_frames(orientation, (nz, ny, nx), torch, phase_dev.device)
+            bad = np.zeros(5, dtype=np.int32)
+            check(lib.vxl_local_rows(nz, ny, nx, phase_dev.element_size(), dptr(phase_dev), lo, span,
+                                     slot.ctypes.data_as(ip), int(local.sum()),
+                                     cloc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), mode, dptr(a),
+                                     dptr(b) if b is not None else None, stride, dptr(rows_dev), L,
+                                     bad.ctypes.data_as(ip), stream), "vxl_local_rows")
+            for flag, message in zip(bad, _FRAME_ERRORS):
+                if flag:
+                    raise ValueError(message)
+    return DeviceField(phase_rows, index, kind, labels, tensors, local, used, samples, rows_dev)
 
 
 def build_field(phase, materials, orientation=None, *, deduplicate=True):

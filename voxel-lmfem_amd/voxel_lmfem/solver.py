@@ -24,20 +24,13 @@ GPU_BACKENDS = ("gpu", "hip", "cuda")
 TOL = 1e-8
 
 
-def _prepare(phase, table, domain, orientation):
-    """Host bookkeeping (local-frame cells): labels, frames and Bond-rotated rows."""
-    field = mat.build_field(phase, table, orientation)
-    h = domain / np.array(phase.shape[::-1], dtype=float)
-    lam, mu, meta = select(field, phase, table)
-    return field, h, lam, mu, meta
-
-
-def _prepare_device(phase, table, domain):
-    """Device bookkeeping (global-frame cells): labels uploaded once, rows on the GPU."""
+def _prepare_device(phase, table, domain, orientation=None):
+    """Device bookkeeping: labels uploaded once; label -> row, and the frames and
+    Bond-rotated rows of local-frame phases, computed on the GPU."""
     import torch
     small = phase.min() >= np.iinfo(np.int32).min and phase.max() <= np.iinfo(np.int32).max
     labels = torch.from_numpy(np.ascontiguousarray(phase, dtype=np.int32 if small else np.int64)).to("cuda")
-    field = mat.build_field_device(labels, table)
+    field = mat.build_field_device(labels, table, orientation)
     h = domain / np.array(phase.shape[::-1], dtype=float)
     lam, mu, meta = select(field, phase, table)
     return field, h, lam, mu, meta
@@ -76,10 +69,7 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
         stage.t = now
     stage.t = start
     torch.cuda.reset_peak_memory_stats()
-    if local.any():
-        field, h, lam, mu, meta = _prepare(phase, table, domain, orientation)
-    else:
-        field, h, lam, mu, meta = _prepare_device(phase, table, domain)
+    field, h, lam, mu, meta = _prepare_device(phase, table, domain, orientation)
     stage("material")
     cell = Cell(field, phase.shape, h)
     solver = Solver(cell, lam, mu, mode=precision, nrhs=6)
@@ -99,9 +89,10 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
                   field_statistics=field_stats, peak_allocated_bytes=int(torch.cuda.max_memory_allocated()),
                   peak_reserved_bytes=int(torch.cuda.max_memory_reserved()),
                   device_ms=out["device_ms"])
-    storage = dict(mode="label_rows", n_elements=int(phase.size), pool_rows=int(len(field.rows)),
+    rows = int(getattr(field, "row_count", len(field.rows)))
+    storage = dict(mode="label_rows", n_elements=int(phase.size), pool_rows=rows,
                    index_bytes=int(cell.index.numel() * cell.index.element_size()),
-                   coefficient_bytes=int(field.rows.size * 8),
+                   coefficient_bytes=rows * 21 * 8,
                    front_end="device" if getattr(field, "index_dev", None) is not None else "host")
     operator = dict(variant="modal_fi_lds_stencil", precision=precision, reference_medium=meta)
     solver.close()
