@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--preconditioner", default="fft", choices=["fft", "fft_bf16", "jacobi"],
+                    help="mixed-mode preconditioner of the timed solve (fft: FP32 spectra; fft_bf16: BF16-stored "
+                         "spectra; jacobi: 1/diag K control)")
     ap.add_argument("--cpu-sample", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--stencil-only", action="store_true",
                     help="only run 10 six-RHS FP32 stencil applies and a 1 GiB device copy "
@@ -152,7 +155,8 @@ def main():
         field = pm.build_field_device(labels, mats)
         lam, mu, rmeta = select(field, phase, mats)
         cell = Cell(field, phase.shape, h)
-        return cell, Solver(cell, lam, mu, mode=mode, nrhs=6), rmeta
+        kind = args.preconditioner if mode == "mixed" else "fft"
+        return cell, Solver(cell, lam, mu, mode=mode, nrhs=6, preconditioner=kind), rmeta
 
     def step(mode):
         """One full C_eff: device label front end, reference medium, operator and
@@ -263,7 +267,7 @@ def main():
         ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f32 inner + f64 refinement", data="synthetic",
         config=dict(workload=f"voronoi_polycrystal_{n}^3_200grains_cubic_6loads", grid=[n, n, n], grains=200,
-                    loads=6, precision="mixed", tol=1e-8, iterations=its,
+                    loads=6, precision="mixed", preconditioner=args.preconditioner, tol=1e-8, iterations=its,
                     step=("device label front end + reference medium + operator/preconditioner set-up + "
                           "RHS + block PCG + FP64 certificate + stress + teardown; labels resident in HBM"),
                     resident_operator_ms_per_solve=resident_ms, reference_medium=rmeta),

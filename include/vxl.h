@@ -34,10 +34,16 @@ enum {
 
 enum { VXL_INDEX_U8 = 1, VXL_INDEX_U16 = 2, VXL_INDEX_I32 = 4 };
 enum { VXL_F32 = 0, VXL_F64 = 1 };
+/* Preconditioner precision only: FP32 arithmetic with the spectrum stored as
+ * BF16 pairs between passes (power-of-two axes >= 8). */
+enum { VXL_F32_BF16 = 2 };
 /* Solve modes.  FP64: FP64 operator, FP64 FFT preconditioner, FP64 CG.
  * MIXED: FP32 operator, FP32 FFT preconditioner and FP32 CG with FP64
  * reliable updates (true residual f - K u, dots and stress in FP64). */
 enum { VXL_SOLVE_FP64 = 0, VXL_SOLVE_MIXED = 1 };
+/* Preconditioners of the solver: the FFT Green operator (default) or the
+ * scalar Jacobi control. */
+enum { VXL_PRECOND_FFT = 0, VXL_PRECOND_JACOBI = 1, VXL_PRECOND_FFT_BF16 = 2 };
 
 typedef struct vxl_cell vxl_cell;
 typedef struct vxl_precond vxl_precond;
@@ -181,7 +187,8 @@ int vxl_project_zero_mean(vxl_cell* cell, int nrhs, double* u_dev);
  * Replaces enable_fi_reference/fi_coeff (formula_adapter.py:40-68,
  * formula_kernels.cu:104-137, 199-208), contest prepare/spectral
  * (contest_kernels.cu:46-94) and NativeFFT (native_fft.cu:11-58).
- * precision: VXL_F32 or VXL_F64 (arithmetic and storage of the spectra). */
+ * precision: VXL_F32 or VXL_F64 (arithmetic and storage of the spectra), or
+ * VXL_F32_BF16. */
 int vxl_precond_create(vxl_precond** out, vxl_cell* cell, int precision, double lam, double mu,
                        int nrhs_max);
 /* r_dev, z_dev: nrhs fields in the precond's precision. */
@@ -204,6 +211,18 @@ int vxl_solver_create(vxl_solver** out, vxl_cell* cell, int mode, int nrhs, doub
 int vxl_solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter,
                    double* ch_host, int* iterations, int* restarts, double* residuals,
                    double* u_dev_out);
+/* out_dev (3 x N FP64 planes) = 1 / diag K, the operator's diagonal assembled
+ * from the 8 elements around each node (2x2x2 Gauss rule); the reference
+ * caches the same diagonal for its Jacobi control (op._diag_cache,
+ * chapter_aligned/study.py:84-87). */
+int vxl_cell_inverse_diagonal(vxl_cell* cell, double* out_dev);
+/* Selects the solver's preconditioner.  VXL_PRECOND_FFT_BF16 (mixed mode):
+ * the FFT Green preconditioner with its spectrum stored in BF16 between
+ * passes (VXL_F32_BF16).  VXL_PRECOND_JACOBI is the
+ * reference's control (repro/bench/chapter_aligned/study.py:84-90):
+ * z = project_zero_mean(r / diag K) with the assembled diagonal of K; the
+ * diagonal is assembled here on the device (once per solver). */
+int vxl_solver_set_preconditioner(vxl_solver* s, int kind);
 /* Runs count independent solvers (each bound to its own cell and stream)
  * concurrently from `workers` host threads, largest cells first, so small
  * cells overlap on the GPU instead of waiting on each other's host checks.

@@ -24,6 +24,7 @@ int solver_create(vxl_solver* s);
 int solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter, double* ch_host,
                int* iterations, int* restarts, double* residuals, double* u_out);
 int solver_release(vxl_solver* s);
+int solver_set_preconditioner(vxl_solver* s, int kind);
 
 namespace {
 
@@ -415,7 +416,8 @@ int vxl_precond_create(vxl_precond** out, vxl_cell* cell, int precision, double 
                        int nrhs_max) {
   g_error.clear();
   if (!out || !cell || nrhs_max < 1) return fail("vxl_precond_create: invalid arguments");
-  if (precision != VXL_F32 && precision != VXL_F64) return fail("precision must be VXL_F32 or VXL_F64");
+  if (precision != VXL_F32 && precision != VXL_F64 && precision != VXL_F32_BF16)
+    return fail("precision must be VXL_F32, VXL_F64 or VXL_F32_BF16");
   *out = nullptr;
   auto* pc = new vxl_precond();
   pc->cell = cell;
@@ -524,6 +526,23 @@ int vxl_solver_run_batch(vxl_solver* const* solvers, int count, const double* ma
       return rc[i];
     }
   return VXL_OK;
+}
+
+int vxl_cell_inverse_diagonal(vxl_cell* cell, double* out_dev) {
+  g_error.clear();
+  if (!cell || !out_dev) return fail("vxl_cell_inverse_diagonal: invalid arguments");
+  VXL_TRY(launch_jacobi_diag(cell, out_dev, nullptr, cell->stream));
+  VXL_HIP(hipStreamSynchronize(cell->stream));
+  return VXL_OK;
+}
+
+int vxl_solver_set_preconditioner(vxl_solver* s, int kind) {
+  g_error.clear();
+  if (!s || (kind != VXL_PRECOND_FFT && kind != VXL_PRECOND_JACOBI && kind != VXL_PRECOND_FFT_BF16))
+    return fail("vxl_solver_set_preconditioner: invalid arguments");
+  if (kind == VXL_PRECOND_FFT_BF16 && s->mode != VXL_SOLVE_MIXED)
+    return fail("the BF16-stored preconditioner is a mixed-mode option");
+  return solver_set_preconditioner(s, kind);
 }
 
 int vxl_solver_timings(vxl_solver* s, double* ms3) {

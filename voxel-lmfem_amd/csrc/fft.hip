@@ -64,6 +64,35 @@ template <typename T> struct CT;
 template <> struct CT<float> { using type = float2; };
 template <> struct CT<double> { using type = double2; };
 
+
+// Spectrum storage element: the compute complex type itself, or a pair of
+// BF16 values (FP32 arithmetic, half the spectrum traffic; round to nearest
+// even on store).
+struct bf16x2 {
+  unsigned short x, y;
+};
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+  const unsigned int b = __float_as_uint(f);
+  return static_cast<unsigned short>((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float from_bf16(unsigned short h) { return __uint_as_float(static_cast<unsigned int>(h) << 16); }
+template <typename C, typename SC>
+__device__ __forceinline__ C sload(const SC* p) {
+  if constexpr (std::is_same<SC, bf16x2>::value) {
+    const bf16x2 v = *p;
+    return C{from_bf16(v.x), from_bf16(v.y)};
+  } else {
+    return *p;
+  }
+}
+template <typename C, typename SC>
+__device__ __forceinline__ void sstore(SC* p, C v) {
+  if constexpr (std::is_same<SC, bf16x2>::value) {
+    *p = bf16x2{to_bf16(v.x), to_bf16(v.y)};
+  } else {
+    *p = v;
+  }
+}
 template <typename C>
 __device__ __forceinline__ C cadd(C a, C b) { return {a.x + b.x, a.y + b.y}; }
 template <typename C>
@@ -573,10 +602,10 @@ __device__ __forceinline__ void four_step_2(const C* L, int TJ, int p, int t, co
   for (int j2 = 0; j2 < N2; ++j2) store(t + N1 * j2, u[j2]);
 }
 
-template <bool INV, typename T, int N1, int N2>
+template <bool INV, typename T, int N1, int N2, typename SC>
 __global__ void __launch_bounds__(256)
-fft_y4(typename CT<T>::type* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict__ w,
-       int64_t spec_stride, const int* active) {
+fft_y4(SC* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict__ w, int64_t spec_stride,
+       const int* active) {
   using C = typename CT<T>::type;
   constexpr int n = N1 * N2;
   extern __shared__ double2 smem_raw[];
@@ -584,24 +613,24 @@ fft_y4(typename CT<T>::type* __restrict__ S, int W, int TJ, const typename CT<T>
   C* L = reinterpret_cast<C*>(smem_raw);
   const int j0 = blockIdx.x * TJ;
   const int tj = min(TJ, W - j0);
-  C* plane = S + int64_t(blockIdx.z) * spec_stride + int64_t(blockIdx.y) * n * W + j0;
+  SC* plane = S + int64_t(blockIdx.z) * spec_stride + int64_t(blockIdx.y) * n * W + j0;
   const int p = threadIdx.x % TJ, t = threadIdx.x / TJ;
   const bool col = p < tj;
   if (t < N2)
     four_step_1<INV, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C {
-      return col ? plane[int64_t(t + N2 * k1) * W + p] : C{T(0), T(0)};
+      return col ? sload<C>(plane + int64_t(t + N2 * k1) * W + p) : C{T(0), T(0)};
     });
   __syncthreads();
   if (t < N1 && col)
-    four_step_2<INV, T, N1, N2>(L, TJ, p, t, w, [&](int e, C v) { plane[int64_t(e) * W + p] = v; });
+    four_step_2<INV, T, N1, N2>(L, TJ, p, t, w, [&](int e, C v) { sstore(plane + int64_t(e) * W + p, v); });
 }
 
 // Launched with a multiple of 64 threads; threads past 3 TK lanes-per-column
 // only take part in the Green loop and the reduction.  kDot as in fft_z_green.
-template <typename T, int N1, int N2, bool kDot>
+template <typename T, int N1, int N2, bool kDot, typename SC>
 __global__ void __launch_bounds__(256)
-fft_z4_green(typename CT<T>::type* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T mu,
-             T inv_vn, int64_t spec_stride, const int* active, int nyq, double* partial) {
+fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T mu, T inv_vn,
+             int64_t spec_stride, const int* active, int nyq, double* partial) {
   using C = typename CT<T>::type;
   constexpr int nz = N1 * N2;
   extern __shared__ double2 smem_raw[];
@@ -614,14 +643,14 @@ fft_z4_green(typename CT<T>::type* __restrict__ S, int ny, int nxh, int TK, Tabl
   const int tk = min(TK, nxh - k0);
   const int W = 3 * nxh;
   const int64_t zstride = int64_t(ny) * W;
-  C* col = S + int64_t(blockIdx.z) * spec_stride + int64_t(y) * W + 3 * k0;
+  SC* col = S + int64_t(blockIdx.z) * spec_stride + int64_t(y) * W + 3 * k0;
   const int p = threadIdx.x % TJ, t = threadIdx.x / TJ;
   const bool live = p < 3 * tk;
   const C* w = tb.wz;
   // forward z transform: global -> L (natural kz order)
   if (t < N2)
     four_step_1<false, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C {
-      return live ? col[(t + N2 * k1) * zstride + p] : C{T(0), T(0)};
+      return live ? sload<C>(col + (t + N2 * k1) * zstride + p) : C{T(0), T(0)};
     });
   __syncthreads();
   C u[N2];
@@ -665,7 +694,7 @@ fft_z4_green(typename CT<T>::type* __restrict__ S, int ny, int nxh, int TK, Tabl
     four_step_1<true, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C { return v[k1]; });
   __syncthreads();
   if (t < N1 && live)
-    four_step_2<true, T, N1, N2>(L, TJ, p, t, w, [&](int e, C val) { col[e * zstride + p] = val; });
+    four_step_2<true, T, N1, N2>(L, TJ, p, t, w, [&](int e, C val) { sstore(col + e * zstride + p, val); });
 }
 
 // ---- four-step x passes for power-of-two nx -------------------------------
@@ -698,9 +727,9 @@ struct XTile {
 // kUpd: the CG update r -= alpha q, x += alpha p is applied to every entry as it
 // is loaded (r and x written back), the updated r is transformed, and the
 // FP64 ||r||^2 of the workgroup goes to upd.partial[rhs * gridDim.x + blockIdx.x].
-template <typename T, int N1, int N2, bool kUpd>
+template <typename T, int N1, int N2, bool kUpd, typename SC>
 __global__ void __launch_bounds__(256)
-fft_x4_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int64_t rows, int PP,
+fft_x4_r2c(const T* __restrict__ r, SC* __restrict__ S, int64_t rows, int PP,
            const typename CT<T>::type* __restrict__ w, int64_t field_stride, int64_t spec_stride,
            const int* active, XUpdate<T> upd) {
   using C = typename CT<T>::type;
@@ -715,7 +744,7 @@ fft_x4_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int64_
   const int pp = p / 3, q = p - 3 * pp;
   const int64_t plane = rows * n;
   const T* f = r + int64_t(blockIdx.y) * field_stride;
-  C* out = S + int64_t(blockIdx.y) * spec_stride;
+  SC* out = S + int64_t(blockIdx.y) * spec_stride;
   const int64_t rho = 2 * (int64_t(blockIdx.x) * PP + pp);
   const bool live = p < P;
   // component/row of the real and imaginary part of this pencil
@@ -786,16 +815,16 @@ fft_x4_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int64_
       sp[qq][1] = C{T(0.5) * df.y, T(-0.5) * df.x};
     }
     if (rr < rows) {
-      C* o = out + (rr * nxh + k) * 3;
-      o[0] = sp[0][0];
-      o[1] = sp[0][1];
-      o[2] = sp[2][0];
+      SC* o = out + (rr * nxh + k) * 3;
+      sstore(o + 0, sp[0][0]);
+      sstore(o + 1, sp[0][1]);
+      sstore(o + 2, sp[2][0]);
     }
     if (rr + 1 < rows) {
-      C* o = out + ((rr + 1) * nxh + k) * 3;
-      o[0] = sp[1][0];
-      o[1] = sp[1][1];
-      o[2] = sp[2][1];
+      SC* o = out + ((rr + 1) * nxh + k) * 3;
+      sstore(o + 0, sp[1][0]);
+      sstore(o + 1, sp[1][1]);
+      sstore(o + 2, sp[2][1]);
     }
   }
   if constexpr (kUpd) {
@@ -805,9 +834,9 @@ fft_x4_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int64_
 }
 
 // kDir as in fft_x_c2r: the output is p = z + beta[rhs] p, in place.
-template <typename T, int N1, int N2, bool kDir>
+template <typename T, int N1, int N2, bool kDir, typename SC>
 __global__ void __launch_bounds__(256)
-fft_x4_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int64_t rows, int PP,
+fft_x4_c2r(const SC* __restrict__ S, T* __restrict__ z, int64_t rows, int PP,
            const typename CT<T>::type* __restrict__ w, int64_t field_stride, int64_t spec_stride,
            const int* active, const double* beta) {
   using C = typename CT<T>::type;
@@ -817,7 +846,7 @@ fft_x4_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int64_
   if (active && !active[blockIdx.y]) return;
   C* L = reinterpret_cast<C*>(smem_raw);
   const int P = 3 * PP;
-  const C* in = S + int64_t(blockIdx.y) * spec_stride;
+  const SC* in = S + int64_t(blockIdx.y) * spec_stride;
   // Hermitian extension and packing of the three pencils per row pair.
   for (int idx = threadIdx.x; idx < PP * n; idx += blockDim.x) {
     const int a = idx / n, k = idx - a * n;
@@ -831,7 +860,7 @@ fft_x4_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int64_
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         C v = {T(0), T(0)};
-        if (rr + h < rows) v = in[((rr + h) * nxh + kk) * 3 + c];
+        if (rr + h < rows) v = sload<C>(in + ((rr + h) * nxh + kk) * 3 + c);
         if (realonly) v.y = T(0);
         if (mirror) v = cconj(v);
         x[h][c] = v;
@@ -894,6 +923,7 @@ struct PrecondImpl {
   int pairs_per_block, tj, tk;
   size_t lds_x, lds_y, lds_z;
   // four-step kernels for power-of-two y / z axes (0: use the Stockham pass)
+  int bf16;          // spectrum stored as BF16 pairs (FP32 arithmetic, four-step passes only)
   int x4;            // nx handled by fft_x4_r2c / fft_x4_c2r (0: Stockham x passes)
   int pp4, thr_x4;   // row pairs and threads per x4 workgroup
   size_t lds_x4;
@@ -919,12 +949,12 @@ inline int four_step_lanes(int n) {
   return 0;
 }
 
-template <bool INV, typename T>
-void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, typename CT<T>::type* S, int W, int TJ,
+template <bool INV, typename T, typename SC>
+void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int W, int TJ,
                const typename CT<T>::type* w, int64_t stride, const int* active) {
 #define VXL_Y4(N, A, B)                                                                         \
   case N:                                                                                     \
-    hipLaunchKernelGGL((fft_y4<INV, T, A, B>), grid, dim3(thr), lds, s, S, W, TJ, w, stride, active); \
+    hipLaunchKernelGGL((fft_y4<INV, T, A, B, SC>), grid, dim3(thr), lds, s, S, W, TJ, w, stride, active); \
     break;
   switch (n) {
     VXL_Y4(2, 2, 1)
@@ -940,13 +970,13 @@ void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, typename CT
 #undef VXL_Y4
 }
 
-template <typename T, bool kDot>
-void launch_z4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, typename CT<T>::type* S, int ny, int nxh,
+template <typename T, bool kDot, typename SC>
+void launch_z4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int ny, int nxh,
                int TK, Tables<T> tb, T lam, T mu, T inv_vn, int64_t stride, const int* active, int nyq,
                double* partial) {
 #define VXL_Z4(N, A, B)                                                                              \
   case N:                                                                                          \
-    hipLaunchKernelGGL((fft_z4_green<T, A, B, kDot>), grid, dim3(thr), lds, s, S, ny, nxh, TK, tb, lam, mu, \
+    hipLaunchKernelGGL((fft_z4_green<T, A, B, kDot, SC>), grid, dim3(thr), lds, s, S, ny, nxh, TK, tb, lam, mu, \
                        inv_vn, stride, active, nyq, partial);                                      \
     break;
   switch (n) {
@@ -975,26 +1005,26 @@ size_t x4_stride(int n) {
   return 0;
 }
 
-template <typename T, bool kUpd>
-void launch_x4_r2c(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const T* r, typename CT<T>::type* S,
+template <typename T, bool kUpd, typename SC>
+void launch_x4_r2c(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const T* r, SC* S,
                    int64_t rows, int PP, const typename CT<T>::type* w, int64_t fstride, int64_t sstride,
                    const int* active, const XUpdate<T>& upd) {
 #define VXL_X4R(N, A, B)                                                                                     \
   case N:                                                                                                  \
-    hipLaunchKernelGGL((fft_x4_r2c<T, A, B, kUpd>), grid, dim3(thr), lds, s, r, S, rows, PP, w, fstride,     \
+    hipLaunchKernelGGL((fft_x4_r2c<T, A, B, kUpd, SC>), grid, dim3(thr), lds, s, r, S, rows, PP, w, fstride, \
                        sstride, active, upd);                                                              \
     break;
   switch (n) { VXL_X4_CASES(VXL_X4R) }
 #undef VXL_X4R
 }
 
-template <typename T, bool kDir>
-void launch_x4_c2r(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const typename CT<T>::type* S, T* z,
+template <typename T, bool kDir, typename SC>
+void launch_x4_c2r(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const SC* S, T* z,
                    int64_t rows, int PP, const typename CT<T>::type* w, int64_t fstride, int64_t sstride,
                    const int* active, const double* beta) {
 #define VXL_X4C(N, A, B)                                                                                      \
   case N:                                                                                                   \
-    hipLaunchKernelGGL((fft_x4_c2r<T, A, B, kDir>), grid, dim3(thr), lds, s, S, z, rows, PP, w, fstride,     \
+    hipLaunchKernelGGL((fft_x4_c2r<T, A, B, kDir, SC>), grid, dim3(thr), lds, s, S, z, rows, PP, w, fstride, \
                        sstride, active, beta);                                                              \
     break;
   switch (n) { VXL_X4_CASES(VXL_X4C) }
@@ -1119,7 +1149,11 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   }
   im->spec_stride = int64_t(im->nz) * im->ny * im->nxh * 3;
   VXL_TRY(upload_tables<T>(im, cell->h));
-  VXL_HIP(dev_malloc(&im->spec, sizeof(C) * size_t(im->spec_stride) * pc->nrhs_max));
+  im->bf16 = pc->precision == VXL_F32_BF16 ? 1 : 0;
+  if (im->bf16 && !(im->x4 && im->y4 && im->z4))
+    return fail("BF16 spectrum storage needs power-of-two axes of at least 8 (four-step passes)");
+  const size_t elem = im->bf16 ? sizeof(bf16x2) : sizeof(C);
+  VXL_HIP(dev_malloc(&im->spec, elem * size_t(im->spec_stride) * pc->nrhs_max));
   return VXL_OK;
 }
 
@@ -1138,12 +1172,14 @@ enum Stages { kForwardX = 1, kRest = 2, kAllStages = 3 };
 
 // Runs the x forward pass (kForwardX, optionally fused with the CG update upd,
 // four-step x passes only) and/or the remaining passes (kRest).
-template <typename T>
-int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s,
-               const int* active, const CgHooks* cg, int stages = kAllStages, const XUpdate<T>* upd = nullptr) {
+// SC is the spectrum storage type: CT<T>::type, or bf16x2 (four-step passes only).
+template <typename T, typename SC>
+int apply_stages(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s, const int* active,
+                 const CgHooks* cg, int stages, const XUpdate<T>* upd) {
   using C = typename CT<T>::type;
+  constexpr bool kFull = std::is_same<SC, C>::value;
   const Tables<T> tb = tables_of<T>(im);
-  C* S = static_cast<C*>(im->spec);
+  SC* S = static_cast<SC*>(im->spec);
   const int64_t fstride = 3 * int64_t(im->nx) * im->rows;
   const int64_t pairs = (im->rows + 1) / 2;
   const unsigned bx = unsigned((pairs + im->pairs_per_block - 1) / im->pairs_per_block);
@@ -1152,12 +1188,12 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
   if (stages & kForwardX) {
     if (upd && !im->x4) return fail("fused CG update needs the four-step x pass");
     if (upd)
-      launch_x4_r2c<T, true>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, S, im->rows, im->pp4, tb.wx,
+      launch_x4_r2c<T, true, SC>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, S, im->rows, im->pp4, tb.wx,
                              fstride, im->spec_stride, active, *upd);
     else if (im->x4)
-      launch_x4_r2c<T, false>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, S, im->rows, im->pp4, tb.wx,
+      launch_x4_r2c<T, false, SC>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, S, im->rows, im->pp4, tb.wx,
                               fstride, im->spec_stride, active, XUpdate<T>{});
-    else
+    else if constexpr (kFull)
       hipLaunchKernelGGL((fft_x_r2c<T>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, r, S, im->nx, im->rows,
                          im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active);
     VXL_HIP(hipGetLastError());
@@ -1167,9 +1203,11 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
     if (im->y4) {
       const dim3 g((W + im->tj4 - 1) / im->tj4, im->nz, nrhs);
       if (inverse)
-        launch_y4<true, T>(im->y4, g, im->thr_y4, im->lds_y4, s, S, W, im->tj4, tb.wy, im->spec_stride, active);
+        launch_y4<true, T, SC>(im->y4, g, im->thr_y4, im->lds_y4, s, S, W, im->tj4, tb.wy, im->spec_stride, active);
       else
-        launch_y4<false, T>(im->y4, g, im->thr_y4, im->lds_y4, s, S, W, im->tj4, tb.wy, im->spec_stride, active);
+        launch_y4<false, T, SC>(im->y4, g, im->thr_y4, im->lds_y4, s, S, W, im->tj4, tb.wy, im->spec_stride, active);
+    } else if constexpr (!kFull) {
+      return fail("BF16 spectrum storage needs power-of-two axes");
     } else if (inverse) {
       hipLaunchKernelGGL((fft_y<true, T>), dim3((W + im->tj - 1) / im->tj, im->nz, nrhs), dim3(kThreads),
                          im->lds_y, s, S, im->ny, im->nz, W, im->tj, im->axes, tb, im->spec_stride, active);
@@ -1189,12 +1227,12 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
   if (im->z4) {
     const dim3 g((im->nxh + im->tk4 - 1) / im->tk4, im->ny, nrhs);
     if (cg)
-      launch_z4<T, true>(im->z4, g, im->thr_z4, im->lds_z4, s, S, im->ny, im->nxh, im->tk4, tb, lam, mu, inv_vn,
+      launch_z4<T, true, SC>(im->z4, g, im->thr_z4, im->lds_z4, s, S, im->ny, im->nxh, im->tk4, tb, lam, mu, inv_vn,
                          im->spec_stride, active, nyq, part);
     else
-      launch_z4<T, false>(im->z4, g, im->thr_z4, im->lds_z4, s, S, im->ny, im->nxh, im->tk4, tb, lam, mu, inv_vn,
+      launch_z4<T, false, SC>(im->z4, g, im->thr_z4, im->lds_z4, s, S, im->ny, im->nxh, im->tk4, tb, lam, mu, inv_vn,
                           im->spec_stride, active, nyq, part);
-  } else {
+  } else if constexpr (kFull) {
     const dim3 g((im->nxh + im->tk - 1) / im->tk, im->ny, nrhs);
     if (cg)
       hipLaunchKernelGGL((fft_z_green<T, true>), g, dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh,
@@ -1207,11 +1245,13 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
   if (cg) VXL_TRY(cg->mid());
   VXL_TRY(y_pass(true));
   if (im->x4 && cg)
-    launch_x4_c2r<T, true>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, S, z, im->rows, im->pp4, tb.wx,
+    launch_x4_c2r<T, true, SC>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, S, z, im->rows, im->pp4, tb.wx,
                            fstride, im->spec_stride, active, cg->beta);
   else if (im->x4)
-    launch_x4_c2r<T, false>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, S, z, im->rows, im->pp4, tb.wx,
+    launch_x4_c2r<T, false, SC>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, S, z, im->rows, im->pp4, tb.wx,
                             fstride, im->spec_stride, active, nullptr);
+  else if constexpr (!kFull)
+    return fail("BF16 spectrum storage needs power-of-two axes");
   else if (cg)
     hipLaunchKernelGGL((fft_x_c2r<T, true>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
                        im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active,
@@ -1222,6 +1262,15 @@ int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hip
                        nullptr);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
+}
+
+template <typename T>
+int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s,
+               const int* active, const CgHooks* cg, int stages = kAllStages, const XUpdate<T>* upd = nullptr) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (im->bf16) return apply_stages<T, bf16x2>(pc, im, nrhs, r, z, s, active, cg, stages, upd);
+  }
+  return apply_stages<T, typename CT<T>::type>(pc, im, nrhs, r, z, s, active, cg, stages, upd);
 }
 
 }  // namespace
@@ -1285,8 +1334,10 @@ template int precond_cg_rest<double>(vxl_precond*, int, double*, double*, const 
 
 int precond_fused_update_blocks(const vxl_precond* pc) {
   const auto* im = static_cast<const PrecondImpl*>(pc->impl);
+  // Off unless VXL_FUSED_UPDATE=1: folding the update into the x pass raised
+  // that pass's register pressure by more than the separate pass costs.
   const char* env = std::getenv("VXL_FUSED_UPDATE");
-  if (!im->x4 || (env && env[0] == '0')) return 0;
+  if (!im->x4 || !env || env[0] != '1') return 0;
   const int64_t pairs = (im->rows + 1) / 2;
   return int((pairs + im->pp4 - 1) / im->pp4);
 }

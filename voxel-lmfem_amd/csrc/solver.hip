@@ -39,7 +39,8 @@ struct Inner {
 int free_all(vxl_solver* s) {
   if (s->cell) (void)hipStreamSynchronize(s->cell->stream);
   double* bufs[] = {s->f, s->x64, s->r64, s->q64, s->scal, s->partial, s->macro_dev,
-                    s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz, s->rhs_sig, s->partial_fx};
+                    s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz, s->rhs_sig, s->partial_fx,
+                    s->dinv64, s->jmean, s->jrz};
   for (double* b : bufs)
     if (b) dev_free(b);
   if (s->mode == VXL_SOLVE_MIXED) {
@@ -49,6 +50,7 @@ int free_all(vxl_solver* s) {
   } else if (s->p) {
     dev_free(s->p);
   }
+  if (s->dinv32) dev_free(s->dinv32);
   if (s->active_dev) dev_free(s->active_dev);
   if (s->restart_dev) dev_free(s->restart_dev);
   if (s->stop_dev) dev_free(s->stop_dev);
@@ -146,7 +148,17 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   const std::function<int()> form_beta = [&]() -> int {
     return launch_finalize(s->partial_rz, nblk_rz, kOpBeta, scal, R, act, s->restart_dev, st);
   };
-  VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
+  const bool jacobi = s->precond_kind == VXL_PRECOND_JACOBI;
+  const T* dinv = jacobi ? (std::is_same<T, double>::value ? reinterpret_cast<const T*>(s->dinv64)
+                                                            : reinterpret_cast<const T*>(s->dinv32))
+                         : nullptr;
+  auto jacobi_step = [&]() -> int {
+    return jacobi_cg_step<T>(r, p, dinv, n, R, partial, s->jmean, s->jrz, scal, act, s->restart_dev, st);
+  };
+  if (jacobi)
+    VXL_TRY(jacobi_step());
+  else
+    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
   VXL_HIP(hipEventRecord(s->ev[1], st));
 
   int it = 0;
@@ -189,7 +201,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   // With four-step x passes the x/r update and ||r||^2 ride on the forward x
   // pass of the next preconditioner apply (its input is the updated r); a
   // certification that replaces r repeats that pass on the new r.
-  const int nblk_fx = precond_fused_update_blocks(&s->pc);
+  const int nblk_fx = jacobi ? 0 : precond_fused_update_blocks(&s->pc);
   while (any_active() && it < max_iter) {
     VXL_TRY(launch_apply_cg<T>(cell, R, p, q, act, s->partial_st, nullptr, st));
     VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st));
@@ -237,10 +249,14 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
       VXL_TRY(upload_flags());
       if (!any_active()) break;
     }
-    if (!forward_done)
-      VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, nullptr, nullptr, nullptr, act,
-                                    st));
-    VXL_TRY(precond_cg_rest<T>(&s->pc, R, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
+    if (jacobi) {
+      VXL_TRY(jacobi_step());
+    } else {
+      if (!forward_done)
+        VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, nullptr, nullptr, nullptr,
+                                      act, st));
+      VXL_TRY(precond_cg_rest<T>(&s->pc, R, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
+    }
     if (std::any_of(restart.begin(), restart.end(), [](int v) { return v; })) {
       std::fill(restart.begin(), restart.end(), 0);
       VXL_TRY(upload_flags());
@@ -345,5 +361,35 @@ int solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter
 }
 
 int solver_release(vxl_solver* s) { return free_all(s); }
+
+int solver_set_preconditioner(vxl_solver* s, int kind) {
+  // FFT storage precision: rebuild the preconditioner when it changes.
+  const int want = kind == VXL_PRECOND_FFT_BF16 ? VXL_F32_BF16
+                                                : (s->mode == VXL_SOLVE_MIXED ? VXL_F32 : VXL_F64);
+  if (kind != VXL_PRECOND_JACOBI && s->pc.precision != want) {
+    precond_release(&s->pc);
+    s->pc.precision = want;
+    const int rc = precond_create(&s->pc);
+    if (rc != VXL_OK) {  // restore the default so the solver stays usable
+      const std::string msg = vxl_last_error();
+      precond_release(&s->pc);
+      s->pc.precision = s->mode == VXL_SOLVE_MIXED ? VXL_F32 : VXL_F64;
+      VXL_TRY(precond_create(&s->pc));
+      set_error(msg);
+      return rc;
+    }
+  }
+  if (kind == VXL_PRECOND_JACOBI && !s->dinv64) {
+    const int64_t len = 3 * s->cell->grid.n;
+    VXL_HIP(dev_malloc(&s->dinv64, sizeof(double) * size_t(len)));
+    if (s->mode == VXL_SOLVE_MIXED) VXL_HIP(dev_malloc(&s->dinv32, sizeof(float) * size_t(len)));
+    VXL_HIP(dev_malloc(&s->jmean, sizeof(double) * 3 * s->R));
+    VXL_HIP(dev_malloc(&s->jrz, sizeof(double) * size_t(jacobi_blocks(s->cell->grid.n)) * s->R));
+    VXL_TRY(launch_jacobi_diag(s->cell, s->dinv64, s->dinv32, s->cell->stream));
+    VXL_HIP(hipStreamSynchronize(s->cell->stream));
+  }
+  s->precond_kind = kind;
+  return VXL_OK;
+}
 
 }  // namespace vxl

@@ -233,6 +233,18 @@ __device__ __forceinline__ BlockId xcd_block() {
   return b;
 }
 
+// Output-node tile of one workgroup: 32 x TY, TY chosen so that a workgroup
+// is a whole number of waves that pairs up on a CU at the kernel's register
+// budget (VXL_TILE_Y_F32 / VXL_TILE_Y_F64 override at build time).
+#ifndef VXL_TILE_Y_F32
+#define VXL_TILE_Y_F32 16
+#endif
+#ifndef VXL_TILE_Y_F64
+#define VXL_TILE_Y_F64 8
+#endif
+constexpr int kTileYF32 = VXL_TILE_Y_F32;
+constexpr int kTileYF64 = VXL_TILE_Y_F64;
+
 template <int TX, int TY>
 struct Tile {
   static constexpr int PX = TX + 2, PY = TY + 2, PLANE = PX * PY;
@@ -882,33 +894,33 @@ int default_stencil_variant(int precision) {
 
 template <>
 int launch_apply<float>(const vxl_cell* cell, int nrhs, const float* u, float* out, hipStream_t s) {
-  return dispatch_apply<float, 32, 16>(cell, nrhs, u, out, s);
+  return dispatch_apply<float, 32, kTileYF32>(cell, nrhs, u, out, s);
 }
 template <>
 int launch_apply<double>(const vxl_cell* cell, int nrhs, const double* u, double* out, hipStream_t s) {
-  return dispatch_apply<double, 32, 8>(cell, nrhs, u, out, s);
+  return dispatch_apply<double, 32, kTileYF64>(cell, nrhs, u, out, s);
 }
 
 template <>
 int launch_apply_cg<float>(const vxl_cell* cell, int nrhs, const float* p, float* q, const int* active,
                            double* partial, int* blocks, hipStream_t s) {
-  return dispatch_apply<float, 32, 16>(cell, nrhs, p, q, s, active, partial, blocks);
+  return dispatch_apply<float, 32, kTileYF32>(cell, nrhs, p, q, s, active, partial, blocks);
 }
 template <>
 int launch_apply_cg<double>(const vxl_cell* cell, int nrhs, const double* p, double* q, const int* active,
                             double* partial, int* blocks, hipStream_t s) {
-  return dispatch_apply<double, 32, 8>(cell, nrhs, p, q, s, active, partial, blocks);
+  return dispatch_apply<double, 32, kTileYF64>(cell, nrhs, p, q, s, active, partial, blocks);
 }
 template <>
 int apply_cg_blocks<float>(const vxl_cell* cell, int nrhs) {
   int tx, tiles, chunks, zc;
-  apply_grid<float, 32, 16>(cell, nrhs, tx, tiles, chunks, zc);
+  apply_grid<float, 32, kTileYF32>(cell, nrhs, tx, tiles, chunks, zc);
   return tiles * chunks;
 }
 template <>
 int apply_cg_blocks<double>(const vxl_cell* cell, int nrhs) {
   int tx, tiles, chunks, zc;
-  apply_grid<double, 32, 8>(cell, nrhs, tx, tiles, chunks, zc);
+  apply_grid<double, 32, kTileYF64>(cell, nrhs, tx, tiles, chunks, zc);
   return tiles * chunks;
 }
 

@@ -36,6 +36,16 @@ int launch_fill_zero(T* dst, int64_t len, int nrhs, const int* active, hipStream
 int launch_project_zero_mean(double* u, int64_t n, int nrhs, double* partial, const int* active,
                              hipStream_t s);
 
+// Scalar Jacobi control preconditioner (jacobi.hip).  dinv = 1 / diag K per
+// DOF (component planes); one CG step forms z = dinv r - mean(dinv r), the
+// r.z partials, beta (kOpBeta) and p = z + beta p.  partial: >= 3 * nrhs *
+// jacobi_blocks(n) doubles; rz_partial: nrhs * jacobi_blocks(n); mean: 3 nrhs.
+int jacobi_blocks(int64_t n);
+int launch_jacobi_diag(const vxl_cell* cell, double* dinv64, float* dinv32, hipStream_t s);
+template <typename T>
+int jacobi_cg_step(const T* r, T* p, const T* dinv, int64_t n, int nrhs, double* partial, double* mean,
+                   double* rz_partial, double* scal, int* active, const int* restart, hipStream_t s);
+
 }  // namespace vxl
 
 // FFT Green preconditioner handle (fft.hip).
@@ -95,6 +105,11 @@ struct vxl_solver {
   double* partial_st;  // stencil p.q partials
   double* partial_rz;  // FFT z-pass r.z partials (Parseval)
   double* partial_fx;  // ||r||^2 partials of the fused forward x pass (four-step x only)
+  int precond_kind;    // VXL_PRECOND_FFT or VXL_PRECOND_JACOBI
+  double* dinv64;      // Jacobi: 1 / diag K (3N FP64)
+  float* dinv32;       // Jacobi, mixed mode: FP32 copy
+  double* jmean;       // Jacobi: per-component means of dinv r (3 R)
+  double* jrz;         // Jacobi: r.z partials
   int nblk;
   int* active_dev;
   int* restart_dev;

@@ -16,7 +16,8 @@ VXL_ERR_ARG = -1
 VXL_ERR_HIP = -2
 VXL_ERR_CONVERGENCE = -3
 VXL_INDEX_U8, VXL_INDEX_U16, VXL_INDEX_I32 = 1, 2, 4
-VXL_F32, VXL_F64 = 0, 1
+VXL_F32, VXL_F64, VXL_F32_BF16 = 0, 1, 2
+VXL_PRECOND_FFT, VXL_PRECOND_JACOBI, VXL_PRECOND_FFT_BF16 = 0, 1, 2
 VXL_SOLVE_FP64, VXL_SOLVE_MIXED = 0, 1
 
 _P = ctypes.c_void_p
@@ -54,6 +55,8 @@ SIGNATURES = {
     "vxl_solver_create": (_I, [_PP, _P, _I, _I, _D, _D]),
     "vxl_solver_run": (_I, [_P, _DP, _D, _I, _DP, _IP, _IP, _DP, _P]),
     "vxl_solver_run_batch": (_I, [_PP, _I, _DP, _D, _I, _DP, _IP, _IP, _DP, _I]),
+    "vxl_cell_inverse_diagonal": (_I, [_P, _P]),
+    "vxl_solver_set_preconditioner": (_I, [_P, _I]),
     "vxl_solver_timings": (_I, [_P, _DP]),
     "vxl_solver_trace": (_I, [_P, _I, _IP, _DP, _I]),
     "vxl_solver_destroy": (_I, [_P]),

@@ -150,6 +150,12 @@ class Cell:
               "vxl_center_fields")
         return eps, sig, w
 
+    def inverse_diagonal_planes(self):
+        """(3, N) FP64: 1 / diag K, assembled from the 8 elements around each node."""
+        out = self.torch.empty((3, self.n), dtype=self.torch.float64, device=self.device)
+        check(self.lib.vxl_cell_inverse_diagonal(self.handle, dptr(out)), "vxl_cell_inverse_diagonal")
+        return out
+
     def project_planes(self, u):
         check(self.lib.vxl_project_zero_mean(self.handle, u.shape[0], dptr(u)), "vxl_project_zero_mean")
         return u
@@ -194,8 +200,11 @@ class Preconditioner:
     def __init__(self, cell, lam, mu, *, precision="float32", nrhs_max=6):
         self.cell = cell
         torch = cell.torch
-        self.dtype = torch.float32 if precision in ("float32", "fp32", "mixed") else torch.float64
-        prec = _native.VXL_F32 if self.dtype == torch.float32 else _native.VXL_F64
+        if precision not in ("float32", "fp32", "mixed", "bf16", "float64", "fp64"):
+            raise ValueError("precision must be float32, bf16 (FP32 arithmetic, BF16 spectra) or float64")
+        self.dtype = torch.float64 if precision in ("float64", "fp64") else torch.float32
+        prec = {"bf16": _native.VXL_F32_BF16}.get(
+            precision, _native.VXL_F32 if self.dtype == torch.float32 else _native.VXL_F64)
         handle = ctypes.c_void_p()
         check(cell.lib.vxl_precond_create(ctypes.byref(handle), cell.handle, prec, float(lam), float(mu),
                                           int(nrhs_max)), "vxl_precond_create")
@@ -231,7 +240,10 @@ class Preconditioner:
 class Solver:
     """Block multi-RHS C_eff solver (mode 'mixed' or 'fp64')."""
 
-    def __init__(self, cell, lam, mu, *, mode="mixed", nrhs=6):
+    PRECONDITIONERS = {"fft": _native.VXL_PRECOND_FFT, "jacobi": _native.VXL_PRECOND_JACOBI,
+                       "fft_bf16": _native.VXL_PRECOND_FFT_BF16}
+
+    def __init__(self, cell, lam, mu, *, mode="mixed", nrhs=6, preconditioner="fft"):
         self.cell = cell
         self.mode = mode
         code = {"mixed": _native.VXL_SOLVE_MIXED, "fp64": _native.VXL_SOLVE_FP64}.get(mode)
@@ -242,6 +254,18 @@ class Solver:
                                          float(mu)), "vxl_solver_create")
         self.handle = handle
         self.nrhs = nrhs
+        self.preconditioner = "fft"
+        if preconditioner != "fft":
+            self.set_preconditioner(preconditioner)
+
+    def set_preconditioner(self, kind):
+        """'fft' (Green operator of the reference medium), 'fft_bf16' (same,
+        spectra stored in BF16; mixed mode) or 'jacobi' (1 / diag K control)."""
+        code = self.PRECONDITIONERS.get(kind)
+        if code is None:
+            raise ValueError(f"preconditioner must be one of {sorted(self.PRECONDITIONERS)}")
+        check(self.cell.lib.vxl_solver_set_preconditioner(self.handle, code), "vxl_solver_set_preconditioner")
+        self.preconditioner = kind
 
     def run(self, macro=None, *, tol=1e-8, max_iter=10000, return_fields=False):
         torch = self.cell.torch

@@ -11,6 +11,10 @@ of the GPU backend -- no CUDA code is involved).  "cpu" names the reference's
 sparse-direct oracle, which this GPU build does not ship.
 precision: "mixed" (FP32 operator/preconditioner/CG with FP64 reliable
 updates, the default) or "fp64" (everything FP64).
+preconditioner: "fft" (Green operator of the isotropic reference medium, the
+default), "fft_bf16" (same, spectra stored in BF16 between FFT passes; mixed
+precision only) or "jacobi" (1 / diag K, the reference's scalar control in
+repro/bench/chapter_aligned/study.py:84-90).
 """
 import time
 
@@ -55,7 +59,7 @@ def _field_statistics(cell, u, phase, loads=(0, 5)):
     return stats, slices
 
 
-def _workload(phase, table, domain, orientation, precision, tol, local, fields=False):
+def _workload(phase, table, domain, orientation, precision, tol, local, fields=False, preconditioner="fft"):
     from .gpu import Cell, Solver
     import torch
     torch.cuda.synchronize()
@@ -72,7 +76,7 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
     field, h, lam, mu, meta = _prepare_device(phase, table, domain, orientation)
     stage("material")
     cell = Cell(field, phase.shape, h)
-    solver = Solver(cell, lam, mu, mode=precision, nrhs=6)
+    solver = Solver(cell, lam, mu, mode=precision, nrhs=6, preconditioner=preconditioner)
     stage("operator")
     out = solver.run(np.eye(6), tol=tol, return_fields=fields)
     stage("solve")
@@ -94,7 +98,8 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
                    index_bytes=int(cell.index.numel() * cell.index.element_size()),
                    coefficient_bytes=rows * 21 * 8,
                    front_end="device" if getattr(field, "index_dev", None) is not None else "host")
-    operator = dict(variant="modal_fi_lds_stencil", precision=precision, reference_medium=meta)
+    operator = dict(variant="modal_fi_lds_stencil", precision=precision, preconditioner=preconditioner,
+                    reference_medium=meta)
     solver.close()
     cell.close()
     if fields:
@@ -103,7 +108,7 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
 
 
 def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="gpu", warmup=False,
-               precision="mixed", tol=TOL, fields=False):
+               precision="mixed", tol=TOL, fields=False, preconditioner="fft"):
     """Return effective stiffness in engineering-Voigt order 11,22,33,23,13,12.
 
     fields=True adds the reference workload's field diagnostics for loads 0
@@ -113,6 +118,10 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="g
         raise ValueError("backend must be 'gpu', 'hip', 'cuda' or 'cpu'")
     if precision not in ("mixed", "fp64"):
         raise ValueError("precision must be 'mixed' or 'fp64'")
+    if preconditioner not in ("fft", "fft_bf16", "jacobi"):
+        raise ValueError("preconditioner must be 'fft', 'fft_bf16' or 'jacobi'")
+    if preconditioner == "fft_bf16" and precision != "mixed":
+        raise ValueError("preconditioner='fft_bf16' needs precision='mixed'")
     phase, table, domain, orientation, labels, tensors, local = validate(
         phase_id, materials, domain_size, orientation)
     if backend == "cpu":
@@ -130,11 +139,14 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="g
     if not torch.cuda.is_available():
         raise RuntimeError("No GPU is available. This backend runs only on an MI355X (gfx950).")
     if warmup:
-        _workload(phase, table, domain, orientation, precision, tol, local, fields)
+        _workload(phase, table, domain, orientation, precision, tol, local, fields, preconditioner)
     start = time.perf_counter()
-    result, storage, operator = _workload(phase, table, domain, orientation, precision, tol, local, fields)
+    result, storage, operator = _workload(phase, table, domain, orientation, precision, tol, local, fields,
+                                          preconditioner)
     result.update(backend="gpu", storage=storage, operator=operator,
-                  scope=("FI HEX8 matrix-free stiffness; FFT Green preconditioner; "
+                  scope=("FI HEX8 matrix-free stiffness; "
+                         + {"fft": "FFT Green preconditioner; ", "fft_bf16": "FFT Green preconditioner (BF16 spectra); ",
+                            "jacobi": "Jacobi preconditioner; "}[preconditioner]
                          + ("FP32 PCG with FP64 reliable updates" if precision == "mixed" else "FP64 PCG")
                          + "; FP64 true-residual certificate; six unit engineering strains"))
     residuals = [r for call in result["calls"] for r in call["true_residuals"]]
