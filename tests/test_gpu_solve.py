@@ -97,3 +97,22 @@ def test_archived_woven_certify(torch_gpu, grid):
     print(f"  rel err {err:.2e} its {[c['iterations'] for c in out['calls']]} "
           f"seconds {out['seconds']:.3f}", flush=True)
     assert err < 1e-7
+
+
+@pytest.mark.parametrize("precision,preconditioner", [("mixed", "fft"), ("fp64", "fft"), ("mixed", "jacobi")])
+def test_graph_blocks_match_eager_launches(torch_gpu, monkeypatch, precision, preconditioner):
+    """Blocks between host checks replayed from one captured graph give the
+    same bits, iterations and residual history as launching them one by one."""
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import voronoi_polycrystal
+    (phase, mats, dom, _), _ = voronoi_polycrystal(n=24, grains=10, seed=8)
+    print(f"case graph vs eager {precision} {preconditioner} 24^3", flush=True)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("VXL_GRAPH", flag)
+        out[flag] = homogenize(phase, mats, dom, precision=precision, preconditioner=preconditioner)
+    a, b = out["0"], out["1"]
+    assert np.array_equal(np.asarray(a["CH"]), np.asarray(b["CH"]))
+    assert [c["iterations"] for c in a["calls"]] == [c["iterations"] for c in b["calls"]]
+    assert [c["trace"] for c in a["calls"]] == [c["trace"] for c in b["calls"]]
+    assert max(c["iterations"] for c in b["calls"]) > 8  # at least one graph block ran

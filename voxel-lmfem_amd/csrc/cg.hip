@@ -155,7 +155,7 @@ subtract_mean_kernel(double* u, int64_t n, const double* mean, const int* active
 // then the scalar update selected by op.
 __global__ void __launch_bounds__(kBlock)
 finalize_kernel(const double* partial, int nblk, int op, double* scal, int nrhs, int* active,
-                const int* restart, int iteration, double tol, int* stop) {
+                int* restart, int iteration, double tol, int* stop) {
   const int k = blockIdx.x;
   if (active && !active[k]) return;
   double acc = 0;
@@ -192,7 +192,10 @@ finalize_kernel(const double* partial, int nblk, int op, double* scal, int nrhs,
     case kOpBeta: {
       const double rho_old = S[kRho * nrhs + k];
       double beta = rho_old > 0.0 ? acc / rho_old : 0.0;
-      if (restart && restart[k]) beta = 0.0;
+      if (restart && restart[k]) {  // consumed here, so a captured block needs no host reset
+        beta = 0.0;
+        restart[k] = 0;
+      }
       if (!isfinite(beta)) beta = 0.0;
       S[kBeta * nrhs + k] = beta;
       S[kRho * nrhs + k] = acc;
@@ -226,7 +229,7 @@ template int launch_dot<float, float>(const float*, const float*, int64_t, int, 
 template int launch_dot<double, double>(const double*, const double*, int64_t, int, double*, const int*, hipStream_t);
 
 int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, int* active,
-                    const int* restart, hipStream_t s, int iteration, double tol, int* stop) {
+                    int* restart, hipStream_t s, int iteration, double tol, int* stop) {
   hipLaunchKernelGGL(finalize_kernel, dim3(nrhs), dim3(kBlock), 0, s, partial, nblk, op, scal, nrhs,
                      active, restart, iteration, tol, stop);
   VXL_HIP(hipGetLastError());

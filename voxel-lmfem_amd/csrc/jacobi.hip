@@ -160,7 +160,7 @@ int launch_jacobi_diag(const vxl_cell* cell, double* dinv64, float* dinv32, hipS
 
 template <typename T>
 int jacobi_cg_step(const T* r, T* p, const T* dinv, int64_t n, int nrhs, double* partial, double* mean,
-                   double* rz_partial, double* scal, int* active, const int* restart, hipStream_t s) {
+                   double* rz_partial, double* scal, int* active, int* restart, hipStream_t s) {
   const int nb = jacobi_blocks(n);
   const dim3 grid = dim3(unsigned(nb), unsigned(nrhs));
   hipLaunchKernelGGL((jacobi_sums_kernel<T>), grid, dim3(kJBlock), 0, s, r, dinv, n, partial, active);
@@ -177,8 +177,8 @@ int jacobi_cg_step(const T* r, T* p, const T* dinv, int64_t n, int nrhs, double*
   return VXL_OK;
 }
 template int jacobi_cg_step<float>(const float*, float*, const float*, int64_t, int, double*, double*, double*,
-                                   double*, int*, const int*, hipStream_t);
+                                   double*, int*, int*, hipStream_t);
 template int jacobi_cg_step<double>(const double*, double*, const double*, int64_t, int, double*, double*, double*,
-                                    double*, int*, const int*, hipStream_t);
+                                    double*, int*, int*, hipStream_t);
 
 }  // namespace vxl

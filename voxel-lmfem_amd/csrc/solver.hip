@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 
 namespace vxl {
@@ -31,6 +32,13 @@ namespace {
 constexpr double kReliable = 1e-3;  // residual drop between FP64 updates (mixed)
 constexpr int kCheckStride = 4;
 
+// Graph replay of the blocks between host checks; VXL_GRAPH=0 (read at
+// every run) launches them one kernel at a time instead.
+bool graphs_enabled() {
+  const char* e = std::getenv("VXL_GRAPH");
+  return !(e && e[0] == '0');
+}
+
 template <typename T>
 struct Inner {
   static constexpr int precision = std::is_same<T, double>::value ? VXL_F64 : VXL_F32;
@@ -38,6 +46,8 @@ struct Inner {
 
 int free_all(vxl_solver* s) {
   if (s->cell) (void)hipStreamSynchronize(s->cell->stream);
+  if (s->block_graph) (void)hipGraphExecDestroy(s->block_graph);
+  s->block_graph = nullptr;
   double* bufs[] = {s->f, s->x64, s->r64, s->q64, s->scal, s->partial, s->macro_dev,
                     s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz, s->rhs_sig, s->partial_fx,
                     s->dinv64, s->jmean, s->jrz};
@@ -202,20 +212,82 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   // pass of the next preconditioner apply (its input is the updated r); a
   // certification that replaces r repeats that pass on the new r.
   const int nblk_fx = jacobi ? 0 : precond_fused_update_blocks(&s->pc);
-  while (any_active() && it < max_iter) {
+  // [stencil, alpha, update, ||r||^2 with the latch at iteration `iter`]
+  auto cg_update = [&](int iter, bool& forward_done) -> int {
     VXL_TRY(launch_apply_cg<T>(cell, R, p, q, act, s->partial_st, nullptr, st));
     VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st));
-    bool forward_done = false;
+    forward_done = false;
     if (nblk_fx > 0) {
       VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, x, p, q, scal + kAlpha * R, s->partial_fx, act, st));
-      ++it;
-      VXL_TRY(launch_finalize(s->partial_fx, nblk_fx, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
+      VXL_TRY(launch_finalize(s->partial_fx, nblk_fx, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
       forward_done = true;
     } else {
       VXL_TRY(launch_update_xr<T>(x, r, p, q, len, R, scal + kAlpha * R, partial, act, st));
-      ++it;
-      VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st, it, tol, s->stop_dev));
+      VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
     }
+    return VXL_OK;
+  };
+  // [FFT passes of M r, beta, p = M r + beta p] (or the Jacobi step); the
+  // beta kernel consumes the restart flags on the device.
+  auto precond_step = [&](bool forward_done) -> int {
+    if (jacobi) return jacobi_step();
+    if (!forward_done)
+      VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, nullptr, nullptr, nullptr, act,
+                                    st));
+    return precond_cg_rest<T>(&s->pc, R, p, s->partial_rz, scal + kBeta * R, form_beta, act, st);
+  };
+  // Blocks of kCheckStride iterations between host checks replay one
+  // captured graph: no host work happens inside a block (flags, scalars and
+  // the latch live on the device), and the latch records the iteration
+  // relative to the block's start.  The fused forward pass is not captured.
+  const bool graphs = nblk_fx == 0 && graphs_enabled();
+  auto block_graph = [&]() -> int {
+    if (s->block_graph && s->block_graph_tol == tol) return VXL_OK;
+    if (s->block_graph) {
+      VXL_HIP(hipGraphExecDestroy(s->block_graph));
+      s->block_graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    VXL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = precond_step(false);
+    for (int j = 1; rc == VXL_OK && j <= kCheckStride; ++j) {
+      bool fd = false;
+      rc = cg_update(j, fd);
+      if (rc == VXL_OK && j < kCheckStride) rc = precond_step(fd);
+    }
+    const hipError_t end = hipStreamEndCapture(st, &g);
+    if (rc != VXL_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    VXL_HIP(end);
+    const hipError_t inst = hipGraphInstantiate(&s->block_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (inst != hipSuccess) s->block_graph = nullptr;
+    VXL_HIP(inst);
+    s->block_graph_tol = tol;  // the latch threshold is a kernel argument
+    return VXL_OK;
+  };
+
+  bool forward_done = false, pending = false;  // pending: M r of iteration `it` not formed yet
+  while (any_active() && it < max_iter) {
+    int base = -1;  // >= 0: this block ran as a graph starting after iteration `base`
+    if (pending && graphs && it % kCheckStride == 0 && it + kCheckStride <= max_iter) {
+      VXL_TRY(block_graph());
+      VXL_HIP(hipGraphLaunch(s->block_graph, st));
+      base = it;
+      it += kCheckStride;
+      std::fill(restart.begin(), restart.end(), 0);
+    } else {
+      if (pending) {
+        VXL_TRY(precond_step(forward_done));
+        if (std::any_of(restart.begin(), restart.end(), [](int v) { return v; }))
+          std::fill(restart.begin(), restart.end(), 0);  // consumed by the beta kernel
+      }
+      ++it;
+      VXL_TRY(cg_update(it, forward_done));
+    }
+    pending = true;
     if (it == 1 || it % kCheckStride == 0 || it == max_iter) {
       VXL_TRY(read_slot(kRR, rr));
       VXL_HIP(hipMemcpyAsync(pin_stop, s->stop_dev, sizeof(int) * R, hipMemcpyDeviceToHost, st));
@@ -225,8 +297,9 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
       for (int k = 0; k < R; ++k) {
         if (!active[k]) continue;
         if (!std::isfinite(rr[k])) return fail("Nonfinite residual in CG recursion");
-        s->trace[k].push_back({pin_stop[k] ? pin_stop[k] : it, std::sqrt(rr[k] / ff[k])});
-        its[k] = pin_stop[k] ? pin_stop[k] : it;
+        const int at = pin_stop[k] ? (base >= 0 ? base + pin_stop[k] : pin_stop[k]) : it;
+        s->trace[k].push_back({at, std::sqrt(rr[k] / ff[k])});
+        its[k] = at;
         latched[k] = pin_stop[k] != 0;
         const bool drop = mixed && rr[k] < kReliable * kReliable * rr_ref[k];
         if (latched[k] || drop) {
@@ -244,22 +317,10 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
         // search direction is stale: restart it (beta = 0) if it continues.
         for (int k = 0; k < R; ++k)
           if (active[k] && latched[k]) restart[k] = 1;
-        VXL_HIP(hipMemsetAsync(s->stop_dev, 0, sizeof(int) * R, st));
+        VXL_HIP(hipMemsetAsync(s->stop_dev, 0, sizeof(int) * R, st));  // eager, never captured
       }
       VXL_TRY(upload_flags());
       if (!any_active()) break;
-    }
-    if (jacobi) {
-      VXL_TRY(jacobi_step());
-    } else {
-      if (!forward_done)
-        VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, nullptr, nullptr, nullptr,
-                                      act, st));
-      VXL_TRY(precond_cg_rest<T>(&s->pc, R, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
-    }
-    if (std::any_of(restart.begin(), restart.end(), [](int v) { return v; })) {
-      std::fill(restart.begin(), restart.end(), 0);
-      VXL_TRY(upload_flags());
     }
   }
   VXL_HIP(hipEventRecord(s->ev[2], st));
@@ -363,6 +424,11 @@ int solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter
 int solver_release(vxl_solver* s) { return free_all(s); }
 
 int solver_set_preconditioner(vxl_solver* s, int kind) {
+  if (s->block_graph) {  // captured with the old preconditioner's kernels
+    VXL_HIP(hipStreamSynchronize(s->cell->stream));
+    VXL_HIP(hipGraphExecDestroy(s->block_graph));
+    s->block_graph = nullptr;
+  }
   // FFT storage precision: rebuild the preconditioner when it changes.
   const int want = kind == VXL_PRECOND_FFT_BF16 ? VXL_F32_BF16
                                                 : (s->mode == VXL_SOLVE_MIXED ? VXL_F32 : VXL_F64);

@@ -62,20 +62,23 @@ __device__ __forceinline__ void cmul(const T* c, const T (&e)[6], T (&s)[6]) {
 // One strain mode.  MX/MY/MZ name the Hadamard mode whose coefficient vector
 // forms gradient column x/y/z (-1: column absent); CLS selects the weight
 // class.  Accumulates the back-projected modal forces into P.
-template <int MX, int MY, int MZ, int CLS, typename T>
+// kIso (cubic voxels, gx = gy = gz within a class): the gradient scale is
+// moved onto the back-projection, P += g^2 C e(q), and g holds g^2.
+template <int MX, int MY, int MZ, int CLS, bool kIso, typename T>
 __device__ __forceinline__ void strain_mode(const T* c, const G9<T>& g, const T (&q)[3][8],
                                             T (&P)[3][8]) {
   constexpr bool HX = MX >= 0, HY = MY >= 0, HZ = MZ >= 0;
   constexpr int MASK = (HX ? 1 : 0) | (HY ? 2 : 0) | (HZ ? 4 : 0) | ((HY || HZ) ? 8 : 0) |
                        ((HX || HZ) ? 16 : 0) | ((HX || HY) ? 32 : 0);
-  const T gx = g.g[CLS][0], gy = g.g[CLS][1], gz = g.g[CLS][2];
+  const T bx = g.g[CLS][0], by = g.g[CLS][1], bz = g.g[CLS][2];
+  const T gx = kIso ? T(1) : bx, gy = kIso ? T(1) : by, gz = kIso ? T(1) : bz;
   // Gradient columns col_j[i] = d u_i / d x_j.
   T cx[3], cy[3], cz[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    cx[i] = HX ? gx * q[i][HX ? MX : 0] : T(0);
-    cy[i] = HY ? gy * q[i][HY ? MY : 0] : T(0);
-    cz[i] = HZ ? gz * q[i][HZ ? MZ : 0] : T(0);
+    cx[i] = HX ? (kIso ? q[i][HX ? MX : 0] : gx * q[i][HX ? MX : 0]) : T(0);
+    cy[i] = HY ? (kIso ? q[i][HY ? MY : 0] : gy * q[i][HY ? MY : 0]) : T(0);
+    cz[i] = HZ ? (kIso ? q[i][HZ ? MZ : 0] : gz * q[i][HZ ? MZ : 0]) : T(0);
   }
   T e[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
   if constexpr (HX) e[0] = cx[0];
@@ -94,19 +97,19 @@ __device__ __forceinline__ void strain_mode(const T* c, const G9<T>& g, const T 
   cmul<MASK>(c, e, s);
   // Back-projection: P[m_j] += g_j * (sigma_0j, sigma_1j, sigma_2j).
   if constexpr (HX) {
-    P[0][MX] = fma(gx, s[0], P[0][MX]);
-    P[1][MX] = fma(gx, s[5], P[1][MX]);
-    P[2][MX] = fma(gx, s[4], P[2][MX]);
+    P[0][MX] = fma(bx, s[0], P[0][MX]);
+    P[1][MX] = fma(bx, s[5], P[1][MX]);
+    P[2][MX] = fma(bx, s[4], P[2][MX]);
   }
   if constexpr (HY) {
-    P[0][MY] = fma(gy, s[5], P[0][MY]);
-    P[1][MY] = fma(gy, s[1], P[1][MY]);
-    P[2][MY] = fma(gy, s[3], P[2][MY]);
+    P[0][MY] = fma(by, s[5], P[0][MY]);
+    P[1][MY] = fma(by, s[1], P[1][MY]);
+    P[2][MY] = fma(by, s[3], P[2][MY]);
   }
   if constexpr (HZ) {
-    P[0][MZ] = fma(gz, s[4], P[0][MZ]);
-    P[1][MZ] = fma(gz, s[3], P[1][MZ]);
-    P[2][MZ] = fma(gz, s[2], P[2][MZ]);
+    P[0][MZ] = fma(bz, s[4], P[0][MZ]);
+    P[1][MZ] = fma(bz, s[3], P[1][MZ]);
+    P[2][MZ] = fma(bz, s[2], P[2][MZ]);
   }
 }
 
@@ -140,7 +143,7 @@ __device__ __forceinline__ void hadamard_transpose(T (&v)[8]) {
 
 // Modal element action: Hadamard coefficients q[c][m] in, modal forces P[c][m]
 // out (nodal forces f = H^T P).  c: 21 packed stiffness.
-template <typename T>
+template <bool kIso = false, typename T>
 __device__ __forceinline__ void element_modal(const T (&q)[3][8], const T* c, const G9<T>& g,
                                               T (&P)[3][8]) {
 #pragma unroll
@@ -148,13 +151,13 @@ __device__ __forceinline__ void element_modal(const T (&q)[3][8], const T* c, co
 #pragma unroll
     for (int m = 0; m < 8; ++m) P[i][m] = T(0);
   // Mode numbering: bit 0 = x, bit 1 = y, bit 2 = z.
-  strain_mode<1, 2, 4, 0>(c, g, q, P);    // constant
-  strain_mode<-1, 3, 5, 1>(c, g, q, P);   // xi
-  strain_mode<3, -1, 6, 1>(c, g, q, P);   // eta
-  strain_mode<5, 6, -1, 1>(c, g, q, P);   // zeta
-  strain_mode<7, -1, -1, 2>(c, g, q, P);  // eta*zeta
-  strain_mode<-1, 7, -1, 2>(c, g, q, P);  // xi*zeta
-  strain_mode<-1, -1, 7, 2>(c, g, q, P);  // xi*eta
+  strain_mode<1, 2, 4, 0, kIso>(c, g, q, P);    // constant
+  strain_mode<-1, 3, 5, 1, kIso>(c, g, q, P);   // xi
+  strain_mode<3, -1, 6, 1, kIso>(c, g, q, P);   // eta
+  strain_mode<5, 6, -1, 1, kIso>(c, g, q, P);   // zeta
+  strain_mode<7, -1, -1, 2, kIso>(c, g, q, P);  // eta*zeta
+  strain_mode<-1, 7, -1, 2, kIso>(c, g, q, P);  // xi*zeta
+  strain_mode<-1, -1, 7, 2, kIso>(c, g, q, P);  // xi*eta
 }
 
 // v[c][a]: nodal displacements in, nodal forces out.
@@ -233,11 +236,13 @@ __device__ __forceinline__ BlockId xcd_block() {
   return b;
 }
 
-// Output-node tile of one workgroup: 32 x TY, TY chosen so that a workgroup
-// is a whole number of waves that pairs up on a CU at the kernel's register
-// budget (VXL_TILE_Y_F32 / VXL_TILE_Y_F64 override at build time).
+// Output-node tile of one workgroup: 32 x TY.  FP32: 32 x 14 nodes = 33 x 15
+// elements = 8 waves, so two workgroups (16 waves, 4 per SIMD) share a CU at
+// up to 128 VGPRs; 32 x 16 (9 waves) fits only one at the 96 VGPRs that
+// pairing would need, and ran 1.55x slower.  VXL_TILE_Y_F32 / VXL_TILE_Y_F64
+// override at build time.
 #ifndef VXL_TILE_Y_F32
-#define VXL_TILE_Y_F32 16
+#define VXL_TILE_Y_F32 14
 #endif
 #ifndef VXL_TILE_Y_F64
 #define VXL_TILE_Y_F64 8
@@ -380,8 +385,10 @@ apply_kernel(ApplyArgs<T> a) {
 // (4 nodes), runs the modal element action, and exchanges only the 4 corner
 // forces of one face through LDS.  One staged node plane; two barriers per
 // plane.  Same arithmetic as the full 8-point transform, regrouped.
-template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse, bool kPF>
-__global__ void __launch_bounds__((Tile<TX, TY>::NT), (sizeof(T) == 4 ? 5 : 0))
+// kIso: cubic voxels, a.g holds the squared modal factors (strain_mode).
+// FP32: the register budget lets two workgroups share a CU (waves per SIMD).
+template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse, bool kPF, bool kIso>
+__global__ void __launch_bounds__((Tile<TX, TY>::NT), (sizeof(T) == 4 ? (2 * Tile<TX, TY>::NT / 64 + 3) / 4 : 0))
 apply_kernel_v2(ApplyArgs<T> a) {
   using TL = Tile<TX, TY>;
   __shared__ T su[3][TL::PLANE];
@@ -437,10 +444,10 @@ apply_kernel_v2(ApplyArgs<T> a) {
     }
   };
   T Bf[3][4], Ftp[3][4];
-  // Element of layer z from its bottom (Bf) and top (Tf) face modes; returns the
+  // Element of layer z from its bottom (Bb) and top (Tf) face modes; returns the
   // bottom-face force (Fb) and top-face force (Ft) in face-mode form.
   auto row_of = [&](int z) { return load_row<Idx>(a.index, int64_t(wrapmod(z, nz)) * nxy + e_xy); };
-  auto element = [&](int z, int row, const T (&Tf)[3][4], T (&Fb)[3][4], T (&Ft)[3][4]) {
+  auto element = [&](int z, int row, const T (&Bb)[3][4], const T (&Tf)[3][4], T (&Fb)[3][4], T (&Ft)[3][4]) {
     const int64_t voxel = int64_t(wrapmod(z, nz)) * nxy + e_xy;
     const T* crow = a.table + int64_t(row) * kRowWidth;
     T c[21];
@@ -456,10 +463,10 @@ apply_kernel_v2(ApplyArgs<T> a) {
     for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        q[cc][m] = Tf[cc][m] + Bf[cc][m];
-        q[cc][m + 4] = Tf[cc][m] - Bf[cc][m];
+        q[cc][m] = Tf[cc][m] + Bb[cc][m];
+        q[cc][m + 4] = Tf[cc][m] - Bb[cc][m];
       }
-    element_modal(q, c, a.g, P);
+    element_modal<kIso>(q, c, a.g, P);
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
@@ -513,7 +520,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
   if (is_elem) {
     T Tf[3][4], Fb[3][4];
     face_modes(Tf);
-    element(z0 - 1, row_of(z0 - 1), Tf, Fb, Ftp);
+    element(z0 - 1, row_of(z0 - 1), Bf, Tf, Fb, Ftp);
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
@@ -528,8 +535,11 @@ apply_kernel_v2(ApplyArgs<T> a) {
   // label -> row -> coefficients chain never stalls a step on two latencies.
   // (Plain form only: the CG form is at its register limit.)
   int row_next = (!kFuse && is_elem) ? row_of(z0) : 0;
-  for (int k = 0; z0 + k < z1; ++k) {
-    const int z = z0 + k;
+  // One layer.  Bc/Fp: this layer's bottom face modes and the force the layer
+  // below put on that face; Bn/Fn receive the top face modes and top force,
+  // which are the next layer's Bc/Fp.  The loop alternates two register sets,
+  // so no copies are needed between layers.
+  auto layer = [&](int z, const T (&Bc)[3][4], const T (&Fp)[3][4], T (&Bn)[3][4], T (&Fn)[3][4]) {
     int row_cur = row_next;
     if constexpr (kFuse) {
       if (is_elem) row_cur = row_of(z);
@@ -542,22 +552,19 @@ apply_kernel_v2(ApplyArgs<T> a) {
       __syncthreads();
     }
     // LDS holds plane z+1: read everything this step needs from it first.
-    T Tf[3][4];
-    if (is_elem) face_modes(Tf);
+    if (is_elem) face_modes(Bn);
     if (more) prefetch(z + 2);
     if (is_elem) {
-      T Fb[3][4], Ft[3][4];
-      element(z, row_cur, Tf, Fb, Ft);
+      T Fb[3][4];
+      element(z, row_cur, Bc, Bn, Fb, Fn);
       T w = T(0);  // kFuse: u_e . f_e of this element
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
         T F[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-          if constexpr (kFuse) w = fma(Tf[cc][m], Ft[cc][m], fma(Bf[cc][m], Fb[cc][m], w));
-          F[m] = Ftp[cc][m] + Fb[cc][m];
-          Ftp[cc][m] = Ft[cc][m];
-          Bf[cc][m] = Tf[cc][m];
+          if constexpr (kFuse) w = fma(Bn[cc][m], Fn[cc][m], fma(Bc[cc][m], Fb[cc][m], w));
+          F[m] = Fp[cc][m] + Fb[cc][m];
         }
         face_transpose(F);
 #pragma unroll
@@ -583,6 +590,11 @@ apply_kernel_v2(ApplyArgs<T> a) {
       for (int cc = 0; cc < 3; ++cc) out[cc * n + node] = acc[cc];
     }
     if constexpr (kPF) __syncthreads();
+  };
+  T Bf2[3][4], Ftp2[3][4];
+  for (int z = z0; z < z1; z += 2) {
+    layer(z, Bf, Ftp, Bf2, Ftp2);
+    if (z + 1 < z1) layer(z + 1, Bf2, Ftp2, Bf, Ftp);
   }
   if constexpr (kFuse) {
     const double total = block_reduce_sum<TL::NT>(double(pq), red);
@@ -643,22 +655,37 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
   a.nz = cell->grid.nz;
   int tiles = 0, chunks = 0;
   apply_grid<T, TX, TY>(cell, nrhs, a.tiles_x, tiles, chunks, a.zchunk);
-  a.g = cast_g<T>(modal_factors(cell->h));
+  const ModalFactors mf = modal_factors(cell->h);
+  a.g = cast_g<T>(mf);
   dim3 grid(tiles, chunks, nrhs);
   if (blocks) *blocks = tiles * chunks;
   const int v = cell->variant[std::is_same<T, double>::value ? 1 : 0];
+  // Cubic voxels: the squared factors go to the back-projection (one multiply
+  // per gradient entry less); schedule 1 keeps the general form.
+  const bool iso = cell->h[0] == cell->h[1] && cell->h[1] == cell->h[2] && v != 1;
+  if (iso)
+    for (int k = 0; k < 3; ++k)
+      for (int d = 0; d < 3; ++d) a.g.g[k][d] = T(mf.g[k][d] * mf.g[k][d]);
+#define VXL_V2(FUSE, PF)                                                                                   \
+  do {                                                                                                     \
+    if (iso)                                                                                               \
+      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, FUSE, PF, true>), grid, dim3(TL::NT), 0, s, a); \
+    else                                                                                                   \
+      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, FUSE, PF, false>), grid, dim3(TL::NT), 0, s, a); \
+  } while (0)
   if (partial) {
     if (v == 3)
-      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, true, true>), grid, dim3(TL::NT), 0, s, a);
+      VXL_V2(true, true);
     else
-      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, true, false>), grid, dim3(TL::NT), 0, s, a);
+      VXL_V2(true, false);
   } else if (v == 1) {
     hipLaunchKernelGGL((apply_kernel<T, Idx, kScale, TX, TY>), grid, dim3(TL::NT), 0, s, a);
   } else if (v == 3) {
-    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, false, true>), grid, dim3(TL::NT), 0, s, a);
+    VXL_V2(false, true);
   } else {
-    hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, false, false>), grid, dim3(TL::NT), 0, s, a);
+    VXL_V2(false, false);
   }
+#undef VXL_V2
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }

@@ -20,7 +20,7 @@ int launch_dot(const A* a, const B* b, int64_t len, int nrhs, double* partial, c
 // kOpRR with `stop` latches the first iteration below tol: stop[k] = iteration,
 // active[k] = 0 (on the device).
 int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, int* active,
-                    const int* restart, hipStream_t s, int iteration = 0, double tol = 0.0,
+                    int* restart, hipStream_t s, int iteration = 0, double tol = 0.0,
                     int* stop = nullptr);
 template <typename T>
 int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, const double* alpha,
@@ -44,7 +44,7 @@ int jacobi_blocks(int64_t n);
 int launch_jacobi_diag(const vxl_cell* cell, double* dinv64, float* dinv32, hipStream_t s);
 template <typename T>
 int jacobi_cg_step(const T* r, T* p, const T* dinv, int64_t n, int nrhs, double* partial, double* mean,
-                   double* rz_partial, double* scal, int* active, const int* restart, hipStream_t s);
+                   double* rz_partial, double* scal, int* active, int* restart, hipStream_t s);
 
 }  // namespace vxl
 
@@ -123,6 +123,12 @@ struct vxl_solver {
   hipEvent_t flag_ev;
   int* pinned_flags;
   double* pinned_scal;
+  // Instantiated graph of one block between host checks: the preconditioner
+  // of the checked iteration, then kCheckStride iterations up to their
+  // ||r||^2 (nullptr until first needed; rebuilt when the preconditioner
+  // changes).
+  hipGraphExec_t block_graph;
+  double block_graph_tol;
   // Per-system residual history of the last run: (iteration, relative
   // residual); a negative iteration marks an FP64 true-residual evaluation.
   std::vector<std::vector<std::pair<int, double>>> trace;

@@ -75,12 +75,24 @@ def periodic_spheres(n=32, radius=3.0, fraction=0.30, seed=1, E=(3.0, 70.0), nu=
     c = np.arange(n) + .5
     inside = np.zeros((n, n, n), dtype=bool)
     centres = []
-    while inside.mean() < fraction:
+    count, total = 0, n ** 3
+    reach = int(np.ceil(radius)) + 1
+    while count / total < fraction:
         p = rng.uniform(0, n, size=3)
         centres.append(p)
-        d = [np.abs(c - p[a]) for a in (2, 1, 0)]  # z, y, x
-        d = [np.minimum(v, n - v) for v in d]
-        inside |= (d[0][:, None, None] ** 2 + d[1][None, :, None] ** 2 + d[2][None, None, :] ** 2) <= radius ** 2
+        # Only the sphere's periodic bounding box can change; the distance
+        # test is evaluated there exactly as over the whole grid.
+        win = []
+        for a in (2, 1, 0):  # z, y, x
+            lo = int(np.floor(p[a])) - reach
+            idx = np.unique(np.arange(lo, lo + 2 * reach + 2) % n)
+            v = np.abs(c[idx] - p[a])
+            win.append((idx, np.minimum(v, n - v)))
+        (iz, dz), (iy, dy), (ix, dx) = win
+        hit = (dz[:, None, None] ** 2 + dy[None, :, None] ** 2 + dx[None, None, :] ** 2) <= radius ** 2
+        box = np.ix_(iz, iy, ix)
+        count += int((hit & ~inside[box]).sum())
+        inside[box] |= hit
     phase = inside.astype(np.uint8)
     materials = {0: dict(kind="isotropic", E=float(E[0]), nu=float(nu[0])),
                  1: dict(kind="isotropic", E=float(E[1]), nu=float(nu[1]))}
