@@ -223,6 +223,11 @@ int vxl_cell_inverse_diagonal(vxl_cell* cell, double* out_dev);
  * z = project_zero_mean(r / diag K) with the assembled diagonal of K; the
  * diagonal is assembled here on the device (once per solver). */
 int vxl_solver_set_preconditioner(vxl_solver* s, int kind);
+/* Number of four-iteration blocks the last vxl_solver_run replayed from its
+ * captured HIP graph (0 with VXL_GRAPH=0, the fused forward pass, or when
+ * capture is unavailable).  No reference counterpart (the reference launches
+ * through PyTorch eagerly). */
+int vxl_solver_graph_blocks(vxl_solver* s);
 /* Runs count independent solvers (each bound to its own cell and stream)
  * concurrently from `workers` host threads, largest cells first, so small
  * cells overlap on the GPU instead of waiting on each other's host checks.

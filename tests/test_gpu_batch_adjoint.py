@@ -28,6 +28,7 @@ def test_batch_equals_individual_solves(torch_gpu):
         assert got["max_true_residual"] <= 1e-8
         assert np.array_equal(np.asarray(got["CH"]), np.asarray(one["CH"]))  # same kernels, same bits
         assert [c["iterations"] for c in got["calls"]] == [c["iterations"] for c in one["calls"]]
+        assert got["graph_blocks"] == one["graph_blocks"]
 
 
 def test_woven_128_mixed_vs_fp64(torch_gpu):

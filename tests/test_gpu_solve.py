@@ -115,4 +115,5 @@ def test_graph_blocks_match_eager_launches(torch_gpu, monkeypatch, precision, pr
     assert np.array_equal(np.asarray(a["CH"]), np.asarray(b["CH"]))
     assert [c["iterations"] for c in a["calls"]] == [c["iterations"] for c in b["calls"]]
     assert [c["trace"] for c in a["calls"]] == [c["trace"] for c in b["calls"]]
-    assert max(c["iterations"] for c in b["calls"]) > 8  # at least one graph block ran
+    assert max(c["iterations"] for c in b["calls"]) > 8
+    assert a["graph_blocks"] == 0 and b["graph_blocks"] > 0  # the blocks really were replayed

@@ -25,6 +25,7 @@ int solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter
                int* iterations, int* restarts, double* residuals, double* u_out);
 int solver_release(vxl_solver* s);
 int solver_set_preconditioner(vxl_solver* s, int kind);
+int solver_prepare_graph(vxl_solver* s, double tol);
 
 namespace {
 
@@ -499,6 +500,12 @@ int vxl_solver_run_batch(vxl_solver* const* solvers, int count, const double* ma
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return solvers[a]->cell->grid.n * solvers[a]->R > solvers[b]->cell->grid.n * solvers[b]->R;
   });
+  // Graphs are captured here, before any worker runs: a capture in progress
+  // makes other threads' synchronising HIP calls fail.
+  for (int i = 0; i < count; ++i) {
+    (void)hipSetDevice(solvers[i]->cell->device);
+    VXL_TRY(solver_prepare_graph(solvers[i], tol));
+  }
   std::atomic<int> next{0};
   std::vector<int> rc(count, VXL_OK);
   std::vector<std::string> err(count);
@@ -543,6 +550,12 @@ int vxl_solver_set_preconditioner(vxl_solver* s, int kind) {
   if (kind == VXL_PRECOND_FFT_BF16 && s->mode != VXL_SOLVE_MIXED)
     return fail("the BF16-stored preconditioner is a mixed-mode option");
   return solver_set_preconditioner(s, kind);
+}
+
+int vxl_solver_graph_blocks(vxl_solver* s) {
+  g_error.clear();
+  if (!s) return fail("vxl_solver_graph_blocks: invalid arguments");
+  return s->graph_blocks;
 }
 
 int vxl_solver_timings(vxl_solver* s, double* ms3) {

@@ -73,6 +73,121 @@ int free_all(vxl_solver* s) {
   return VXL_OK;
 }
 
+// The kernels of one PCG iteration, bound to a solver's buffers.
+// One iteration: [stencil: q = K p, p.q] [alpha] [x += alpha p, r -= alpha q,
+// ||r||^2] [latch] (host check / FP64 certification) [FFT passes of M r; the
+// z pass reduces r.z] [beta] [last pass: p = M r + beta p].  With the fused
+// four-step forward pass the x/r update and ||r||^2 ride on the forward x
+// pass of the next preconditioner apply (its input is the updated r); a
+// certification that replaces r repeats that pass on the new r.
+template <typename T>
+struct Steps {
+  vxl_solver* s;
+  double tol;
+  hipStream_t st;
+  int R;
+  int64_t len, n;
+  T *x, *r, *p, *q;
+  int nblk, nblk_st, nblk_rz, nblk_fx;
+  bool jacobi;
+  const T* dinv;
+  std::function<int()> form_beta;
+
+  Steps(vxl_solver* sv, double t)
+      : s(sv), tol(t), st(sv->cell->stream), R(sv->R), len(sv->len), n(sv->cell->grid.n),
+        x(static_cast<T*>(sv->x)), r(static_cast<T*>(sv->r)), p(static_cast<T*>(sv->p)),
+        q(static_cast<T*>(sv->q)), nblk(sv->nblk), nblk_st(apply_cg_blocks<T>(sv->cell, sv->R)),
+        nblk_rz(precond_rz_blocks(&sv->pc)), jacobi(sv->precond_kind == VXL_PRECOND_JACOBI),
+        dinv(nullptr) {
+    if (jacobi)
+      dinv = std::is_same<T, double>::value ? reinterpret_cast<const T*>(s->dinv64)
+                                            : reinterpret_cast<const T*>(s->dinv32);
+    nblk_fx = jacobi ? 0 : precond_fused_update_blocks(&s->pc);
+    vxl_solver* so = s;
+    const int nrz = nblk_rz;
+    hipStream_t stream = st;
+    form_beta = [so, nrz, stream]() -> int {
+      return launch_finalize(so->partial_rz, nrz, kOpBeta, so->scal, so->R, so->active_dev, so->restart_dev, stream);
+    };
+  }
+  int jacobi_step() {
+    return jacobi_cg_step<T>(r, p, dinv, n, R, s->partial, s->jmean, s->jrz, s->scal, s->active_dev,
+                             s->restart_dev, st);
+  }
+  // [stencil, alpha, update, ||r||^2 with the latch at iteration `iter`]
+  int cg_update(int iter, bool& forward_done) {
+    int* act = s->active_dev;
+    double* scal = s->scal;
+    VXL_TRY(launch_apply_cg<T>(s->cell, R, p, q, act, s->partial_st, nullptr, st));
+    VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st));
+    forward_done = false;
+    if (nblk_fx > 0) {
+      VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, x, p, q, scal + kAlpha * R, s->partial_fx, act, st));
+      VXL_TRY(launch_finalize(s->partial_fx, nblk_fx, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
+      forward_done = true;
+    } else {
+      VXL_TRY(launch_update_xr<T>(x, r, p, q, len, R, scal + kAlpha * R, s->partial, act, st));
+      VXL_TRY(launch_finalize(s->partial, nblk, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
+    }
+    return VXL_OK;
+  }
+  // [FFT passes of M r, beta, p = M r + beta p] (or the Jacobi step); the
+  // beta kernel consumes the restart flags on the device.
+  int precond_step(bool forward_done) {
+    if (jacobi) return jacobi_step();
+    int* act = s->active_dev;
+    if (!forward_done)
+      VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, nullptr, nullptr, nullptr, act,
+                                    st));
+    return precond_cg_rest<T>(&s->pc, R, p, s->partial_rz, s->scal + kBeta * R, form_beta, act, st);
+  }
+};
+
+// Blocks of kCheckStride iterations between host checks replay one captured
+// graph: the preconditioner of the checked iteration, then kCheckStride
+// iterations up to their ||r||^2.  No host work happens inside a block
+// (flags, scalars and the latch live on the device), and the latch records
+// the iteration relative to the block's start.  The graph bakes in the
+// kernels and their arguments, so it is keyed on the latch tolerance and the
+// stencil schedule.  The fused forward pass is not captured.
+int graph_key(const vxl_solver* s) { return s->cell->variant[s->mode == VXL_SOLVE_MIXED ? 0 : 1]; }
+
+template <typename T>
+int prepare_block_graph(vxl_solver* s, double tol) {
+  if (s->block_graph && s->block_graph_tol == tol && s->block_graph_key == graph_key(s)) return VXL_OK;
+  if (s->block_graph) {
+    VXL_HIP(hipGraphExecDestroy(s->block_graph));
+    s->block_graph = nullptr;
+  }
+  Steps<T> steps(s, tol);
+  if (steps.nblk_fx > 0) return VXL_OK;
+  hipStream_t st = s->cell->stream;
+  hipGraph_t g = nullptr;
+  VXL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  int rc = steps.precond_step(false);
+  for (int j = 1; rc == VXL_OK && j <= kCheckStride; ++j) {
+    bool fd = false;
+    rc = steps.cg_update(j, fd);
+    if (rc == VXL_OK && j < kCheckStride) rc = steps.precond_step(fd);
+  }
+  const hipError_t end = hipStreamEndCapture(st, &g);
+  hipGraphExec_t exec = nullptr;
+  hipError_t inst = hipErrorUnknown;
+  if (rc == VXL_OK && end == hipSuccess && g) inst = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (rc != VXL_OK || end != hipSuccess || inst != hipSuccess) {
+    // Capture is an optimisation: on any failure this solver launches its
+    // blocks kernel by kernel from now on.
+    (void)hipGetLastError();
+    s->graph_failed = true;
+    return VXL_OK;
+  }
+  s->block_graph = exec;
+  s->block_graph_tol = tol;
+  s->block_graph_key = graph_key(s);
+  return VXL_OK;
+}
+
 template <typename T>
 int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, double* ch_host,
              int* iterations, int* restarts, double* residuals, double* u_out) {
@@ -153,22 +268,12 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     VXL_TRY(launch_convert<double, double>(s->r64, s->f, len, R, act, st));
   }
   // p = z = M r with rho = r.z: beta = 0 because rho_old = 0 (slots cleared).
-  const int nblk_st = apply_cg_blocks<T>(cell, R);
-  const int nblk_rz = precond_rz_blocks(&s->pc);
-  const std::function<int()> form_beta = [&]() -> int {
-    return launch_finalize(s->partial_rz, nblk_rz, kOpBeta, scal, R, act, s->restart_dev, st);
-  };
-  const bool jacobi = s->precond_kind == VXL_PRECOND_JACOBI;
-  const T* dinv = jacobi ? (std::is_same<T, double>::value ? reinterpret_cast<const T*>(s->dinv64)
-                                                            : reinterpret_cast<const T*>(s->dinv32))
-                         : nullptr;
-  auto jacobi_step = [&]() -> int {
-    return jacobi_cg_step<T>(r, p, dinv, n, R, partial, s->jmean, s->jrz, scal, act, s->restart_dev, st);
-  };
+  Steps<T> steps(s, tol);
+  const bool jacobi = steps.jacobi;
   if (jacobi)
-    VXL_TRY(jacobi_step());
+    VXL_TRY(steps.jacobi_step());
   else
-    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, p, s->partial_rz, scal + kBeta * R, form_beta, act, st));
+    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, p, s->partial_rz, scal + kBeta * R, steps.form_beta, act, st));
   VXL_HIP(hipEventRecord(s->ev[1], st));
 
   int it = 0;
@@ -204,77 +309,23 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     return VXL_OK;
   };
 
-  // One iteration: [stencil: q = K p, p.q] [alpha] [x += alpha p,
-  // r -= alpha q, ||r||^2] [latch] (host check / FP64 certification)
-  // [FFT passes of M r; the z pass reduces r.z] [beta] [last pass:
-  // p = M r + beta p].
-  // With four-step x passes the x/r update and ||r||^2 ride on the forward x
-  // pass of the next preconditioner apply (its input is the updated r); a
-  // certification that replaces r repeats that pass on the new r.
-  const int nblk_fx = jacobi ? 0 : precond_fused_update_blocks(&s->pc);
-  // [stencil, alpha, update, ||r||^2 with the latch at iteration `iter`]
-  auto cg_update = [&](int iter, bool& forward_done) -> int {
-    VXL_TRY(launch_apply_cg<T>(cell, R, p, q, act, s->partial_st, nullptr, st));
-    VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st));
-    forward_done = false;
-    if (nblk_fx > 0) {
-      VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, x, p, q, scal + kAlpha * R, s->partial_fx, act, st));
-      VXL_TRY(launch_finalize(s->partial_fx, nblk_fx, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
-      forward_done = true;
-    } else {
-      VXL_TRY(launch_update_xr<T>(x, r, p, q, len, R, scal + kAlpha * R, partial, act, st));
-      VXL_TRY(launch_finalize(partial, nblk, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
-    }
-    return VXL_OK;
-  };
-  // [FFT passes of M r, beta, p = M r + beta p] (or the Jacobi step); the
-  // beta kernel consumes the restart flags on the device.
-  auto precond_step = [&](bool forward_done) -> int {
-    if (jacobi) return jacobi_step();
-    if (!forward_done)
-      VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, nullptr, nullptr, nullptr, act,
-                                    st));
-    return precond_cg_rest<T>(&s->pc, R, p, s->partial_rz, scal + kBeta * R, form_beta, act, st);
-  };
-  // Blocks of kCheckStride iterations between host checks replay one
-  // captured graph: no host work happens inside a block (flags, scalars and
-  // the latch live on the device), and the latch records the iteration
-  // relative to the block's start.  The fused forward pass is not captured.
-  const bool graphs = nblk_fx == 0 && graphs_enabled();
-  auto block_graph = [&]() -> int {
-    if (s->block_graph && s->block_graph_tol == tol) return VXL_OK;
-    if (s->block_graph) {
-      VXL_HIP(hipGraphExecDestroy(s->block_graph));
-      s->block_graph = nullptr;
-    }
-    hipGraph_t g = nullptr;
-    VXL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = precond_step(false);
-    for (int j = 1; rc == VXL_OK && j <= kCheckStride; ++j) {
-      bool fd = false;
-      rc = cg_update(j, fd);
-      if (rc == VXL_OK && j < kCheckStride) rc = precond_step(fd);
-    }
-    const hipError_t end = hipStreamEndCapture(st, &g);
-    if (rc != VXL_OK) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    VXL_HIP(end);
-    const hipError_t inst = hipGraphInstantiate(&s->block_graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (inst != hipSuccess) s->block_graph = nullptr;
-    VXL_HIP(inst);
-    s->block_graph_tol = tol;  // the latch threshold is a kernel argument
-    return VXL_OK;
-  };
+  const bool graphs = steps.nblk_fx == 0 && graphs_enabled() && !s->graph_failed;
+  auto cg_update = [&](int iter, bool& fd) { return steps.cg_update(iter, fd); };
+  auto precond_step = [&](bool fd) { return steps.precond_step(fd); };
+  auto block_graph = [&]() { return prepare_block_graph<T>(s, tol); };
 
   bool forward_done = false, pending = false;  // pending: M r of iteration `it` not formed yet
+  s->graph_blocks = 0;
   while (any_active() && it < max_iter) {
     int base = -1;  // >= 0: this block ran as a graph starting after iteration `base`
-    if (pending && graphs && it % kCheckStride == 0 && it + kCheckStride <= max_iter) {
+    bool replay = pending && graphs && it % kCheckStride == 0 && it + kCheckStride <= max_iter;
+    if (replay) {
       VXL_TRY(block_graph());
+      replay = s->block_graph != nullptr;
+    }
+    if (replay) {
       VXL_HIP(hipGraphLaunch(s->block_graph, st));
+      ++s->graph_blocks;
       base = it;
       it += kCheckStride;
       std::fill(restart.begin(), restart.end(), 0);
@@ -422,6 +473,11 @@ int solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter
 }
 
 int solver_release(vxl_solver* s) { return free_all(s); }
+
+int solver_prepare_graph(vxl_solver* s, double tol) {
+  if (!graphs_enabled() || s->graph_failed) return VXL_OK;
+  return s->mode == VXL_SOLVE_MIXED ? prepare_block_graph<float>(s, tol) : prepare_block_graph<double>(s, tol);
+}
 
 int solver_set_preconditioner(vxl_solver* s, int kind) {
   if (s->block_graph) {  // captured with the old preconditioner's kernels

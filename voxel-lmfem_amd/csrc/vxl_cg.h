@@ -129,6 +129,9 @@ struct vxl_solver {
   // changes).
   hipGraphExec_t block_graph;
   double block_graph_tol;
+  int block_graph_key;  // stencil schedule (and tile) the graph was captured with
+  bool graph_failed;    // capture unavailable: blocks run kernel by kernel
+  int graph_blocks;     // blocks replayed from the graph in the last run
   // Per-system residual history of the last run: (iteration, relative
   // residual); a negative iteration marks an FP64 true-residual evaluation.
   std::vector<std::vector<std::pair<int, double>>> trace;

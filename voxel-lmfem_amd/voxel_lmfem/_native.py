@@ -57,6 +57,7 @@ SIGNATURES = {
     "vxl_solver_run_batch": (_I, [_PP, _I, _DP, _D, _I, _DP, _IP, _IP, _DP, _I]),
     "vxl_cell_inverse_diagonal": (_I, [_P, _P]),
     "vxl_solver_set_preconditioner": (_I, [_P, _I]),
+    "vxl_solver_graph_blocks": (_I, [_P]),
     "vxl_solver_timings": (_I, [_P, _DP]),
     "vxl_solver_trace": (_I, [_P, _I, _IP, _DP, _I]),
     "vxl_solver_destroy": (_I, [_P]),

@@ -287,6 +287,7 @@ class Solver:
         self.cell.lib.vxl_solver_timings(self.handle, ms)
         out = dict(CH=ch, iterations=list(its), restarts=list(rst), true_residuals=list(res),
                    device_ms=dict(setup_rhs=ms[0], iterations=ms[1], certify_stress=ms[2]),
+                   graph_blocks=int(self.cell.lib.vxl_solver_graph_blocks(self.handle)),
                    trace=[self.trace(k) for k in range(self.nrhs)])
         if fields is not None:
             out["fields"] = fields
@@ -342,8 +343,9 @@ def run_batch(solvers, macro=None, *, tol=1e-8, max_iter=10000, workers=4):
                                    ch.ctypes.data_as(_native._DP), its, rst, res, int(workers)),
           "vxl_solver_run_batch")
     out, o = [], 0
-    for R in Rs:
+    for s, R in zip(solvers, Rs):
         out.append(dict(CH=ch[6 * o:6 * (o + R)].reshape(6, R).copy(), iterations=list(its[o:o + R]),
-                        restarts=list(rst[o:o + R]), true_residuals=list(res[o:o + R])))
+                        restarts=list(rst[o:o + R]), true_residuals=list(res[o:o + R]),
+                        graph_blocks=int(lib.vxl_solver_graph_blocks(s.handle))))
         o += R
     return out

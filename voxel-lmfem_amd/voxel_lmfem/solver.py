@@ -92,7 +92,7 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
     result = dict(seconds=seconds, CH=np.asarray(out["CH"]).tolist(), calls=calls, stages=stages,
                   field_statistics=field_stats, peak_allocated_bytes=int(torch.cuda.max_memory_allocated()),
                   peak_reserved_bytes=int(torch.cuda.max_memory_reserved()),
-                  device_ms=out["device_ms"])
+                  device_ms=out["device_ms"], graph_blocks=out["graph_blocks"])
     rows = int(getattr(field, "row_count", len(field.rows)))
     storage = dict(mode="label_rows", n_elements=int(phase.size), pool_rows=rows,
                    index_bytes=int(cell.index.numel() * cell.index.element_size()),
@@ -192,6 +192,7 @@ def homogenize_batch(cells, *, precision="mixed", tol=TOL, streams=4, workers=4)
         if not np.isfinite(residuals).all() or max(residuals) > tol:
             raise RuntimeError(f"True-residual check failed: {residuals}")
         results.append(dict(CH=out["CH"].tolist(), grid_shape=list(shape), max_true_residual=max(residuals),
+                            graph_blocks=out["graph_blocks"],
                             calls=[dict(iterations=int(i), restarts=int(r), true_residuals=[float(t)])
                                    for i, r, t in zip(out["iterations"], out["restarts"], out["true_residuals"])],
                             reference_medium=meta, voigt_order=["11", "22", "33", "23", "13", "12"]))
