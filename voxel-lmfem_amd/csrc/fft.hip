@@ -609,11 +609,12 @@ fft_y4(SC* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict
   using C = typename CT<T>::type;
   constexpr int n = N1 * N2;
   extern __shared__ double2 smem_raw[];
-  if (active && !active[blockIdx.z]) return;
+  const uint3 b = xcd_block3();
+  if (active && !active[b.z]) return;
   C* L = reinterpret_cast<C*>(smem_raw);
-  const int j0 = blockIdx.x * TJ;
+  const int j0 = b.x * TJ;
   const int tj = min(TJ, W - j0);
-  SC* plane = S + int64_t(blockIdx.z) * spec_stride + int64_t(blockIdx.y) * n * W + j0;
+  SC* plane = S + int64_t(b.z) * spec_stride + int64_t(b.y) * n * W + j0;
   const int p = threadIdx.x % TJ, t = threadIdx.x / TJ;
   const bool col = p < tj;
   if (t < N2)
@@ -635,15 +636,16 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
   constexpr int nz = N1 * N2;
   extern __shared__ double2 smem_raw[];
   __shared__ double red[4];
-  if (active && !active[blockIdx.z]) return;
+  const uint3 b = xcd_block3();
+  if (active && !active[b.z]) return;
   C* L = reinterpret_cast<C*>(smem_raw);
   const int TJ = 3 * TK;
-  const int k0 = blockIdx.x * TK;
-  const int y = blockIdx.y;
+  const int k0 = b.x * TK;
+  const int y = b.y;
   const int tk = min(TK, nxh - k0);
   const int W = 3 * nxh;
   const int64_t zstride = int64_t(ny) * W;
-  SC* col = S + int64_t(blockIdx.z) * spec_stride + int64_t(y) * W + 3 * k0;
+  SC* col = S + int64_t(b.z) * spec_stride + int64_t(y) * W + 3 * k0;
   const int p = threadIdx.x % TJ, t = threadIdx.x / TJ;
   const bool live = p < 3 * tk;
   const C* w = tb.wz;
@@ -679,7 +681,7 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
   if constexpr (kDot) {
     const double total = block_reduce_dyn(dot, red);
     if (threadIdx.x == 0)
-      partial[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = total;
+      partial[(int64_t(b.z) * gridDim.y + b.y) * gridDim.x + b.x] = total;
   } else {
     __syncthreads();
   }

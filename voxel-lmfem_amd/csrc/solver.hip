@@ -48,6 +48,8 @@ int free_all(vxl_solver* s) {
   if (s->cell) (void)hipStreamSynchronize(s->cell->stream);
   if (s->block_graph) (void)hipGraphExecDestroy(s->block_graph);
   s->block_graph = nullptr;
+  if (s->capture_stream) (void)hipStreamDestroy(s->capture_stream);
+  s->capture_stream = nullptr;
   double* bufs[] = {s->f, s->x64, s->r64, s->q64, s->scal, s->partial, s->macro_dev,
                     s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz, s->rhs_sig, s->partial_fx,
                     s->dinv64, s->jmean, s->jrz};
@@ -93,8 +95,8 @@ struct Steps {
   const T* dinv;
   std::function<int()> form_beta;
 
-  Steps(vxl_solver* sv, double t)
-      : s(sv), tol(t), st(sv->cell->stream), R(sv->R), len(sv->len), n(sv->cell->grid.n),
+  Steps(vxl_solver* sv, double t, hipStream_t stream)
+      : s(sv), tol(t), st(stream), R(sv->R), len(sv->len), n(sv->cell->grid.n),
         x(static_cast<T*>(sv->x)), r(static_cast<T*>(sv->r)), p(static_cast<T*>(sv->p)),
         q(static_cast<T*>(sv->q)), nblk(sv->nblk), nblk_st(apply_cg_blocks<T>(sv->cell, sv->R)),
         nblk_rz(precond_rz_blocks(&sv->pc)), jacobi(sv->precond_kind == VXL_PRECOND_JACOBI),
@@ -105,9 +107,9 @@ struct Steps {
     nblk_fx = jacobi ? 0 : precond_fused_update_blocks(&s->pc);
     vxl_solver* so = s;
     const int nrz = nblk_rz;
-    hipStream_t stream = st;
-    form_beta = [so, nrz, stream]() -> int {
-      return launch_finalize(so->partial_rz, nrz, kOpBeta, so->scal, so->R, so->active_dev, so->restart_dev, stream);
+    hipStream_t on = st;
+    form_beta = [so, nrz, on]() -> int {
+      return launch_finalize(so->partial_rz, nrz, kOpBeta, so->scal, so->R, so->active_dev, so->restart_dev, on);
     };
   }
   int jacobi_step() {
@@ -152,6 +154,9 @@ struct Steps {
 // stencil schedule.  The fused forward pass is not captured.
 int graph_key(const vxl_solver* s) { return s->cell->variant[s->mode == VXL_SOLVE_MIXED ? 0 : 1]; }
 
+// The block is recorded on a private non-blocking stream (the cell's stream
+// may be the legacy default stream, which cannot be captured) and replayed
+// on the cell's stream.
 template <typename T>
 int prepare_block_graph(vxl_solver* s, double tol) {
   if (s->block_graph && s->block_graph_tol == tol && s->block_graph_key == graph_key(s)) return VXL_OK;
@@ -159,11 +164,20 @@ int prepare_block_graph(vxl_solver* s, double tol) {
     VXL_HIP(hipGraphExecDestroy(s->block_graph));
     s->block_graph = nullptr;
   }
-  Steps<T> steps(s, tol);
+  auto give_up = [&]() {  // capture is an optimisation: launch kernel by kernel instead
+    (void)hipGetLastError();
+    s->graph_failed = true;
+    return VXL_OK;
+  };
+  if (!s->capture_stream && hipStreamCreateWithFlags(&s->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+    s->capture_stream = nullptr;
+    return give_up();
+  }
+  hipStream_t st = s->capture_stream;
+  Steps<T> steps(s, tol, st);
   if (steps.nblk_fx > 0) return VXL_OK;
-  hipStream_t st = s->cell->stream;
   hipGraph_t g = nullptr;
-  VXL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return give_up();
   int rc = steps.precond_step(false);
   for (int j = 1; rc == VXL_OK && j <= kCheckStride; ++j) {
     bool fd = false;
@@ -176,11 +190,8 @@ int prepare_block_graph(vxl_solver* s, double tol) {
   if (rc == VXL_OK && end == hipSuccess && g) inst = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
   if (g) (void)hipGraphDestroy(g);
   if (rc != VXL_OK || end != hipSuccess || inst != hipSuccess) {
-    // Capture is an optimisation: on any failure this solver launches its
-    // blocks kernel by kernel from now on.
-    (void)hipGetLastError();
-    s->graph_failed = true;
-    return VXL_OK;
+    set_error("");
+    return give_up();
   }
   s->block_graph = exec;
   s->block_graph_tol = tol;
@@ -268,7 +279,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     VXL_TRY(launch_convert<double, double>(s->r64, s->f, len, R, act, st));
   }
   // p = z = M r with rho = r.z: beta = 0 because rho_old = 0 (slots cleared).
-  Steps<T> steps(s, tol);
+  Steps<T> steps(s, tol, st);
   const bool jacobi = steps.jacobi;
   if (jacobi)
     VXL_TRY(steps.jacobi_step());

@@ -131,6 +131,7 @@ struct vxl_solver {
   double block_graph_tol;
   int block_graph_key;  // stencil schedule (and tile) the graph was captured with
   bool graph_failed;    // capture unavailable: blocks run kernel by kernel
+  hipStream_t capture_stream;  // private stream the block is recorded on (replayed on the cell's)
   int graph_blocks;     // blocks replayed from the graph in the last run
   // Per-system residual history of the last run: (iteration, relative
   // residual); a negative iteration marks an FP64 true-residual evaluation.

@@ -27,6 +27,20 @@ __host__ __device__ constexpr int pidx(int i, int j) {
   return i <= j ? i * 6 - (i * (i - 1)) / 2 + (j - i) : j * 6 - (j * (j - 1)) / 2 + (i - j);
 }
 
+// XCD-aware block order.  Workgroups are dealt to the 8 XCDs round-robin in
+// linear block order and each XCD has its own L2; this remap gives every XCD
+// a contiguous run of logical blocks (x fastest), so neighbouring blocks whose
+// rows share cache lines run on the same XCD at about the same time.  A
+// bijection on the grid (the last total % 8 blocks keep their ids).
+__device__ __forceinline__ uint3 xcd_block3() {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned total = gx * gy * gridDim.z;
+  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned full = total & ~7u;
+  if (lin < full) lin = (lin & 7u) * (full >> 3) + (lin >> 3);
+  return make_uint3(lin % gx, (lin / gx) % gy, lin / (gx * gy));
+}
+
 template <typename Idx>
 __device__ __forceinline__ int load_row(const void* index, int64_t v) {
   return int(static_cast<const Idx*>(index)[v]);

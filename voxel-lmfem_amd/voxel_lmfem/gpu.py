@@ -283,6 +283,9 @@ class Solver:
         rc = self.cell.lib.vxl_solver_run(self.handle, mp, float(tol), int(max_iter),
                                           ch.ctypes.data_as(_native._DP), its, rst, res,
                                           dptr(fields) if fields is not None else None)
+        if rc not in (_native.VXL_OK, _native.VXL_ERR_CONVERGENCE):
+            check(rc, "vxl_solver_run")  # before any other call replaces the error text
+        error = None if rc == _native.VXL_OK else self.cell.lib.vxl_last_error()
         ms = (ctypes.c_double * 3)()
         self.cell.lib.vxl_solver_timings(self.handle, ms)
         out = dict(CH=ch, iterations=list(its), restarts=list(rst), true_residuals=list(res),
@@ -291,7 +294,8 @@ class Solver:
                    trace=[self.trace(k) for k in range(self.nrhs)])
         if fields is not None:
             out["fields"] = fields
-        check(rc, "vxl_solver_run")
+        if error is not None:
+            raise _native.ConvergenceError(f"vxl_solver_run: {error.decode(errors='replace')}")
         return out
 
     def trace(self, k):
