@@ -103,6 +103,7 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8):
     largest cells first, `streams` HIP streams, `workers` host threads) and
     teardown."""
     import torch
+    from voxel_lmfem import autotune as tuner
     from voxel_lmfem.cells import random_inclusion_batch
     from voxel_lmfem import materials as pm
     from voxel_lmfem.gpu import Cell, Solver, run_batch
@@ -112,7 +113,9 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8):
     print(f"# generated {count} inclusion cells in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     resident = [(torch.from_numpy(ph.astype(np.int32)).to("cuda"), ph, mats, dom) for (ph, mats, dom, _), _ in cells]
     pool = [torch.cuda.Stream() for _ in range(streams)]
+    entries = tuner.load()  # autotuned stencil configurations, per grid shape
     torch.cuda.synchronize()
+    configs = {}
 
     def step():
         built = []
@@ -120,6 +123,8 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8):
             field = pm.build_field_device(lab, mats)
             lam, mu, _ = select(field, ph, mats)
             cell = Cell(field, ph.shape, dom / np.array(ph.shape[::-1], dtype=float), stream=pool[k % streams])
+            got = tuner.tune(cell, "float32", 6, entries=entries)  # tunes only on a cache miss (warm-up)
+            configs[ph.shape[0]] = (got["tile_y"], got["schedule"], got["zmin"])
             built.append((cell, Solver(cell, lam, mu, mode="mixed", nrhs=6)))
         outs = run_batch([b[1] for b in built], tol=1e-8, workers=workers)
         for cell, solver in built:
@@ -129,6 +134,7 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8):
 
     for _ in range(warmup):
         step()
+    tuner.save(entries)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -141,6 +147,9 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8):
     out = dict(workload=f"random_inclusion_batch_{count}_cells_edges_32-96_6loads", cells=count, voxels=voxels,
                cells_per_s=count / dt, ms_per_step=1e3 * dt, voxel_loads_per_s=6 * voxels / dt, steps=steps,
                warmup=warmup, streams=streams, workers=workers, iterations_mean=float(np.mean(its)),
+               stencil_config_by_edge={str(k): dict(tile=f"32x{t}", schedule=v, zmin=z)
+                                       for k, (t, v, z) in sorted(configs.items())},
+               graph_blocks=int(sum(o["graph_blocks"] for o in outs)),
                iterations_max=max(its), max_true_residual=worst)
     print(f"# batch: {1e3 * dt:.1f} ms per {count}-cell batch ({count / dt:.1f} cells/s), iterations mean "
           f"{np.mean(its):.1f} max {max(its)}, worst residual {worst:.2e}", file=sys.stderr, flush=True)

@@ -143,6 +143,25 @@ int vxl_set_stencil_variant(vxl_cell* cell, int precision, int variant);
 int vxl_get_stencil_variant(vxl_cell* cell, int precision);
 int vxl_tune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev, int reps,
                      int* best, double* ms3);
+/* Full stencil configuration per precision: tile height (workgroup tile of
+ * 32 x tile_y output nodes; VXL_F32: 14 (default), 8 or 6; VXL_F64: 8
+ * (default), 6 or 4), schedule (1-3, as above) and the fewest z planes per
+ * workgroup chunk (2..64, default 8; each chunk recomputes one element
+ * layer, and shallower chunks give small cells more workgroups).  Every
+ * configuration computes the same element sums.  No reference counterpart
+ * (see vxl_tune_stencil). */
+int vxl_set_stencil_config(vxl_cell* cell, int precision, int tile_y, int variant, int zmin);
+int vxl_get_stencil_config(vxl_cell* cell, int precision, int* tile_y, int* variant, int* zmin);
+/* Autotuner: times all 27 (tile, schedule, chunk depth in {8, 4, 2})
+ * configurations on the given buffers (reps launches each, HIP events on the
+ * cell's stream), keeps the fastest on the cell and returns it.  ms_table
+ * (27 entries, may be NULL) receives the per-launch milliseconds at
+ * [(tile_index * 3 + schedule - 1) * 3 + depth_index], tiles in the order
+ * listed above.  The persistent cache of tuned configurations (JSON keyed by
+ * grid, precision, load count and material index width) lives in the Python
+ * layer (voxel_lmfem.autotune). */
+int vxl_autotune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev, int reps,
+                         int* tile_y, int* variant, int* zmin, double* ms_table);
 
 /* f = -V sum_e B0^T C_e E for nload macro strains (host, nload x 6), FP64.
  * Replaces `rhs`  hybrid_kernels.cu:128-137 (rhs_kernel :49-68). */

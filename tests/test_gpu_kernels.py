@@ -207,3 +207,66 @@ def test_precond_matches_oracle(torch_gpu, shape, monkeypatch):
             assert err < tol
             pc.close()
     cell.close()
+
+
+TILES = {"float32": (14, 8, 6), "float64": (8, 6, 4)}
+
+
+@pytest.mark.parametrize("shape,cubic", [((9, 20, 40), True), ((7, 17, 33), False)])
+def test_every_stencil_config_matches_oracle(torch_gpu, shape, cubic):
+    """All 27 (tile, schedule, chunk depth) configurations per precision; cubic
+    voxels take the folded-factor element form."""
+    torch = torch_gpu
+    from voxel_lmfem.gpu import Cell
+    print(f"case stencil configs shape={shape} cubic={cubic}", flush=True)
+    field, full, h = make_cell(shape, seed=sum(shape) + 1)
+    if cubic:
+        h = np.full(3, 0.7)
+    cell = Cell(field, shape, h)
+    op = fem.MatrixFree(full, shape, h)
+    n = int(np.prod(shape))
+    U = np.random.default_rng(2).normal(size=(2, n, 3))
+    want = np.stack([planes(op.apply(u)) for u in U])
+    scale = np.abs(want).max()
+    u64 = torch.tensor(np.stack([planes(u) for u in U]), device=cell.device)
+    worst = {"float32": 0.0, "float64": 0.0}
+    for precision, tol in (("float64", 1e-13), ("float32", 2e-6)):
+        uu = u64 if precision == "float64" else u64.float()
+        for ty in TILES[precision]:
+            for v in (1, 2, 3):
+                for z in (8, 4, 2):
+                    cell.set_stencil_config(precision, ty, v, z)
+                    assert cell.stencil_config(precision) == (ty, v, z)
+                    got = cell.apply_planes(uu).double().cpu().numpy()
+                    err = np.abs(got - want).max() / scale
+                    worst[precision] = max(worst[precision], err)
+                    assert err < tol, (precision, ty, v, z, err)
+    print(f"  worst fp64 {worst['float64']:.2e} fp32 {worst['float32']:.2e}", flush=True)
+    with pytest.raises(ValueError):
+        cell.set_stencil_config("float32", 16, 2, 8)   # not a supported tile
+    with pytest.raises(ValueError):
+        cell.set_stencil_config("float64", 8, 2, 1)    # chunk depth below 2
+    cell.close()
+
+
+def test_autotune_and_persistent_cache(torch_gpu, tmp_path):
+    from voxel_lmfem import autotune
+    from voxel_lmfem.gpu import Cell
+    shape = (16, 24, 32)
+    print(f"case autotune shape={shape}", flush=True)
+    field, full, h = make_cell(shape, seed=5)
+    cell = Cell(field, shape, h)
+    (ty, v, z), table = cell.autotune("float32", nrhs=6, reps=2)
+    assert (table > 0).all()
+    i, j, k = np.unravel_index(np.argmin(table), table.shape)
+    assert (TILES["float32"][i], j + 1, (8, 4, 2)[k]) == (ty, v, z)
+    assert cell.stencil_config("float32") == (ty, v, z)
+    path = tmp_path / "tuning.json"
+    first = autotune.tune(cell, "float32", 6, path=path, reps=2)
+    assert first["tuned"] and path.is_file()
+    cell.set_stencil_config("float32", 14, 1, 8)
+    second = autotune.tune(cell, "float32", 6, path=path)
+    assert not second["tuned"]
+    assert cell.stencil_config("float32") == (first["tile_y"], first["schedule"], first["zmin"])
+    print(f"  tuned {first}", flush=True)
+    cell.close()

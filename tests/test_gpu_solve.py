@@ -117,3 +117,28 @@ def test_graph_blocks_match_eager_launches(torch_gpu, monkeypatch, precision, pr
     assert [c["trace"] for c in a["calls"]] == [c["trace"] for c in b["calls"]]
     assert max(c["iterations"] for c in b["calls"]) > 8
     assert a["graph_blocks"] == 0 and b["graph_blocks"] > 0  # the blocks really were replayed
+
+
+def test_solve_under_other_stencil_configs(torch_gpu):
+    """The CG form of the stencil (p.Kp from element energies) under other tiles,
+    schedules and chunk depths: same certified C_eff; the block graph is
+    re-captured for the new configuration."""
+    from voxel_lmfem import materials as pm
+    from voxel_lmfem.cells import voronoi_polycrystal
+    from voxel_lmfem.gpu import Cell, Solver
+    (phase, mats, dom, _), _ = voronoi_polycrystal(n=24, grains=10, seed=8)
+    field = pm.build_field(phase, mats)
+    cell = Cell(field, phase.shape, dom / 24)
+    solver = Solver(cell, 100.0, 40.0, mode="mixed")
+    base = solver.run()
+    print("case stencil configs in the solve", flush=True)
+    for ty, v, z in ((8, 3, 2), (6, 2, 4), (14, 3, 8)):
+        cell.set_stencil_config("float32", ty, v, z)
+        cell.set_stencil_config("float64", (8, 6, 4)[(14, 8, 6).index(ty)], v, z)
+        out = solver.run()
+        rel = np.linalg.norm(out["CH"] - base["CH"]) / np.linalg.norm(base["CH"])
+        print(f"  tile {ty} schedule {v} zmin {z}: rel {rel:.2e}, its {out['iterations']}", flush=True)
+        assert rel < 1e-8 and max(out["true_residuals"]) <= 1e-8
+        assert out["graph_blocks"] > 0
+    solver.close()
+    cell.close()

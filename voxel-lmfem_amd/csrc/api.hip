@@ -119,6 +119,9 @@ int cell_create_common(vxl_cell** out, int nz, int ny, int nx, double hx, double
   c->c32_owned = nullptr;
   c->variant[0] = default_stencil_variant(VXL_F32);
   c->variant[1] = default_stencil_variant(VXL_F64);
+  c->tile_y[0] = default_stencil_tile(VXL_F32);
+  c->tile_y[1] = default_stencil_tile(VXL_F64);
+  c->zmin[0] = c->zmin[1] = kDefaultChunkPlanes;
   (void)hipGetDevice(&c->device);
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(c->stream);
@@ -344,6 +347,84 @@ int vxl_tune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev,
   if (best) *best = best_v;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  return VXL_OK;
+}
+
+int vxl_set_stencil_config(vxl_cell* cell, int precision, int tile_y, int variant, int zmin) {
+  g_error.clear();
+  if (!cell || (precision != VXL_F32 && precision != VXL_F64))
+    return fail("vxl_set_stencil_config: invalid arguments");
+  if (!stencil_tile_supported(precision, tile_y)) return fail("vxl_set_stencil_config: unsupported tile height");
+  if (variant < 1 || variant > 3) return fail("vxl_set_stencil_config: schedule must be 1, 2 or 3");
+  if (zmin < kMinChunkPlanes || zmin > 64) return fail("vxl_set_stencil_config: chunk depth must lie in [2, 64]");
+  const int slot = precision == VXL_F64 ? 1 : 0;
+  cell->tile_y[slot] = tile_y;
+  cell->variant[slot] = variant;
+  cell->zmin[slot] = zmin;
+  return VXL_OK;
+}
+
+int vxl_get_stencil_config(vxl_cell* cell, int precision, int* tile_y, int* variant, int* zmin) {
+  g_error.clear();
+  if (!cell || (precision != VXL_F32 && precision != VXL_F64))
+    return fail("vxl_get_stencil_config: invalid arguments");
+  const int slot = precision == VXL_F64 ? 1 : 0;
+  if (tile_y) *tile_y = cell->tile_y[slot];
+  if (variant) *variant = cell->variant[slot];
+  if (zmin) *zmin = cell->zmin[slot];
+  return VXL_OK;
+}
+
+int vxl_autotune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev, int reps,
+                         int* tile_y, int* variant, int* zmin, double* ms_table) {
+  g_error.clear();
+  if (!cell || !u_dev || !out_dev || nrhs < 1 || reps < 1 || (precision != VXL_F32 && precision != VXL_F64))
+    return fail("vxl_autotune_stencil: invalid arguments");
+  const int slot = precision == VXL_F64 ? 1 : 0;
+  const int depths[3] = {kDefaultChunkPlanes, 4, kMinChunkPlanes};
+  hipEvent_t e0, e1;
+  VXL_HIP(hipEventCreate(&e0));
+  VXL_HIP(hipEventCreate(&e1));
+  const int keep[3] = {cell->tile_y[slot], cell->variant[slot], cell->zmin[slot]};
+  int best[3] = {keep[0], keep[1], keep[2]};
+  double best_ms = 1e300;
+  int rc = VXL_OK;
+  for (int t = 0; t < 3 && rc == VXL_OK; ++t)
+    for (int v = 1; v <= 3 && rc == VXL_OK; ++v)
+      for (int d = 0; d < 3 && rc == VXL_OK; ++d) {
+        cell->tile_y[slot] = stencil_tile_option(precision, t);
+        cell->variant[slot] = v;
+        cell->zmin[slot] = depths[d];
+        rc = vxl_apply(cell, precision, nrhs, u_dev, out_dev);  // warm-up
+        if (rc != VXL_OK) break;
+        if (hipEventRecord(e0, cell->stream) != hipSuccess) rc = fail("hipEventRecord failed");
+        for (int i = 0; i < reps && rc == VXL_OK; ++i) rc = vxl_apply(cell, precision, nrhs, u_dev, out_dev);
+        if (rc != VXL_OK) break;
+        float ms = 0;
+        if (hipEventRecord(e1, cell->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = fail("HIP event timing failed in vxl_autotune_stencil");
+          break;
+        }
+        const double per = double(ms) / reps;
+        if (ms_table) ms_table[(t * 3 + (v - 1)) * 3 + d] = per;
+        if (per < best_ms) {
+          best_ms = per;
+          best[0] = cell->tile_y[slot];
+          best[1] = v;
+          best[2] = depths[d];
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const int* use = rc == VXL_OK ? best : keep;
+  cell->tile_y[slot] = use[0];
+  cell->variant[slot] = use[1];
+  cell->zmin[slot] = use[2];
+  if (rc != VXL_OK) return rc;
+  if (tile_y) *tile_y = best[0];
+  if (variant) *variant = best[1];
+  if (zmin) *zmin = best[2];
   return VXL_OK;
 }
 

@@ -152,7 +152,10 @@ struct Steps {
 // the iteration relative to the block's start.  The graph bakes in the
 // kernels and their arguments, so it is keyed on the latch tolerance and the
 // stencil schedule.  The fused forward pass is not captured.
-int graph_key(const vxl_solver* s) { return s->cell->variant[s->mode == VXL_SOLVE_MIXED ? 0 : 1]; }
+int graph_key(const vxl_solver* s) {
+  const int k = s->mode == VXL_SOLVE_MIXED ? 0 : 1;
+  return s->cell->variant[k] + 4 * (s->cell->tile_y[k] + 64 * s->cell->zmin[k]);
+}
 
 // The block is recorded on a private non-blocking stream (the cell's stream
 // may be the legacy default stream, which cannot be captured) and replayed
@@ -448,8 +451,8 @@ int solver_create(vxl_solver* s) {
   const size_t np = size_t(std::max(s->nblk * R, 3 * R * (nblk_rows + 1)));
   VXL_HIP(dev_malloc(&s->partial, sizeof(double) * np));
   {
-    const int st_blocks = s->mode == VXL_SOLVE_MIXED ? apply_cg_blocks<float>(cell, R)
-                                                     : apply_cg_blocks<double>(cell, R);
+    // sized for any tile / chunk depth the cell can be switched to
+    const int st_blocks = apply_cg_blocks_max(cell, R, s->mode == VXL_SOLVE_MIXED ? VXL_F32 : VXL_F64);
     VXL_HIP(dev_malloc(&s->partial_st, sizeof(double) * size_t(st_blocks) * R));
   }
   VXL_HIP(dev_malloc(&s->scal, sizeof(double) * kNumSlots * R));

@@ -236,19 +236,14 @@ __device__ __forceinline__ BlockId xcd_block() {
   return b;
 }
 
-// Output-node tile of one workgroup: 32 x TY.  FP32: 32 x 14 nodes = 33 x 15
-// elements = 8 waves, so two workgroups (16 waves, 4 per SIMD) share a CU at
-// up to 128 VGPRs; 32 x 16 (9 waves) fits only one at the 96 VGPRs that
-// pairing would need, and ran 1.55x slower.  VXL_TILE_Y_F32 / VXL_TILE_Y_F64
-// override at build time.
-#ifndef VXL_TILE_Y_F32
-#define VXL_TILE_Y_F32 14
-#endif
-#ifndef VXL_TILE_Y_F64
-#define VXL_TILE_Y_F64 8
-#endif
-constexpr int kTileYF32 = VXL_TILE_Y_F32;
-constexpr int kTileYF64 = VXL_TILE_Y_F64;
+// Output-node tile of one workgroup: 32 x TY, TY from a short list per
+// precision (the first entry is the default; the autotuner picks per cell).
+// FP32 32 x 14 nodes = 33 x 15 elements = 8 waves, so two workgroups (16
+// waves, 4 per SIMD) share a CU at up to 128 VGPRs; 32 x 16 (9 waves) fits
+// only one at the 96 VGPRs that pairing would need, and ran 1.55x slower.
+// Smaller tiles give small cells more workgroups.
+constexpr int kTilesF32[3] = {14, 8, 6};
+constexpr int kTilesF64[3] = {8, 6, 4};
 
 template <int TX, int TY>
 struct Tile {
@@ -624,14 +619,17 @@ G9<T> cast_g(const ModalFactors& f) {
 }
 
 template <typename T, int TX, int TY>
-void apply_grid(const vxl_cell* cell, int nrhs, int& tiles_x, int& tiles, int& chunks, int& zchunk) {
+void apply_grid(const vxl_cell* cell, int nrhs, int& tiles_x, int& tiles, int& chunks, int& zchunk,
+                int zmin = 0) {
   const int nx = cell->grid.nx, ny = cell->grid.ny, nz = cell->grid.nz;
   tiles_x = (nx + TX - 1) / TX;
   const int tiles_y = (ny + TY - 1) / TY;
   tiles = tiles_x * tiles_y;
-  // Enough workgroups to fill 256 CUs several times; chunks of >= 8 planes.
+  // Enough workgroups to fill 256 CUs several times; chunks of >= zmin
+  // planes (default 8: the recomputed layer per chunk stays <= 1/8).
+  if (zmin <= 0) zmin = cell->zmin[std::is_same<T, double>::value ? 1 : 0];
   chunks = (4096 + tiles * nrhs - 1) / (tiles * nrhs);
-  chunks = std::max(1, std::min(chunks, nz / 8));
+  chunks = std::max(1, std::min(chunks, nz / std::max(1, zmin)));
   zchunk = (nz + chunks - 1) / chunks;
   chunks = (nz + zchunk - 1) / zchunk;
 }
@@ -919,37 +917,86 @@ int default_stencil_variant(int precision) {
   return precision == VXL_F64 ? 3 : 2;
 }
 
+// Runtime tile choice -> template instance.
+template <typename T, typename F>
+int with_tile(const vxl_cell* cell, F&& f) {
+  constexpr bool d = std::is_same<T, double>::value;
+  const int ty = cell->tile_y[d ? 1 : 0];
+  const auto& list = d ? kTilesF64 : kTilesF32;
+  if (ty == list[0]) return f(std::integral_constant<int, (d ? kTilesF64[0] : kTilesF32[0])>());
+  if (ty == list[1]) return f(std::integral_constant<int, (d ? kTilesF64[1] : kTilesF32[1])>());
+  if (ty == list[2]) return f(std::integral_constant<int, (d ? kTilesF64[2] : kTilesF32[2])>());
+  return fail("unsupported stencil tile height");
+}
+
+template <typename T>
+int launch_apply_t(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s) {
+  return with_tile<T>(cell, [&](auto ty) { return dispatch_apply<T, 32, decltype(ty)::value>(cell, nrhs, u, out, s); });
+}
+template <typename T>
+int launch_apply_cg_t(const vxl_cell* cell, int nrhs, const T* p, T* q, const int* active, double* partial,
+                      int* blocks, hipStream_t s) {
+  return with_tile<T>(cell, [&](auto ty) {
+    return dispatch_apply<T, 32, decltype(ty)::value>(cell, nrhs, p, q, s, active, partial, blocks);
+  });
+}
+template <typename T>
+int cg_blocks_t(const vxl_cell* cell, int nrhs, int zmin) {
+  return with_tile<T>(cell, [&](auto ty) {
+    int tx, tiles, chunks, zc;
+    apply_grid<T, 32, decltype(ty)::value>(cell, nrhs, tx, tiles, chunks, zc, zmin);
+    return tiles * chunks;
+  });
+}
+
 template <>
 int launch_apply<float>(const vxl_cell* cell, int nrhs, const float* u, float* out, hipStream_t s) {
-  return dispatch_apply<float, 32, kTileYF32>(cell, nrhs, u, out, s);
+  return launch_apply_t<float>(cell, nrhs, u, out, s);
 }
 template <>
 int launch_apply<double>(const vxl_cell* cell, int nrhs, const double* u, double* out, hipStream_t s) {
-  return dispatch_apply<double, 32, kTileYF64>(cell, nrhs, u, out, s);
+  return launch_apply_t<double>(cell, nrhs, u, out, s);
 }
 
 template <>
 int launch_apply_cg<float>(const vxl_cell* cell, int nrhs, const float* p, float* q, const int* active,
                            double* partial, int* blocks, hipStream_t s) {
-  return dispatch_apply<float, 32, kTileYF32>(cell, nrhs, p, q, s, active, partial, blocks);
+  return launch_apply_cg_t<float>(cell, nrhs, p, q, active, partial, blocks, s);
 }
 template <>
 int launch_apply_cg<double>(const vxl_cell* cell, int nrhs, const double* p, double* q, const int* active,
                             double* partial, int* blocks, hipStream_t s) {
-  return dispatch_apply<double, 32, kTileYF64>(cell, nrhs, p, q, s, active, partial, blocks);
+  return launch_apply_cg_t<double>(cell, nrhs, p, q, active, partial, blocks, s);
 }
 template <>
 int apply_cg_blocks<float>(const vxl_cell* cell, int nrhs) {
-  int tx, tiles, chunks, zc;
-  apply_grid<float, 32, kTileYF32>(cell, nrhs, tx, tiles, chunks, zc);
-  return tiles * chunks;
+  return cg_blocks_t<float>(cell, nrhs, 0);
 }
 template <>
 int apply_cg_blocks<double>(const vxl_cell* cell, int nrhs) {
-  int tx, tiles, chunks, zc;
-  apply_grid<double, 32, kTileYF64>(cell, nrhs, tx, tiles, chunks, zc);
-  return tiles * chunks;
+  return cg_blocks_t<double>(cell, nrhs, 0);
 }
+
+// Partial sums the CG form can need for any tile and chunk depth the cell may
+// be switched to later (the solver allocates them once).
+int apply_cg_blocks_max(const vxl_cell* cell, int nrhs, int precision) {
+  vxl_cell probe = *cell;
+  int most = 0;
+  const bool d = precision == VXL_F64;
+  for (int t = 0; t < 3; ++t) {
+    probe.tile_y[d ? 1 : 0] = d ? kTilesF64[t] : kTilesF32[t];
+    const int b = d ? cg_blocks_t<double>(&probe, nrhs, kMinChunkPlanes) : cg_blocks_t<float>(&probe, nrhs, kMinChunkPlanes);
+    most = std::max(most, b);
+  }
+  return most;
+}
+
+bool stencil_tile_supported(int precision, int tile_y) {
+  const auto& list = precision == VXL_F64 ? kTilesF64 : kTilesF32;
+  return tile_y == list[0] || tile_y == list[1] || tile_y == list[2];
+}
+int default_stencil_tile(int precision) { return precision == VXL_F64 ? kTilesF64[0] : kTilesF32[0]; }
+int stencil_tile_option(int precision, int k) { return precision == VXL_F64 ? kTilesF64[k] : kTilesF32[k]; }
 
 int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, double* sig, hipStream_t s) {
   const auto& g = cell->grid;

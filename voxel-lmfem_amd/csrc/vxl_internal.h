@@ -145,6 +145,8 @@ struct vxl_cell {
   hipStream_t stream;
   int device;
   int variant[2];  // stencil schedule per precision (VXL_F32, VXL_F64): 1, 2 or 3
+  int tile_y[2];   // stencil tile height (output nodes along y; the tile is 32 wide)
+  int zmin[2];     // fewest z planes per workgroup chunk (each chunk recomputes one layer)
 };
 
 namespace vxl {
@@ -160,6 +162,12 @@ int launch_apply_cg(const vxl_cell* cell, int nrhs, const T* p, T* q, const int*
                     int* blocks, hipStream_t s);
 template <typename T>
 int apply_cg_blocks(const vxl_cell* cell, int nrhs);
+constexpr int kDefaultChunkPlanes = 8;  // default fewest z planes per stencil chunk
+constexpr int kMinChunkPlanes = 2;      // smallest chunk depth a cell may be set to
+int apply_cg_blocks_max(const vxl_cell* cell, int nrhs, int precision);
+bool stencil_tile_supported(int precision, int tile_y);
+int default_stencil_tile(int precision);
+int stencil_tile_option(int precision, int k);  // k = 0, 1, 2
 int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, double* sig, hipStream_t s);
 // Doubles of the `sig` scratch of launch_rhs (rows x nload x 6 element
 // stresses); 0 means pass sig = nullptr (direct per-element form).

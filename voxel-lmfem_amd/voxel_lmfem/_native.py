@@ -44,6 +44,9 @@ SIGNATURES = {
     "vxl_set_stencil_variant": (_I, [_P, _I, _I]),
     "vxl_get_stencil_variant": (_I, [_P, _I]),
     "vxl_tune_stencil": (_I, [_P, _I, _I, _P, _P, _I, _IP, _DP]),
+    "vxl_set_stencil_config": (_I, [_P, _I, _I, _I, _I]),
+    "vxl_get_stencil_config": (_I, [_P, _I, _IP, _IP, _IP]),
+    "vxl_autotune_stencil": (_I, [_P, _I, _I, _P, _P, _I, _IP, _IP, _IP, _DP]),
     "vxl_rhs": (_I, [_P, _I, _DP, _P]),
     "vxl_avg_stress": (_I, [_P, _I, _P, _DP, _DP]),
     "vxl_center_fields": (_I, [_P, _I, _P, _DP, _P, _P, _P]),

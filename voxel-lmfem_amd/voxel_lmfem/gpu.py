@@ -117,6 +117,36 @@ class Cell:
         code = _native.VXL_F64 if precision in ("float64", "fp64") else _native.VXL_F32
         check(self.lib.vxl_set_stencil_variant(self.handle, code, int(variant)), "vxl_set_stencil_variant")
 
+    @staticmethod
+    def _code(precision):
+        return _native.VXL_F64 if precision in ("float64", "fp64") else _native.VXL_F32
+
+    def stencil_config(self, precision="float32"):
+        """(tile_y, schedule, fewest z planes per chunk) of the stencil."""
+        t, v, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self.lib.vxl_get_stencil_config(self.handle, self._code(precision), ctypes.byref(t), ctypes.byref(v),
+                                              ctypes.byref(z)), "vxl_get_stencil_config")
+        return t.value, v.value, z.value
+
+    def set_stencil_config(self, precision, tile_y, variant, zmin):
+        check(self.lib.vxl_set_stencil_config(self.handle, self._code(precision), int(tile_y), int(variant),
+                                              int(zmin)), "vxl_set_stencil_config")
+
+    def autotune(self, precision="float32", nrhs=6, reps=3):
+        """Time all 27 (tile, schedule, chunk depth) stencil configurations on
+        this cell and keep the fastest.  Returns (tile_y, schedule, zmin) and
+        the 3 x 3 x 3 table of milliseconds per launch."""
+        torch = self.torch
+        dtype = torch.float64 if precision in ("float64", "fp64") else torch.float32
+        u = torch.randn(nrhs, 3, self.n, dtype=dtype, device=self.device)
+        out = torch.empty_like(u)
+        t, v, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = (ctypes.c_double * 27)()
+        check(self.lib.vxl_autotune_stencil(self.handle, self._code(precision), int(nrhs), dptr(u), dptr(out),
+                                            int(reps), ctypes.byref(t), ctypes.byref(v), ctypes.byref(z), ms),
+              "vxl_autotune_stencil")
+        return (t.value, v.value, z.value), np.array(ms[:]).reshape(3, 3, 3)
+
     def rhs_planes(self, macro):
         macro, p = darray(np.atleast_2d(macro))
         out = self.torch.empty((len(macro), 3, self.n), dtype=self.torch.float64, device=self.device)

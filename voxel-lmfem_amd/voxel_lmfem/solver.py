@@ -160,19 +160,26 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="g
     return result
 
 
-def homogenize_batch(cells, *, precision="mixed", tol=TOL, streams=4, workers=4):
+def homogenize_batch(cells, *, precision="mixed", tol=TOL, streams=4, workers=4, autotune=True, tune_cache=None):
     """C_eff of many global-frame cells of uneven size, solved concurrently.
 
     cells: iterable of (phase_id, materials, domain_size).  Each cell gets the
     device label front end, its own reference medium, and a solver on one of
     `streams` HIP streams; ``vxl_solver_run_batch`` then drives them from
     `workers` host threads, largest first, so small cells overlap on the GPU.
+    autotune: pick each cell's stencil tile, schedule and chunk depth with
+    the autotuner (tuned once per grid shape and machine, then read from the
+    JSON cache `tune_cache`, default $VXL_TUNE_CACHE or
+    ~/.cache/voxel_lmfem/stencil_tuning.json).
     Returns one dict per cell (CH, calls, max_true_residual, grid_shape).
     """
+    from . import autotune as tuner
     import torch
     from .gpu import Cell, Solver, run_batch
     pool = [torch.cuda.Stream() for _ in range(max(1, int(streams)))]
     built = []
+    entries = tuner.load(tune_cache) if autotune else None
+    tuned_any = False
     try:
         for k, (phase_id, materials, domain_size) in enumerate(cells):
             phase, table, domain, orientation, labels, tensors, local = validate(phase_id, materials, domain_size)
@@ -180,7 +187,12 @@ def homogenize_batch(cells, *, precision="mixed", tol=TOL, streams=4, workers=4)
                 raise ValueError("homogenize_batch takes global-frame cells; use homogenize for local frames")
             field, h, lam, mu, meta = _prepare_device(phase, table, domain)
             cell = Cell(field, phase.shape, h, stream=pool[k % len(pool)])
+            if autotune:
+                prec = "float32" if precision == "mixed" else "float64"
+                tuned_any |= tuner.tune(cell, prec, 6, entries=entries)["tuned"]
             built.append((cell, Solver(cell, lam, mu, mode=precision, nrhs=6), phase.shape, meta))
+        if tuned_any:
+            tuner.save(entries, tune_cache)
         outs = run_batch([b[1] for b in built], tol=tol, workers=workers)
     finally:
         for cell, solver, _, _ in built:
