@@ -15,20 +15,25 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_batch_equals_individual_solves(torch_gpu):
+def test_batch_equals_individual_solves(torch_gpu, tmp_path):
     from voxel_lmfem import homogenize
     from voxel_lmfem.cells import random_inclusion_batch
     from voxel_lmfem.solver import homogenize_batch
     cells = random_inclusion_batch(count=5, seed=11, sizes=(16, 24, 32))
     print("case batch", [m["n"] for _, m in cells], flush=True)
     inputs = [(phase, mats, dom) for (phase, mats, dom, _), _ in cells]
-    batch = homogenize_batch(inputs, streams=3, workers=3)
-    for (phase, mats, dom), got in zip(inputs, batch):
+    batch = homogenize_batch(inputs, streams=3, workers=3, autotune=False)
+    tuned = homogenize_batch(inputs, streams=3, workers=3, tune_cache=tmp_path / "tuning.json")
+    assert (tmp_path / "tuning.json").is_file()
+    for (phase, mats, dom), got, alt in zip(inputs, batch, tuned):
         one = homogenize(phase, mats, dom)
         assert got["max_true_residual"] <= 1e-8
         assert np.array_equal(np.asarray(got["CH"]), np.asarray(one["CH"]))  # same kernels, same bits
         assert [c["iterations"] for c in got["calls"]] == [c["iterations"] for c in one["calls"]]
         assert got["graph_blocks"] == one["graph_blocks"]
+        # autotuned stencil configurations: other summation orders, same certified answer
+        assert alt["max_true_residual"] <= 1e-8
+        assert rel(alt["CH"], one["CH"]) < 1e-8
 
 
 def test_woven_128_mixed_vs_fp64(torch_gpu):
