@@ -43,8 +43,20 @@ def load(path=None):
     return data["entries"]
 
 
-def save(entries, path=None):
-    """Atomically replace the cache file with `entries`."""
+def save(entries, path=None, *, strict=True):
+    """Atomically replace the cache file with `entries`.  strict=False turns a
+    failure to write (read-only home, full disk) into a warning: the tuned
+    configurations still apply to the cells of this process."""
+    try:
+        _write(entries, path)
+    except OSError as exc:
+        if strict:
+            raise
+        import warnings
+        warnings.warn(f"stencil tuning cache not written: {exc}", RuntimeWarning, stacklevel=2)
+
+
+def _write(entries, path):
     p = cache_path(path)
     p.parent.mkdir(parents=True, exist_ok=True)
     text = json.dumps(dict(schema=SCHEMA, entries=entries), indent=1, sort_keys=True) + "\n"
@@ -91,5 +103,5 @@ def tune(cell, precision="float32", nrhs=6, *, path=None, reps=3, refresh=False,
                  default_ms=float(table[0, 1 if precision not in ("float64", "fp64") else 2, 0]))
     entries[k] = entry
     if own:
-        save(entries, path)
+        save(entries, path, strict=False)
     return dict(entry, tuned=True)

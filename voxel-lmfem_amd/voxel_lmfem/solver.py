@@ -192,7 +192,7 @@ def homogenize_batch(cells, *, precision="mixed", tol=TOL, streams=4, workers=4,
                 tuned_any |= tuner.tune(cell, prec, 6, entries=entries)["tuned"]
             built.append((cell, Solver(cell, lam, mu, mode=precision, nrhs=6), phase.shape, meta))
         if tuned_any:
-            tuner.save(entries, tune_cache)
+            tuner.save(entries, tune_cache, strict=False)
         outs = run_batch([b[1] for b in built], tol=tol, workers=workers)
     finally:
         for cell, solver, _, _ in built:
