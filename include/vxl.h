@@ -144,8 +144,8 @@ int vxl_get_stencil_variant(vxl_cell* cell, int precision);
 int vxl_tune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev, int reps,
                      int* best, double* ms3);
 /* Full stencil configuration per precision: tile height (workgroup tile of
- * 32 x tile_y output nodes; VXL_F32: 14 (default), 8 or 6; VXL_F64: 8
- * (default), 6 or 4), schedule (1-3, as above) and the fewest z planes per
+ * 32 x tile_y output nodes; VXL_F32: 14 (default), 8 or 6; VXL_F64: 6
+ * (default), 8 or 4), schedule (1-3, as above) and the fewest z planes per
  * workgroup chunk (2..64, default 8; each chunk recomputes one element
  * layer, and shallower chunks give small cells more workgroups).  Every
  * configuration computes the same element sums.  No reference counterpart

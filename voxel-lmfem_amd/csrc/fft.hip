@@ -849,12 +849,12 @@ fft_x4_c2r(const SC* __restrict__ S, T* __restrict__ z, int64_t rows, int PP,
   C* L = reinterpret_cast<C*>(smem_raw);
   const int P = 3 * PP;
   const SC* in = S + int64_t(blockIdx.y) * spec_stride;
-  // Hermitian extension and packing of the three pencils per row pair.
-  for (int idx = threadIdx.x; idx < PP * n; idx += blockDim.x) {
-    const int a = idx / n, k = idx - a * n;
+  // Hermitian extension and packing of the three pencils per row pair: each
+  // stored coefficient kk is read once and fills entry kk and, conjugated,
+  // its mirror n - kk.
+  for (int idx = threadIdx.x; idx < PP * nxh; idx += blockDim.x) {
+    const int a = idx / nxh, kk = idx - a * nxh;
     const int64_t rr = 2 * (int64_t(blockIdx.x) * PP + a);
-    const bool mirror = k >= nxh;
-    const int kk = mirror ? n - k : k;
     const bool realonly = kk == 0 || kk == n / 2;
     C x[2][3];
 #pragma unroll
@@ -864,13 +864,20 @@ fft_x4_c2r(const SC* __restrict__ S, T* __restrict__ z, int64_t rows, int PP,
         C v = {T(0), T(0)};
         if (rr + h < rows) v = sload<C>(in + ((rr + h) * nxh + kk) * 3 + c);
         if (realonly) v.y = T(0);
-        if (mirror) v = cconj(v);
         x[h][c] = v;
       }
     }
-    L[(3 * a + 0) * XT::stride + k] = C{x[0][0].x - x[0][1].y, x[0][0].y + x[0][1].x};
-    L[(3 * a + 1) * XT::stride + k] = C{x[1][0].x - x[1][1].y, x[1][0].y + x[1][1].x};
-    L[(3 * a + 2) * XT::stride + k] = C{x[0][2].x - x[1][2].y, x[0][2].y + x[1][2].x};
+    // pencil value at k from the spectra s0, s1 of its two real rows: s0 + i s1
+    C* L0 = L + (3 * a) * XT::stride;
+    L0[kk] = C{x[0][0].x - x[0][1].y, x[0][0].y + x[0][1].x};
+    L0[XT::stride + kk] = C{x[1][0].x - x[1][1].y, x[1][0].y + x[1][1].x};
+    L0[2 * XT::stride + kk] = C{x[0][2].x - x[1][2].y, x[0][2].y + x[1][2].x};
+    if (kk > 0 && kk < n - kk) {  // mirror: conj(s0) + i conj(s1)
+      const int km = n - kk;
+      L0[km] = C{x[0][0].x + x[0][1].y, x[0][1].x - x[0][0].y};
+      L0[XT::stride + km] = C{x[1][0].x + x[1][1].y, x[1][1].x - x[1][0].y};
+      L0[2 * XT::stride + km] = C{x[0][2].x + x[1][2].y, x[1][2].x - x[0][2].y};
+    }
   }
   __syncthreads();
   const int t = threadIdx.x % lanes, p = threadIdx.x / lanes;

@@ -241,9 +241,11 @@ __device__ __forceinline__ BlockId xcd_block() {
 // FP32 32 x 14 nodes = 33 x 15 elements = 8 waves, so two workgroups (16
 // waves, 4 per SIMD) share a CU at up to 128 VGPRs; 32 x 16 (9 waves) fits
 // only one at the 96 VGPRs that pairing would need, and ran 1.55x slower.
-// Smaller tiles give small cells more workgroups.
+// Smaller tiles give small cells more workgroups.  FP64 default 32 x 6
+// (4 waves): measured 2.90 ms against 3.66 ms for 32 x 8 per six-RHS launch
+// at 256^3 (schedule 3).
 constexpr int kTilesF32[3] = {14, 8, 6};
-constexpr int kTilesF64[3] = {8, 6, 4};
+constexpr int kTilesF64[3] = {6, 8, 4};
 
 template <int TX, int TY>
 struct Tile {
