@@ -130,7 +130,7 @@ int vxl_apply(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* 
 
 /* Stencil schedule of the cell per precision: 1 = per-layer 8-node modal
  * transform, 2 = persistent face columns, 3 = 2 with the plane after next
- * prefetched into registers (defaults: 2 for VXL_F32, 3 for VXL_F64; the
+ * prefetched into registers (default 3 for both precisions; the
  * CG form of the stencil used inside vxl_solver_run has schedules 2 and 3
  * only and runs 2 when 1 is set).  All schedules compute the same sums.
  * No reference counterpart: the reference's variant choice is recorded in
@@ -243,8 +243,8 @@ int vxl_cell_inverse_diagonal(vxl_cell* cell, double* out_dev);
  * diagonal is assembled here on the device (once per solver). */
 int vxl_solver_set_preconditioner(vxl_solver* s, int kind);
 /* Number of four-iteration blocks the last vxl_solver_run replayed from its
- * captured HIP graph (0 with VXL_GRAPH=0, the fused forward pass, or when
- * capture is unavailable).  No reference counterpart (the reference launches
+ * captured HIP graph (graph replay is opt-in with VXL_GRAPH=1; 0 without it,
+ * with the fused forward pass, or when capture is unavailable).  No reference counterpart (the reference launches
  * through PyTorch eagerly). */
 int vxl_solver_graph_blocks(vxl_solver* s);
 /* Runs count independent solvers (each bound to its own cell and stream)

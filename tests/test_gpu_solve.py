@@ -119,7 +119,7 @@ def test_graph_blocks_match_eager_launches(torch_gpu, monkeypatch, precision, pr
     assert a["graph_blocks"] == 0 and b["graph_blocks"] > 0  # the blocks really were replayed
 
 
-def test_solve_under_other_stencil_configs(torch_gpu):
+def test_solve_under_other_stencil_configs(torch_gpu, monkeypatch):
     """The CG form of the stencil (p.Kp from element energies) under other tiles,
     schedules and chunk depths: same certified C_eff; the block graph is
     re-captured for the new configuration."""
@@ -127,6 +127,7 @@ def test_solve_under_other_stencil_configs(torch_gpu):
     from voxel_lmfem.cells import voronoi_polycrystal
     from voxel_lmfem.gpu import Cell, Solver
     (phase, mats, dom, _), _ = voronoi_polycrystal(n=24, grains=10, seed=8)
+    monkeypatch.setenv("VXL_GRAPH", "1")
     field = pm.build_field(phase, mats)
     cell = Cell(field, phase.shape, dom / 24)
     solver = Solver(cell, 100.0, 40.0, mode="mixed")
