@@ -44,6 +44,7 @@ enum { VXL_SOLVE_FP64 = 0, VXL_SOLVE_MIXED = 1 };
 /* Preconditioners of the solver: the FFT Green operator (default) or the
  * scalar Jacobi control. */
 enum { VXL_PRECOND_FFT = 0, VXL_PRECOND_JACOBI = 1, VXL_PRECOND_FFT_BF16 = 2 };
+enum { VXL_ELEMENT_FI = 0, VXL_ELEMENT_C3D8R_HG = 1 };
 
 typedef struct vxl_cell vxl_cell;
 typedef struct vxl_precond vxl_precond;
@@ -151,6 +152,18 @@ int vxl_tune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev,
  * configuration computes the same element sums.  No reference counterpart
  * (see vxl_tune_stencil). */
 int vxl_set_stencil_config(vxl_cell* cell, int precision, int tile_y, int variant, int zmin);
+/* Element of the cell's operator: VXL_ELEMENT_FI (default; fully integrated
+ * HEX8, 2x2x2 Gauss) or VXL_ELEMENT_C3D8R_HG, the reference's reduced-
+ * integration control (femlib/c3d8r.py:62-78: centroid rule V B0^T C B0 plus
+ * the hourglass penalty s(C) H H^T, s = coef * max((I1 + 7 I2)/60, 1e-12) *
+ * V / hbar^2; femlib/batched/c3d8r_hourglass_operator.py).  RHS, mean stress
+ * and centre fields are the same for both elements (both use the centroid
+ * strain of a constant macro strain).  The HG element runs the default tile
+ * and schedule 3; preconditioners built afterwards use its symbol (FFT) or
+ * diagonal (Jacobi).  Set it before creating solvers or preconditioners.
+ * hourglass_coefficient must be finite and positive (the reference default
+ * is 0.1); it is ignored for VXL_ELEMENT_FI. */
+int vxl_cell_set_element(vxl_cell* cell, int element, double hourglass_coefficient);
 int vxl_get_stencil_config(vxl_cell* cell, int precision, int* tile_y, int* variant, int* zmin);
 /* Autotuner: times all 27 (tile, schedule, chunk depth in {8, 4, 2})
  * configurations on the given buffers (reps launches each, HIP events on the

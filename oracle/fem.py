@@ -30,6 +30,13 @@ Pinning
     tests/test_solver.py:17-56 fix the full chain through ``assemble``;
   * ``energy_contraction``: central differences of ``assemble``'s C_eff
     with respect to one element's stiffness scale (tests/test_oracle.py);
+  * C3D8R-HG control (``hg_element_matrices``, ``assemble(..., element="hg")``,
+    restating ``femlib/c3d8r.py:18-78``): the element matrix is symmetric,
+    annihilates every linear displacement field (rigid and constant-strain
+    modes reach it only through V B0^T C B0), has each hourglass pattern as an
+    eigenvector with eigenvalue s(C); a uniform cell returns C_eff = C and a
+    z-laminate the Schur formula; the penalty scale of an isotropic medium is
+    coef (lam + 3 mu)/2 V / hbar^2;
   * ``center_fields``: a periodic affine-free field (u = 0) gives strain = E
     and stress = C_e E in every element; a rigid translation leaves both
     unchanged; a node-local displacement enters the 8 adjacent elements with
@@ -104,13 +111,58 @@ def assemble_matrix(full, shape, h):
     return sparse.coo_matrix((Ks.ravel(), (rows, cols)), shape=(3 * n, 3 * n)).tocsr()
 
 
-def assemble(full, shape, h):
-    """independent_validation.py:77-121: K, rhs (3N,6), u (N,3,6), CH, recover."""
+def hourglass_basis():
+    """(24, 12) C3D8R-HG basis (femlib/c3d8r.py:18-45): for displacement
+    component c (DOF 3 a + c) the corner patterns xy, yz, zx, xyz of the corner
+    signs, over sqrt(8)."""
+    S = SIGNS.astype(np.float64)
+    pats = np.stack([S[:, 0] * S[:, 1], S[:, 1] * S[:, 2], S[:, 2] * S[:, 0], S[:, 0] * S[:, 1] * S[:, 2]],
+                    axis=1) / np.sqrt(8.0)
+    H = np.zeros((24, 12))
+    column = 0
+    for c in range(3):
+        for m in range(4):
+            H[c::3, column] = pats[:, m]
+            column += 1
+    return H
+
+
+def hourglass_scale(C, h, coef):
+    """femlib/c3d8r.py:48-59 for one (6, 6) tensor or a stack (..., 6, 6)."""
+    C = np.asarray(C, dtype=np.float64)
+    h = np.asarray(h, dtype=np.float64)
+    hbar = h.sum() / 3.0
+    normal = C[..., 0, 0] + C[..., 1, 1] + C[..., 2, 2]
+    first = normal + 2.0 * (C[..., 0, 1] + C[..., 0, 2] + C[..., 1, 2])
+    second = normal + 2.0 * (C[..., 3, 3] + C[..., 4, 4] + C[..., 5, 5])
+    material = (first + 7.0 * second) / 60.0
+    return coef * np.maximum(material, 1e-12) * np.prod(h) / (hbar * hbar)
+
+
+def hg_element_matrices(full, h, coef=0.1):
+    """(N, 24, 24) C3D8R-HG element stiffness (femlib/c3d8r.py:62-78):
+    V B0^T C B0 at the centroid plus s(C) H H^T, symmetrised."""
+    B0 = centroid_B(h)
+    K = np.einsum("ai,eab,bj->eij", B0, full, B0) * np.prod(h)
+    H = hourglass_basis()
+    K = K + hourglass_scale(full, h, coef)[:, None, None] * (H @ H.T)[None]
+    return 0.5 * (K + K.transpose(0, 2, 1))
+
+
+def assemble(full, shape, h, element="fi", coef=0.1):
+    """independent_validation.py:77-121: K, rhs (3N,6), u (N,3,6), CH, recover.
+    element="hg": the C3D8R-HG element (centroid rule + hourglass penalty),
+    whose RHS and stress recovery use the centroid B0."""
     nz, ny, nx = shape
     n = int(np.prod(shape))
-    Bs = gauss_B(h)
-    Ks = np.einsum("gai,eab,gbj->eij", Bs, full, Bs) * np.prod(h) / 8
-    Fs = -np.einsum("gai,eab->eib", Bs, full) * np.prod(h) / 8
+    if element == "hg":
+        Bs = centroid_B(h)[None]
+        Ks = hg_element_matrices(full, h, coef)
+        Fs = -np.einsum("gai,eab->eib", Bs, full) * np.prod(h)
+    else:
+        Bs = gauss_B(h)
+        Ks = np.einsum("gai,eab,gbj->eij", Bs, full, Bs) * np.prod(h) / 8
+        Fs = -np.einsum("gai,eab->eib", Bs, full) * np.prod(h) / 8
     nodes = element_nodes(shape)
     dofs = (3 * nodes[..., None] + np.arange(3)).reshape(n, 24)
     rows = np.broadcast_to(dofs[:, :, None], (n, 24, 24)).ravel()
@@ -247,3 +299,22 @@ def apply_uniform_stencil(u, shape, stencil):
                    + np.roll(field, tuple(-d[::-1]), axis=(0, 1, 2)) - 2 * field)
         out += np.einsum("ij,...j->...i", S, shifted)
     return out.reshape(-1, 3)
+
+
+class MatrixFreeHG:
+    """FP64 periodic C3D8R-HG operator: sum_e Ke_hg u_e with the dense
+    element matrices of ``hg_element_matrices``.  Fields are (N, 3)."""
+
+    def __init__(self, full, shape, h, coef=0.1):
+        self.shape = tuple(int(s) for s in shape)
+        self.n = int(np.prod(self.shape))
+        self.nodes = element_nodes(self.shape)
+        self.Ke = hg_element_matrices(np.asarray(full, dtype=np.float64), h, coef)
+
+    def apply(self, u):
+        u = np.asarray(u, dtype=np.float64).reshape(self.n, 3)
+        local = u[self.nodes].reshape(self.n, 24)
+        force = np.einsum("eij,ej->ei", self.Ke, local)
+        out = np.zeros((self.n, 3))
+        np.add.at(out, self.nodes, force.reshape(self.n, 8, 3))
+        return out

@@ -15,9 +15,17 @@ Symbol, per axis i with theta_i = 2 pi k_i / n_i:
     v_i = sin theta_i/(h_i m_i),  d_i = D_i - v_i^2,
     K(k) = V prod_i m_i [diag(mu sum_j D_j + (lam+mu) d_i) + (lam+mu) v v^T].
 
+C3D8R-HG control (``symbol_hg``, ``green_hg_dense``): with s_i = sin(theta_i/2),
+c_i = cos(theta_i/2), w_x = (2/h_x) s_x c_y c_z (cyclic),
+    K(k) = V (mu |w|^2 I + (lam+mu) w w^T) + s0 H(k) I,
+    H = 8 (s_x^2 s_y^2 c_z^2 + s_y^2 s_z^2 c_x^2 + s_z^2 s_x^2 c_y^2 + s_x^2 s_y^2 s_z^2),
+s0 the hourglass scale of the isotropic medium (femlib/c3d8r.py:48-59).
+
 Pinning
   * ``symbol`` equals the periodic FI operator (oracle.fem, Gauss-B form) of a
     uniform isotropic cell applied to plane waves (independent derivation);
+  * ``symbol_hg`` equals the assembled C3D8R-HG operator (oracle.fem dense
+    element matrices) of a uniform isotropic cell applied to plane waves;
   * ``green_apply`` (Sherman-Morrison) equals the dense 3x3 inverse path;
   * K(k) G(k) = I on every k != 0.
 """
@@ -126,3 +134,41 @@ def green_apply(r, shape, h, lam, mu):
     out = q - beta * (a * v) * s
     z = np.fft.irfftn(out, s=shape, axes=(1, 2, 3))
     return np.moveaxis(z, 0, -1).reshape(-1, 3)
+
+
+def symbol_hg(shape, h, lam, mu, coef=0.1):
+    """Dense (Nz, Ny, Nx, 3, 3) symbol of the C3D8R-HG element."""
+    nz, ny, nx = shape
+    h = np.asarray(h, dtype=np.float64)
+    V = float(np.prod(h))
+    hbar = h.sum() / 3
+    s0 = coef * max((lam + 3 * mu) / 2, 1e-12) * V / (hbar * hbar)
+    th = [2 * np.pi * np.arange(n) / n for n in (nx, ny, nz)]
+    sx, sy, sz = (np.sin(t / 2) for t in th)
+    cx, cy, cz = (np.cos(t / 2) for t in th)
+    X = lambda a: a[None, None, :]
+    Y = lambda a: a[None, :, None]
+    Z = lambda a: a[:, None, None]
+    w = [2 / h[0] * X(sx) * Y(cy) * Z(cz), 2 / h[1] * X(cx) * Y(sy) * Z(cz), 2 / h[2] * X(cx) * Y(cy) * Z(sz)]
+    w = [np.broadcast_to(v, (nz, ny, nx)) for v in w]
+    ww = w[0] ** 2 + w[1] ** 2 + w[2] ** 2
+    Hk = 8 * (X(sx) ** 2 * Y(sy) ** 2 * Z(cz) ** 2 + Y(sy) ** 2 * Z(sz) ** 2 * X(cx) ** 2
+              + Z(sz) ** 2 * X(sx) ** 2 * Y(cy) ** 2 + X(sx) ** 2 * Y(sy) ** 2 * Z(sz) ** 2)
+    out = np.empty((nz, ny, nx, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            out[..., i, j] = V * (lam + mu) * w[i] * w[j]
+        out[..., i, i] += V * mu * ww + s0 * Hk
+    return out
+
+
+def green_hg_dense(r, shape, h, lam, mu, coef=0.1):
+    """z = IFFT(K_hg^-1 FFT(r)) with the dense per-wavevector inverse, k = 0 -> 0."""
+    S = symbol_hg(shape, h, lam, mu, coef)
+    inv = np.zeros_like(S)
+    flat = inv.reshape(-1, 3, 3)
+    flat[1:] = np.linalg.inv(S.reshape(-1, 3, 3)[1:])
+    field = np.asarray(r, dtype=np.float64).reshape(*shape, 3)
+    spectrum = np.fft.fftn(field, axes=(0, 1, 2))
+    out = np.fft.ifftn(np.einsum("...ij,...j->...i", inv, spectrum), axes=(0, 1, 2)).real
+    return out.reshape(-1, 3)

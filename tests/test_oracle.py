@@ -355,3 +355,58 @@ def test_archived_woven_8_record():
     assert np.linalg.norm(out["CH"] - CH) / np.linalg.norm(CH) < 1e-9
     its = [c["iterations"] for c in out["calls"]]
     assert all(abs(a - b) <= 1 for a, b in zip(its, rec["iterations"]))
+
+
+# ---- C3D8R-HG control --------------------------------------------------------
+def test_hg_element_properties():
+    h = np.array([.7, .9, 1.1])
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(6, 6))
+    C = X @ X.T + 6 * np.eye(6)
+    K = fem.hg_element_matrices(C[None], h, 0.1)[0]
+    assert np.array_equal(K, K.T)
+    B0 = fem.centroid_B(h)
+    x = fem.OFFSETS * h
+    for A in (np.eye(3), rng.normal(size=(3, 3)), np.zeros((3, 3))):  # linear fields (incl. translations)
+        u = (x @ A.T + rng.normal(size=3)).reshape(-1)
+        assert np.abs(K @ u - B0.T @ C @ (B0 @ u) * np.prod(h)).max() < 1e-12 * np.abs(K).max()
+    H = fem.hourglass_basis()
+    s = fem.hourglass_scale(C, h, 0.1)
+    assert np.abs(H.T @ H - np.eye(12)).max() < 1e-15
+    assert np.abs(B0 @ H).max() < 1e-15
+    assert np.abs(K @ H - s * H).max() < 1e-12 * s
+    lam, mu = 2.0, 0.7
+    Ci = iso(mu * (3 * lam + 2 * mu) / (lam + mu), lam / (2 * (lam + mu)))
+    hbar = h.sum() / 3
+    assert fem.hourglass_scale(Ci, h, 0.25) == pytest.approx(0.25 * (lam + 3 * mu) / 2 * np.prod(h) / hbar ** 2,
+                                                            rel=1e-12)
+
+
+def test_hg_patch_and_laminate_kats():
+    n = 4 * 3 * 2
+    C = iso(3500., .35)
+    _, _, _, CH, _ = fem.assemble(np.broadcast_to(C, (n, 6, 6)).copy(), (4, 3, 2), [.5, .7, 1.1], element="hg")
+    np.testing.assert_allclose(CH, C, rtol=1e-12, atol=1e-9)
+    phase = np.zeros((4, 2, 3), dtype=np.uint8)
+    phase[:2] = 1
+    mats = [iso(3500., .35), iso(70000., .2)]
+    full = np.stack([mats[p] for p in phase.reshape(-1)])
+    _, _, _, CH, _ = fem.assemble(full, phase.shape, [1., 1., 1.], element="hg")
+    np.testing.assert_allclose(CH, laminate_expected(mats), rtol=2e-12, atol=2e-9)
+
+
+def test_hg_symbol_matches_operator_on_plane_waves():
+    shape, h = (6, 5, 4), np.array([.3, .5, .7])
+    C = iso(3., .3)
+    lam, mu = C[0, 1], C[3, 3]
+    n = int(np.prod(shape))
+    op = fem.MatrixFreeHG(np.broadcast_to(C, (n, 6, 6)).copy(), shape, h, coef=0.3)
+    S = spectral.symbol_hg(shape, h, lam, mu, coef=0.3)
+    z, y, x = np.indices(shape)
+    rng = np.random.default_rng(3)
+    for kz, ky, kx in [(1, 2, 3), (0, 0, 1), (2, 4, 1), (3, 0, 0), (5, 4, 2), (3, 0, 2)]:
+        ph = np.exp(2j * np.pi * (kx * x / shape[2] + ky * y / shape[1] + kz * z / shape[0])).ravel()
+        amp = rng.normal(size=3)
+        Ku = op.apply(np.real(ph[:, None] * amp)) + 1j * op.apply(np.imag(ph[:, None] * amp))
+        pred = ph[:, None] * (S[kz, ky, kx] @ amp)
+        assert np.abs(Ku - pred).max() < 1e-13 * np.abs(pred).max()

@@ -122,6 +122,8 @@ int cell_create_common(vxl_cell** out, int nz, int ny, int nx, double hx, double
   c->tile_y[0] = default_stencil_tile(VXL_F32);
   c->tile_y[1] = default_stencil_tile(VXL_F64);
   c->zmin[0] = c->zmin[1] = kDefaultChunkPlanes;
+  c->element = VXL_ELEMENT_FI;
+  c->hg_coef = 0.1;
   (void)hipGetDevice(&c->device);
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(c->stream);
@@ -350,6 +352,17 @@ int vxl_tune_stencil(vxl_cell* cell, int precision, int nrhs, const void* u_dev,
   return VXL_OK;
 }
 
+int vxl_cell_set_element(vxl_cell* cell, int element, double hourglass_coefficient) {
+  g_error.clear();
+  if (!cell || (element != VXL_ELEMENT_FI && element != VXL_ELEMENT_C3D8R_HG))
+    return fail("vxl_cell_set_element: invalid arguments");
+  if (element == VXL_ELEMENT_C3D8R_HG && !(std::isfinite(hourglass_coefficient) && hourglass_coefficient > 0.0))
+    return fail("hourglass_coefficient must be finite and positive");
+  cell->element = element;
+  if (element == VXL_ELEMENT_C3D8R_HG) cell->hg_coef = hourglass_coefficient;
+  return VXL_OK;
+}
+
 int vxl_set_stencil_config(vxl_cell* cell, int precision, int tile_y, int variant, int zmin) {
   g_error.clear();
   if (!cell || (precision != VXL_F32 && precision != VXL_F64))
@@ -476,6 +489,7 @@ int vxl_energy_gradient(vxl_cell* cell, int nload, const double* u_dev, const do
   g_error.clear();
   if (!cell || nload < 1 || nload > 64 || !u_dev || !macro_host || !w_host || !out_dev)
     return fail("vxl_energy_gradient: invalid arguments");
+  if (cell->element != VXL_ELEMENT_FI) return fail("vxl_energy_gradient: the contraction is for the FI element");
   for (int i = 0; i < nload; ++i)
     for (int j = 0; j < nload; ++j)
       if (!std::isfinite(w_host[i * nload + j]) || w_host[i * nload + j] != w_host[j * nload + i])
@@ -542,6 +556,7 @@ int vxl_solver_create(vxl_solver** out, vxl_cell* cell, int mode, int nrhs, doub
   s->R = nrhs;
   s->pc.lam = lam;
   s->pc.mu = mu;
+  s->element = cell->element;
   int rc = solver_create(s);
   if (rc != VXL_OK) {
     std::string msg = g_error;
@@ -560,6 +575,8 @@ int vxl_solver_run(vxl_solver* s, const double* macro_host, double tol, int max_
   if (!s || !macro_host || !ch_host) return fail("vxl_solver_run: invalid arguments");
   if (!(tol > 0 && tol < 1)) return fail("tol must lie in (0, 1)");
   if (max_iter < 1) return fail("max_iter must be >= 1");
+  if (s->element != s->cell->element)
+    return fail("the cell's element changed after this solver was created; create a new solver");
   return solver_run(s, macro_host, tol, max_iter, ch_host, iterations, restarts, residuals, u_dev_out);
 }
 
@@ -573,6 +590,8 @@ int vxl_solver_run_batch(vxl_solver* const* solvers, int count, const double* ma
   std::vector<int64_t> first(count + 1, 0);  // first load of each solver in the packed arrays
   for (int i = 0; i < count; ++i) {
     if (!solvers[i]) return fail("vxl_solver_run_batch: null solver");
+    if (solvers[i]->element != solvers[i]->cell->element)
+      return fail("the cell's element changed after a solver was created; create a new solver");
     first[i + 1] = first[i] + solvers[i]->R;
   }
   // Largest cells first (LPT): a worker that takes a small cell late finishes soon.

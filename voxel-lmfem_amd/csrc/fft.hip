@@ -263,9 +263,10 @@ struct Tables {
   const typename CT<T>::type* wx;
   const typename CT<T>::type* wy;
   const typename CT<T>::type* wz;
-  const T* gx;  // 4 x (nx/2+1): 1/m, D, v, d
+  const T* gx;  // 7 x (nx/2+1): 1/m, D, v, d (FI symbol); 2 sin(t/2)/h, cos(t/2), sin(t/2) (C3D8R-HG)
   const T* gy;
   const T* gz;
+  T hg0;        // > 0: C3D8R-HG symbol with hourglass penalty hg0 = s(C0) / V
 };
 
 // ---- pass 1: x real-to-complex ---------------------------------------------
@@ -469,6 +470,55 @@ __device__ __forceinline__ double green_point(C& q0, C& q1, C& q2, const Tables<
   return d;
 }
 
+// Green solve of the C3D8R-HG element (one-point rule + hourglass penalty,
+// femlib/c3d8r.py:62-78) for the isotropic reference medium, derived from its
+// plane-wave symbol.  The centroid gradient of a plane wave is i w (times a
+// common phase) with w_x = (2/hx) sx cy cz (s = sin(t/2), c = cos(t/2) per
+// axis, cyclic for y and z), and the hourglass pattern m has |h_m|^2 =
+// 8 prod_{d in m} s_d^2 prod_{d not in m} c_d^2, so per unit volume
+//   K(k) = (mu |w|^2 + hg0 H) I + (lam + mu) w w^T,
+//   H = 8 (sx^2 sy^2 cz^2 + sy^2 sz^2 cx^2 + sz^2 sx^2 cy^2 + sx^2 sy^2 sz^2),
+// inverted by Sherman-Morrison; k = 0 (a = 0) is removed.
+template <typename T, typename C>
+__device__ __forceinline__ double green_point_hg(C& q0, C& q1, C& q2, const Tables<T>& tb, int kx, int nxh, int y,
+                                                 int ny, int kz, int nz, T lam, T mu, T inv_vn) {
+  const T Sx = tb.gx[4 * nxh + kx], cx = tb.gx[5 * nxh + kx], sx = tb.gx[6 * nxh + kx];
+  const T Sy = tb.gy[4 * ny + y], cy = tb.gy[5 * ny + y], sy = tb.gy[6 * ny + y];
+  const T Sz = tb.gz[4 * nz + kz], cz = tb.gz[5 * nz + kz], sz = tb.gz[6 * nz + kz];
+  const T wx = Sx * cy * cz, wy = cx * Sy * cz, wz = cx * cy * Sz;
+  const T ww = wx * wx + wy * wy + wz * wz;
+  const T sx2 = sx * sx, sy2 = sy * sy, sz2 = sz * sz, cx2 = cx * cx, cy2 = cy * cy, cz2 = cz * cz;
+  const T H = T(8) * (sx2 * sy2 * cz2 + sy2 * sz2 * cx2 + sz2 * sx2 * cy2 + sx2 * sy2 * sz2);
+  const T a = mu * ww + tb.hg0 * H;
+  if (!(a > T(0))) {
+    q0 = q1 = q2 = C{T(0), T(0)};
+    return 0.0;
+  }
+  const T lm = lam + mu;
+  const T ia = rcp(a);
+  const T g = lm * rcp(a + lm * ww);
+  const C wd = {wx * q0.x + wy * q1.x + wz * q2.x, wx * q0.y + wy * q1.y + wz * q2.y};  // w . q
+  const T sc = inv_vn * ia;
+  const C o0 = {sc * (q0.x - g * wx * wd.x), sc * (q0.y - g * wx * wd.y)};
+  const C o1 = {sc * (q1.x - g * wy * wd.x), sc * (q1.y - g * wy * wd.y)};
+  const C o2 = {sc * (q2.x - g * wz * wd.x), sc * (q2.y - g * wz * wd.y)};
+  const double d = double(q0.x) * o0.x + double(q0.y) * o0.y + double(q1.x) * o1.x + double(q1.y) * o1.y +
+                   double(q2.x) * o2.x + double(q2.y) * o2.y;
+  q0 = o0;
+  q1 = o1;
+  q2 = o2;
+  return d;
+}
+
+// Green solve of the cell's element at one wavevector.
+template <typename T, typename C>
+__device__ __forceinline__ double green_solve(C& q0, C& q1, C& q2, const Tables<T>& tb, int kx, int nxh, int y,
+                                              int ny, T my, T Dy, T vy, T dy, int kz, int nz, T lam, T mu,
+                                              T inv_vn) {
+  if (tb.hg0 > T(0)) return green_point_hg<T>(q0, q1, q2, tb, kx, nxh, y, ny, kz, nz, lam, mu, inv_vn);
+  return green_point<T>(q0, q1, q2, tb, kx, nxh, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
+}
+
 // ---- pass 3: z forward, Green solve, z inverse ---------------------------------
 // kDot: r . z by Parseval, sum over the half spectrum of w(kx) Re(conj(r^) z^)
 // with w = 1 at kx = 0 and kx = nyq (nx even), 2 elsewhere; z^ = G r^ already
@@ -507,8 +557,8 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
     const int kz = idx / TK, kk = idx % TK;
     if (kk >= tk) continue;
     const int kx = k0 + kk;
-    const double d = green_point<T>(F[kz * TJ + 3 * kk + 0], F[kz * TJ + 3 * kk + 1], F[kz * TJ + 3 * kk + 2],
-                                    tb, kx, nxh, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
+    const double d = green_solve<T>(F[kz * TJ + 3 * kk + 0], F[kz * TJ + 3 * kk + 1], F[kz * TJ + 3 * kk + 2],
+                                    tb, kx, nxh, y, ny, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
     if constexpr (kDot) dot += (kx == 0 || kx == nyq) ? d : 2.0 * d;
   }
   if constexpr (kDot) {
@@ -674,8 +724,8 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
     const int kz = idx / TK, kk = idx % TK;
     if (kk >= tk) continue;
     const int kx = k0 + kk;
-    const double d = green_point<T>(L[kz * TJ + 3 * kk + 0], L[kz * TJ + 3 * kk + 1], L[kz * TJ + 3 * kk + 2],
-                                    tb, kx, nxh, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
+    const double d = green_solve<T>(L[kz * TJ + 3 * kk + 0], L[kz * TJ + 3 * kk + 1], L[kz * TJ + 3 * kk + 2],
+                                    tb, kx, nxh, y, ny, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
     if constexpr (kDot) dot += (kx == 0 || kx == nyq) ? d : 2.0 * d;
   }
   if constexpr (kDot) {
@@ -922,6 +972,7 @@ fft_x4_c2r(const SC* __restrict__ S, T* __restrict__ z, int64_t rows, int PP,
 }
 
 struct PrecondImpl {
+  double hg0;  // C3D8R-HG cell: hourglass penalty of the reference medium per unit volume (0: FI)
   int nx, ny, nz, nxh;
   int64_t rows;
   Axes axes;
@@ -1055,7 +1106,7 @@ int upload_tables(PrecondImpl* im, const double h[3]) {
     VXL_HIP(dev_malloc(&im->tw[a], sizeof(C) * n));
     VXL_HIP(hipMemcpy(im->tw[a], w.data(), sizeof(C) * n, hipMemcpyHostToDevice));
     const int cnt = counts[a];
-    std::vector<T> tab(4 * size_t(cnt));
+    std::vector<T> tab(7 * size_t(cnt));
     for (int k = 0; k < cnt; ++k) {
       const double th = 2.0 * M_PI * double(k) / double(n);
       const double sh = std::sin(0.5 * th);
@@ -1065,6 +1116,9 @@ int upload_tables(PrecondImpl* im, const double h[3]) {
       tab[cnt + k] = T(4.0 * sh * sh / (hh * hh * m));
       tab[2 * cnt + k] = T(std::sin(th) / (hh * m));
       tab[3 * cnt + k] = T(4.0 * sh * sh * sh * sh / (3.0 * hh * hh * m * m));
+      tab[4 * cnt + k] = T(2.0 * sh / hh);  // C3D8R-HG factors
+      tab[5 * cnt + k] = T(std::cos(0.5 * th));
+      tab[6 * cnt + k] = T(sh);
     }
     VXL_HIP(dev_malloc(&im->tab[a], sizeof(T) * tab.size()));
     VXL_HIP(hipMemcpy(im->tab[a], tab.data(), sizeof(T) * tab.size(), hipMemcpyHostToDevice));
@@ -1082,6 +1136,7 @@ Tables<T> tables_of(const PrecondImpl* im) {
   t.gx = static_cast<const T*>(im->tab[0]);
   t.gy = static_cast<const T*>(im->tab[1]);
   t.gz = static_cast<const T*>(im->tab[2]);
+  t.hg0 = T(im->hg0);
   return t;
 }
 
@@ -1158,6 +1213,12 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   }
   im->spec_stride = int64_t(im->nz) * im->ny * im->nxh * 3;
   VXL_TRY(upload_tables<T>(im, cell->h));
+  // Isotropic reference medium: (I1 + 7 I2) / 60 = (lam + 3 mu) / 2.
+  im->hg0 = 0.0;
+  if (cell->element == VXL_ELEMENT_C3D8R_HG) {
+    const double hbar = (cell->h[0] + cell->h[1] + cell->h[2]) / 3.0;
+    im->hg0 = cell->hg_coef * std::max(0.5 * (pc->lam + 3.0 * pc->mu), 1e-12) / (hbar * hbar);
+  }
   im->bf16 = pc->precision == VXL_F32_BF16 ? 1 : 0;
   if (im->bf16 && !(im->x4 && im->y4 && im->z4))
     return fail("BF16 spectrum storage needs power-of-two axes of at least 8 (four-step passes)");

@@ -3,7 +3,9 @@
 // r / diag K) with the assembled diagonal).  diag K is assembled here from the
 // 8 elements around each node with the operator's 2x2x2 Gauss rule:
 //   K_(n c),(n c) = sum_e sum_g (V/8) b^T C_e b,
-// b the column of B_g for node n's corner in element e and component c.
+// b the column of B_g for node n's corner in element e and component c.  For
+// the C3D8R-HG element the rule is the centroid with weight V, plus the
+// diagonal of the hourglass penalty s_e H H^T: s_e * 4 patterns * (1/8).
 #include "vxl_cg.h"
 
 namespace vxl {
@@ -14,13 +16,15 @@ constexpr int kJBlock = 256;
 template <typename Idx, bool kScale>
 __global__ void __launch_bounds__(128)
 jacobi_diag_kernel(const void* index, const double* table, const float* scale, int nx, int ny, int nz, double hx,
-                   double hy, double hz, double* dinv64, float* dinv32) {
+                   double hy, double hz, double hgk, double* dinv64, float* dinv32) {
   const int64_t nxy = int64_t(nx) * ny, n = nxy * nz;
   const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (node >= n) return;
   const int x = int(node % nx), y = int((node / nx) % ny), z = int(node / nxy);
-  const double gp = 0.57735026918962576451;  // 1/sqrt(3)
-  const double wv = hx * hy * hz / 8.0;
+  const bool hg = hgk > 0.0;  // C3D8R-HG: centroid rule + hourglass penalty
+  const double gp = hg ? 0.0 : 0.57735026918962576451;  // 1/sqrt(3)
+  const int npts = hg ? 1 : 8;
+  const double wv = hx * hy * hz / double(npts);
   double d[3] = {0, 0, 0};
   for (int a = 0; a < 8; ++a) {  // the node is corner a of element (x - ax, y - ay, z - az)
     const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
@@ -29,7 +33,14 @@ jacobi_diag_kernel(const void* index, const double* table, const float* scale, i
     const double* C = table + int64_t(load_row<Idx>(index, e)) * kRowWidth;
     const double s = kScale ? double(scale[e]) : 1.0;
     const double sx = ax ? 1.0 : -1.0, sy = ay ? 1.0 : -1.0, sz = az ? 1.0 : -1.0;
-    for (int g = 0; g < 8; ++g) {
+    if (hg) {
+      const double normal = C[pidx(0, 0)] + C[pidx(1, 1)] + C[pidx(2, 2)];
+      const double i1 = normal + 2 * (C[pidx(0, 1)] + C[pidx(0, 2)] + C[pidx(1, 2)]);
+      const double i2 = normal + 2 * (C[pidx(3, 3)] + C[pidx(4, 4)] + C[pidx(5, 5)]);
+      const double pen = hgk * fmax(s * (i1 + 7 * i2) / 60.0, 1e-12) * 0.5;  // scaled C, as the stencil
+      for (int c = 0; c < 3; ++c) d[c] += pen;
+    }
+    for (int g = 0; g < npts; ++g) {
       const double xi = (g & 1) ? gp : -gp, eta = ((g >> 1) & 1) ? gp : -gp, zeta = (g >> 2) ? gp : -gp;
       const double fx = 1 + sx * xi, fy = 1 + sy * eta, fz = 1 + sz * zeta;
       const double gx = sx * fy * fz / (4 * hx), gy = sy * fx * fz / (4 * hy), gz = sz * fx * fy / (4 * hz);
@@ -138,9 +149,10 @@ int jacobi_blocks(int64_t n) { return int(std::max<int64_t>(1, std::min<int64_t>
 int launch_jacobi_diag(const vxl_cell* cell, double* dinv64, float* dinv32, hipStream_t s) {
   const auto& g = cell->grid;
   const unsigned blocks = unsigned((g.n + 127) / 128);
+  const double hgk = cell->element == VXL_ELEMENT_C3D8R_HG ? hg_factor(cell) : 0.0;
 #define VXL_JD(I, SC)                                                                                         \
   hipLaunchKernelGGL((jacobi_diag_kernel<I, SC>), dim3(blocks), dim3(128), 0, s, cell->mat.index, cell->mat.c64, \
-                     cell->mat.scale, g.nx, g.ny, g.nz, cell->h[0], cell->h[1], cell->h[2], dinv64, dinv32)
+                     cell->mat.scale, g.nx, g.ny, g.nz, cell->h[0], cell->h[1], cell->h[2], hgk, dinv64, dinv32)
   const bool sc = cell->mat.scale != nullptr;
   switch (cell->mat.index_kind) {
     case VXL_INDEX_U8:

@@ -160,6 +160,32 @@ __device__ __forceinline__ void element_modal(const T (&q)[3][8], const T* c, co
   strain_mode<-1, -1, 7, 2, kIso>(c, g, q, P);  // xi*eta
 }
 
+// C3D8R-HG element (femlib/c3d8r.py:62-78): one-point (centroid) integration,
+// V B0^T C B0, which is exactly the constant strain mode above, plus the
+// hourglass penalty s(C) H H^T.  H holds, per displacement component, the
+// corner patterns xy, yz, zx and xyz over sqrt(8); in Hadamard coefficients
+// the penalty is P[c][m] += s / 8 q[c][m] for the modes m = 3, 6, 5, 7.
+// hg = coef V / hbar^2 / 8; s / 8 = hg * max((I1 + 7 I2) / 60, 1e-12).
+template <bool kIso = false, typename T>
+__device__ __forceinline__ void element_hg(const T (&q)[3][8], const T* c, const G9<T>& g, T hg, T (&P)[3][8]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int m = 0; m < 8; ++m) P[i][m] = T(0);
+  strain_mode<1, 2, 4, 0, kIso>(c, g, q, P);
+  const T normal = c[pidx(0, 0)] + c[pidx(1, 1)] + c[pidx(2, 2)];
+  const T i1 = normal + T(2) * (c[pidx(0, 1)] + c[pidx(0, 2)] + c[pidx(1, 2)]);
+  const T i2 = normal + T(2) * (c[pidx(3, 3)] + c[pidx(4, 4)] + c[pidx(5, 5)]);
+  const T k = hg * fmax((i1 + T(7) * i2) * T(1.0 / 60.0), T(1e-12));
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    P[i][3] = k * q[i][3];
+    P[i][5] = k * q[i][5];
+    P[i][6] = k * q[i][6];
+    P[i][7] = k * q[i][7];
+  }
+}
+
 // v[c][a]: nodal displacements in, nodal forces out.
 template <typename T>
 __device__ __forceinline__ void element_action(T (&v)[3][8], const T* c, const G9<T>& g) {
@@ -205,6 +231,7 @@ struct ApplyArgs {
   int zchunk;
   int tiles_x;
   G9<T> g;
+  T hg;  // C3D8R-HG: coef * V / hbar^2 / 8 (hourglass penalty per unit material scale)
   // kFuse: systems with active[r] == 0 are skipped and u . (K u) of each
   // workgroup is written to partial[(r * chunks + chunk) * tiles + tile].
   const int* active;
@@ -383,8 +410,9 @@ apply_kernel(ApplyArgs<T> a) {
 // forces of one face through LDS.  One staged node plane; two barriers per
 // plane.  Same arithmetic as the full 8-point transform, regrouped.
 // kIso: cubic voxels, a.g holds the squared modal factors (strain_mode).
+// kHG: the C3D8R-HG element instead of the fully integrated one.
 // FP32: the register budget lets two workgroups share a CU (waves per SIMD).
-template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse, bool kPF, bool kIso>
+template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse, bool kPF, bool kIso, bool kHG = false>
 __global__ void __launch_bounds__((Tile<TX, TY>::NT), (sizeof(T) == 4 ? (2 * Tile<TX, TY>::NT / 64 + 3) / 4 : 0))
 apply_kernel_v2(ApplyArgs<T> a) {
   using TL = Tile<TX, TY>;
@@ -463,7 +491,10 @@ apply_kernel_v2(ApplyArgs<T> a) {
         q[cc][m] = Tf[cc][m] + Bb[cc][m];
         q[cc][m + 4] = Tf[cc][m] - Bb[cc][m];
       }
-    element_modal<kIso>(q, c, a.g, P);
+    if constexpr (kHG)
+      element_hg<kIso>(q, c, a.g, a.hg, P);
+    else
+      element_modal<kIso>(q, c, a.g, P);
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
@@ -636,7 +667,7 @@ void apply_grid(const vxl_cell* cell, int nrhs, int& tiles_x, int& tiles, int& c
   chunks = (nz + zchunk - 1) / zchunk;
 }
 
-template <typename T, typename Idx, bool kScale, int TX, int TY>
+template <typename T, typename Idx, bool kScale, int TX, int TY, bool kHG>
 int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s,
                       const int* active = nullptr, double* partial = nullptr, int* blocks = nullptr) {
   using TL = Tile<TX, TY>;
@@ -659,7 +690,9 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
   a.g = cast_g<T>(mf);
   dim3 grid(tiles, chunks, nrhs);
   if (blocks) *blocks = tiles * chunks;
-  const int v = cell->variant[std::is_same<T, double>::value ? 1 : 0];
+  // The C3D8R-HG element runs schedule 3 only.
+  const int v = kHG ? 3 : cell->variant[std::is_same<T, double>::value ? 1 : 0];
+  a.hg = T(kHG ? hg_factor(cell) / 8.0 : 0.0);
   // Cubic voxels: the squared factors go to the back-projection (one multiply
   // per gradient entry less); schedule 1 keeps the general form.
   const bool iso = cell->h[0] == cell->h[1] && cell->h[1] == cell->h[2] && v != 1;
@@ -669,11 +702,18 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
 #define VXL_V2(FUSE, PF)                                                                                   \
   do {                                                                                                     \
     if (iso)                                                                                               \
-      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, FUSE, PF, true>), grid, dim3(TL::NT), 0, s, a); \
+      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, FUSE, PF, true, kHG>), grid, dim3(TL::NT), 0, s, \
+                         a);                                                                               \
     else                                                                                                   \
-      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, FUSE, PF, false>), grid, dim3(TL::NT), 0, s, a); \
+      hipLaunchKernelGGL((apply_kernel_v2<T, Idx, kScale, TX, TY, FUSE, PF, false, kHG>), grid, dim3(TL::NT), 0, \
+                         s, a);                                                                            \
   } while (0)
-  if (partial) {
+  if constexpr (kHG) {
+    if (partial)
+      VXL_V2(true, true);
+    else
+      VXL_V2(false, true);
+  } else if (partial) {
     if (v == 3)
       VXL_V2(true, true);
     else
@@ -690,19 +730,19 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
   return VXL_OK;
 }
 
-template <typename T, int TX, int TY, typename... Extra>
+template <typename T, int TX, int TY, bool kHG = false, typename... Extra>
 int dispatch_apply(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s, Extra... extra) {
   const bool sc = cell->mat.scale != nullptr;
   switch (cell->mat.index_kind) {
     case VXL_INDEX_U8:
-      return sc ? launch_apply_impl<T, uint8_t, true, TX, TY>(cell, nrhs, u, out, s, extra...)
-                : launch_apply_impl<T, uint8_t, false, TX, TY>(cell, nrhs, u, out, s, extra...);
+      return sc ? launch_apply_impl<T, uint8_t, true, TX, TY, kHG>(cell, nrhs, u, out, s, extra...)
+                : launch_apply_impl<T, uint8_t, false, TX, TY, kHG>(cell, nrhs, u, out, s, extra...);
     case VXL_INDEX_U16:
-      return sc ? launch_apply_impl<T, uint16_t, true, TX, TY>(cell, nrhs, u, out, s, extra...)
-                : launch_apply_impl<T, uint16_t, false, TX, TY>(cell, nrhs, u, out, s, extra...);
+      return sc ? launch_apply_impl<T, uint16_t, true, TX, TY, kHG>(cell, nrhs, u, out, s, extra...)
+                : launch_apply_impl<T, uint16_t, false, TX, TY, kHG>(cell, nrhs, u, out, s, extra...);
     case VXL_INDEX_I32:
-      return sc ? launch_apply_impl<T, int32_t, true, TX, TY>(cell, nrhs, u, out, s, extra...)
-                : launch_apply_impl<T, int32_t, false, TX, TY>(cell, nrhs, u, out, s, extra...);
+      return sc ? launch_apply_impl<T, int32_t, true, TX, TY, kHG>(cell, nrhs, u, out, s, extra...)
+                : launch_apply_impl<T, int32_t, false, TX, TY, kHG>(cell, nrhs, u, out, s, extra...);
   }
   return fail("unknown material index kind");
 }
@@ -920,11 +960,12 @@ int default_stencil_variant(int precision) {
   return 3;
 }
 
-// Runtime tile choice -> template instance.
+// Runtime tile choice -> template instance (the C3D8R-HG element runs the
+// default tile only).
 template <typename T, typename F>
 int with_tile(const vxl_cell* cell, F&& f) {
   constexpr bool d = std::is_same<T, double>::value;
-  const int ty = cell->tile_y[d ? 1 : 0];
+  const int ty = cell->element == VXL_ELEMENT_C3D8R_HG ? (d ? kTilesF64[0] : kTilesF32[0]) : cell->tile_y[d ? 1 : 0];
   const auto& list = d ? kTilesF64 : kTilesF32;
   if (ty == list[0]) return f(std::integral_constant<int, (d ? kTilesF64[0] : kTilesF32[0])>());
   if (ty == list[1]) return f(std::integral_constant<int, (d ? kTilesF64[1] : kTilesF32[1])>());
@@ -933,12 +974,21 @@ int with_tile(const vxl_cell* cell, F&& f) {
 }
 
 template <typename T>
+constexpr int default_tile_y() {
+  return std::is_same<T, double>::value ? kTilesF64[0] : kTilesF32[0];
+}
+
+template <typename T>
 int launch_apply_t(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s) {
+  if (cell->element == VXL_ELEMENT_C3D8R_HG)
+    return dispatch_apply<T, 32, default_tile_y<T>(), true>(cell, nrhs, u, out, s);
   return with_tile<T>(cell, [&](auto ty) { return dispatch_apply<T, 32, decltype(ty)::value>(cell, nrhs, u, out, s); });
 }
 template <typename T>
 int launch_apply_cg_t(const vxl_cell* cell, int nrhs, const T* p, T* q, const int* active, double* partial,
                       int* blocks, hipStream_t s) {
+  if (cell->element == VXL_ELEMENT_C3D8R_HG)
+    return dispatch_apply<T, 32, default_tile_y<T>(), true>(cell, nrhs, p, q, s, active, partial, blocks);
   return with_tile<T>(cell, [&](auto ty) {
     return dispatch_apply<T, 32, decltype(ty)::value>(cell, nrhs, p, q, s, active, partial, blocks);
   });

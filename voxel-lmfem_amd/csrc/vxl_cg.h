@@ -106,6 +106,7 @@ struct vxl_solver {
   double* partial_rz;  // FFT z-pass r.z partials (Parseval)
   double* partial_fx;  // ||r||^2 partials of the fused forward x pass (four-step x only)
   int precond_kind;    // VXL_PRECOND_FFT or VXL_PRECOND_JACOBI
+  int element;         // element of the cell when the preconditioner was built
   double* dinv64;      // Jacobi: 1 / diag K (3N FP64)
   float* dinv32;       // Jacobi, mixed mode: FP32 copy
   double* jmean;       // Jacobi: per-component means of dinv r (3 R)

@@ -147,6 +147,8 @@ struct vxl_cell {
   int variant[2];  // stencil schedule per precision (VXL_F32, VXL_F64): 1, 2 or 3
   int tile_y[2];   // stencil tile height (output nodes along y; the tile is 32 wide)
   int zmin[2];     // fewest z planes per workgroup chunk (each chunk recomputes one layer)
+  int element;     // VXL_ELEMENT_FI (2x2x2 Gauss) or VXL_ELEMENT_C3D8R_HG (one point + hourglass)
+  double hg_coef;  // hourglass coefficient of the C3D8R-HG element
 };
 
 namespace vxl {
@@ -168,6 +170,16 @@ int apply_cg_blocks_max(const vxl_cell* cell, int nrhs, int precision);
 bool stencil_tile_supported(int precision, int tile_y);
 int default_stencil_tile(int precision);
 int stencil_tile_option(int precision, int k);  // k = 0, 1, 2
+
+// C3D8R-HG hourglass penalty per element: s(C) = coef * max((I1 + 7 I2) / 60,
+// 1e-12) * V / hbar^2 with I1 = sum C_ii + 2 sum_{i<j<3} C_ij, I2 = sum_{i<3}
+// C_ii + 2 sum_{i>=3} C_ii, hbar the mean voxel edge
+// (femlib/c3d8r.py:48-59).  hg_factor = coef * V / hbar^2; the per-element
+// material part is applied where C is known.
+inline double hg_factor(const vxl_cell* cell) {
+  const double hbar = (cell->h[0] + cell->h[1] + cell->h[2]) / 3.0;
+  return cell->hg_coef * cell->volume / (hbar * hbar);
+}
 int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, double* sig, hipStream_t s);
 // Doubles of the `sig` scratch of launch_rhs (rows x nload x 6 element
 // stresses); 0 means pass sig = nullptr (direct per-element form).

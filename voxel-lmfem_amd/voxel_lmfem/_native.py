@@ -18,6 +18,7 @@ VXL_ERR_CONVERGENCE = -3
 VXL_INDEX_U8, VXL_INDEX_U16, VXL_INDEX_I32 = 1, 2, 4
 VXL_F32, VXL_F64, VXL_F32_BF16 = 0, 1, 2
 VXL_PRECOND_FFT, VXL_PRECOND_JACOBI, VXL_PRECOND_FFT_BF16 = 0, 1, 2
+VXL_ELEMENT_FI, VXL_ELEMENT_C3D8R_HG = 0, 1
 VXL_SOLVE_FP64, VXL_SOLVE_MIXED = 0, 1
 
 _P = ctypes.c_void_p
@@ -45,6 +46,7 @@ SIGNATURES = {
     "vxl_get_stencil_variant": (_I, [_P, _I]),
     "vxl_tune_stencil": (_I, [_P, _I, _I, _P, _P, _I, _IP, _DP]),
     "vxl_set_stencil_config": (_I, [_P, _I, _I, _I, _I]),
+    "vxl_cell_set_element": (_I, [_P, _I, _D]),
     "vxl_get_stencil_config": (_I, [_P, _I, _IP, _IP, _IP]),
     "vxl_autotune_stencil": (_I, [_P, _I, _I, _P, _P, _I, _IP, _IP, _IP, _DP]),
     "vxl_rhs": (_I, [_P, _I, _DP, _P]),

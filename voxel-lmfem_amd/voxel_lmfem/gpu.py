@@ -26,9 +26,16 @@ def _torch():
 class Cell:
     """Grid + material field resident on the current device."""
 
-    def __init__(self, field, shape, h, *, scale=None, device=None, stream=None):
+    ELEMENTS = {"fi": _native.VXL_ELEMENT_FI, "c3d8r_hg": _native.VXL_ELEMENT_C3D8R_HG}
+
+    def __init__(self, field, shape, h, *, scale=None, device=None, stream=None, element="fi",
+                 hourglass_coefficient=0.1):
         """stream: torch.cuda.Stream the cell (and everything built on it) runs
-        on; default the current stream."""
+        on; default the current stream.  element: "fi" (fully integrated HEX8)
+        or "c3d8r_hg" (the reference's reduced-integration control with an
+        hourglass penalty of `hourglass_coefficient`)."""
+        if element not in self.ELEMENTS:
+            raise ValueError(f"element must be one of {sorted(self.ELEMENTS)}")
         torch = _torch()
         self.stream = torch.cuda.current_stream() if stream is None else stream
         self.lib = _native.load()
@@ -68,6 +75,10 @@ class Cell:
                                            field.index_kind, dptr(self.index), len(rows), rows_p, scale_p,
                                            ctypes.c_void_p(self.stream.cuda_stream)), "vxl_cell_create")
         self.handle = handle
+        self.element = element
+        if element != "fi":
+            check(self.lib.vxl_cell_set_element(handle, self.ELEMENTS[element], float(hourglass_coefficient)),
+                  "vxl_cell_set_element")
         self.mesh = SimpleNamespace(Nz=nz, Ny=ny, Nx=nx, dx=self.h[0], dy=self.h[1], dz=self.h[2],
                                     n_elem=self.n, n_nodes=self.n, n_dofs=3 * self.n,
                                     V_elem=float(np.prod(self.h)))

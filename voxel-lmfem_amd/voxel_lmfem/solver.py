@@ -11,6 +11,10 @@ of the GPU backend -- no CUDA code is involved).  "cpu" names the reference's
 sparse-direct oracle, which this GPU build does not ship.
 precision: "mixed" (FP32 operator/preconditioner/CG with FP64 reliable
 updates, the default) or "fp64" (everything FP64).
+element: "fi" (fully integrated HEX8, the reference's element; default) or
+"c3d8r_hg" (the reference's reduced-integration control, femlib/c3d8r.py:62-78,
+with `hourglass_coefficient`, default 0.1; its FFT preconditioner uses that
+element's symbol).
 preconditioner: "fft" (Green operator of the isotropic reference medium, the
 default), "fft_bf16" (same, spectra stored in BF16 between FFT passes; mixed
 precision only) or "jacobi" (1 / diag K, the reference's scalar control in
@@ -59,7 +63,8 @@ def _field_statistics(cell, u, phase, loads=(0, 5)):
     return stats, slices
 
 
-def _workload(phase, table, domain, orientation, precision, tol, local, fields=False, preconditioner="fft"):
+def _workload(phase, table, domain, orientation, precision, tol, local, fields=False, preconditioner="fft",
+              element="fi", hourglass_coefficient=0.1):
     from .gpu import Cell, Solver
     import torch
     torch.cuda.synchronize()
@@ -75,7 +80,7 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
     torch.cuda.reset_peak_memory_stats()
     field, h, lam, mu, meta = _prepare_device(phase, table, domain, orientation)
     stage("material")
-    cell = Cell(field, phase.shape, h)
+    cell = Cell(field, phase.shape, h, element=element, hourglass_coefficient=hourglass_coefficient)
     solver = Solver(cell, lam, mu, mode=precision, nrhs=6, preconditioner=preconditioner)
     stage("operator")
     out = solver.run(np.eye(6), tol=tol, return_fields=fields)
@@ -98,8 +103,10 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
                    index_bytes=int(cell.index.numel() * cell.index.element_size()),
                    coefficient_bytes=rows * 21 * 8,
                    front_end="device" if getattr(field, "index_dev", None) is not None else "host")
-    operator = dict(variant="modal_fi_lds_stencil", precision=precision, preconditioner=preconditioner,
-                    reference_medium=meta)
+    operator = dict(variant="modal_fi_lds_stencil" if element == "fi" else "modal_c3d8r_hg_lds_stencil",
+                    element=element, precision=precision, preconditioner=preconditioner, reference_medium=meta)
+    if element != "fi":
+        operator["hourglass_coefficient"] = float(hourglass_coefficient)
     solver.close()
     cell.close()
     if fields:
@@ -108,7 +115,8 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
 
 
 def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="gpu", warmup=False,
-               precision="mixed", tol=TOL, fields=False, preconditioner="fft"):
+               precision="mixed", tol=TOL, fields=False, preconditioner="fft", element="fi",
+               hourglass_coefficient=0.1):
     """Return effective stiffness in engineering-Voigt order 11,22,33,23,13,12.
 
     fields=True adds the reference workload's field diagnostics for loads 0
@@ -122,6 +130,10 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="g
         raise ValueError("preconditioner must be 'fft', 'fft_bf16' or 'jacobi'")
     if preconditioner == "fft_bf16" and precision != "mixed":
         raise ValueError("preconditioner='fft_bf16' needs precision='mixed'")
+    if element not in ("fi", "c3d8r_hg"):
+        raise ValueError("element must be 'fi' or 'c3d8r_hg'")
+    if element == "c3d8r_hg" and not (np.isfinite(hourglass_coefficient) and hourglass_coefficient > 0):
+        raise ValueError("hourglass_coefficient must be finite and positive")
     phase, table, domain, orientation, labels, tensors, local = validate(
         phase_id, materials, domain_size, orientation)
     if backend == "cpu":
@@ -139,12 +151,13 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="g
     if not torch.cuda.is_available():
         raise RuntimeError("No GPU is available. This backend runs only on an MI355X (gfx950).")
     if warmup:
-        _workload(phase, table, domain, orientation, precision, tol, local, fields, preconditioner)
+        _workload(phase, table, domain, orientation, precision, tol, local, fields, preconditioner, element,
+                  hourglass_coefficient)
     start = time.perf_counter()
     result, storage, operator = _workload(phase, table, domain, orientation, precision, tol, local, fields,
-                                          preconditioner)
+                                          preconditioner, element, hourglass_coefficient)
     result.update(backend="gpu", storage=storage, operator=operator,
-                  scope=("FI HEX8 matrix-free stiffness; "
+                  scope=(("FI HEX8" if element == "fi" else "C3D8R-HG") + " matrix-free stiffness; "
                          + {"fft": "FFT Green preconditioner; ", "fft_bf16": "FFT Green preconditioner (BF16 spectra); ",
                             "jacobi": "Jacobi preconditioner; "}[preconditioner]
                          + ("FP32 PCG with FP64 reliable updates" if precision == "mixed" else "FP64 PCG")
