@@ -67,27 +67,42 @@ def cpu_sample(n=64, budget_s=12.0):
     return dict(n=n, reps=reps, t_apply=t_apply, t_green=t_green)
 
 
-def cpu_baseline_child():
-    """Run cpu_sample in a child process pinned to one BLAS/OpenMP thread.
+def host_info():
+    """Host cores visible to this process and the CPU model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo", encoding="utf-8"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return dict(cpu_count=os.cpu_count(), affinity=affinity, model=model)
+
+
+def cpu_baseline_child(threads=1):
+    """Run cpu_sample in a child process with `threads` BLAS/OpenMP threads
+    (NumPy's FFT and element loops run on one thread either way).
 
     Called before the GPU is initialised; the child never touches the GPU."""
     import subprocess
     env = dict(os.environ)
     for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
-        env[k] = "1"
+        env[k] = str(threads)
     out = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-sample"], env=env,
                          capture_output=True, text=True, timeout=600, check=True)
     return json.loads(out.stdout.strip().splitlines()[-1])
 
 
-def cpu_baseline(sample, iterations_total, n_target):
+def cpu_baseline(sample, iterations_total, n_target, threads=1):
     """Extrapolate the per-iteration oracle sample to a full n_target^3 solve."""
     n = sample["n"]
     scale = (n_target / n) ** 3
     per_iter = (sample["t_apply"] + sample["t_green"]) * scale
     seconds = per_iter * iterations_total
-    return dict(value=1.0 / seconds, unit="solves/s", cores=1, kind="port",
-                sample=(f"FP64 NumPy oracle in a child process with OMP/OpenBLAS/MKL threads = 1: Gauss-B "
+    return dict(value=1.0 / seconds, unit="solves/s", cores=threads, kind="port",
+                sample=(f"FP64 NumPy oracle in a child process with OMP/OpenBLAS/MKL threads = {threads}: Gauss-B "
                         f"operator apply {sample['t_apply']:.3f} s + FFT Green apply {sample['t_green']:.3f} s per "
                         f"RHS-iteration at {n}^3 (200-grain polycrystal, {sample['reps']} reps each), x{scale:.0f} to "
                         f"{n_target}^3, x{iterations_total} RHS-iterations of the GPU run; extrapolated"))
@@ -200,11 +215,18 @@ def main():
         torch.cuda.set_device(0)
         print(json.dumps(dict(batch=batch_leg(steps=args.steps, warmup=args.warmup))), flush=True)
         return
-    sample = None
+    sample = sample_all = None
+    host = host_info()
+    all_threads = max(1, min(16, host["affinity"] or host["cpu_count"] or 1))
     if not (args.skip_cpu or args.stencil_only):
         t0 = time.perf_counter()
-        sample = cpu_baseline_child()
+        sample = cpu_baseline_child(1)
         print(f"# cpu sample {sample} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        if all_threads > 1:
+            t0 = time.perf_counter()
+            sample_all = cpu_baseline_child(all_threads)
+            print(f"# cpu sample, {all_threads} threads {sample_all} ({time.perf_counter() - t0:.1f} s)",
+                  file=sys.stderr, flush=True)
     import torch
     from voxel_lmfem.cells import voronoi_polycrystal
     from voxel_lmfem import materials as pm
@@ -256,8 +278,11 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     out = rmeta = None
+    step_ms = []
     for _ in range(args.steps):
-        out, rmeta = step("mixed")
+        t1 = time.perf_counter()
+        out, rmeta = step("mixed")  # ends with teardown, which synchronises the stream
+        step_ms.append(1e3 * (time.perf_counter() - t1))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
@@ -326,10 +351,13 @@ def main():
           file=sys.stderr, flush=True)
     cell.close()
 
-    cpu = None
+    cpu = cpu_all = None
     if sample is not None:
         cpu = cpu_baseline(sample, int(sum(its)), n)
+        cpu["host"] = host
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
+    if sample_all is not None:
+        cpu_all = cpu_baseline(sample_all, int(sum(its)), n, threads=all_threads)
 
     batch = None
     if not args.skip_batch:
@@ -345,7 +373,8 @@ def main():
                     loads=6, precision="mixed", preconditioner=args.preconditioner, tol=1e-8, iterations=its,
                     step=("device label front end + reference medium + operator/preconditioner set-up + "
                           "RHS + block PCG + FP64 certificate + stress + teardown; labels resident in HBM"),
-                    resident_operator_ms_per_solve=resident_ms, reference_medium=rmeta),
+                    resident_operator_ms_per_solve=resident_ms, reference_medium=rmeta,
+                    step_ms=[round(t, 2) for t in step_ms], median_ms_per_step=float(np.median(step_ms))),
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_src,
                       kernel="apply_kernel_v2<float,uint8> R=6, 32x14 tile, cubic-voxel form (schedule 2)",
@@ -353,6 +382,7 @@ def main():
         fp64=fp64,
         batch=batch,
         cpu_baseline=cpu,
+        cpu_baseline_all_cores=cpu_all,
     )
     print(json.dumps(line), flush=True)
 
