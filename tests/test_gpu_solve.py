@@ -108,6 +108,7 @@ def test_graph_blocks_match_eager_launches(torch_gpu, monkeypatch, precision, pr
     (phase, mats, dom, _), _ = voronoi_polycrystal(n=24, grains=10, seed=8)
     print(f"case graph vs eager {precision} {preconditioner} 24^3", flush=True)
     out = {}
+    monkeypatch.setenv("VXL_DIRECTION_RING", "0")  # graph replay runs the per-iteration x update
     for flag in ("0", "1"):
         monkeypatch.setenv("VXL_GRAPH", flag)
         out[flag] = homogenize(phase, mats, dom, precision=precision, preconditioner=preconditioner)
@@ -143,3 +144,25 @@ def test_solve_under_other_stencil_configs(torch_gpu, monkeypatch):
         assert out["graph_blocks"] > 0
     solver.close()
     cell.close()
+
+
+@pytest.mark.parametrize("precision,preconditioner", [("mixed", "fft"), ("fp64", "fft"), ("mixed", "jacobi")])
+def test_direction_ring_matches_per_iteration_update(torch_gpu, monkeypatch, precision, preconditioner):
+    """x advanced from the stored directions in one FP64 pass per certification
+    against x += alpha p every iteration: same iterations, C_eff to rounding."""
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import voronoi_polycrystal
+    (phase, mats, dom, _), _ = voronoi_polycrystal(n=24, grains=10, seed=9)
+    print(f"case direction ring {precision} {preconditioner} 24^3", flush=True)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("VXL_DIRECTION_RING", flag)
+        out[flag] = homogenize(phase, mats, dom, precision=precision, preconditioner=preconditioner)
+    a, b = out["0"], out["1"]
+    rel = np.linalg.norm(np.asarray(a["CH"]) - np.asarray(b["CH"])) / np.linalg.norm(np.asarray(a["CH"]))
+    its_a = [c["iterations"] for c in a["calls"]]
+    its_b = [c["iterations"] for c in b["calls"]]
+    print(f"  rel {rel:.2e} iterations {its_a} vs {its_b}", flush=True)
+    assert rel < 1e-9
+    assert max(abs(x - y) for x, y in zip(its_a, its_b)) <= 1
+    assert b["max_true_residual"] <= 1e-8

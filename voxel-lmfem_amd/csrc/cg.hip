@@ -69,6 +69,44 @@ update_xr_kernel(T* x, T* r, const T* p, const T* q, int64_t len, const double* 
   if (threadIdx.x == 0) partial[int64_t(k) * gridDim.x + blockIdx.x] = acc;
 }
 
+// r -= alpha q; partial ||r||^2 (x is advanced later from the stored
+// directions, see flush_kernel).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+update_r_kernel(T* r, const T* q, int64_t len, const double* alpha, double* partial, const int* active) {
+  const int k = blockIdx.y;
+  if (active && !active[k]) return;
+  const T a = T(alpha[k]);
+  const int64_t off = int64_t(k) * len;
+  double acc = 0;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock) {
+    const T rn = fma(-a, q[off + i], r[off + i]);
+    r[off + i] = rn;
+    acc += double(rn) * double(rn);
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[int64_t(k) * gridDim.x + blockIdx.x] = acc;
+}
+
+// x64 += sum_t alpha_t p_t over the stored search directions of every system
+// (alpha_t = 0 where a system was not active), accumulated in FP64.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+flush_kernel(double* x64, DirectionSet<T> d, const double* alpha, int nrhs, int64_t len) {
+  const int k = blockIdx.y;
+  const int64_t off = int64_t(k) * len;
+  double a[kMaxDirections];
+#pragma unroll
+  for (int t = 0; t < kMaxDirections; ++t) a[t] = t < d.count ? alpha[t * nrhs + k] : 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock) {
+    double acc = x64[off + i];
+#pragma unroll
+    for (int t = 0; t < kMaxDirections; ++t)
+      if (t < d.count) acc = fma(a[t], double(d.p[t][off + i]), acc);
+    x64[off + i] = acc;
+  }
+}
+
 // r64 = f - q64; partial ||r64||^2; optional r32 = float(r64).
 __global__ void __launch_bounds__(kBlock)
 residual_kernel(double* r, const double* f, const double* q, float* r32, int64_t len,
@@ -155,9 +193,12 @@ subtract_mean_kernel(double* u, int64_t n, const double* mean, const int* active
 // then the scalar update selected by op.
 __global__ void __launch_bounds__(kBlock)
 finalize_kernel(const double* partial, int nblk, int op, double* scal, int nrhs, int* active,
-                int* restart, int iteration, double tol, int* stop) {
+                int* restart, int iteration, double tol, int* stop, double* hist) {
   const int k = blockIdx.x;
-  if (active && !active[k]) return;
+  if (active && !active[k]) {
+    if (hist && threadIdx.x == 0) hist[k] = 0.0;  // no step for an inactive system
+    return;
+  }
   double acc = 0;
   for (int b = threadIdx.x; b < nblk; b += kBlock) acc += partial[int64_t(k) * nblk + b];
   acc = block_sum(acc);
@@ -187,6 +228,7 @@ finalize_kernel(const double* partial, int nblk, int op, double* scal, int nrhs,
       S[kPQ * nrhs + k] = acc;
       const bool bad = !isfinite(acc) || acc <= 0.0;
       S[kAlpha * nrhs + k] = bad ? 0.0 : S[kRho * nrhs + k] / fmax(acc, 1e-300);
+      if (hist) hist[k] = S[kAlpha * nrhs + k];
       break;
     }
     case kOpBeta: {
@@ -229,12 +271,35 @@ template int launch_dot<float, float>(const float*, const float*, int64_t, int, 
 template int launch_dot<double, double>(const double*, const double*, int64_t, int, double*, const int*, hipStream_t);
 
 int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, int* active,
-                    int* restart, hipStream_t s, int iteration, double tol, int* stop) {
+                    int* restart, hipStream_t s, int iteration, double tol, int* stop, double* hist) {
   hipLaunchKernelGGL(finalize_kernel, dim3(nrhs), dim3(kBlock), 0, s, partial, nblk, op, scal, nrhs,
-                     active, restart, iteration, tol, stop);
+                     active, restart, iteration, tol, stop, hist);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }
+
+template <typename T>
+int launch_update_r(T* r, const T* q, int64_t len, int nrhs, const double* alpha, double* partial,
+                    const int* active, hipStream_t s) {
+  hipLaunchKernelGGL((update_r_kernel<T>), vgrid(len, nrhs), dim3(kBlock), 0, s, r, q, len, alpha, partial, active);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+template int launch_update_r<float>(float*, const float*, int64_t, int, const double*, double*, const int*,
+                                    hipStream_t);
+template int launch_update_r<double>(double*, const double*, int64_t, int, const double*, double*, const int*,
+                                     hipStream_t);
+
+template <typename T>
+int launch_flush(double* x64, const DirectionSet<T>& d, const double* alpha, int nrhs, int64_t len, hipStream_t s) {
+  if (d.count <= 0) return VXL_OK;
+  if (d.count > kMaxDirections) return fail("too many stored search directions");
+  hipLaunchKernelGGL((flush_kernel<T>), vgrid(len, nrhs), dim3(kBlock), 0, s, x64, d, alpha, nrhs, len);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+template int launch_flush<float>(double*, const DirectionSet<float>&, const double*, int, int64_t, hipStream_t);
+template int launch_flush<double>(double*, const DirectionSet<double>&, const double*, int, int64_t, hipStream_t);
 
 template <typename T>
 int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, const double* alpha,

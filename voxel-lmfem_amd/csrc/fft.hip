@@ -338,13 +338,13 @@ fft_x_r2c(const T* __restrict__ r, typename CT<T>::type* __restrict__ S, int nx,
 }
 
 // ---- pass 5: x complex-to-real -----------------------------------------------
-// kDir: the output is the CG search direction, updated in place as
-// p = z + beta[rhs] p (beta = 0: p = z, the old p is not read).
+// kDir: the output is the CG search direction p = z + beta[rhs] p_old with
+// p_old read from pin (pin == z: in place; beta = 0: p = z, p_old not read).
 template <typename T, bool kDir>
 __global__ void __launch_bounds__(kThreads)
-fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx, int64_t rows,
+fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* z, int nx, int64_t rows,
           int nxh, int pairs_per_block, Axes ax, Tables<T> tb, int64_t field_stride,
-          int64_t spec_stride, const int* active, const double* beta) {
+          int64_t spec_stride, const int* active, const double* beta, const T* pin) {
   using C = typename CT<T>::type;
   extern __shared__ double2 smem_raw[];
   if (active && !active[blockIdx.y]) return;
@@ -384,7 +384,8 @@ fft_x_c2r(const typename CT<T>::type* __restrict__ S, T* __restrict__ z, int nx,
   C* F = lds_fft<true, T>(A, B, P, ax.px, tb.wx);
   const T b = kDir ? T(beta[blockIdx.y]) : T(0);
   const bool keep = kDir && b != T(0);
-  auto put = [&](int64_t i, T v) { f[i] = keep ? fma(b, f[i], v) : v; };
+  const T* g = (pin ? pin : z) + int64_t(blockIdx.y) * field_stride;
+  auto put = [&](int64_t i, T v) { f[i] = keep ? fma(b, g[i], v) : v; };
   for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x) {
     const int pp = idx / nx, x = idx % nx;
     const int64_t rho = 2 * (pair0 + pp);
@@ -885,12 +886,13 @@ fft_x4_r2c(const T* __restrict__ r, SC* __restrict__ S, int64_t rows, int PP,
   }
 }
 
-// kDir as in fft_x_c2r: the output is p = z + beta[rhs] p, in place.
+// kDir as in fft_x_c2r: the output is p = z + beta[rhs] p_old, p_old from pin
+// (pin == z: in place).
 template <typename T, int N1, int N2, bool kDir, typename SC>
 __global__ void __launch_bounds__(256)
-fft_x4_c2r(const SC* __restrict__ S, T* __restrict__ z, int64_t rows, int PP,
+fft_x4_c2r(const SC* __restrict__ S, T* z, int64_t rows, int PP,
            const typename CT<T>::type* __restrict__ w, int64_t field_stride, int64_t spec_stride,
-           const int* active, const double* beta) {
+           const int* active, const double* beta, const T* pin) {
   using C = typename CT<T>::type;
   using XT = XTile<N1, N2>;
   constexpr int n = XT::n, nxh = n / 2 + 1, lanes = FourStep<N1, N2>::lanes;
@@ -963,11 +965,12 @@ fft_x4_c2r(const SC* __restrict__ S, T* __restrict__ z, int64_t rows, int PP,
   const bool im_ok = rho + (q == 0 ? 0 : 1) < rows;
   const T b = kDir ? T(beta[blockIdx.y]) : T(0);
   const bool keep = kDir && b != T(0);
+  const T* g = (pin ? pin : z) + int64_t(blockIdx.y) * field_stride;
 #pragma unroll
   for (int j2 = 0; j2 < N2; ++j2) {
     const int e = t + N1 * j2;
-    if (re_ok) f[re_off + e] = keep ? fma(b, f[re_off + e], u[j2].x) : u[j2].x;
-    if (im_ok) f[im_off + e] = keep ? fma(b, f[im_off + e], u[j2].y) : u[j2].y;
+    if (re_ok) f[re_off + e] = keep ? fma(b, g[re_off + e], u[j2].x) : u[j2].x;
+    if (im_ok) f[im_off + e] = keep ? fma(b, g[im_off + e], u[j2].y) : u[j2].y;
   }
 }
 
@@ -1081,11 +1084,11 @@ void launch_x4_r2c(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const T
 template <typename T, bool kDir, typename SC>
 void launch_x4_c2r(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const SC* S, T* z,
                    int64_t rows, int PP, const typename CT<T>::type* w, int64_t fstride, int64_t sstride,
-                   const int* active, const double* beta) {
+                   const int* active, const double* beta, const T* pin) {
 #define VXL_X4C(N, A, B)                                                                                      \
   case N:                                                                                                   \
     hipLaunchKernelGGL((fft_x4_c2r<T, A, B, kDir, SC>), grid, dim3(thr), lds, s, S, z, rows, PP, w, fstride, \
-                       sstride, active, beta);                                                              \
+                       sstride, active, beta, pin);                                                         \
     break;
   switch (n) { VXL_X4_CASES(VXL_X4C) }
 #undef VXL_X4C
@@ -1236,6 +1239,7 @@ struct CgHooks {
   double* rz_partial;
   const double* beta;
   std::function<int()> mid;
+  const void* pin;  // previous search direction (nullptr: the output buffer, in place)
 };
 
 enum Stages { kForwardX = 1, kRest = 2, kAllStages = 3 };
@@ -1316,20 +1320,20 @@ int apply_stages(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, h
   VXL_TRY(y_pass(true));
   if (im->x4 && cg)
     launch_x4_c2r<T, true, SC>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, S, z, im->rows, im->pp4, tb.wx,
-                           fstride, im->spec_stride, active, cg->beta);
+                           fstride, im->spec_stride, active, cg->beta, static_cast<const T*>(cg->pin));
   else if (im->x4)
     launch_x4_c2r<T, false, SC>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, S, z, im->rows, im->pp4, tb.wx,
-                            fstride, im->spec_stride, active, nullptr);
+                            fstride, im->spec_stride, active, nullptr, static_cast<const T*>(nullptr));
   else if constexpr (!kFull)
     return fail("BF16 spectrum storage needs power-of-two axes");
   else if (cg)
     hipLaunchKernelGGL((fft_x_c2r<T, true>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
                        im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active,
-                       cg->beta);
+                       cg->beta, static_cast<const T*>(cg->pin));
   else
     hipLaunchKernelGGL((fft_x_c2r<T, false>), dim3(bx, nrhs), dim3(kThreads), im->lds_x, s, S, z, im->nx,
                        im->rows, im->nxh, im->pairs_per_block, im->axes, tb, fstride, im->spec_stride, active,
-                       nullptr);
+                       nullptr, static_cast<const T*>(nullptr));
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }
@@ -1365,7 +1369,7 @@ int precond_apply_cg(vxl_precond* pc, int nrhs, const T* r, T* p, double* rz_par
   auto* im = static_cast<PrecondImpl*>(pc->impl);
   if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
   if ((pc->precision == VXL_F64) != std::is_same<T, double>::value) return fail("precond precision mismatch");
-  const CgHooks cg{rz_partial, beta, mid};
+  const CgHooks cg{rz_partial, beta, mid, nullptr};
   return apply_impl<T>(pc, im, nrhs, r, p, s, active, &cg);
 }
 template int precond_apply_cg<float>(vxl_precond*, int, const float*, float*, double*, const double*,
@@ -1390,17 +1394,17 @@ template int precond_cg_forward<double>(vxl_precond*, int, double*, double*, con
 
 template <typename T>
 int precond_cg_rest(vxl_precond* pc, int nrhs, T* p, double* rz_partial, const double* beta,
-                    const std::function<int()>& mid, const int* active, hipStream_t s) {
+                    const std::function<int()>& mid, const int* active, hipStream_t s, const T* p_old) {
   auto* im = static_cast<PrecondImpl*>(pc->impl);
   if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
   if ((pc->precision == VXL_F64) != std::is_same<T, double>::value) return fail("precond precision mismatch");
-  const CgHooks cg{rz_partial, beta, mid};
+  const CgHooks cg{rz_partial, beta, mid, p_old};
   return apply_impl<T>(pc, im, nrhs, static_cast<const T*>(nullptr), p, s, active, &cg, kRest);
 }
 template int precond_cg_rest<float>(vxl_precond*, int, float*, double*, const double*, const std::function<int()>&,
-                                    const int*, hipStream_t);
+                                    const int*, hipStream_t, const float*);
 template int precond_cg_rest<double>(vxl_precond*, int, double*, double*, const double*,
-                                     const std::function<int()>&, const int*, hipStream_t);
+                                     const std::function<int()>&, const int*, hipStream_t, const double*);
 
 int precond_fused_update_blocks(const vxl_precond* pc) {
   const auto* im = static_cast<const PrecondImpl*>(pc->impl);

@@ -123,21 +123,23 @@ jacobi_rz_kernel(const T* r, const T* dinv, const double* mean, int64_t n, doubl
   if (threadIdx.x == 0) partial[int64_t(k) * gridDim.x + blockIdx.x] = acc;
 }
 
-// p = (dinv r - mean) + beta p  (beta = 0: p = z, the old p is not read).
+// p = (dinv r - mean) + beta p_old, p_old from pin (pin == p: in place;
+// beta = 0: p = z, the old p is not read).
 template <typename T>
 __global__ void __launch_bounds__(kJBlock)
-jacobi_direction_kernel(T* p, const T* r, const T* dinv, const double* mean, const double* beta, int64_t n,
-                        const int* active) {
+jacobi_direction_kernel(T* p, const T* pin, const T* r, const T* dinv, const double* mean, const double* beta,
+                        int64_t n, const int* active) {
   const int k = blockIdx.y;
   if (active && !active[k]) return;
   const T b = T(beta[k]);
   T* pk = p + int64_t(k) * 3 * n;
+  const T* gk = pin + int64_t(k) * 3 * n;
   const T* rk = r + int64_t(k) * 3 * n;
   for (int c = 0; c < 3; ++c) {
     const T m = T(mean[k * 3 + c]);
     for (int64_t i = int64_t(blockIdx.x) * kJBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kJBlock) {
       const T zv = dinv[c * n + i] * rk[c * n + i] - m;
-      pk[c * n + i] = b != T(0) ? fma(b, pk[c * n + i], zv) : zv;
+      pk[c * n + i] = b != T(0) ? fma(b, gk[c * n + i], zv) : zv;
     }
   }
 }
@@ -172,7 +174,7 @@ int launch_jacobi_diag(const vxl_cell* cell, double* dinv64, float* dinv32, hipS
 
 template <typename T>
 int jacobi_cg_step(const T* r, T* p, const T* dinv, int64_t n, int nrhs, double* partial, double* mean,
-                   double* rz_partial, double* scal, int* active, int* restart, hipStream_t s) {
+                   double* rz_partial, double* scal, int* active, int* restart, hipStream_t s, const T* p_old) {
   const int nb = jacobi_blocks(n);
   const dim3 grid = dim3(unsigned(nb), unsigned(nrhs));
   hipLaunchKernelGGL((jacobi_sums_kernel<T>), grid, dim3(kJBlock), 0, s, r, dinv, n, partial, active);
@@ -183,14 +185,14 @@ int jacobi_cg_step(const T* r, T* p, const T* dinv, int64_t n, int nrhs, double*
   hipLaunchKernelGGL((jacobi_rz_kernel<T>), grid, dim3(kJBlock), 0, s, r, dinv, mean, n, rz_partial, active);
   VXL_HIP(hipGetLastError());
   VXL_TRY(launch_finalize(rz_partial, nb, kOpBeta, scal, nrhs, active, restart, s));
-  hipLaunchKernelGGL((jacobi_direction_kernel<T>), grid, dim3(kJBlock), 0, s, p, r, dinv, mean,
+  hipLaunchKernelGGL((jacobi_direction_kernel<T>), grid, dim3(kJBlock), 0, s, p, p_old ? p_old : p, r, dinv, mean,
                      scal + kBeta * nrhs, n, active);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }
 template int jacobi_cg_step<float>(const float*, float*, const float*, int64_t, int, double*, double*, double*,
-                                   double*, int*, int*, hipStream_t);
+                                   double*, int*, int*, hipStream_t, const float*);
 template int jacobi_cg_step<double>(const double*, double*, const double*, int64_t, int, double*, double*, double*,
-                                    double*, int*, int*, hipStream_t);
+                                    double*, int*, int*, hipStream_t, const double*);
 
 }  // namespace vxl

@@ -64,6 +64,10 @@ int free_all(vxl_solver* s) {
     dev_free(s->p);
   }
   if (s->dinv32) dev_free(s->dinv32);
+  for (int i = 1; i < s->nslot; ++i)
+    if (s->pring[i]) dev_free(s->pring[i]);
+  s->nslot = 0;
+  if (s->hist) dev_free(s->hist);
   if (s->active_dev) dev_free(s->active_dev);
   if (s->restart_dev) dev_free(s->restart_dev);
   if (s->stop_dev) dev_free(s->stop_dev);
@@ -95,13 +99,20 @@ struct Steps {
   bool jacobi;
   const T* dinv;
   std::function<int()> form_beta;
+  // Direction ring (ring == true): the stencil reads pcur, the preconditioner
+  // step writes pnext = M r + beta pcur, alpha goes to hist_row and x is not
+  // touched.  Otherwise pcur = pnext = p (in place) and x += alpha p here.
+  bool ring = false;
+  T* pcur;
+  T* pnext;
+  double* hist_row = nullptr;
 
   Steps(vxl_solver* sv, double t, hipStream_t stream)
       : s(sv), tol(t), st(stream), R(sv->R), len(sv->len), n(sv->cell->grid.n),
         x(static_cast<T*>(sv->x)), r(static_cast<T*>(sv->r)), p(static_cast<T*>(sv->p)),
         q(static_cast<T*>(sv->q)), nblk(sv->nblk), nblk_st(apply_cg_blocks<T>(sv->cell, sv->R)),
         nblk_rz(precond_rz_blocks(&sv->pc)), jacobi(sv->precond_kind == VXL_PRECOND_JACOBI),
-        dinv(nullptr) {
+        dinv(nullptr), pcur(p), pnext(p) {
     if (jacobi)
       dinv = std::is_same<T, double>::value ? reinterpret_cast<const T*>(s->dinv64)
                                             : reinterpret_cast<const T*>(s->dinv32);
@@ -114,17 +125,21 @@ struct Steps {
     };
   }
   int jacobi_step() {
-    return jacobi_cg_step<T>(r, p, dinv, n, R, s->partial, s->jmean, s->jrz, s->scal, s->active_dev,
-                             s->restart_dev, st);
+    return jacobi_cg_step<T>(r, pnext, dinv, n, R, s->partial, s->jmean, s->jrz, s->scal, s->active_dev,
+                             s->restart_dev, st, pcur);
   }
   // [stencil, alpha, update, ||r||^2 with the latch at iteration `iter`]
   int cg_update(int iter, bool& forward_done) {
     int* act = s->active_dev;
     double* scal = s->scal;
-    VXL_TRY(launch_apply_cg<T>(s->cell, R, p, q, act, s->partial_st, nullptr, st));
-    VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st));
+    VXL_TRY(launch_apply_cg<T>(s->cell, R, pcur, q, act, s->partial_st, nullptr, st));
+    VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st, 0, 0.0, nullptr,
+                            ring ? hist_row : nullptr));
     forward_done = false;
-    if (nblk_fx > 0) {
+    if (ring) {
+      VXL_TRY(launch_update_r<T>(r, q, len, R, scal + kAlpha * R, s->partial, act, st));
+      VXL_TRY(launch_finalize(s->partial, nblk, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
+    } else if (nblk_fx > 0) {
       VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, x, p, q, scal + kAlpha * R, s->partial_fx, act, st));
       VXL_TRY(launch_finalize(s->partial_fx, nblk_fx, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
       forward_done = true;
@@ -142,7 +157,8 @@ struct Steps {
     if (!forward_done)
       VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, nullptr, nullptr, nullptr, act,
                                     st));
-    return precond_cg_rest<T>(&s->pc, R, p, s->partial_rz, s->scal + kBeta * R, form_beta, act, st);
+    return precond_cg_rest<T>(&s->pc, R, pnext, s->partial_rz, s->scal + kBeta * R, form_beta, act, st,
+                              pcur == pnext ? nullptr : pcur);
   }
 };
 
@@ -285,10 +301,30 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   // p = z = M r with rho = r.z: beta = 0 because rho_old = 0 (slots cleared).
   Steps<T> steps(s, tol, st);
   const bool jacobi = steps.jacobi;
+  // Direction ring (unless graph replay or the fused forward pass is on, or
+  // VXL_DIRECTION_RING=0 asks for the per-iteration x update).
+  const char* ring_env = std::getenv("VXL_DIRECTION_RING");
+  const bool ring = s->nslot >= 2 && !graphs_enabled() && steps.nblk_fx == 0 && !(ring_env && ring_env[0] == '0');
+  steps.ring = ring;
+  std::vector<T*> slots(size_t(std::max(1, s->nslot)));
+  for (int i = 0; i < s->nslot; ++i) slots[i] = static_cast<T*>(s->pring[i]);
+  if (s->nslot < 1) slots[0] = p;
+  int cur = 0;
+  std::vector<int> pend;  // slots whose alpha_t p_t are not in x64 yet (hist rows in order)
+  auto flush = [&]() -> int {
+    if (!ring || pend.empty()) return VXL_OK;
+    DirectionSet<T> d{};
+    for (size_t t = 0; t < pend.size(); ++t) d.p[t] = slots[pend[t]];
+    d.count = int(pend.size());
+    VXL_TRY(launch_flush<T>(s->x64, d, s->hist, R, len, st));
+    pend.clear();
+    return VXL_OK;
+  };
+  steps.pcur = steps.pnext = slots[0];
   if (jacobi)
     VXL_TRY(steps.jacobi_step());
   else
-    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, p, s->partial_rz, scal + kBeta * R, steps.form_beta, act, st));
+    VXL_TRY(precond_apply_cg<T>(&s->pc, R, r, slots[0], s->partial_rz, scal + kBeta * R, steps.form_beta, act, st));
   VXL_HIP(hipEventRecord(s->ev[1], st));
 
   int it = 0;
@@ -297,7 +333,9 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     std::vector<int> mask(R, 0);
     for (int k = 0; k < R; ++k) mask[k] = which[k] && active[k];
     VXL_TRY(upload_mask(mask));
-    if (mixed)
+    if (ring)
+      VXL_TRY(flush());
+    else if (mixed)
       VXL_TRY(launch_accumulate(s->x64, reinterpret_cast<float*>(x), len, R, act, st));
     // K annihilates the mean, so the zero-mean projection of x64 is left to
     // the end of the run; mixed mode needs the residual only in FP32.
@@ -325,8 +363,29 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   };
 
   const bool graphs = steps.nblk_fx == 0 && graphs_enabled() && !s->graph_failed;
-  auto cg_update = [&](int iter, bool& fd) { return steps.cg_update(iter, fd); };
-  auto precond_step = [&](bool fd) { return steps.precond_step(fd); };
+  auto cg_update = [&](int iter, bool& fd) -> int {
+    if (ring) {
+      steps.pcur = slots[cur];
+      steps.hist_row = s->hist + int64_t(pend.size()) * R;
+      VXL_TRY(steps.cg_update(iter, fd));
+      pend.push_back(cur);
+      return VXL_OK;
+    }
+    return steps.cg_update(iter, fd);
+  };
+  auto precond_step = [&](bool fd) -> int {
+    if (!ring) return steps.precond_step(fd);
+    if (int(pend.size()) >= s->nslot) VXL_TRY(flush());  // no free slot for the next direction
+    int next = -1;
+    for (int i = 0; i < s->nslot && next < 0; ++i)
+      if (i != cur && std::find(pend.begin(), pend.end(), i) == pend.end()) next = i;
+    if (next < 0) return fail("search-direction ring exhausted");
+    steps.pcur = slots[cur];
+    steps.pnext = slots[next];
+    VXL_TRY(steps.precond_step(fd));
+    cur = next;
+    return VXL_OK;
+  };
   auto block_graph = [&]() { return prepare_block_graph<T>(s, tol); };
 
   bool forward_done = false, pending = false;  // pending: M r of iteration `it` not formed yet
@@ -389,6 +448,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
       if (!any_active()) break;
     }
   }
+  VXL_TRY(flush());
   VXL_HIP(hipEventRecord(s->ev[2], st));
   const bool converged = !any_active();
   // Systems that never started (zero RHS) keep x64 = 0.  Final projection and
@@ -447,6 +507,23 @@ int solver_create(vxl_solver* s) {
     s->q = s->q64;
     VXL_HIP(dev_malloc(&s->p, b64));
   }
+  // Direction ring: slot 0 is p; the other slots are optional (a failed
+  // allocation leaves a shorter ring, and fewer than two slots mean the
+  // solver updates x every iteration instead).
+  s->pring[0] = s->p;
+  s->nslot = 1;
+  {
+    const size_t bp = (s->mode == VXL_SOLVE_MIXED ? sizeof(float) : sizeof(double)) * size_t(s->len) * R;
+    for (int i = 1; i < kMaxDirections; ++i) {
+      if (dev_malloc(&s->pring[i], bp) != hipSuccess) {
+        (void)hipGetLastError();
+        s->pring[i] = nullptr;
+        break;
+      }
+      s->nslot = i + 1;
+    }
+  }
+  VXL_HIP(dev_malloc(&s->hist, sizeof(double) * kMaxDirections * R));
   s->nblk = vec_blocks(s->len);
   const int nblk_rows = vec_blocks(cell->grid.n);
   const size_t np = size_t(std::max(s->nblk * R, 3 * R * (nblk_rows + 1)));

@@ -21,7 +21,21 @@ int launch_dot(const A* a, const B* b, int64_t len, int nrhs, double* partial, c
 // active[k] = 0 (on the device).
 int launch_finalize(const double* partial, int nblk, int op, double* scal, int nrhs, int* active,
                     int* restart, hipStream_t s, int iteration = 0, double tol = 0.0,
-                    int* stop = nullptr);
+                    int* stop = nullptr, double* hist = nullptr);
+// Stored search directions p_t (nrhs fields each) whose steps alpha_t are
+// still to be added to x64.
+constexpr int kMaxDirections = 8;
+template <typename T>
+struct DirectionSet {
+  const T* p[kMaxDirections];
+  int count;
+};
+template <typename T>
+int launch_update_r(T* r, const T* q, int64_t len, int nrhs, const double* alpha, double* partial,
+                    const int* active, hipStream_t s);
+// x64 += sum_t alpha[t * nrhs + k] p_t for every system k.
+template <typename T>
+int launch_flush(double* x64, const DirectionSet<T>& d, const double* alpha, int nrhs, int64_t len, hipStream_t s);
 template <typename T>
 int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, const double* alpha,
                      double* partial, const int* active, hipStream_t s);
@@ -44,7 +58,8 @@ int jacobi_blocks(int64_t n);
 int launch_jacobi_diag(const vxl_cell* cell, double* dinv64, float* dinv32, hipStream_t s);
 template <typename T>
 int jacobi_cg_step(const T* r, T* p, const T* dinv, int64_t n, int nrhs, double* partial, double* mean,
-                   double* rz_partial, double* scal, int* active, int* restart, hipStream_t s);
+                   double* rz_partial, double* scal, int* active, int* restart, hipStream_t s,
+                   const T* p_old = nullptr);
 
 }  // namespace vxl
 
@@ -78,10 +93,11 @@ int precond_rz_blocks(const vxl_precond* pc);
 template <typename T>
 int precond_cg_forward(vxl_precond* pc, int nrhs, T* r, T* x, const T* p, const T* q, const double* alpha,
                        double* rr_partial, const int* active, hipStream_t s);
-// y, z (+ Green, r.z), mid(), y^-1 and the x inverse pass writing p = z + beta p.
+// y, z (+ Green, r.z), mid(), y^-1 and the x inverse pass writing
+// p = z + beta p_old (p_old == nullptr: p itself, in place).
 template <typename T>
 int precond_cg_rest(vxl_precond* pc, int nrhs, T* p, double* rz_partial, const double* beta,
-                    const std::function<int()>& mid, const int* active, hipStream_t s);
+                    const std::function<int()>& mid, const int* active, hipStream_t s, const T* p_old = nullptr);
 // Workgroups per RHS of the fused forward pass, 0 if it is not available.
 int precond_fused_update_blocks(const vxl_precond* pc);
 // z = M r for the systems with active[k] != 0 only.
@@ -134,6 +150,12 @@ struct vxl_solver {
   bool graph_failed;    // capture unavailable: blocks run kernel by kernel
   hipStream_t capture_stream;  // private stream the block is recorded on (replayed on the cell's)
   int graph_blocks;     // blocks replayed from the graph in the last run
+  // Search-direction ring: the CG steps alpha_t p_t are added to x64 in one
+  // FP64 pass (at a certification, when the ring is full, and at the end)
+  // instead of every iteration; slot 0 is p.
+  int nslot;
+  void* pring[vxl::kMaxDirections];
+  double* hist;  // alpha_t per slot use and system, kMaxDirections x R
   // Per-system residual history of the last run: (iteration, relative
   // residual); a negative iteration marks an FP64 true-residual evaluation.
   std::vector<std::vector<std::pair<int, double>>> trace;
