@@ -107,6 +107,47 @@ flush_kernel(double* x64, DirectionSet<T> d, const double* alpha, int nrhs, int6
   }
 }
 
+// Block sums of NV values per thread; partial[v * gridDim.x + blockIdx.x].
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double* partial) {
+  __shared__ double red[kBlock / 64][NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double w = wave_sum(v[i]);
+    if (lane == 0) red[wave][i] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    partial[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// Gram matrix of six stacked fields f and u (len entries each):
+// partial[(i * 6 + j) * gridDim.x + block] = sum f_i . u_j over the block.
+__global__ void __launch_bounds__(kBlock)
+gram6_kernel(const double* f, const double* u, int64_t len, double* partial) {
+  double acc[36];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) acc[i] = 0.0;
+  for (int64_t e = int64_t(blockIdx.x) * kBlock + threadIdx.x; e < len; e += int64_t(gridDim.x) * kBlock) {
+    double a[6], b[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      a[i] = f[int64_t(i) * len + e];
+      b[i] = u[int64_t(i) * len + e];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc[i * 6 + j] = fma(a[i], b[j], acc[i * 6 + j]);
+  }
+  block_sums<36>(acc, partial);
+}
+
 // r64 = f - q64; partial ||r64||^2; optional r32 = float(r64).
 __global__ void __launch_bounds__(kBlock)
 residual_kernel(double* r, const double* f, const double* q, float* r32, int64_t len,
@@ -274,6 +315,21 @@ int launch_finalize(const double* partial, int nblk, int op, double* scal, int n
                     int* restart, hipStream_t s, int iteration, double tol, int* stop, double* hist) {
   hipLaunchKernelGGL(finalize_kernel, dim3(nrhs), dim3(kBlock), 0, s, partial, nblk, op, scal, nrhs,
                      active, restart, iteration, tol, stop, hist);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+
+int launch_gram6(const double* f, const double* u, int64_t len, double* partial, double scale, double* out,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(gram6_kernel, dim3(kGramBlocks), dim3(kBlock), 0, s, f, u, len, partial);
+  VXL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(row_mean_kernel, dim3(36), dim3(kBlock), 0, s, partial, kGramBlocks, scale, out);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+
+int launch_row_totals(const double* partial, int rows, int nblk, double scale, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(row_mean_kernel, dim3(rows), dim3(kBlock), 0, s, partial, nblk, scale, out);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }

@@ -68,6 +68,7 @@ int free_all(vxl_solver* s) {
     if (s->pring[i]) dev_free(s->pring[i]);
   s->nslot = 0;
   if (s->hist) dev_free(s->hist);
+  if (s->gram) dev_free(s->gram);
   if (s->active_dev) dev_free(s->active_dev);
   if (s->restart_dev) dev_free(s->restart_dev);
   if (s->stop_dev) dev_free(s->stop_dev);
@@ -451,14 +452,35 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   VXL_TRY(flush());
   VXL_HIP(hipEventRecord(s->ev[2], st));
   const bool converged = !any_active();
-  // Systems that never started (zero RHS) keep x64 = 0.  Final projection and
-  // stress recovery for every system.
+  // Systems that never started (zero RHS) keep x64 = 0.  Mean stress of
+  // every system; the zero-mean projection only for returned fields (B0 and
+  // the zero-mean RHS both annihilate constants).
   std::fill(active.begin(), active.end(), 1);
   VXL_TRY(upload_flags());
-  VXL_TRY(launch_project_zero_mean(s->x64, n, R, partial, nullptr, st));
-  VXL_TRY(launch_avg_stress(cell, R, s->x64, s->macro_dev, s->stress_partial, s->ch_dev, st));
+  if (u_out) VXL_TRY(launch_project_zero_mean(s->x64, n, R, partial, nullptr, st));
+  // Six unit strains: <sigma_i>(e_k) = <C>_ik - f_i . u_k / |Omega| (f_i = -sum_e
+  // V B0^T C_e e_i, the RHS of load i), two streaming passes instead of the
+  // element gather.
+  bool unit = R == 6;
+  for (int k = 0; unit && k < 6; ++k)
+    for (int i = 0; i < 6; ++i) unit = unit && macro_host[k * 6 + i] == (i == k ? 1.0 : 0.0);
   std::vector<double> ch(6 * R);
-  VXL_HIP(hipMemcpyAsync(ch.data(), s->ch_dev, sizeof(double) * 6 * R, hipMemcpyDeviceToHost, st));
+  double* gp = s->gram;                                 // 36 + 21 rows of kGramBlocks partials
+  double* gsum = s->gram + 57 * size_t(kGramBlocks);    // 36 + 21 totals
+  if (unit) {
+    const double omega = double(n) * cell->volume;
+    VXL_TRY(launch_gram6(s->f, s->x64, len, gp, -1.0 / omega, gsum, st));
+    VXL_TRY(launch_stiffness_sums(cell, gp + 36 * size_t(kGramBlocks), kGramBlocks, st));
+    VXL_TRY(launch_row_totals(gp + 36 * size_t(kGramBlocks), 21, kGramBlocks, 1.0 / double(n), gsum + 36, st));
+    std::vector<double> g(57);
+    VXL_HIP(hipMemcpyAsync(g.data(), gsum, sizeof(double) * 57, hipMemcpyDeviceToHost, st));
+    VXL_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 6; ++k)
+      for (int i = 0; i < 6; ++i) ch[k * 6 + i] = g[36 + pidx(i, k)] + g[i * 6 + k];
+  } else {
+    VXL_TRY(launch_avg_stress(cell, R, s->x64, s->macro_dev, s->stress_partial, s->ch_dev, st));
+    VXL_HIP(hipMemcpyAsync(ch.data(), s->ch_dev, sizeof(double) * 6 * R, hipMemcpyDeviceToHost, st));
+  }
   if (u_out)
     VXL_HIP(hipMemcpyAsync(u_out, s->x64, sizeof(double) * len * R, hipMemcpyDeviceToDevice, st));
   VXL_HIP(hipEventRecord(s->ev[3], st));
@@ -524,6 +546,7 @@ int solver_create(vxl_solver* s) {
     }
   }
   VXL_HIP(dev_malloc(&s->hist, sizeof(double) * kMaxDirections * R));
+  VXL_HIP(dev_malloc(&s->gram, sizeof(double) * (57 * size_t(kGramBlocks) + 64)));
   s->nblk = vec_blocks(s->len);
   const int nblk_rows = vec_blocks(cell->grid.n);
   const size_t np = size_t(std::max(s->nblk * R, 3 * R * (nblk_rows + 1)));

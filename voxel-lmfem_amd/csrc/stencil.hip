@@ -1090,6 +1090,35 @@ int64_t rhs_scratch_size(const vxl_cell* cell, int nload) {
   return int64_t(cell->rows) * nload * 6;
 }
 
+// Per-block sums of the 21 coefficients (times the per-voxel scale) over
+// the elements: the volume average of C is their total / N.
+template <typename Idx, bool kScale>
+__global__ void __launch_bounds__(256)
+stiffness_sums_kernel(const void* index, const double* table, const float* scale, int64_t n, double* partial) {
+  double acc[21];
+#pragma unroll
+  for (int c = 0; c < 21; ++c) acc[c] = 0.0;
+  for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < n; e += int64_t(gridDim.x) * 256) {
+    const double* row = table + int64_t(load_row<Idx>(index, e)) * kRowWidth;
+    const double sc = kScale ? double(scale[e]) : 1.0;
+#pragma unroll
+    for (int c = 0; c < 21; ++c) acc[c] = fma(sc, row[c], acc[c]);
+  }
+  __shared__ double red[4][21];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 21; ++c) {
+    double v = acc[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 21)
+    partial[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
 int64_t stress_partial_size(const vxl_cell* cell, int nload) {
   const int64_t blocks = (cell->grid.n + kStressBlock - 1) / kStressBlock;
   return blocks * 6 * nload;
@@ -1131,6 +1160,20 @@ static int stress_common(const vxl_cell* cell, int nload, const double* u, const
     VXL_HIP(hipGetLastError());
   }
   return VXL_OK;
+}
+
+int launch_stiffness_sums(const vxl_cell* cell, double* partial, int blocks, hipStream_t s) {
+  return by_index(cell, [&](auto tag) -> int {
+    using Idx = decltype(tag);
+    if (cell->mat.scale)
+      hipLaunchKernelGGL((stiffness_sums_kernel<Idx, true>), dim3(unsigned(blocks)), dim3(256), 0, s, cell->mat.index,
+                         cell->mat.c64, cell->mat.scale, cell->grid.n, partial);
+    else
+      hipLaunchKernelGGL((stiffness_sums_kernel<Idx, false>), dim3(unsigned(blocks)), dim3(256), 0, s, cell->mat.index,
+                         cell->mat.c64, cell->mat.scale, cell->grid.n, partial);
+    VXL_HIP(hipGetLastError());
+    return VXL_OK;
+  });
 }
 
 int launch_avg_stress(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,

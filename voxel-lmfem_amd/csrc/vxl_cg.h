@@ -33,6 +33,16 @@ struct DirectionSet {
 template <typename T>
 int launch_update_r(T* r, const T* q, int64_t len, int nrhs, const double* alpha, double* partial,
                     const int* active, hipStream_t s);
+// out[i * 6 + j] = scale * f_i . u_j for six stacked FP64 fields (partial:
+// 36 * kGramBlocks doubles).
+constexpr int kGramBlocks = 256;
+int launch_gram6(const double* f, const double* u, int64_t len, double* partial, double scale, double* out,
+                 hipStream_t s);
+// out[row] = scale * sum of partial[row * nblk + 0 .. nblk) (fixed order).
+int launch_row_totals(const double* partial, int rows, int nblk, double scale, double* out, hipStream_t s);
+// partial[c * blocks + block]: per-block sums of the 21 stiffness
+// coefficients (times the per-voxel scale) over all elements.
+int launch_stiffness_sums(const vxl_cell* cell, double* partial, int blocks, hipStream_t s);
 // x64 += sum_t alpha[t * nrhs + k] p_t for every system k.
 template <typename T>
 int launch_flush(double* x64, const DirectionSet<T>& d, const double* alpha, int nrhs, int64_t len, hipStream_t s);
@@ -156,6 +166,7 @@ struct vxl_solver {
   int nslot;
   void* pring[vxl::kMaxDirections];
   double* hist;  // alpha_t per slot use and system, kMaxDirections x R
+  double* gram;  // mean-stress partials and totals of the six-unit-strain path
   // Per-system residual history of the last run: (iteration, relative
   // residual); a negative iteration marks an FP64 true-residual evaluation.
   std::vector<std::vector<std::pair<int, double>>> trace;
