@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--preconditioner", default="fft", choices=["fft", "fft_bf16", "jacobi"],
                     help="mixed-mode preconditioner of the timed solve (fft: FP32 spectra; fft_bf16: BF16-stored "
                          "spectra; jacobi: 1/diag K control)")
+    ap.add_argument("--split", type=int, default=1, choices=[1, 2, 3, 6],
+                    help="solve the six loads as this many concurrent block solves on separate streams")
     ap.add_argument("--skip-batch", action="store_true",
                     help="skip the 64-cell uneven batch reported beside the headline number")
     ap.add_argument("--batch-only", action="store_true", help="run only the 64-cell batch leg")
@@ -251,13 +253,38 @@ def main():
         kind = args.preconditioner if mode == "mixed" else "fft"
         return cell, Solver(cell, lam, mu, mode=mode, nrhs=6, preconditioner=kind), rmeta
 
+    split_streams = [torch.cuda.Stream() for _ in range(args.split)] if args.split > 1 else []
+
     def step(mode):
         """One full C_eff: device label front end, reference medium, operator and
         preconditioner set-up, RHS, block PCG, FP64 certificate, stress, teardown."""
+        if args.split > 1 and mode == "mixed":
+            return split_step(mode)
         cell, solver, rmeta = setup(mode)
         out = solver.run()
         solver.close()
         cell.close()
+        return out, rmeta
+
+    def split_step(mode):
+        """The six loads as `split` independent block solves on their own
+        streams, driven concurrently (vxl_solver_run_batch)."""
+        from voxel_lmfem.gpu import run_batch
+        field = pm.build_field_device(labels, mats)
+        lam, mu, rmeta = select(field, phase, mats)
+        torch.cuda.current_stream().synchronize()
+        per = 6 // args.split
+        cells = [Cell(field, phase.shape, h, stream=st) for st in split_streams]
+        solvers = [Solver(c, lam, mu, mode=mode, nrhs=per) for c in cells]
+        outs = run_batch(solvers, macro=np.eye(6), workers=args.split)
+        for sv, c in zip(solvers, cells):
+            sv.close()
+            c.close()
+        out = dict(CH=np.concatenate([o["CH"] for o in outs], axis=1),
+                   iterations=sum((o["iterations"] for o in outs), []),
+                   restarts=sum((o["restarts"] for o in outs), []),
+                   true_residuals=sum((o["true_residuals"] for o in outs), []),
+                   device_ms={}, graph_blocks=sum(o["graph_blocks"] for o in outs))
         return out, rmeta
 
     if args.stencil_only:
