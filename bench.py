@@ -197,6 +197,7 @@ def main():
                          "spectra; jacobi: 1/diag K control)")
     ap.add_argument("--split", type=int, default=1, choices=[1, 2, 3, 6],
                     help="solve the six loads as this many concurrent block solves on separate streams")
+    ap.add_argument("--breakdown", action="store_true", help="time the parts of one step (diagnostic)")
     ap.add_argument("--skip-batch", action="store_true",
                     help="skip the 64-cell uneven batch reported beside the headline number")
     ap.add_argument("--batch-only", action="store_true", help="run only the 64-cell batch leg")
@@ -286,6 +287,31 @@ def main():
                    true_residuals=sum((o["true_residuals"] for o in outs), []),
                    device_ms={}, graph_blocks=sum(o["graph_blocks"] for o in outs))
         return out, rmeta
+
+    if args.breakdown:
+        # Host-side split of one step (synchronising between the parts).
+        for _ in range(3):
+            marks = [("start", time.perf_counter())]
+
+            def mark(name):
+                torch.cuda.synchronize()
+                marks.append((name, time.perf_counter()))
+            field = pm.build_field_device(labels, mats)
+            mark("label front end")
+            lam, mu, _ = select(field, phase, mats)
+            mark("reference medium")
+            cell = Cell(field, phase.shape, h)
+            mark("cell")
+            solver = Solver(cell, lam, mu, mode="mixed", nrhs=6)
+            mark("solver + preconditioner set-up")
+            solver.run()
+            mark("run")
+            solver.close()
+            cell.close()
+            mark("teardown")
+            print("# breakdown ms: " + ", ".join(f"{marks[i][0]} {1e3 * (marks[i][1] - marks[i - 1][1]):.2f}"
+                                                 for i in range(1, len(marks))), file=sys.stderr, flush=True)
+        return
 
     if args.stencil_only:
         cell, solver, _ = setup("mixed")
