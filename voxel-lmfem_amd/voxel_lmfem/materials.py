@@ -80,6 +80,35 @@ def stiffness(spec):
     return C
 
 
+def _stiffness_all(specs):
+    """stiffness() of every spec; the anisotropic ones are checked as one
+    stack (same arithmetic and result), and any spec the stack rejects is
+    re-run alone for its error message."""
+    out = [None] * len(specs)
+    batch = [i for i, s in enumerate(specs)
+             if s.get("kind") == "anisotropic" and s.get("frame", "global") in ("global", "local")]
+    if len(batch) >= 8:
+        try:
+            C = np.stack([np.asarray(specs[i]["C"], dtype=np.float64) for i in batch])
+        except (ValueError, TypeError):
+            C = None
+        if C is not None and C.shape[1:] == (6, 6) and np.isfinite(C).all():
+            scale = np.abs(C).max(axis=(1, 2))
+            sym = (np.abs(C - C.transpose(0, 2, 1)) <= (1e-12 * scale)[:, None, None]).all(axis=(1, 2))
+            if sym.all():
+                C = (C + C.transpose(0, 2, 1)) * .5
+                try:
+                    np.linalg.cholesky(C)
+                    for j, i in enumerate(batch):
+                        out[i] = C[j]
+                except np.linalg.LinAlgError:
+                    pass
+    for i, s in enumerate(specs):
+        if out[i] is None:
+            out[i] = stiffness(s)
+    return np.stack(out)
+
+
 def phase_table(materials):
     """Sorted int64 labels, (L,6,6) tensors, local-frame flags."""
     if not isinstance(materials, Mapping) or not materials:
@@ -88,7 +117,7 @@ def phase_table(materials):
         raise ValueError("Material labels must be integers")
     labels = np.array(sorted(materials), dtype=np.int64)
     specs = [materials[int(k)] for k in labels]
-    tensors = np.stack([stiffness(s) for s in specs])
+    tensors = _stiffness_all(specs)
     local = np.array([s["kind"] != "isotropic" and s.get("frame", "global") == "local" for s in specs])
     return labels, tensors, local
 

@@ -13,7 +13,6 @@ Same selection as the reference chain:
 The choice changes iteration counts, never the converged C_eff.
 """
 import numpy as np
-from scipy.linalg import eigvalsh
 
 
 def _iso(lam, mu):
@@ -33,29 +32,47 @@ def zeman(field, shape):
     return _iso(max(0., K - 2 * mu / 3.), mu)
 
 
+def _spectra(phases, reference):
+    """(k, 6) ascending generalized eigenvalues of each phase against the SPD
+    reference (Cholesky reduction L^-1 C L^-T, as LAPACK's sygv)."""
+    L = np.linalg.cholesky(reference)
+    Li = np.linalg.inv(L)
+    M = Li @ np.asarray(phases) @ Li.T
+    return np.linalg.eigvalsh(0.5 * (M + M.transpose(0, 2, 1)))
+
+
 def _bound(phases, reference):
-    spectra = [eigvalsh(c, reference, check_finite=False) for c in phases]
-    return float(max(s[-1] for s in spectra)) / float(min(s[0] for s in spectra))
+    spectra = _spectra(phases, reference)
+    return float(spectra[:, -1].max()) / float(spectra[:, 0].min())
 
 
 def _is_isotropic(C, rtol=1e-10):
-    bulk = float(np.sum(C[:3, :3]) / 9.0)
-    mu = float(np.trace(C[3:, 3:]) / 3.0)
-    cand = _iso(bulk - 2.0 * mu / 3.0, mu)
-    return float(np.max(np.abs(C - cand)) / np.max(np.abs(C))) <= rtol
+    """Per tensor of a (k, 6, 6) stack (or one 6x6): equal to the isotropic
+    tensor with its own bulk and shear moduli to rtol."""
+    C = np.asarray(C, dtype=float)
+    one = C.ndim == 2
+    C = C[None] if one else C
+    bulk = C[:, :3, :3].sum(axis=(1, 2)) / 9.0
+    mu = np.trace(C[:, 3:, 3:], axis1=1, axis2=2) / 3.0
+    lam = bulk - 2.0 * mu / 3.0
+    cand = np.zeros_like(C)
+    cand[:, :3, :3] = lam[:, None, None]
+    cand[:, np.arange(3), np.arange(3)] += 2 * mu[:, None]
+    cand[:, np.arange(3, 6), np.arange(3, 6)] = mu[:, None]
+    ok = np.abs(C - cand).max(axis=(1, 2)) / np.abs(C).max(axis=(1, 2)) <= rtol
+    return bool(ok[0]) if one else ok
 
 
 def design(phases, baseline, *, order_rtol=1e-10, tie_rtol=1e-10):
     """Return (reference 6x6, status in {'closed_form','baseline','unsupported'})."""
-    phases = [0.5 * (np.asarray(c, float) + np.asarray(c, float).T) for c in phases]
+    phases = np.asarray(phases, dtype=float)
+    phases = 0.5 * (phases + phases.transpose(0, 2, 1))
     base_bound = _bound(phases, baseline)
     mu = float(baseline[3, 3])
     best = None
-    for soft in phases:
-        if not _is_isotropic(soft):
-            continue
-        spectra = [eigvalsh(c, soft, check_finite=False) for c in phases]
-        if min(v[0] for v in spectra) < 1.0 - order_rtol:
+    for soft in phases[_is_isotropic(phases)]:
+        spectra = _spectra(phases, soft)
+        if float(spectra[:, 0].min()) < 1.0 - order_rtol:
             continue
         ref = (mu / float(soft[3, 3])) * soft
         b = _bound(phases, ref)
@@ -70,10 +87,9 @@ def design(phases, baseline, *, order_rtol=1e-10, tie_rtol=1e-10):
 
 def select(field, phase, materials):
     """(lam0, mu0, metadata) for a cell."""
-    from .materials import stiffness
     base = zeman(field, phase.shape)
-    used = field.used_labels(phase)
-    phases = [stiffness(materials[int(k)]) for k in used]
+    used = np.asarray(field.used_labels(phase))
+    phases = field.tensors[np.searchsorted(field.labels, used)]  # validated when the field was built
     ref, status = design(phases, base)
     lam, mu = float(ref[0, 1]), float(ref[3, 3])
     return lam, mu, dict(status=status, lam=lam, mu=mu, zeman_lam=float(base[0, 1]),
