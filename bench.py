@@ -430,7 +430,7 @@ def main():
                     step_ms=[round(t, 2) for t in step_ms], median_ms_per_step=float(np.median(step_ms))),
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_src,
-                      kernel="apply_kernel_v2<float,uint8> R=6, 32x14 tile, cubic-voxel form (schedule 2)",
+                      kernel="apply_kernel_v2<float,uint8> R=6, 32x14 tile, cubic-voxel form (schedule 3)",
                       ms_per_launch=t_apply * 1e3, algorithmic_bytes=alg_bytes),
         fp64=fp64,
         batch=batch,

@@ -230,6 +230,24 @@ subtract_mean_kernel(double* u, int64_t n, const double* mean, const int* active
     ur[i] -= m;
 }
 
+// f -= mean (per RHS and component), r = TR(f), partial ||f||^2: the RHS
+// projection, its norm and the first residual in one pass.
+template <typename TR>
+__global__ void __launch_bounds__(kBlock)
+center_copy_kernel(double* f, TR* r, const double* mean, int64_t n, double* partial) {
+  const int k = blockIdx.y;
+  const int64_t len = 3 * n, off = int64_t(k) * len;
+  double acc = 0;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock) {
+    const double v = f[off + i] - mean[k * 3 + int(i / n)];
+    f[off + i] = v;
+    r[off + i] = TR(v);
+    acc += v * v;
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[int64_t(k) * gridDim.x + blockIdx.x] = acc;
+}
+
 // One workgroup per RHS: total = sum partial[k][0..nblk) in a fixed order,
 // then the scalar update selected by op.
 __global__ void __launch_bounds__(kBlock)
@@ -400,6 +418,25 @@ int launch_fill_zero(T* dst, int64_t len, int nrhs, const int* active, hipStream
 }
 template int launch_fill_zero<float>(float*, int64_t, int, const int*, hipStream_t);
 template int launch_fill_zero<double>(double*, int64_t, int, const int*, hipStream_t);
+
+template <typename TR>
+int launch_center_copy(double* f, TR* r, int64_t n, int nrhs, double* partial, int* nblk_out, hipStream_t s) {
+  const dim3 g = vgrid(n, 3 * nrhs);
+  double* mean = partial + int64_t(3) * nrhs * g.x;  // 3*nrhs slots after the row partials
+  hipLaunchKernelGGL(row_sum_kernel, g, dim3(kBlock), 0, s, f, n, partial);
+  VXL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(row_mean_kernel, dim3(3 * nrhs), dim3(kBlock), 0, s, partial, int(g.x), 1.0 / double(n), mean);
+  VXL_HIP(hipGetLastError());
+  // The norm partials go after the means (the row partials are consumed).
+  double* norms = mean + 3 * nrhs;
+  const dim3 gv = vgrid(3 * n, nrhs);
+  hipLaunchKernelGGL((center_copy_kernel<TR>), gv, dim3(kBlock), 0, s, f, r, mean, n, norms);
+  VXL_HIP(hipGetLastError());
+  *nblk_out = int(gv.x);
+  return VXL_OK;
+}
+template int launch_center_copy<float>(double*, float*, int64_t, int, double*, int*, hipStream_t);
+template int launch_center_copy<double>(double*, double*, int64_t, int, double*, int*, hipStream_t);
 
 int launch_project_zero_mean(double* u, int64_t n, int nrhs, double* partial, const int* active,
                              hipStream_t s) {

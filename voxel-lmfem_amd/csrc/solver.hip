@@ -278,11 +278,15 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   VXL_HIP(hipMemsetAsync(scal, 0, sizeof(double) * kNumSlots * R, st));
   VXL_HIP(hipMemsetAsync(s->stop_dev, 0, sizeof(int) * R, st));
   VXL_TRY(upload_flags());
-  // f = P0 rhs(E); ||f||^2
+  // f = P0 rhs(E), r = f and ||f||^2 (one pass after the component means)
   VXL_TRY(launch_rhs(cell, R, s->macro_dev, s->f, s->rhs_sig, st));
-  VXL_TRY(launch_project_zero_mean(s->f, n, R, partial, nullptr, st));
-  VXL_TRY(launch_dot<double, double>(s->f, s->f, len, R, partial, nullptr, st));
-  VXL_TRY(launch_finalize(partial, nblk, kOpFF, scal, R, nullptr, nullptr, st));
+  int nblk_ff = 0;
+  if (mixed)
+    VXL_TRY(launch_center_copy<float>(s->f, reinterpret_cast<float*>(r), n, R, partial, &nblk_ff, st));
+  else
+    VXL_TRY(launch_center_copy<double>(s->f, s->r64, n, R, partial, &nblk_ff, st));
+  VXL_TRY(launch_finalize(partial + int64_t(3) * R * (vec_blocks(n) + 1), nblk_ff, kOpFF, scal, R, nullptr, nullptr,
+                          st));
   VXL_TRY(read_slot(kFF, ff));
   for (int k = 0; k < R; ++k) {
     if (!(ff[k] > 0.0)) {  // zero RHS: exact zero solution (benchmark.py:79)
@@ -293,12 +297,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   }
   VXL_TRY(upload_flags());
   VXL_TRY(launch_fill_zero<double>(s->x64, len, R, nullptr, st));
-  if (mixed) {
-    VXL_TRY(launch_convert<double, float>(reinterpret_cast<float*>(r), s->f, len, R, act, st));
-    VXL_TRY(launch_fill_zero<float>(reinterpret_cast<float*>(x), len, R, act, st));
-  } else {
-    VXL_TRY(launch_convert<double, double>(s->r64, s->f, len, R, act, st));
-  }
+  if (mixed) VXL_TRY(launch_fill_zero<float>(reinterpret_cast<float*>(x), len, R, act, st));
   // p = z = M r with rho = r.z: beta = 0 because rho_old = 0 (slots cleared).
   Steps<T> steps(s, tol, st);
   const bool jacobi = steps.jacobi;
@@ -549,7 +548,8 @@ int solver_create(vxl_solver* s) {
   VXL_HIP(dev_malloc(&s->gram, sizeof(double) * (57 * size_t(kGramBlocks) + 64)));
   s->nblk = vec_blocks(s->len);
   const int nblk_rows = vec_blocks(cell->grid.n);
-  const size_t np = size_t(std::max(s->nblk * R, 3 * R * (nblk_rows + 1)));
+  // row sums and means of the projection, then one ||f||^2 partial per block
+  const size_t np = size_t(3 * R * (nblk_rows + 1) + s->nblk * R);
   VXL_HIP(dev_malloc(&s->partial, sizeof(double) * np));
   {
     // sized for any tile / chunk depth the cell can be switched to

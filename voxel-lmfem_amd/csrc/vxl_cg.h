@@ -38,6 +38,11 @@ int launch_update_r(T* r, const T* q, int64_t len, int nrhs, const double* alpha
 constexpr int kGramBlocks = 256;
 int launch_gram6(const double* f, const double* u, int64_t len, double* partial, double scale, double* out,
                  hipStream_t s);
+// f -= its per-(RHS, component) mean, r = f (in r's precision) and the
+// ||f||^2 partials, which start at partial + 3 nrhs (nblk_x + 1) with
+// *nblk_out blocks per RHS.
+template <typename TR>
+int launch_center_copy(double* f, TR* r, int64_t n, int nrhs, double* partial, int* nblk_out, hipStream_t s);
 // out[row] = scale * sum of partial[row * nblk + 0 .. nblk) (fixed order).
 int launch_row_totals(const double* partial, int rows, int nblk, double scale, double* out, hipStream_t s);
 // partial[c * blocks + block]: per-block sums of the 21 stiffness
