@@ -69,6 +69,7 @@ int free_all(vxl_solver* s) {
   s->nslot = 0;
   if (s->hist) dev_free(s->hist);
   if (s->gram) dev_free(s->gram);
+  if (s->partial_cert) dev_free(s->partial_cert);
   if (s->active_dev) dev_free(s->active_dev);
   if (s->restart_dev) dev_free(s->restart_dev);
   if (s->stop_dev) dev_free(s->stop_dev);
@@ -339,10 +340,10 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
       VXL_TRY(launch_accumulate(s->x64, reinterpret_cast<float*>(x), len, R, act, st));
     // K annihilates the mean, so the zero-mean projection of x64 is left to
     // the end of the run; mixed mode needs the residual only in FP32.
-    VXL_TRY(launch_apply<double>(cell, R, s->x64, s->q64, st));
-    VXL_TRY(launch_residual(mixed ? nullptr : s->r64, s->f, s->q64, mixed ? reinterpret_cast<float*>(r) : nullptr, len,
-                            R, partial, act, st));
-    VXL_TRY(launch_finalize(partial, nblk, kOpStore, scal, R, act, nullptr, st));
+    int rblocks = 0;
+    VXL_TRY(launch_residual_apply(cell, R, s->x64, s->f, mixed ? reinterpret_cast<float*>(r) : nullptr,
+                                  mixed ? nullptr : s->r64, act, s->partial_cert, &rblocks, st));
+    VXL_TRY(launch_finalize(s->partial_cert, rblocks, kOpStore, scal, R, act, nullptr, st));
     std::vector<double> t(R);
     VXL_TRY(read_slot(kTmp, t));
     for (int k = 0; k < R; ++k) {
@@ -546,6 +547,7 @@ int solver_create(vxl_solver* s) {
   }
   VXL_HIP(dev_malloc(&s->hist, sizeof(double) * kMaxDirections * R));
   VXL_HIP(dev_malloc(&s->gram, sizeof(double) * (57 * size_t(kGramBlocks) + 64)));
+  VXL_HIP(dev_malloc(&s->partial_cert, sizeof(double) * size_t(residual_apply_blocks(cell, R)) * R));
   s->nblk = vec_blocks(s->len);
   const int nblk_rows = vec_blocks(cell->grid.n);
   // row sums and means of the projection, then one ||f||^2 partial per block

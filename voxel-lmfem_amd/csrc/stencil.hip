@@ -232,6 +232,10 @@ struct ApplyArgs {
   int tiles_x;
   G9<T> g;
   T hg;  // C3D8R-HG: coef * V / hbar^2 / 8 (hourglass penalty per unit material scale)
+  // kResid: r = f - K u written as r32 (if set) or r64, ||r||^2 per workgroup.
+  const double* f;
+  float* r32;
+  double* r64;
   // kFuse: systems with active[r] == 0 are skipped and u . (K u) of each
   // workgroup is written to partial[(r * chunks + chunk) * tiles + tile].
   const int* active;
@@ -412,7 +416,9 @@ apply_kernel(ApplyArgs<T> a) {
 // kIso: cubic voxels, a.g holds the squared modal factors (strain_mode).
 // kHG: the C3D8R-HG element instead of the fully integrated one.
 // FP32: the register budget lets two workgroups share a CU (waves per SIMD).
-template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse, bool kPF, bool kIso, bool kHG = false>
+// kResid: the certification form r = f - K u with ||r||^2 (FP64 operator).
+template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse, bool kPF, bool kIso, bool kHG = false,
+          bool kResid = false>
 __global__ void __launch_bounds__((Tile<TX, TY>::NT), (sizeof(T) == 4 ? (2 * Tile<TX, TY>::NT / 64 + 3) / 4 : 0))
 apply_kernel_v2(ApplyArgs<T> a) {
   using TL = Tile<TX, TY>;
@@ -428,7 +434,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
   const int z0 = bid.chunk * a.zchunk;
   const int z1 = min(z0 + a.zchunk, a.nz);
   if (z0 >= a.nz) return;
-  if constexpr (kFuse) {
+  if constexpr (kFuse || kResid) {
     if (a.active && !a.active[bid.rhs]) return;
   }
   const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -513,6 +519,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
   // their transposes to the forces).  Each element is counted by the tile and
   // z chunk that own it: local columns 1..TX, rows 1..TY, inside the grid.
   const bool own_elem = is_elem && exl >= 1 && eyl >= 1 && x0 - 1 + exl < nx && y0 - 1 + eyl < ny;
+  double rr = 0.0;  // kResid: sum of r^2 over the nodes this thread writes
   T pq = T(0);  // at most zchunk elements per thread; FP64 from the block sum on
 
   // Next-plane prefetch: raw values of the plane after next are loaded into
@@ -614,8 +621,21 @@ apply_kernel_v2(ApplyArgs<T> a) {
           for (int cc = 0; cc < 3; ++cc) acc[cc] += sc[corner * 3 + cc][e];
         }
       const int64_t node = int64_t(z) * nxy + n_xy;
+      if constexpr (kResid) {
+        const int64_t base = int64_t(bid.rhs) * a.rhs_stride;
 #pragma unroll
-      for (int cc = 0; cc < 3; ++cc) out[cc * n + node] = acc[cc];
+        for (int cc = 0; cc < 3; ++cc) {
+          const double v = a.f[base + cc * n + node] - double(acc[cc]);
+          if (a.r32)
+            a.r32[base + cc * n + node] = float(v);
+          else
+            a.r64[base + cc * n + node] = v;
+          rr = fma(v, v, rr);
+        }
+      } else {
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) out[cc * n + node] = acc[cc];
+      }
     }
     if constexpr (kPF) __syncthreads();
   };
@@ -624,8 +644,8 @@ apply_kernel_v2(ApplyArgs<T> a) {
     layer(z, Bf, Ftp, Bf2, Ftp2);
     if (z + 1 < z1) layer(z + 1, Bf2, Ftp2, Bf, Ftp);
   }
-  if constexpr (kFuse) {
-    const double total = block_reduce_sum<TL::NT>(double(pq), red);
+  if constexpr (kFuse || kResid) {
+    const double total = block_reduce_sum<TL::NT>(kResid ? rr : double(pq), red);
     if (tid == 0)
       a.partial[(int64_t(bid.rhs) * gridDim.y + bid.chunk) * gridDim.x + bid.tile] = total;
   }
@@ -671,7 +691,7 @@ template <typename T, typename Idx, bool kScale, int TX, int TY, bool kHG>
 int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s,
                       const int* active = nullptr, double* partial = nullptr, int* blocks = nullptr) {
   using TL = Tile<TX, TY>;
-  ApplyArgs<T> a;
+  ApplyArgs<T> a{};
   a.u = u;
   a.out = out;
   a.active = active;
@@ -958,6 +978,79 @@ int default_stencil_variant(int precision) {
   if (const int v = env_stencil_variant()) return v;
   (void)precision;
   return 3;
+}
+
+// Certification residual r = f - K u (FP64 operator, default FP64 tile,
+// schedule 3, chunks of kDefaultChunkPlanes planes): u FP64, r into r32 or
+// r64, ||r||^2 partials per workgroup; systems with active == 0 are skipped.
+template <typename Idx, bool kScale>
+int launch_resid_impl(const vxl_cell* cell, int nrhs, const double* u, const double* f, float* r32, double* r64,
+                      const int* active, double* partial, int* blocks, hipStream_t s) {
+  constexpr int TY = kTilesF64[0];
+  using TL = Tile<32, TY>;
+  ApplyArgs<double> a{};
+  a.u = u;
+  a.out = nullptr;
+  a.f = f;
+  a.r32 = r32;
+  a.r64 = r64;
+  a.active = active;
+  a.partial = partial;
+  a.rhs_stride = 3 * cell->grid.n;
+  a.index = cell->mat.index;
+  a.table = cell->mat.c64;
+  a.scale = cell->mat.scale;
+  a.nx = cell->grid.nx;
+  a.ny = cell->grid.ny;
+  a.nz = cell->grid.nz;
+  int tiles = 0, chunks = 0;
+  apply_grid<double, 32, TY>(cell, nrhs, a.tiles_x, tiles, chunks, a.zchunk, kDefaultChunkPlanes);
+  const ModalFactors mf = modal_factors(cell->h);
+  a.g = cast_g<double>(mf);
+  const bool hg = cell->element == VXL_ELEMENT_C3D8R_HG;
+  a.hg = hg ? hg_factor(cell) / 8.0 : 0.0;
+  const bool iso = cell->h[0] == cell->h[1] && cell->h[1] == cell->h[2];
+  if (iso)
+    for (int k = 0; k < 3; ++k)
+      for (int d = 0; d < 3; ++d) a.g.g[k][d] = mf.g[k][d] * mf.g[k][d];
+  const dim3 grid = dim3(tiles, chunks, nrhs);
+  if (blocks) *blocks = tiles * chunks;
+#define VXL_R(ISO, HG) \
+  hipLaunchKernelGGL((apply_kernel_v2<double, Idx, kScale, 32, TY, false, true, ISO, HG, true>), grid, dim3(TL::NT), 0, s, a)
+  if (iso && hg)
+    VXL_R(true, true);
+  else if (iso)
+    VXL_R(true, false);
+  else if (hg)
+    VXL_R(false, true);
+  else
+    VXL_R(false, false);
+#undef VXL_R
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+
+int launch_residual_apply(const vxl_cell* cell, int nrhs, const double* u, const double* f, float* r32, double* r64,
+                          const int* active, double* partial, int* blocks, hipStream_t s) {
+  const bool sc = cell->mat.scale != nullptr;
+  switch (cell->mat.index_kind) {
+    case VXL_INDEX_U8:
+      return sc ? launch_resid_impl<uint8_t, true>(cell, nrhs, u, f, r32, r64, active, partial, blocks, s)
+                : launch_resid_impl<uint8_t, false>(cell, nrhs, u, f, r32, r64, active, partial, blocks, s);
+    case VXL_INDEX_U16:
+      return sc ? launch_resid_impl<uint16_t, true>(cell, nrhs, u, f, r32, r64, active, partial, blocks, s)
+                : launch_resid_impl<uint16_t, false>(cell, nrhs, u, f, r32, r64, active, partial, blocks, s);
+    case VXL_INDEX_I32:
+      return sc ? launch_resid_impl<int32_t, true>(cell, nrhs, u, f, r32, r64, active, partial, blocks, s)
+                : launch_resid_impl<int32_t, false>(cell, nrhs, u, f, r32, r64, active, partial, blocks, s);
+  }
+  return fail("unknown material index kind");
+}
+
+int residual_apply_blocks(const vxl_cell* cell, int nrhs) {
+  int tx, tiles, chunks, zc;
+  apply_grid<double, 32, kTilesF64[0]>(cell, nrhs, tx, tiles, chunks, zc, kDefaultChunkPlanes);
+  return tiles * chunks;
 }
 
 // Runtime tile choice -> template instance (the C3D8R-HG element runs the

@@ -35,7 +35,7 @@ int launch_update_r(T* r, const T* q, int64_t len, int nrhs, const double* alpha
                     const int* active, hipStream_t s);
 // out[i * 6 + j] = scale * f_i . u_j for six stacked FP64 fields (partial:
 // 36 * kGramBlocks doubles).
-constexpr int kGramBlocks = 256;
+constexpr int kGramBlocks = 1024;
 int launch_gram6(const double* f, const double* u, int64_t len, double* partial, double scale, double* out,
                  hipStream_t s);
 // f -= its per-(RHS, component) mean, r = f (in r's precision) and the
@@ -172,6 +172,7 @@ struct vxl_solver {
   void* pring[vxl::kMaxDirections];
   double* hist;  // alpha_t per slot use and system, kMaxDirections x R
   double* gram;  // mean-stress partials and totals of the six-unit-strain path
+  double* partial_cert;  // ||f - K x||^2 partials of the certification stencil
   // Per-system residual history of the last run: (iteration, relative
   // residual); a negative iteration marks an FP64 true-residual evaluation.
   std::vector<std::vector<std::pair<int, double>>> trace;

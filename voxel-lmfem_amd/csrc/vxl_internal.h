@@ -168,6 +168,12 @@ constexpr int kDefaultChunkPlanes = 8;  // default fewest z planes per stencil c
 constexpr int kMinChunkPlanes = 2;      // smallest chunk depth a cell may be set to
 int apply_cg_blocks_max(const vxl_cell* cell, int nrhs, int precision);
 bool stencil_tile_supported(int precision, int tile_y);
+// Certification residual r = f - K u with the FP64 operator (r32 or r64
+// written; ||r||^2 partials, *blocks per system); systems with active == 0
+// are skipped.
+int launch_residual_apply(const vxl_cell* cell, int nrhs, const double* u, const double* f, float* r32, double* r64,
+                          const int* active, double* partial, int* blocks, hipStream_t s);
+int residual_apply_blocks(const vxl_cell* cell, int nrhs);
 int default_stencil_tile(int precision);
 int stencil_tile_option(int precision, int k);  // k = 0, 1, 2
 
