@@ -901,6 +901,30 @@ fft_x4_c2r(const SC* __restrict__ S, T* z, int64_t rows, int PP,
   C* L = reinterpret_cast<C*>(smem_raw);
   const int P = 3 * PP;
   const SC* in = S + int64_t(blockIdx.y) * spec_stride;
+  // Output position of this thread (pencil p, lane t) and, for the direction
+  // update, the previous direction there, loaded first so that its latency
+  // overlaps the transform.
+  const int t = threadIdx.x % lanes, p = threadIdx.x / lanes;
+  const int pp = p / 3, q = p - 3 * pp;
+  const bool live = p < P;
+  const int64_t plane = rows * n;
+  const int64_t rho = 2 * (int64_t(blockIdx.x) * PP + pp);
+  const int64_t re_off = q == 2 ? 2 * plane + rho * n : rho * n + (q == 1 ? n : 0);
+  const int64_t im_off = q == 2 ? 2 * plane + (rho + 1) * n : plane + rho * n + (q == 1 ? n : 0);
+  const bool re_ok = q == 1 ? rho + 1 < rows : rho < rows;
+  const bool im_ok = rho + (q == 0 ? 0 : 1) < rows;
+  const T b = kDir ? T(beta[blockIdx.y]) : T(0);
+  const bool keep = kDir && b != T(0);
+  T prev[2][N2];
+  if (keep && live && t < N1) {
+    const T* g = (pin ? pin : z) + int64_t(blockIdx.y) * field_stride;
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) {
+      const int e = t + N1 * j2;
+      prev[0][j2] = re_ok ? g[re_off + e] : T(0);
+      prev[1][j2] = im_ok ? g[im_off + e] : T(0);
+    }
+  }
   // Hermitian extension and packing of the three pencils per row pair: each
   // stored coefficient kk is read once and fills entry kk and, conjugated,
   // its mirror n - kk.
@@ -932,9 +956,6 @@ fft_x4_c2r(const SC* __restrict__ S, T* z, int64_t rows, int PP,
     }
   }
   __syncthreads();
-  const int t = threadIdx.x % lanes, p = threadIdx.x / lanes;
-  const int pp = p / 3, q = p - 3 * pp;
-  const bool live = p < P;
   C* X = L + p * XT::stride;
   C v[N1];
   if (live && t < N2) {
@@ -956,21 +977,12 @@ fft_x4_c2r(const SC* __restrict__ S, T* z, int64_t rows, int PP,
 #pragma unroll
   for (int k2 = 0; k2 < N2; ++k2) u[k2] = X[t * (N2 + 1) + k2];
   dft_reg<N2, true, T>(u, w, N1);
-  const int64_t plane = rows * n;
   T* f = z + int64_t(blockIdx.y) * field_stride;
-  const int64_t rho = 2 * (int64_t(blockIdx.x) * PP + pp);
-  const int64_t re_off = q == 2 ? 2 * plane + rho * n : rho * n + (q == 1 ? n : 0);
-  const int64_t im_off = q == 2 ? 2 * plane + (rho + 1) * n : plane + rho * n + (q == 1 ? n : 0);
-  const bool re_ok = q == 1 ? rho + 1 < rows : rho < rows;
-  const bool im_ok = rho + (q == 0 ? 0 : 1) < rows;
-  const T b = kDir ? T(beta[blockIdx.y]) : T(0);
-  const bool keep = kDir && b != T(0);
-  const T* g = (pin ? pin : z) + int64_t(blockIdx.y) * field_stride;
 #pragma unroll
   for (int j2 = 0; j2 < N2; ++j2) {
     const int e = t + N1 * j2;
-    if (re_ok) f[re_off + e] = keep ? fma(b, g[re_off + e], u[j2].x) : u[j2].x;
-    if (im_ok) f[im_off + e] = keep ? fma(b, g[im_off + e], u[j2].y) : u[j2].y;
+    if (re_ok) f[re_off + e] = keep ? fma(b, prev[0][j2], u[j2].x) : u[j2].x;
+    if (im_ok) f[im_off + e] = keep ? fma(b, prev[1][j2], u[j2].y) : u[j2].y;
   }
 }
 
