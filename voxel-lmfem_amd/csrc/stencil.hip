@@ -582,6 +582,16 @@ apply_kernel_v2(ApplyArgs<T> a) {
       if (is_elem && z + 1 < z1) row_next = row_of(z + 1);
     }
     const bool more = kPF && z + 1 < z1;  // another step follows: it needs plane z+2
+    // kResid: the right-hand side at this thread's output node, loaded before
+    // the element work so that its latency is hidden.
+    double fpre[3] = {0.0, 0.0, 0.0};
+    if constexpr (kResid) {
+      if (node_valid) {
+        const int64_t base = int64_t(bid.rhs) * a.rhs_stride + int64_t(z) * nxy + n_xy;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) fpre[cc] = a.f[base + cc * n];
+      }
+    }
     if constexpr (!kPF) {
       load_plane(z + 1);
       __syncthreads();
@@ -625,7 +635,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
         const int64_t base = int64_t(bid.rhs) * a.rhs_stride;
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
-          const double v = a.f[base + cc * n + node] - double(acc[cc]);
+          const double v = fpre[cc] - double(acc[cc]);
           if (a.r32)
             a.r32[base + cc * n + node] = float(v);
           else
