@@ -653,8 +653,11 @@ __device__ __forceinline__ void four_step_2(const C* L, int TJ, int p, int t, co
   for (int j2 = 0; j2 < N2; ++j2) store(t + N1 * j2, u[j2]);
 }
 
+// Threads per four-step y / z workgroup, at most.
+constexpr int kThreads4 = 512;
+
 template <bool INV, typename T, int N1, int N2, typename SC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kThreads4)
 fft_y4(SC* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict__ w, int64_t spec_stride,
        const int* active) {
   using C = typename CT<T>::type;
@@ -680,13 +683,13 @@ fft_y4(SC* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict
 // Launched with a multiple of 64 threads; threads past 3 TK lanes-per-column
 // only take part in the Green loop and the reduction.  kDot as in fft_z_green.
 template <typename T, int N1, int N2, bool kDot, typename SC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kThreads4)
 fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T mu, T inv_vn,
              int64_t spec_stride, const int* active, int nyq, double* partial) {
   using C = typename CT<T>::type;
   constexpr int nz = N1 * N2;
   extern __shared__ double2 smem_raw[];
-  __shared__ double red[4];
+  __shared__ double red[kThreads4 / 64];
   const uint3 b = xcd_block3();
   if (active && !active[b.z]) return;
   C* L = reinterpret_cast<C*>(smem_raw);
@@ -1199,17 +1202,17 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
     im->z4 = (enable && lz) ? im->nz : 0;
     if (im->y4) {
       im->tj4 = std::max(1, std::min({cols, 256 / ly, 3 * im->nxh}));
-      if (const int v = env_int("VXL_FFT_YCOLS")) im->tj4 = std::min({v, 256 / ly, 3 * im->nxh});
+      if (const int v = env_int("VXL_FFT_YCOLS")) im->tj4 = std::min({v, kThreads4 / ly, 3 * im->nxh});
       im->thr_y4 = im->tj4 * ly;
       im->lds_y4 = sizeof(C) * size_t(im->ny) * im->tj4;
       if (im->lds_y4 > kMaxLds) im->y4 = 0;
     }
     if (im->z4) {
       im->tk4 = std::max(1, std::min({(cols + 2) / 3, 256 / (3 * lz), im->nxh}));
-      if (const int v = env_int("VXL_FFT_ZCOLS")) im->tk4 = std::min({v, 256 / (3 * lz), im->nxh});
+      if (const int v = env_int("VXL_FFT_ZCOLS")) im->tk4 = std::min({v, kThreads4 / (3 * lz), im->nxh});
       im->thr_z4 = (3 * im->tk4 * lz + 63) / 64 * 64;  // whole waves for the block reduction
       im->lds_z4 = sizeof(C) * size_t(im->nz) * 3 * im->tk4;
-      if (im->lds_z4 > kMaxLds || im->thr_z4 > 256 || im->thr_z4 < 1) im->z4 = 0;
+      if (im->lds_z4 > kMaxLds || im->thr_z4 > kThreads4 || im->thr_z4 < 1) im->z4 = 0;
     }
   }
   {
