@@ -166,3 +166,38 @@ def test_direction_ring_matches_per_iteration_update(torch_gpu, monkeypatch, pre
     assert rel < 1e-9
     assert max(abs(x - y) for x, y in zip(its_a, its_b)) <= 1
     assert b["max_true_residual"] <= 1e-8
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_zero_load_among_others(torch_gpu, mode):
+    """A zero macro strain among five others (benchmark.py:79: zero RHS -> 0):
+    its displacement stays exactly zero through the direction ring and its
+    stress column is zero; the other columns equal the six-load solve's.
+    Solved twice on one solver, so the second run starts from direction
+    slots the first one filled."""
+    import torch
+    from voxel_lmfem import materials as pm
+    from voxel_lmfem.cells import voronoi_polycrystal
+    from voxel_lmfem.gpu import Cell, Solver
+    (phase, mats, dom, _), _ = voronoi_polycrystal(n=24, grains=10, seed=4)
+    print(f"case zero load {mode} 24^3", flush=True)
+    field = pm.build_field(phase, mats)
+    h = dom / np.array(phase.shape[::-1], dtype=float)
+    cell = Cell(field, phase.shape, h)
+    solver = Solver(cell, 100.0, 40.0, mode=mode, nrhs=6)
+    full = solver.run(return_fields=True)
+    macro = np.eye(6)
+    macro[3] = 0.0
+    for _ in range(2):
+        out = solver.run(macro, return_fields=True)
+        u = out["fields"]
+        assert torch.count_nonzero(u[3]).item() == 0
+        assert np.all(out["CH"][:, 3] == 0.0) and out["true_residuals"][3] == 0.0
+        keep = [0, 1, 2, 4, 5]
+        err = rel(out["CH"][:, keep], full["CH"][:, keep])
+        du = float((u[keep] - full["fields"][keep]).abs().max() / full["fields"][keep].abs().max())
+        print(f"  CH rel {err:.2e}, u rel {du:.2e}", flush=True)
+        assert err < 1e-8 and du < 1e-5
+        assert max(out["true_residuals"]) <= 1e-8
+    solver.close()
+    cell.close()

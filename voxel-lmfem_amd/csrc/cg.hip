@@ -88,9 +88,11 @@ update_r_kernel(T* r, const T* q, int64_t len, const double* alpha, double* part
   if (threadIdx.x == 0) partial[int64_t(k) * gridDim.x + blockIdx.x] = acc;
 }
 
-// x64 += sum_t alpha_t p_t over the stored search directions of every system
-// (alpha_t = 0 where a system was not active), accumulated in FP64.
-template <typename T>
+// x64 += sum_t alpha_t p_t over the stored search directions of every system,
+// accumulated in FP64 (kAssign: x64 = sum_t alpha_t p_t, x64 not read).
+// alpha_t = 0 where a system was not active; such terms are skipped, so a
+// direction slot that a system never wrote does not reach its x64.
+template <typename T, bool kAssign>
 __global__ void __launch_bounds__(kBlock)
 flush_kernel(double* x64, DirectionSet<T> d, const double* alpha, int nrhs, int64_t len) {
   const int k = blockIdx.y;
@@ -99,10 +101,10 @@ flush_kernel(double* x64, DirectionSet<T> d, const double* alpha, int nrhs, int6
 #pragma unroll
   for (int t = 0; t < kMaxDirections; ++t) a[t] = t < d.count ? alpha[t * nrhs + k] : 0.0;
   for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock) {
-    double acc = x64[off + i];
+    double acc = kAssign ? 0.0 : x64[off + i];
 #pragma unroll
     for (int t = 0; t < kMaxDirections; ++t)
-      if (t < d.count) acc = fma(a[t], double(d.p[t][off + i]), acc);
+      if (t < d.count && a[t] != 0.0) acc = fma(a[t], double(d.p[t][off + i]), acc);
     x64[off + i] = acc;
   }
 }
@@ -124,7 +126,7 @@ __device__ __forceinline__ void load4(const double* p, double (&o)[4]) {
   o[3] = v1.y;
 }
 
-template <typename T>
+template <typename T, bool kAssign>
 __global__ void __launch_bounds__(kBlock)
 flush4_kernel(double* x64, DirectionSet<T> d, const double* alpha, int nrhs, int64_t len) {
   const int k = blockIdx.y;
@@ -135,10 +137,14 @@ flush4_kernel(double* x64, DirectionSet<T> d, const double* alpha, int nrhs, int
   const int64_t len4 = len / 4;
   for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len4; i += int64_t(gridDim.x) * kBlock) {
     double2* xp = reinterpret_cast<double2*>(x64 + off + 4 * i);
-    double2 x0 = xp[0], x1 = xp[1];
+    double2 x0 = {0.0, 0.0}, x1 = {0.0, 0.0};
+    if constexpr (!kAssign) {
+      x0 = xp[0];
+      x1 = xp[1];
+    }
 #pragma unroll
     for (int t = 0; t < kMaxDirections; ++t)
-      if (t < d.count) {
+      if (t < d.count && a[t] != 0.0) {
         double v[4];
         load4(d.p[t] + off + 4 * i, v);
         x0.x = fma(a[t], v[0], x0.x);
@@ -409,18 +415,27 @@ template int launch_update_r<double>(double*, const double*, int64_t, int, const
                                      hipStream_t);
 
 template <typename T>
-int launch_flush(double* x64, const DirectionSet<T>& d, const double* alpha, int nrhs, int64_t len, hipStream_t s) {
+int launch_flush(double* x64, const DirectionSet<T>& d, const double* alpha, int nrhs, int64_t len, bool assign,
+                 hipStream_t s) {
   if (d.count <= 0) return VXL_OK;
   if (d.count > kMaxDirections) return fail("too many stored search directions");
-  if (len % 4 == 0)
-    hipLaunchKernelGGL((flush4_kernel<T>), vgrid(len / 4, nrhs), dim3(kBlock), 0, s, x64, d, alpha, nrhs, len);
+  const bool v4 = len % 4 == 0;
+  const dim3 g = vgrid(v4 ? len / 4 : len, nrhs);
+  if (v4 && assign)
+    hipLaunchKernelGGL((flush4_kernel<T, true>), g, dim3(kBlock), 0, s, x64, d, alpha, nrhs, len);
+  else if (v4)
+    hipLaunchKernelGGL((flush4_kernel<T, false>), g, dim3(kBlock), 0, s, x64, d, alpha, nrhs, len);
+  else if (assign)
+    hipLaunchKernelGGL((flush_kernel<T, true>), g, dim3(kBlock), 0, s, x64, d, alpha, nrhs, len);
   else
-    hipLaunchKernelGGL((flush_kernel<T>), vgrid(len, nrhs), dim3(kBlock), 0, s, x64, d, alpha, nrhs, len);
+    hipLaunchKernelGGL((flush_kernel<T, false>), g, dim3(kBlock), 0, s, x64, d, alpha, nrhs, len);
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }
-template int launch_flush<float>(double*, const DirectionSet<float>&, const double*, int, int64_t, hipStream_t);
-template int launch_flush<double>(double*, const DirectionSet<double>&, const double*, int, int64_t, hipStream_t);
+template int launch_flush<float>(double*, const DirectionSet<float>&, const double*, int, int64_t, bool,
+                                 hipStream_t);
+template int launch_flush<double>(double*, const DirectionSet<double>&, const double*, int, int64_t, bool,
+                                  hipStream_t);
 
 template <typename T>
 int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, const double* alpha,

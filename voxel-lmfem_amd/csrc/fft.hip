@@ -1194,15 +1194,19 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   if (im->lds_x > kMaxLds || im->lds_y > kMaxLds || im->lds_z > kMaxLds)
     return fail("grid axis too long for the LDS pencil FFT");
   {
-    const int cols = sizeof(T) == 4 ? 16 : 8;  // >= 128 B row segments
+    const int cols = sizeof(T) == 4 ? 16 : 8;  // >= 128 B row segments (z pass)
+    // y pass: 256 B row segments (measured 2.5 % faster per apply than
+    // 128 B at 256^3, FP32)
+    const int ycols = sizeof(T) == 4 ? 32 : 16;
     const int ly = four_step_lanes(im->ny), lz = four_step_lanes(im->nz);
     const char* env = std::getenv("VXL_FFT_FOURSTEP");
     const bool enable = !(env && env[0] == '0');
     im->y4 = (enable && ly) ? im->ny : 0;
     im->z4 = (enable && lz) ? im->nz : 0;
     if (im->y4) {
-      im->tj4 = std::max(1, std::min({cols, 256 / ly, 3 * im->nxh}));
+      im->tj4 = std::max(1, std::min({ycols, kThreads4 / ly, 3 * im->nxh}));
       if (const int v = env_int("VXL_FFT_YCOLS")) im->tj4 = std::min({v, kThreads4 / ly, 3 * im->nxh});
+      while (im->tj4 > 1 && sizeof(C) * size_t(im->ny) * im->tj4 > kMaxLds) --im->tj4;
       im->thr_y4 = im->tj4 * ly;
       im->lds_y4 = sizeof(C) * size_t(im->ny) * im->tj4;
       if (im->lds_y4 > kMaxLds) im->y4 = 0;

@@ -297,8 +297,6 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     rr_ref[k] = ff[k];
   }
   VXL_TRY(upload_flags());
-  VXL_TRY(launch_fill_zero<double>(s->x64, len, R, nullptr, st));
-  if (mixed) VXL_TRY(launch_fill_zero<float>(reinterpret_cast<float*>(x), len, R, act, st));
   // p = z = M r with rho = r.z: beta = 0 because rho_old = 0 (slots cleared).
   Steps<T> steps(s, tol, st);
   const bool jacobi = steps.jacobi;
@@ -307,17 +305,28 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   const char* ring_env = std::getenv("VXL_DIRECTION_RING");
   const bool ring = s->nslot >= 2 && !graphs_enabled() && steps.nblk_fx == 0 && !(ring_env && ring_env[0] == '0');
   steps.ring = ring;
+  // x = 0.  With the ring, x64 is first written by the first flush (an
+  // assignment) and the FP32 x is not used.
+  bool x64_set = !ring;
+  if (!ring) VXL_TRY(launch_fill_zero<double>(s->x64, len, R, nullptr, st));
+  if (mixed && !ring) VXL_TRY(launch_fill_zero<float>(reinterpret_cast<float*>(x), len, R, act, st));
   std::vector<T*> slots(size_t(std::max(1, s->nslot)));
   for (int i = 0; i < s->nslot; ++i) slots[i] = static_cast<T*>(s->pring[i]);
   if (s->nslot < 1) slots[0] = p;
   int cur = 0;
   std::vector<int> pend;  // slots whose alpha_t p_t are not in x64 yet (hist rows in order)
   auto flush = [&]() -> int {
-    if (!ring || pend.empty()) return VXL_OK;
+    if (!ring) return VXL_OK;
+    if (pend.empty()) {
+      if (!x64_set) VXL_TRY(launch_fill_zero<double>(s->x64, len, R, nullptr, st));
+      x64_set = true;
+      return VXL_OK;
+    }
     DirectionSet<T> d{};
     for (size_t t = 0; t < pend.size(); ++t) d.p[t] = slots[pend[t]];
     d.count = int(pend.size());
-    VXL_TRY(launch_flush<T>(s->x64, d, s->hist, R, len, st));
+    VXL_TRY(launch_flush<T>(s->x64, d, s->hist, R, len, !x64_set, st));
+    x64_set = true;
     pend.clear();
     return VXL_OK;
   };

@@ -48,9 +48,11 @@ int launch_row_totals(const double* partial, int rows, int nblk, double scale, d
 // partial[c * blocks + block]: per-block sums of the 21 stiffness
 // coefficients (times the per-voxel scale) over all elements.
 int launch_stiffness_sums(const vxl_cell* cell, double* partial, int blocks, hipStream_t s);
-// x64 += sum_t alpha[t * nrhs + k] p_t for every system k.
+// x64 += sum_t alpha[t * nrhs + k] p_t for every system k (assign: x64 =
+// the sum, its old contents ignored); terms with alpha = 0 are skipped.
 template <typename T>
-int launch_flush(double* x64, const DirectionSet<T>& d, const double* alpha, int nrhs, int64_t len, hipStream_t s);
+int launch_flush(double* x64, const DirectionSet<T>& d, const double* alpha, int nrhs, int64_t len, bool assign,
+                 hipStream_t s);
 template <typename T>
 int launch_update_xr(T* x, T* r, const T* p, const T* q, int64_t len, int nrhs, const double* alpha,
                      double* partial, const int* active, hipStream_t s);
