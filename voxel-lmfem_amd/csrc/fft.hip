@@ -436,30 +436,41 @@ fft_y(typename CT<T>::type* __restrict__ S, int ny, int nz, int W, int TJ, Axes 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 
+// Per-axis factors of the Green symbol at one wavenumber: table columns 0-3
+// (1/m, D, v, d) for the FI element, columns 4-6 (S, c, s) in a, b, c for
+// C3D8R-HG.
+template <typename T>
+struct AxisF {
+  T a, b, c, d;
+};
+template <typename T>
+__device__ __forceinline__ AxisF<T> axis_factors(const T* g, int n, int k, bool hg) {
+  if (hg) return {g[4 * n + k], g[5 * n + k], g[6 * n + k], T(0)};
+  return {g[k], g[n + k], g[2 * n + k], g[3 * n + k]};
+}
+
 // Green solve at one wavevector (kx, ky, kz), in place:
 //   out = 1/(t N) [q/E - gamma (v/E) (v . q/E)],  E_i = mu sum D + (lam+mu) d_i,
 //   gamma = (lam+mu)/(1 + (lam+mu) sum v_i^2/E_i),  t N = V N m_x m_y m_z
 // (the axis tables hold 1/m, D, v, d);
 // k = 0 (sum D = 0) maps to 0.  Returns Re(conj(q) . out) in FP64.
 template <typename T, typename C>
-__device__ __forceinline__ double green_point(C& q0, C& q1, C& q2, const Tables<T>& tb, int kx, int nxh, T my,
-                                              T Dy, T vy, T dy, int kz, int nz, T lam, T mu, T inv_vn) {
-  const T mx = tb.gx[kx], Dx = tb.gx[nxh + kx], vx = tb.gx[2 * nxh + kx], dx = tb.gx[3 * nxh + kx];
-  const T mz = tb.gz[kz], Dz = tb.gz[nz + kz], vz = tb.gz[2 * nz + kz], dz = tb.gz[3 * nz + kz];
-  const T sumD = Dx + Dy + Dz;
+__device__ __forceinline__ double green_point(C& q0, C& q1, C& q2, const AxisF<T>& X, const AxisF<T>& Y,
+                                              const AxisF<T>& Z, T lam, T mu, T inv_vn) {
+  const T sumD = X.b + Y.b + Z.b;
   if (!(sumD > T(0))) {
     q0 = q1 = q2 = C{T(0), T(0)};
     return 0.0;
   }
   const T lm = lam + mu;
-  const T i0 = rcp(mu * sumD + lm * dx);
-  const T i1 = rcp(mu * sumD + lm * dy);
-  const T i2 = rcp(mu * sumD + lm * dz);
-  const T gamma = lm * rcp(T(1) + lm * (vx * vx * i0 + vy * vy * i1 + vz * vz * i2));
-  const T scale = inv_vn * mx * my * mz;  // tables hold 1/m
+  const T i0 = rcp(mu * sumD + lm * X.d);
+  const T i1 = rcp(mu * sumD + lm * Y.d);
+  const T i2 = rcp(mu * sumD + lm * Z.d);
+  const T gamma = lm * rcp(T(1) + lm * (X.c * X.c * i0 + Y.c * Y.c * i1 + Z.c * Z.c * i2));
+  const T scale = inv_vn * X.a * Y.a * Z.a;  // tables hold 1/m
   const C a0 = {q0.x * i0, q0.y * i0}, a1 = {q1.x * i1, q1.y * i1}, a2 = {q2.x * i2, q2.y * i2};
-  const C sdot = {vx * a0.x + vy * a1.x + vz * a2.x, vx * a0.y + vy * a1.y + vz * a2.y};
-  const T w0 = gamma * vx * i0, w1 = gamma * vy * i1, w2 = gamma * vz * i2;
+  const C sdot = {X.c * a0.x + Y.c * a1.x + Z.c * a2.x, X.c * a0.y + Y.c * a1.y + Z.c * a2.y};
+  const T w0 = gamma * X.c * i0, w1 = gamma * Y.c * i1, w2 = gamma * Z.c * i2;
   const C o0 = {scale * (a0.x - w0 * sdot.x), scale * (a0.y - w0 * sdot.y)};
   const C o1 = {scale * (a1.x - w1 * sdot.x), scale * (a1.y - w1 * sdot.y)};
   const C o2 = {scale * (a2.x - w2 * sdot.x), scale * (a2.y - w2 * sdot.y)};
@@ -479,18 +490,19 @@ __device__ __forceinline__ double green_point(C& q0, C& q1, C& q2, const Tables<
 // 8 prod_{d in m} s_d^2 prod_{d not in m} c_d^2, so per unit volume
 //   K(k) = (mu |w|^2 + hg0 H) I + (lam + mu) w w^T,
 //   H = 8 (sx^2 sy^2 cz^2 + sy^2 sz^2 cx^2 + sz^2 sx^2 cy^2 + sx^2 sy^2 sz^2),
-// inverted by Sherman-Morrison; k = 0 (a = 0) is removed.
+// inverted by Sherman-Morrison; k = 0 (a = 0) is removed.  Axis factors
+// (a, b, c) = (S = (2/h) s, c, s).
 template <typename T, typename C>
-__device__ __forceinline__ double green_point_hg(C& q0, C& q1, C& q2, const Tables<T>& tb, int kx, int nxh, int y,
-                                                 int ny, int kz, int nz, T lam, T mu, T inv_vn) {
-  const T Sx = tb.gx[4 * nxh + kx], cx = tb.gx[5 * nxh + kx], sx = tb.gx[6 * nxh + kx];
-  const T Sy = tb.gy[4 * ny + y], cy = tb.gy[5 * ny + y], sy = tb.gy[6 * ny + y];
-  const T Sz = tb.gz[4 * nz + kz], cz = tb.gz[5 * nz + kz], sz = tb.gz[6 * nz + kz];
+__device__ __forceinline__ double green_point_hg(C& q0, C& q1, C& q2, const AxisF<T>& X, const AxisF<T>& Y,
+                                                 const AxisF<T>& Z, T hg0, T lam, T mu, T inv_vn) {
+  const T Sx = X.a, cx = X.b, sx = X.c;
+  const T Sy = Y.a, cy = Y.b, sy = Y.c;
+  const T Sz = Z.a, cz = Z.b, sz = Z.c;
   const T wx = Sx * cy * cz, wy = cx * Sy * cz, wz = cx * cy * Sz;
   const T ww = wx * wx + wy * wy + wz * wz;
   const T sx2 = sx * sx, sy2 = sy * sy, sz2 = sz * sz, cx2 = cx * cx, cy2 = cy * cy, cz2 = cz * cz;
   const T H = T(8) * (sx2 * sy2 * cz2 + sy2 * sz2 * cx2 + sz2 * sx2 * cy2 + sx2 * sy2 * sz2);
-  const T a = mu * ww + tb.hg0 * H;
+  const T a = mu * ww + hg0 * H;
   if (!(a > T(0))) {
     q0 = q1 = q2 = C{T(0), T(0)};
     return 0.0;
@@ -513,11 +525,10 @@ __device__ __forceinline__ double green_point_hg(C& q0, C& q1, C& q2, const Tabl
 
 // Green solve of the cell's element at one wavevector.
 template <typename T, typename C>
-__device__ __forceinline__ double green_solve(C& q0, C& q1, C& q2, const Tables<T>& tb, int kx, int nxh, int y,
-                                              int ny, T my, T Dy, T vy, T dy, int kz, int nz, T lam, T mu,
-                                              T inv_vn) {
-  if (tb.hg0 > T(0)) return green_point_hg<T>(q0, q1, q2, tb, kx, nxh, y, ny, kz, nz, lam, mu, inv_vn);
-  return green_point<T>(q0, q1, q2, tb, kx, nxh, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
+__device__ __forceinline__ double green_solve(C& q0, C& q1, C& q2, const Tables<T>& tb, const AxisF<T>& X,
+                                              const AxisF<T>& Y, const AxisF<T>& Z, T lam, T mu, T inv_vn) {
+  if (tb.hg0 > T(0)) return green_point_hg<T>(q0, q1, q2, X, Y, Z, tb.hg0, lam, mu, inv_vn);
+  return green_point<T>(q0, q1, q2, X, Y, Z, lam, mu, inv_vn);
 }
 
 // ---- pass 3: z forward, Green solve, z inverse ---------------------------------
@@ -552,14 +563,16 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
   C* F = lds_fft<false, T>(A, B, TJ, ax.pz, tb.wz);
   C* G = F == A ? B : A;
   // Green solve per (kz, ky, kx), in place in F.
-  const T my = tb.gy[y], Dy = tb.gy[ny + y], vy = tb.gy[2 * ny + y], dy = tb.gy[3 * ny + y];
+  const bool hg = tb.hg0 > T(0);
+  const AxisF<T> ay = axis_factors(tb.gy, ny, y, hg);
   double dot = 0.0;
   for (int idx = threadIdx.x; idx < nz * TK; idx += blockDim.x) {
     const int kz = idx / TK, kk = idx % TK;
     if (kk >= tk) continue;
     const int kx = k0 + kk;
     const double d = green_solve<T>(F[kz * TJ + 3 * kk + 0], F[kz * TJ + 3 * kk + 1], F[kz * TJ + 3 * kk + 2],
-                                    tb, kx, nxh, y, ny, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
+                                    tb, axis_factors(tb.gx, nxh, kx, hg), ay, axis_factors(tb.gz, nz, kz, hg),
+                                    lam, mu, inv_vn);
     if constexpr (kDot) dot += (kx == 0 || kx == nyq) ? d : 2.0 * d;
   }
   if constexpr (kDot) {
@@ -703,6 +716,12 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
   const int p = threadIdx.x % TJ, t = threadIdx.x / TJ;
   const bool live = p < 3 * tk;
   const C* w = tb.wz;
+  // With the block size a multiple of nz, every thread keeps one kz in the
+  // Green pass; its z factors are loaded here, behind the transform's loads.
+  const bool hg = tb.hg0 > T(0);
+  const bool kz_fixed = blockDim.x % nz == 0;
+  AxisF<T> az_own{};
+  if (kz_fixed) az_own = axis_factors(tb.gz, nz, int(threadIdx.x % nz), hg);
   // forward z transform: global -> L (natural kz order)
   if (t < N2)
     four_step_1<false, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C {
@@ -721,15 +740,15 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
     for (int j2 = 0; j2 < N2; ++j2) L[(t + N1 * j2) * TJ + p] = u[j2];
   }
   __syncthreads();
-  // Green solve per (kz, kx) of this tile.
-  const T my = tb.gy[y], Dy = tb.gy[ny + y], vy = tb.gy[2 * ny + y], dy = tb.gy[3 * ny + y];
+  // Green solve per (kz, kx) of this tile, kz fastest over the threads.
+  const AxisF<T> ay = axis_factors(tb.gy, ny, y, hg);
   double dot = 0.0;
-  for (int idx = threadIdx.x; idx < nz * TK; idx += blockDim.x) {
-    const int kz = idx / TK, kk = idx % TK;
-    if (kk >= tk) continue;
+  for (int idx = threadIdx.x; idx < nz * tk; idx += blockDim.x) {
+    const int kz = idx % nz, kk = idx / nz;
     const int kx = k0 + kk;
+    const AxisF<T> az = kz_fixed ? az_own : axis_factors(tb.gz, nz, kz, hg);
     const double d = green_solve<T>(L[kz * TJ + 3 * kk + 0], L[kz * TJ + 3 * kk + 1], L[kz * TJ + 3 * kk + 2],
-                                    tb, kx, nxh, y, ny, my, Dy, vy, dy, kz, nz, lam, mu, inv_vn);
+                                    tb, axis_factors(tb.gx, nxh, kx, hg), ay, az, lam, mu, inv_vn);
     if constexpr (kDot) dot += (kx == 0 || kx == nyq) ? d : 2.0 * d;
   }
   if constexpr (kDot) {
