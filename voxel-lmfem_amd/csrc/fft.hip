@@ -722,6 +722,11 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
   const bool kz_fixed = blockDim.x % nz == 0;
   AxisF<T> az_own{};
   if (kz_fixed) az_own = axis_factors(tb.gz, nz, int(threadIdx.x % nz), hg);
+  // The tile's x factors go to LDS behind the transform data (read after
+  // the transform's barriers); the y factors are uniform.
+  AxisF<T>* xf = reinterpret_cast<AxisF<T>*>(L + nz * TJ);
+  if (int(threadIdx.x) < tk) xf[threadIdx.x] = axis_factors(tb.gx, nxh, k0 + int(threadIdx.x), hg);
+  const AxisF<T> ay = axis_factors(tb.gy, ny, y, hg);
   // forward z transform: global -> L (natural kz order)
   if (t < N2)
     four_step_1<false, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C {
@@ -741,14 +746,14 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
   }
   __syncthreads();
   // Green solve per (kz, kx) of this tile, kz fastest over the threads.
-  const AxisF<T> ay = axis_factors(tb.gy, ny, y, hg);
   double dot = 0.0;
+#pragma unroll 2
   for (int idx = threadIdx.x; idx < nz * tk; idx += blockDim.x) {
     const int kz = idx % nz, kk = idx / nz;
     const int kx = k0 + kk;
     const AxisF<T> az = kz_fixed ? az_own : axis_factors(tb.gz, nz, kz, hg);
     const double d = green_solve<T>(L[kz * TJ + 3 * kk + 0], L[kz * TJ + 3 * kk + 1], L[kz * TJ + 3 * kk + 2],
-                                    tb, axis_factors(tb.gx, nxh, kx, hg), ay, az, lam, mu, inv_vn);
+                                    tb, xf[kk], ay, az, lam, mu, inv_vn);
     if constexpr (kDot) dot += (kx == 0 || kx == nyq) ? d : 2.0 * d;
   }
   if constexpr (kDot) {
@@ -1234,7 +1239,7 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
       im->tk4 = std::max(1, std::min({(cols + 2) / 3, 256 / (3 * lz), im->nxh}));
       if (const int v = env_int("VXL_FFT_ZCOLS")) im->tk4 = std::min({v, kThreads4 / (3 * lz), im->nxh});
       im->thr_z4 = (3 * im->tk4 * lz + 63) / 64 * 64;  // whole waves for the block reduction
-      im->lds_z4 = sizeof(C) * size_t(im->nz) * 3 * im->tk4;
+      im->lds_z4 = sizeof(C) * size_t(im->nz) * 3 * im->tk4 + 4 * sizeof(T) * size_t(im->tk4);  // + x factors
       if (im->lds_z4 > kMaxLds || im->thr_z4 > kThreads4 || im->thr_z4 < 1) im->z4 = 0;
     }
   }
