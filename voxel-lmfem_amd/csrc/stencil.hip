@@ -514,13 +514,15 @@ apply_kernel_v2(ApplyArgs<T> a) {
   const int xl = tid % TX, yl = tid / TX;
   const bool node_valid = is_node && (x0 + xl) < nx && (y0 + yl) < ny;
   const int n_xy = (y0 + yl) * nx + (x0 + xl);
-  // kFuse: u . (K u) = sum over elements of u_e . f_e, and in face-mode form
-  // u_e . f_e = Tf . Ft + Bf . Fb (the face and z transforms are applied with
-  // their transposes to the forces).  Each element is counted by the tile and
-  // z chunk that own it: local columns 1..TX, rows 1..TY, inside the grid.
-  const bool own_elem = is_elem && exl >= 1 && eyl >= 1 && x0 - 1 + exl < nx && y0 - 1 + eyl < ny;
+  // kFuse: u . (K u) as the sum over the nodes this thread writes of
+  // u_node . (K u)_node; u at the node of plane z is read from LDS while that
+  // plane is resident (one layer earlier) and kept in registers.
   double rr = 0.0;  // kResid: sum of r^2 over the nodes this thread writes
-  T pq = T(0);  // at most zchunk elements per thread; FP64 from the block sum on
+  T pq = T(0);  // at most zchunk nodes per thread; FP64 from the block sum on
+  auto node_value = [&](T (&v)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = su[c][(yl + 1) * TL::PX + xl + 1];
+  };
 
   // Next-plane prefetch: raw values of the plane after next are loaded into
   // registers before the element computation and stored to LDS after it, so
@@ -552,6 +554,10 @@ apply_kernel_v2(ApplyArgs<T> a) {
   __syncthreads();
   load_plane(z0);
   __syncthreads();
+  T pnA[3] = {T(0), T(0), T(0)}, pnB[3] = {T(0), T(0), T(0)};  // kFuse: u at this thread's node
+  if constexpr (kFuse) {
+    if (node_valid) node_value(pnA);
+  }
   if (is_elem) {
     T Tf[3][4], Fb[3][4];
     face_modes(Tf);
@@ -574,7 +580,8 @@ apply_kernel_v2(ApplyArgs<T> a) {
   // below put on that face; Bn/Fn receive the top face modes and top force,
   // which are the next layer's Bc/Fp.  The loop alternates two register sets,
   // so no copies are needed between layers.
-  auto layer = [&](int z, const T (&Bc)[3][4], const T (&Fp)[3][4], T (&Bn)[3][4], T (&Fn)[3][4]) {
+  auto layer = [&](int z, const T (&Bc)[3][4], const T (&Fp)[3][4], T (&Bn)[3][4], T (&Fn)[3][4],
+                   const T (&pc)[3], T (&pn)[3]) {
     int row_cur = row_next;
     if constexpr (kFuse) {
       if (is_elem) row_cur = row_of(z);
@@ -598,24 +605,22 @@ apply_kernel_v2(ApplyArgs<T> a) {
     }
     // LDS holds plane z+1: read everything this step needs from it first.
     if (is_elem) face_modes(Bn);
+    if constexpr (kFuse) {
+      if (node_valid && z + 1 < z1) node_value(pn);
+    }
     if (more) prefetch(z + 2);
     if (is_elem) {
       T Fb[3][4];
       element(z, row_cur, Bc, Bn, Fb, Fn);
-      T w = T(0);  // kFuse: u_e . f_e of this element
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
         T F[4];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          if constexpr (kFuse) w = fma(Bn[cc][m], Fn[cc][m], fma(Bc[cc][m], Fb[cc][m], w));
-          F[m] = Fp[cc][m] + Fb[cc][m];
-        }
+        for (int m = 0; m < 4; ++m) F[m] = Fp[cc][m] + Fb[cc][m];
         face_transpose(F);
 #pragma unroll
         for (int a2 = 0; a2 < 4; ++a2) sc[a2 * 3 + cc][tid] = F[a2];
       }
-      if constexpr (kFuse) pq += own_elem ? w : T(0);
     }
     __syncthreads();
     if (more) commit();
@@ -645,14 +650,15 @@ apply_kernel_v2(ApplyArgs<T> a) {
       } else {
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) out[cc * n + node] = acc[cc];
+        if constexpr (kFuse) pq = fma(pc[2], acc[2], fma(pc[1], acc[1], fma(pc[0], acc[0], pq)));
       }
     }
     if constexpr (kPF) __syncthreads();
   };
   T Bf2[3][4], Ftp2[3][4];
   for (int z = z0; z < z1; z += 2) {
-    layer(z, Bf, Ftp, Bf2, Ftp2);
-    if (z + 1 < z1) layer(z + 1, Bf2, Ftp2, Bf, Ftp);
+    layer(z, Bf, Ftp, Bf2, Ftp2, pnA, pnB);
+    if (z + 1 < z1) layer(z + 1, Bf2, Ftp2, Bf, Ftp, pnB, pnA);
   }
   if constexpr (kFuse || kResid) {
     const double total = block_reduce_sum<TL::NT>(kResid ? rr : double(pq), red);
