@@ -201,3 +201,27 @@ def test_zero_load_among_others(torch_gpu, mode):
         assert max(out["true_residuals"]) <= 1e-8
     solver.close()
     cell.close()
+
+
+@pytest.mark.parametrize("ring", ["1", "0"])
+def test_fused_residual_update_matches_separate_pass(torch_gpu, monkeypatch, ring):
+    """r -= alpha q folded into the preconditioner's x pass (with the direction
+    ring, or with x += alpha p there too) against the separate update pass:
+    same iterations to one, C_eff to rounding."""
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import voronoi_polycrystal
+    (phase, mats, dom, _), _ = voronoi_polycrystal(n=32, grains=12, seed=5)
+    print(f"case fused update ring={ring} 32^3", flush=True)
+    monkeypatch.setenv("VXL_DIRECTION_RING", ring)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("VXL_FUSED_UPDATE", flag)
+        out[flag] = homogenize(phase, mats, dom, precision="mixed")
+    a, b = out["0"], out["1"]
+    err = rel(b["CH"], a["CH"])
+    its_a = [c["iterations"] for c in a["calls"]]
+    its_b = [c["iterations"] for c in b["calls"]]
+    print(f"  rel {err:.2e} iterations {its_a} vs {its_b}", flush=True)
+    assert err < 1e-9
+    assert max(abs(x - y) for x, y in zip(its_a, its_b)) <= 1
+    assert b["max_true_residual"] <= 1e-8

@@ -118,7 +118,11 @@ struct Steps {
     if (jacobi)
       dinv = std::is_same<T, double>::value ? reinterpret_cast<const T*>(s->dinv64)
                                             : reinterpret_cast<const T*>(s->dinv32);
-    nblk_fx = jacobi ? 0 : precond_fused_update_blocks(&s->pc);
+    // The fused x pass is not captured in graphs, so graph replay keeps the
+    // separate update pass.
+    nblk_fx = (jacobi || !s->partial_fx || graphs_enabled() || !fused_update_enabled())
+                  ? 0
+                  : precond_fused_update_blocks(&s->pc);
     vxl_solver* so = s;
     const int nrz = nblk_rz;
     hipStream_t on = st;
@@ -138,7 +142,12 @@ struct Steps {
     VXL_TRY(launch_finalize(s->partial_st, nblk_st, kOpAlpha, scal, R, act, nullptr, st, 0, 0.0, nullptr,
                             ring ? hist_row : nullptr));
     forward_done = false;
-    if (ring) {
+    if (ring && nblk_fx > 0) {  // r -= alpha q inside the preconditioner's x pass; x from the ring
+      VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, q, scal + kAlpha * R,
+                                    s->partial_fx, act, st));
+      VXL_TRY(launch_finalize(s->partial_fx, nblk_fx, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
+      forward_done = true;
+    } else if (ring) {
       VXL_TRY(launch_update_r<T>(r, q, len, R, scal + kAlpha * R, s->partial, act, st));
       VXL_TRY(launch_finalize(s->partial, nblk, kOpRR, scal, R, act, nullptr, st, iter, tol, s->stop_dev));
     } else if (nblk_fx > 0) {
@@ -303,7 +312,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   // Direction ring (unless graph replay or the fused forward pass is on, or
   // VXL_DIRECTION_RING=0 asks for the per-iteration x update).
   const char* ring_env = std::getenv("VXL_DIRECTION_RING");
-  const bool ring = s->nslot >= 2 && !graphs_enabled() && steps.nblk_fx == 0 && !(ring_env && ring_env[0] == '0');
+  const bool ring = s->nslot >= 2 && !graphs_enabled() && !(ring_env && ring_env[0] == '0');
   steps.ring = ring;
   // x = 0.  With the ring, x64 is first written by the first flush (an
   // assignment) and the FP32 x is not used.

@@ -103,10 +103,10 @@ int precond_apply_cg(vxl_precond* pc, int nrhs, const T* r, T* p, double* rz_par
                      const std::function<int()>& mid, const int* active, hipStream_t s);
 int precond_rz_blocks(const vxl_precond* pc);
 // The two halves of precond_apply_cg, for a caller that acts between them.
-// Forward x pass of r; with x != nullptr it first applies r -= alpha q,
-// x += alpha p to every entry it loads and writes FP64 ||r||^2 partials
-// (rr_partial[rhs * precond_fused_update_blocks + b]); that form needs the
-// four-step x pass (precond_fused_update_blocks > 0).
+// Forward x pass of r; with q != nullptr it first applies r -= alpha q (and
+// x += alpha p when x != nullptr) to every entry it loads and writes FP64
+// ||r||^2 partials (rr_partial[rhs * precond_fused_update_blocks + b]); that
+// form needs the four-step x pass (precond_fused_update_blocks > 0).
 template <typename T>
 int precond_cg_forward(vxl_precond* pc, int nrhs, T* r, T* x, const T* p, const T* q, const double* alpha,
                        double* rr_partial, const int* active, hipStream_t s);
@@ -117,6 +117,9 @@ int precond_cg_rest(vxl_precond* pc, int nrhs, T* p, double* rz_partial, const d
                     const std::function<int()>& mid, const int* active, hipStream_t s, const T* p_old = nullptr);
 // Workgroups per RHS of the fused forward pass, 0 if it is not available.
 int precond_fused_update_blocks(const vxl_precond* pc);
+// Whether the CG r update runs inside the x pass (VXL_FUSED_UPDATE=0 turns
+// it off); a solver also needs its partial_fx buffer and no graph replay.
+bool fused_update_enabled();
 // z = M r for the systems with active[k] != 0 only.
 int precond_apply_masked(vxl_precond* pc, int nrhs, const void* r, void* z, const int* active,
                          hipStream_t s);
