@@ -843,17 +843,6 @@ fft_x4_r2c(const T* __restrict__ r, SC* __restrict__ S, int64_t rows, int PP,
   T* __restrict__ ux = kUpd && upd.x ? upd.x + base : nullptr;
   const T* __restrict__ up = kUpd && upd.x ? upd.p + base : nullptr;
   const T* __restrict__ uq = kUpd ? upd.q + base : nullptr;
-  auto fetch = [&](int64_t i) -> T {
-    if constexpr (kUpd) {
-      const T v = fma(-alpha, uq[i], ur[i]);
-      ur[i] = v;
-      if (ux) ux[i] = fma(alpha, up[i], ux[i]);
-      rr += double(v) * double(v);
-      return v;
-    } else {
-      return f[i];
-    }
-  };
   if (live && t < N2) {
     C v[N1];
     if constexpr (kUpd) {
@@ -885,7 +874,7 @@ fft_x4_r2c(const T* __restrict__ r, SC* __restrict__ S, int64_t rows, int PP,
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) {
         const int e = t + N2 * k1;
-        v[k1] = C{re_ok ? fetch(re_off + e) : T(0), im_ok ? fetch(im_off + e) : T(0)};
+        v[k1] = C{re_ok ? f[re_off + e] : T(0), im_ok ? f[im_off + e] : T(0)};
       }
     }
     dft_reg<N1, false, T>(v, w, N2);
