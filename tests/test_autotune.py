@@ -28,6 +28,9 @@ class FakeCell:
         self.config = (14, 2, 8)
         self.tunes = 0
 
+    def stencil_config(self, precision="float32"):
+        return self.config
+
     def set_stencil_config(self, precision, tile_y, variant, zmin):
         if tile_y not in (14, 8, 6):
             raise ValueError("unsupported tile")
@@ -47,6 +50,9 @@ def test_tune_records_and_reuses(tmp_path):
     assert first["tuned"] and cell.tunes == 1
     assert (first["tile_y"], first["schedule"], first["zmin"]) == (6, 3, 2)
     assert first["ms"] == 1.0
+    # default_ms is the untuned configuration's entry: tile 14 (axis 0 index 0),
+    # schedule 2, 8 planes (axis 2 index 0) -> table[0, 1, 0] of the reversed ramp
+    assert first["default_ms"] == 27.0 - (0 * 9 + 1 * 3 + 0)
     data = json.loads(path.read_text())
     assert data["schema"] == autotune.SCHEMA
     assert list(data["entries"]) == ["fake-gpu|fp32|32x32x32|r6|u8|plain"]
@@ -79,3 +85,15 @@ def test_shared_entries_are_not_written(tmp_path, monkeypatch):
     assert len(entries) == 1 and not (tmp_path / "env.json").exists()
     autotune.save(entries)
     assert (tmp_path / "env.json").is_file()
+
+
+def test_default_ms_follows_the_cells_configuration(tmp_path):
+    """The recorded default time is the entry of whatever configuration the
+    cell ran before tuning (schedule 3 by default in both precisions)."""
+    table = np.arange(27.0).reshape(3, 3, 3)
+    for precision, config, expect in (("float32", (14, 3, 8), table[0, 2, 0]),
+                                      ("float64", (6, 3, 8), table[0, 2, 0]),
+                                      ("float64", (4, 1, 2), table[2, 0, 2]),
+                                      ("float32", (8, 2, 4), table[1, 1, 1])):
+        assert autotune._table_ms(table, precision, *config) == expect
+    assert np.isnan(autotune._table_ms(table, "float32", 16, 3, 8))

@@ -53,9 +53,16 @@ def _solve(rows, shape, h, precision, tol):
     field = _RowField(rows)
     ref = zeman(field, shape)
     cell = Cell(field, shape, h)
-    solver = Solver(cell, float(ref[0, 1]), float(ref[3, 3]), mode=precision, nrhs=6)
-    out = solver.run(np.eye(6), tol=tol, return_fields=True)
-    solver.close()
+    solver = None
+    try:
+        solver = Solver(cell, float(ref[0, 1]), float(ref[3, 3]), mode=precision, nrhs=6)
+        out = solver.run(np.eye(6), tol=tol, return_fields=True)
+    except BaseException:
+        cell.close()  # the caller only receives (and later closes) the cell of a converged solve
+        raise
+    finally:
+        if solver is not None:
+            solver.close()
     return cell, out
 
 

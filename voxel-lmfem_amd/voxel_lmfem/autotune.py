@@ -70,6 +70,19 @@ def _write(entries, path):
         raise
 
 
+TILES = {"fp32": (14, 8, 6), "fp64": (6, 8, 4)}   # table axis 0, in vxl_autotune_stencil's order
+DEPTHS = (8, 4, 2)                                # table axis 2
+
+
+def _table_ms(table, precision, tile_y, schedule, zmin):
+    """Milliseconds of one configuration in the 3 x 3 x 3 autotune table
+    (NaN if the configuration is not one of the tuned ones)."""
+    tiles = TILES["fp64" if precision in ("float64", "fp64") else "fp32"]
+    if tile_y not in tiles or zmin not in DEPTHS or not 1 <= schedule <= 3:
+        return float("nan")
+    return float(table[tiles.index(tile_y), schedule - 1, DEPTHS.index(zmin)])
+
+
 def key(cell, precision, nrhs):
     """Cache key: device, precision, grid (z, y, x), loads, index width, scale."""
     prec = "fp64" if precision in ("float64", "fp64") else "fp32"
@@ -98,9 +111,10 @@ def tune(cell, precision="float32", nrhs=6, *, path=None, reps=3, refresh=False,
             return dict(hit, tuned=False)
         except (KeyError, TypeError, ValueError):
             pass  # stale or foreign entry: tune again
+    before = cell.stencil_config(precision)   # the configuration the cell runs untuned
     (tile_y, schedule, zmin), table = cell.autotune(precision, nrhs=nrhs, reps=reps)
     entry = dict(tile_y=int(tile_y), schedule=int(schedule), zmin=int(zmin), ms=float(table.min()),
-                 default_ms=float(table[0, 1 if precision not in ("float64", "fp64") else 2, 0]))
+                 default_ms=_table_ms(table, precision, *before))
     entries[k] = entry
     if own:
         save(entries, path, strict=False)

@@ -1,4 +1,4 @@
-"""Command line: ``python -m voxel_lmfem solve ... [--backend gpu|cuda]`` and
+"""Command line: ``python -m voxel_lmfem solve ... [--backend cuda|gpu|hip]`` and
 ``python -m voxel_lmfem import-model ...``.
 
 Mirrors the reference's `solve` subcommand (src/voxel_lmfem/cli.py:166-171,
@@ -60,7 +60,7 @@ def main(argv=None):
     solve.add_argument("--input", type=Path, required=True)
     solve.add_argument("--materials", type=Path, required=True)
     solve.add_argument("--output", type=Path, required=True)
-    solve.add_argument("--backend", choices=("gpu", "hip", "cuda", "cpu"), default="gpu")
+    solve.add_argument("--backend", choices=("cuda", "gpu", "hip", "cpu"), default="cuda")
     solve.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
     solve.add_argument("--warmup", action="store_true")
     solve.add_argument("--fields", action="store_true", help="add element-centre field statistics (loads 0, 5)")

@@ -85,6 +85,10 @@ class Cell:
         self.B = 1
 
     def close(self):
+        pc = getattr(self, "_preconditioner", None)
+        if pc is not None:
+            pc.close()
+            self._preconditioner = None
         if getattr(self, "handle", None):
             self.lib.vxl_cell_destroy(self.handle)
             self.handle = None
@@ -109,6 +113,16 @@ class Cell:
         check(self.lib.vxl_apply(self.handle, precision, u.shape[0], dptr(u), dptr(out)), "vxl_apply")
         return out
 
+    def _timing_buffers(self, nrhs, dtype):
+        """Random input and output fields for the stencil timers, filled on the
+        cell's own stream: libvxl launches the timed kernels there, so they
+        are ordered after the fill whatever stream is current."""
+        torch = self.torch
+        with torch.cuda.stream(self.stream):
+            u = torch.randn(nrhs, 3, self.n, dtype=dtype, device=self.device)
+            out = torch.empty_like(u)
+        return u, out
+
     def tune(self, precision="float32", nrhs=6, reps=5):
         """Time every stencil schedule on this cell and keep the fastest.
 
@@ -116,8 +130,7 @@ class Cell:
         torch = self.torch
         dtype = torch.float32 if precision in ("float32", "fp32", "mixed") else torch.float64
         code = _native.VXL_F32 if dtype == torch.float32 else _native.VXL_F64
-        u = torch.randn(nrhs, 3, self.n, dtype=dtype, device=self.device)
-        out = torch.empty_like(u)
+        u, out = self._timing_buffers(nrhs, dtype)
         best = ctypes.c_int()
         ms = (ctypes.c_double * 3)()
         check(self.lib.vxl_tune_stencil(self.handle, code, nrhs, dptr(u), dptr(out), int(reps),
@@ -149,8 +162,7 @@ class Cell:
         the 3 x 3 x 3 table of milliseconds per launch."""
         torch = self.torch
         dtype = torch.float64 if precision in ("float64", "fp64") else torch.float32
-        u = torch.randn(nrhs, 3, self.n, dtype=dtype, device=self.device)
-        out = torch.empty_like(u)
+        u, out = self._timing_buffers(nrhs, dtype)
         t, v, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = (ctypes.c_double * 27)()
         check(self.lib.vxl_autotune_stencil(self.handle, self._code(precision), int(nrhs), dptr(u), dptr(out),
@@ -202,8 +214,14 @@ class Cell:
         return u
 
     # -- reference operator protocol on (1, 3N) node-major FP64 -----------------
+    def geometry(self):
+        """(Nz, Ny, Nx, dx, dy, dz), as HybridFIOperator.geometry (hybrid_adapter.py:70-72)."""
+        m = self.mesh
+        return (m.Nz, m.Ny, m.Nx, m.dx, m.dy, m.dz)
+
     def _to_planes(self, v):
-        if v.shape != (1, 3 * self.n) or v.dtype != self.torch.float64 or not v.is_contiguous():
+        if (v.shape != (1, 3 * self.n) or v.dtype != self.torch.float64 or v.device != self.device
+                or not v.is_contiguous()):
             raise ValueError("Expected contiguous FP64 displacement (1,3N) on the material device")
         return v.view(self.n, 3).t().contiguous().view(1, 3, self.n)
 
@@ -225,6 +243,45 @@ class Cell:
 
     def project_zero_mean(self, value):
         return self._from_planes(self.project_planes(self._to_planes(value)))
+
+    def _compute_reference_medium(self, strategy="zeman", *, verbose=False):
+        """(1, 6, 6) isotropic Zeman medium from the sampling lattice
+        (HybridFIOperator._compute_reference_medium, hybrid_adapter.py:101-125)."""
+        if strategy != "zeman":
+            raise ValueError("This controlled prototype uses the frozen zeman reference only")
+        from .reference_medium import zeman
+        return zeman(self.field, self.shape)[None]
+
+    def enable_preconditioner(self, lam=None, mu=None, *, precision="float32"):
+        """Attach the FFT Green preconditioner of the isotropic medium (lam, mu)
+        (default: the Zeman medium) and expose the reference's preconditioner
+        protocol on the operator: ``apply_precond`` (FP64 (1,3N) node-major in
+        and out) and ``precond_packed`` (the raw FP32 component-plane apply of
+        chain_adapter.attach_raw, chain_adapter.py:17-31).  Returns it."""
+        if lam is None or mu is None:
+            ref = self._compute_reference_medium()[0]
+            lam, mu = float(ref[0, 1]), float(ref[3, 3])
+        old = getattr(self, "_preconditioner", None)
+        if old is not None:
+            old.close()
+        self._preconditioner = Preconditioner(self, lam, mu, precision=precision, nrhs_max=1)
+        return self._preconditioner
+
+    def _attached(self):
+        pc = getattr(self, "_preconditioner", None)
+        if pc is None:
+            raise RuntimeError("no preconditioner attached; call enable_preconditioner first")
+        return pc
+
+    def apply_precond(self, residual):
+        """z = G0 r on (1, 3N) FP64 node-major (the operator's apply_precond)."""
+        return self._attached().apply_precond(residual)
+
+    def precond_packed(self, packed):
+        """Raw preconditioner apply on FP32 component planes (B, 3, Nz, Ny, Nx)
+        -> new FP32 tensor of the same shape (chain_adapter.py:17-31; the input
+        is not modified)."""
+        return self._attached().precond_packed(packed)
 
     def recover_center_fields(self, u, macro):
         """Reference protocol: element-centre (strain, stress), each (1, N, 6) FP64."""
@@ -265,6 +322,18 @@ class Preconditioner:
         """Reference protocol: (1,3N) FP64 node-major in and out."""
         planes = self.cell._to_planes(residual).to(self.dtype)
         return self.cell._from_planes(self.apply_planes(planes).to(self.cell.torch.float64))
+
+    def precond_packed(self, packed):
+        """(B, 3, Nz, Ny, Nx) component planes in the preconditioner's
+        precision -> new tensor of the same shape and type."""
+        shape = (3,) + self.cell.shape
+        if packed.dim() != 5 or tuple(packed.shape[1:]) != shape or packed.dtype != self.dtype:
+            raise ValueError(f"expected (B, {', '.join(map(str, shape))}) planes in {self.dtype}")
+        if packed.shape[0] > self.nrhs_max:
+            out = [self.precond_packed(packed[b:b + self.nrhs_max]) for b in range(0, packed.shape[0], self.nrhs_max)]
+            return self.cell.torch.cat(out)
+        flat = packed.reshape(packed.shape[0], 3, self.cell.n)
+        return self.apply_planes(flat).view_as(packed)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -336,7 +405,9 @@ class Solver:
         if fields is not None:
             out["fields"] = fields
         if error is not None:
-            raise _native.ConvergenceError(f"vxl_solver_run: {error.decode(errors='replace')}")
+            exc = _native.ConvergenceError(f"vxl_solver_run: {error.decode(errors='replace')}")
+            exc.partial = out   # the last iterate (fields, CH of it, residual history)
+            raise exc
         return out
 
     def trace(self, k):

@@ -6,9 +6,11 @@ src/voxel_lmfem/_core/runtime/benchmark.py:110-111.  The computation runs in
 libvxl (hand-written gfx950 HIP kernels): matrix-free FI HEX8 operator, FFT
 Green preconditioner and block multi-RHS PCG, certified in FP64.
 
-backend: "gpu" (also accepted: "hip", and "cuda" as the reference's spelling
-of the GPU backend -- no CUDA code is involved).  "cpu" names the reference's
-sparse-direct oracle, which this GPU build does not ship.
+backend: "cuda" (the reference's name of its GPU backend and the default;
+no CUDA code is involved), or "gpu" / "hip" for the same backend.  The result's
+``backend`` key repeats the caller's spelling, so a drop-in caller checking
+``result["backend"] == "cuda"`` sees what the reference returns.  "cpu" names
+the reference's sparse-direct oracle, which this GPU build does not ship.
 precision: "mixed" (FP32 operator/preconditioner/CG with FP64 reliable
 updates, the default) or "fp64" (everything FP64).
 element: "fi" (fully integrated HEX8, the reference's element; default) or
@@ -64,7 +66,7 @@ def _field_statistics(cell, u, phase, loads=(0, 5)):
 
 
 def _workload(phase, table, domain, orientation, precision, tol, local, fields=False, preconditioner="fft",
-              element="fi", hourglass_coefficient=0.1):
+              element="fi", hourglass_coefficient=0.1, max_iter=10000):
     from .gpu import Cell, Solver
     import torch
     torch.cuda.synchronize()
@@ -80,15 +82,23 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
     torch.cuda.reset_peak_memory_stats()
     field, h, lam, mu, meta = _prepare_device(phase, table, domain, orientation)
     stage("material")
-    cell = Cell(field, phase.shape, h, element=element, hourglass_coefficient=hourglass_coefficient)
-    solver = Solver(cell, lam, mu, mode=precision, nrhs=6, preconditioner=preconditioner)
-    stage("operator")
-    out = solver.run(np.eye(6), tol=tol, return_fields=fields)
-    stage("solve")
-    field_stats, slices = {}, {}
-    if fields:
-        field_stats, slices = _field_statistics(cell, out["fields"], phase)
-        stage("fields")
+    cell = solver = None
+    try:
+        cell = Cell(field, phase.shape, h, element=element, hourglass_coefficient=hourglass_coefficient)
+        solver = Solver(cell, lam, mu, mode=precision, nrhs=6, preconditioner=preconditioner)
+        stage("operator")
+        out = solver.run(np.eye(6), tol=tol, return_fields=fields, max_iter=max_iter)
+        stage("solve")
+        field_stats, slices = {}, {}
+        if fields:
+            field_stats, slices = _field_statistics(cell, out["fields"], phase)
+            stage("fields")
+        index_bytes = int(cell.index.numel() * cell.index.element_size())
+    finally:  # a failed certificate (ConvergenceError) must not hold the device buffers
+        if solver is not None:
+            solver.close()
+        if cell is not None:
+            cell.close()
     seconds = time.perf_counter() - start
     calls = [dict(iterations=int(out["iterations"][j]), restarts=int(out["restarts"][j]),
                   true_residuals=[float(out["true_residuals"][j])],
@@ -99,25 +109,26 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
                   peak_reserved_bytes=int(torch.cuda.max_memory_reserved()),
                   device_ms=out["device_ms"], graph_blocks=out["graph_blocks"])
     rows = int(getattr(field, "row_count", len(field.rows)))
-    storage = dict(mode="label_rows", n_elements=int(phase.size), pool_rows=rows,
-                   index_bytes=int(cell.index.numel() * cell.index.element_size()),
+    storage = dict(mode="label_rows", n_elements=int(phase.size), pool_rows=rows, index_bytes=index_bytes,
                    coefficient_bytes=rows * 21 * 8,
                    front_end="device" if getattr(field, "index_dev", None) is not None else "host")
     operator = dict(variant="modal_fi_lds_stencil" if element == "fi" else "modal_c3d8r_hg_lds_stencil",
                     element=element, precision=precision, preconditioner=preconditioner, reference_medium=meta)
     if element != "fi":
         operator["hourglass_coefficient"] = float(hourglass_coefficient)
-    solver.close()
-    cell.close()
     if fields:
         result["slices"] = slices
     return result, storage, operator
 
 
-def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="gpu", warmup=False,
+def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="cuda", warmup=False,
                precision="mixed", tol=TOL, fields=False, preconditioner="fft", element="fi",
-               hourglass_coefficient=0.1):
+               hourglass_coefficient=0.1, max_iter=10000):
     """Return effective stiffness in engineering-Voigt order 11,22,33,23,13,12.
+
+    max_iter caps the PCG iterations per load (the reference's 10,000,
+    benchmark.py:81-90); a load that has not certified by then raises
+    RuntimeError, as a residual above the tolerance does (solver.py:78-79).
 
     fields=True adds the reference workload's field diagnostics for loads 0
     and 5 (``field_statistics`` and mid-plane ``slices`` of the element-centre
@@ -134,6 +145,8 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="g
         raise ValueError("element must be 'fi' or 'c3d8r_hg'")
     if element == "c3d8r_hg" and not (np.isfinite(hourglass_coefficient) and hourglass_coefficient > 0):
         raise ValueError("hourglass_coefficient must be finite and positive")
+    if isinstance(max_iter, bool) or not isinstance(max_iter, (int, np.integer)) or max_iter < 1:
+        raise ValueError("max_iter must be a positive integer")
     phase, table, domain, orientation, labels, tensors, local = validate(
         phase_id, materials, domain_size, orientation)
     if backend == "cpu":
@@ -152,11 +165,11 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="g
         raise RuntimeError("No GPU is available. This backend runs only on an MI355X (gfx950).")
     if warmup:
         _workload(phase, table, domain, orientation, precision, tol, local, fields, preconditioner, element,
-                  hourglass_coefficient)
+                  hourglass_coefficient, int(max_iter))
     start = time.perf_counter()
     result, storage, operator = _workload(phase, table, domain, orientation, precision, tol, local, fields,
-                                          preconditioner, element, hourglass_coefficient)
-    result.update(backend="gpu", storage=storage, operator=operator,
+                                          preconditioner, element, hourglass_coefficient, int(max_iter))
+    result.update(backend=backend, storage=storage, operator=operator,
                   scope=(("FI HEX8" if element == "fi" else "C3D8R-HG") + " matrix-free stiffness; "
                          + {"fft": "FFT Green preconditioner; ", "fft_bf16": "FFT Green preconditioner (BF16 spectra); ",
                             "jacobi": "Jacobi preconditioner; "}[preconditioner]
