@@ -84,8 +84,10 @@ def test_spheres_32_mixed_vs_oracle(torch_gpu):
         assert out["max_true_residual"] <= 1e-8
 
 
-@pytest.mark.parametrize("grid", [64])
+@pytest.mark.parametrize("grid", [64, 128, 192, 256])
 def test_archived_woven_certify(torch_gpu, grid):
+    """The reference's certified woven C_eff (certify_woven_{grid}.json, FP64
+    operator, residual <= 1e-8) from the same analytic weave."""
     from voxel_lmfem import homogenize
     from voxel_lmfem.cells import reference_weave
     print(f"case woven{grid} certify", flush=True)
@@ -97,6 +99,7 @@ def test_archived_woven_certify(torch_gpu, grid):
     print(f"  rel err {err:.2e} its {[c['iterations'] for c in out['calls']]} "
           f"seconds {out['seconds']:.3f}", flush=True)
     assert err < 1e-7
+    assert out["max_true_residual"] <= 1e-8
 
 
 @pytest.mark.parametrize("precision,preconditioner", [("mixed", "fft"), ("fp64", "fft"), ("mixed", "jacobi")])
