@@ -35,7 +35,8 @@ def test_cli_solve_backend_cuda(torch_gpu, tmp_path, capsys):
     reference's GPU result keys, prints {output, backend, max_true_residual},
     and refuses to overwrite its output (exit status 2)."""
     from voxel_lmfem.cli import main
-    print("case cli solve --backend cuda 6x5x4", flush=True)
+    with capsys.disabled():  # stdout is the command's own output here
+        print("case cli solve --backend cuda 6x5x4", flush=True)
     phase, table, domain = two_phase(tmp_path)
     out = tmp_path / "result.json"
     argv = ["solve", "--input", str(tmp_path / "model.npz"), "--materials", str(tmp_path / "materials.json"),
