@@ -213,6 +213,16 @@ int vxl_energy_gradient(vxl_cell* cell, int nload, const double* u_dev, const do
 /* Subtract the per-component mean of nrhs FP64 fields in place.
  * Replaces HybridFIOperator.project_zero_mean  hybrid_adapter.py:78-80. */
 int vxl_project_zero_mean(vxl_cell* cell, int nrhs, double* u_dev);
+/* vxl_project_zero_mean without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * partial sums and means use a static per-GPU device array of 768 KiB (held
+ * until the process ends), so nrhs is at most 16 (VXL_ERR_ARG beyond).  The
+ * work is enqueued on the cell's stream and the call returns without waiting
+ * for it.  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (other streams, other threads) race and must be ordered by the caller.
+ * Same kernels and results as vxl_project_zero_mean, bit for bit. */
+int vxl_project_zero_mean_noalloc(vxl_cell* cell, int nrhs, double* u_dev);
 
 /* FFT Green preconditioner of the isotropic reference medium (lam, mu):
  * z = IFFT(K0(k)^-1 FFT(r)), k = 0 mode removed, hand-written LDS pencil FFTs.

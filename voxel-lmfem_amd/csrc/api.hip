@@ -5,6 +5,7 @@
 #include <cstring>
 #include <atomic>
 #include <algorithm>
+#include <mutex>
 #include <thread>
 
 namespace vxl {
@@ -166,6 +167,51 @@ pad_rows_kernel(const double* c21, int64_t rows, double* t64, float* t32, int* b
     t64[i] = v;
     t32[i] = float(v);
   }
+}
+
+}  // namespace
+}  // namespace vxl
+
+// ---- allocation-free zero-mean projection ------------------------------------
+// The partial sums and means of vxl_project_zero_mean_noalloc live in this
+// file-scope device array instead of an allocated workspace.  Each GPU the
+// library runs on holds its own copy (768 KiB) until the process ends; every
+// call on one GPU shares it, so calls that overlap on one GPU race.  The
+// partials and means are written before they are read in every call, so
+// nothing has to be cleared between calls.
+namespace vxl {
+constexpr int kNoAllocMaxRhs = 16;
+constexpr size_t kNoAllocScratchDoubles = size_t(3) * kNoAllocMaxRhs * (kVecBlocks + 1);
+}  // namespace vxl
+
+static __device__ __align__(256) double scratch[vxl::kNoAllocScratchDoubles];
+
+namespace vxl {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with that
+// device current (the host-side name is only a stand-in, never a device
+// address).
+int static_scratch(int device, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (device < 0 || device >= kMaxDevices) return fail("static scratch: device ordinal out of range");
+  if (!g_scratch_ptr[device]) {
+    int prev = -1;
+    VXL_HIP(hipGetDevice(&prev));
+    if (prev != device) VXL_HIP(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev != device) (void)hipSetDevice(prev);
+    if (e != hipSuccess || !ptr)
+      return fail(std::string("HIP error in hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return VXL_OK;
 }
 
 }  // namespace
@@ -505,6 +551,19 @@ int vxl_project_zero_mean(vxl_cell* cell, int nrhs, double* u_dev) {
   VXL_HIP(dev_malloc(&partial.p, sizeof(double) * 3 * nrhs * size_t(vec_blocks(n) + 1)));
   VXL_TRY(launch_project_zero_mean(u_dev, n, nrhs, static_cast<double*>(partial.p), nullptr, cell->stream));
   VXL_HIP(hipStreamSynchronize(cell->stream));
+  return VXL_OK;
+}
+
+int vxl_project_zero_mean_noalloc(vxl_cell* cell, int nrhs, double* u_dev) {
+  g_error.clear();
+  if (!cell || !u_dev || nrhs < 1) return fail("vxl_project_zero_mean_noalloc: invalid arguments");
+  if (nrhs > kNoAllocMaxRhs)
+    return fail("vxl_project_zero_mean_noalloc: at most 16 fields per call (the static scratch holds the "
+                "partial sums of 48 component rows)");
+  double* ptr = nullptr;
+  VXL_TRY(static_scratch(cell->device, &ptr));
+  // 3 * nrhs * (vec_blocks(n) + 1) doubles, within the array for nrhs <= 16.
+  VXL_TRY(launch_project_zero_mean(u_dev, cell->grid.n, nrhs, ptr, nullptr, cell->stream));
   return VXL_OK;
 }
 

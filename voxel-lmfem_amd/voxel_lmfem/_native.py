@@ -54,6 +54,7 @@ SIGNATURES = {
     "vxl_center_fields": (_I, [_P, _I, _P, _DP, _P, _P, _P]),
     "vxl_energy_gradient": (_I, [_P, _I, _P, _DP, _DP, _P, _P]),
     "vxl_project_zero_mean": (_I, [_P, _I, _P]),
+    "vxl_project_zero_mean_noalloc": (_I, [_P, _I, _P]),
     "vxl_precond_create": (_I, [_PP, _P, _I, _D, _D, _I]),
     "vxl_precond_apply": (_I, [_P, _I, _P, _P]),
     "vxl_precond_destroy": (_I, [_P]),
