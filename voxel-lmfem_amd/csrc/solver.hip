@@ -32,12 +32,64 @@ namespace {
 constexpr double kReliable = 1e-3;  // residual drop between FP64 updates (mixed)
 constexpr int kCheckStride = 4;
 
-// Graph replay of the blocks between host checks, opt-in with VXL_GRAPH=1
-// (read at every run).  Off by default: replay measured 2% slower than
-// kernel-by-kernel launches on the 64-cell batch and equal on 256^3.
-bool graphs_enabled() {
+// Graph replay of the blocks between host checks (read at every run):
+// VXL_GRAPH=1 replays each block from one captured graph; VXL_GRAPH=2 runs
+// the same blocks kernel by kernel under the replay's rules (direction ring
+// flushed at every host check), bit-identical to 1 (a diagnostic); anything
+// else launches kernel by kernel with the ring flushed only when full or at
+// a certification.
+int graph_mode() {
   const char* e = std::getenv("VXL_GRAPH");
-  return e && e[0] == '1';
+  if (e && e[0] == '1') return 1;
+  if (e && e[0] == '2') return 2;
+  return 0;
+}
+bool graphs_enabled() { return graph_mode() == 1; }
+
+// Slot of the next search direction: the lowest slot that holds neither the
+// current direction nor a direction whose alpha_t p_t is still pending.
+int next_slot(int cur, const std::vector<int>& pend, int nslot) {
+  for (int i = 0; i < nslot; ++i)
+    if (i != cur && std::find(pend.begin(), pend.end(), i) == pend.end()) return i;
+  return -1;
+}
+
+// A replayed block starts with no pending directions (the ring is flushed at
+// every host check) and rotates the slots with next_slot: kCheckStride
+// preconditioner steps each open a slot and kCheckStride CG updates leave
+// their directions pending.  Returns the slot the block ends on.
+int ring_block_end(int cur, int nslot, std::vector<int>* pend_out) {
+  std::vector<int> pend;
+  for (int j = 0; j < kCheckStride; ++j) {
+    cur = next_slot(cur, pend, nslot);
+    if (cur < 0) return -1;
+    pend.push_back(cur);
+  }
+  if (pend_out) *pend_out = pend;
+  return cur;
+}
+
+// The slot a block ends on when it starts there (iterating ring_block_end
+// from slot 0 reaches it within two blocks for nslot >= kCheckStride), so
+// one captured graph serves every block after the first.
+int ring_block_start(int nslot) {
+  int c = 0;
+  for (int k = 0; k < 2 * kMaxDirections; ++k) {
+    const int e = ring_block_end(c, nslot, nullptr);
+    if (e < 0) return -1;
+    if (e == c) return c;
+    c = e;
+  }
+  return -1;
+}
+
+// The direction ring is on unless VXL_DIRECTION_RING=0 or fewer than two
+// slots could be allocated; with graph replay or its diagnostic twin it needs
+// kCheckStride slots (one block's worth).
+bool ring_on(const vxl_solver* s, int gmode) {
+  const char* e = std::getenv("VXL_DIRECTION_RING");
+  if (e && e[0] == '0') return false;
+  return s->nslot >= (gmode != 0 ? kCheckStride : 2);
 }
 
 template <typename T>
@@ -118,11 +170,7 @@ struct Steps {
     if (jacobi)
       dinv = std::is_same<T, double>::value ? reinterpret_cast<const T*>(s->dinv64)
                                             : reinterpret_cast<const T*>(s->dinv32);
-    // The fused x pass is not captured in graphs, so graph replay keeps the
-    // separate update pass.
-    nblk_fx = (jacobi || !s->partial_fx || graphs_enabled() || !fused_update_enabled())
-                  ? 0
-                  : precond_fused_update_blocks(&s->pc);
+    nblk_fx = (jacobi || !s->partial_fx || !fused_update_enabled()) ? 0 : precond_fused_update_blocks(&s->pc);
     vxl_solver* so = s;
     const int nrz = nblk_rz;
     hipStream_t on = st;
@@ -179,10 +227,17 @@ struct Steps {
 // (flags, scalars and the latch live on the device), and the latch records
 // the iteration relative to the block's start.  The graph bakes in the
 // kernels and their arguments, so it is keyed on the latch tolerance and the
-// stencil schedule.  The fused forward pass is not captured.
+// stencil schedule (and, in mixed mode, the matrix-core switch).  With the
+// direction ring the block is captured from the ring state every block after
+// the first starts in (no pending directions, current direction in slot
+// ring_block_start), and it leaves the pending slots ring_block_end lists.
+// With the fused r update the block starts after the previous iteration's
+// fused pass (the spectrum of r is formed); a block that starts after a
+// certification replaced r forms it eagerly first.
 int graph_key(const vxl_solver* s) {
   const int k = s->mode == VXL_SOLVE_MIXED ? 0 : 1;
-  return s->cell->variant[k] + 4 * (s->cell->tile_y[k] + 64 * s->cell->zmin[k]);
+  const int mfma = k == 0 && s->cell->mfma ? 1 : 0;
+  return s->cell->variant[k] + 4 * (s->cell->tile_y[k] + 64 * (s->cell->zmin[k] + 128 * mfma));
 }
 
 // The block is recorded on a private non-blocking stream (the cell's stream
@@ -190,7 +245,10 @@ int graph_key(const vxl_solver* s) {
 // on the cell's stream.
 template <typename T>
 int prepare_block_graph(vxl_solver* s, double tol) {
-  if (s->block_graph && s->block_graph_tol == tol && s->block_graph_key == graph_key(s)) return VXL_OK;
+  const bool ring = ring_on(s, 1);
+  if (s->block_graph && s->block_graph_tol == tol && s->block_graph_key == graph_key(s) &&
+      s->block_graph_ring == ring)
+    return VXL_OK;
   if (s->block_graph) {
     VXL_HIP(hipGraphExecDestroy(s->block_graph));
     s->block_graph = nullptr;
@@ -206,14 +264,42 @@ int prepare_block_graph(vxl_solver* s, double tol) {
   }
   hipStream_t st = s->capture_stream;
   Steps<T> steps(s, tol, st);
-  if (steps.nblk_fx > 0) return VXL_OK;
+  steps.ring = ring;
+  const bool fused = steps.nblk_fx > 0;
+  int cur = 0;
+  std::vector<int> pend;
+  if (ring) {
+    cur = ring_block_start(s->nslot);
+    if (cur < 0) return give_up();
+  }
+  const int cur0 = cur;
+  auto slot = [&](int i) { return static_cast<T*>(s->pring[i]); };
+  auto precond = [&](bool fd) -> int {
+    if (ring) {
+      const int next = next_slot(cur, pend, s->nslot);
+      if (next < 0) return fail("search-direction ring exhausted");
+      steps.pcur = slot(cur);
+      steps.pnext = slot(next);
+      cur = next;
+    }
+    return steps.precond_step(fd);
+  };
+  auto update = [&](int iter, bool& fd) -> int {
+    if (ring) {
+      steps.pcur = slot(cur);
+      steps.hist_row = s->hist + int64_t(pend.size()) * s->R;
+    }
+    const int rc = steps.cg_update(iter, fd);
+    if (ring) pend.push_back(cur);
+    return rc;
+  };
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return give_up();
-  int rc = steps.precond_step(false);
+  int rc = precond(fused);
   for (int j = 1; rc == VXL_OK && j <= kCheckStride; ++j) {
     bool fd = false;
-    rc = steps.cg_update(j, fd);
-    if (rc == VXL_OK && j < kCheckStride) rc = steps.precond_step(fd);
+    rc = update(j, fd);
+    if (rc == VXL_OK && j < kCheckStride) rc = precond(fd);
   }
   const hipError_t end = hipStreamEndCapture(st, &g);
   hipGraphExec_t exec = nullptr;
@@ -227,6 +313,12 @@ int prepare_block_graph(vxl_solver* s, double tol) {
   s->block_graph = exec;
   s->block_graph_tol = tol;
   s->block_graph_key = graph_key(s);
+  s->block_graph_ring = ring;
+  s->block_graph_fused = fused;
+  s->block_graph_cur0 = cur0;
+  s->block_graph_cur1 = cur;
+  s->block_graph_npend = int(pend.size());
+  for (size_t i = 0; i < pend.size(); ++i) s->block_graph_pend[i] = pend[i];
   return VXL_OK;
 }
 
@@ -309,10 +401,13 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   // p = z = M r with rho = r.z: beta = 0 because rho_old = 0 (slots cleared).
   Steps<T> steps(s, tol, st);
   const bool jacobi = steps.jacobi;
-  // Direction ring (unless graph replay or the fused forward pass is on, or
-  // VXL_DIRECTION_RING=0 asks for the per-iteration x update).
-  const char* ring_env = std::getenv("VXL_DIRECTION_RING");
-  const bool ring = s->nslot >= 2 && !graphs_enabled() && !(ring_env && ring_env[0] == '0');
+  // Direction ring (unless VXL_DIRECTION_RING=0 asks for the per-iteration x
+  // update).  Under graph replay (and its diagnostic twin) the ring is
+  // flushed at every host check, so each block starts with no pending
+  // directions.
+  const int gmode = graph_mode();
+  const bool ring = ring_on(s, gmode);
+  const bool flush_each_check = ring && gmode != 0;
   steps.ring = ring;
   // x = 0.  With the ring, x64 is first written by the first flush (an
   // assignment) and the FP32 x is not used.
@@ -381,7 +476,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     return VXL_OK;
   };
 
-  const bool graphs = steps.nblk_fx == 0 && graphs_enabled() && !s->graph_failed;
+  const bool graphs = gmode == 1 && !s->graph_failed;
   auto cg_update = [&](int iter, bool& fd) -> int {
     if (ring) {
       steps.pcur = slots[cur];
@@ -395,9 +490,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   auto precond_step = [&](bool fd) -> int {
     if (!ring) return steps.precond_step(fd);
     if (int(pend.size()) >= s->nslot) VXL_TRY(flush());  // no free slot for the next direction
-    int next = -1;
-    for (int i = 0; i < s->nslot && next < 0; ++i)
-      if (i != cur && std::find(pend.begin(), pend.end(), i) == pend.end()) next = i;
+    const int next = next_slot(cur, pend, s->nslot);
     if (next < 0) return fail("search-direction ring exhausted");
     steps.pcur = slots[cur];
     steps.pnext = slots[next];
@@ -414,14 +507,23 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
     bool replay = pending && graphs && it % kCheckStride == 0 && it + kCheckStride <= max_iter;
     if (replay) {
       VXL_TRY(block_graph());
-      replay = s->block_graph != nullptr;
+      // The ring must be in the state the block was captured from.
+      replay = s->block_graph != nullptr && (!ring || (pend.empty() && cur == s->block_graph_cur0));
     }
     if (replay) {
+      if (s->block_graph_fused && !forward_done)  // r was replaced: form its spectrum first
+        VXL_TRY(precond_cg_forward<T>(&s->pc, R, r, static_cast<T*>(nullptr), nullptr, nullptr, nullptr, nullptr,
+                                      act, st));
       VXL_HIP(hipGraphLaunch(s->block_graph, st));
       ++s->graph_blocks;
       base = it;
       it += kCheckStride;
       std::fill(restart.begin(), restart.end(), 0);
+      forward_done = s->block_graph_fused;
+      if (ring) {
+        pend.assign(s->block_graph_pend, s->block_graph_pend + s->block_graph_npend);
+        cur = s->block_graph_cur1;
+      }
     } else {
       if (pending) {
         VXL_TRY(precond_step(forward_done));
@@ -463,6 +565,7 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
           if (active[k] && latched[k]) restart[k] = 1;
         VXL_HIP(hipMemsetAsync(s->stop_dev, 0, sizeof(int) * R, st));  // eager, never captured
       }
+      if (flush_each_check) VXL_TRY(flush());
       VXL_TRY(upload_flags());
       if (!any_active()) break;
     }

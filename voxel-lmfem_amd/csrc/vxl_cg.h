@@ -167,6 +167,14 @@ struct vxl_solver {
   hipGraphExec_t block_graph;
   double block_graph_tol;
   int block_graph_key;  // stencil schedule (and tile) the graph was captured with
+  // Direction-ring state the block was captured from and leaves behind: the
+  // slot holding the current direction at the start (cur0) and end (cur1),
+  // and the slots whose alpha_t p_t the block leaves pending, in order.
+  bool block_graph_ring;
+  bool block_graph_fused;  // the block starts after a fused r update (spectrum of r formed)
+  int block_graph_cur0, block_graph_cur1;
+  int block_graph_pend[8];
+  int block_graph_npend;
   bool graph_failed;    // capture unavailable: blocks run kernel by kernel
   hipStream_t capture_stream;  // private stream the block is recorded on (replayed on the cell's)
   int graph_blocks;     // blocks replayed from the graph in the last run

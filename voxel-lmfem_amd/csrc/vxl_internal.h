@@ -149,6 +149,11 @@ struct vxl_cell {
   int zmin[2];     // fewest z planes per workgroup chunk (each chunk recomputes one layer)
   int element;     // VXL_ELEMENT_FI (2x2x2 Gauss) or VXL_ELEMENT_C3D8R_HG (one point + hourglass)
   double hg_coef;  // hourglass coefficient of the C3D8R-HG element
+  // Matrix-core FP32 stencil (mfma.hip): on when mfma != 0; the dense FP16
+  // hi/lo element matrices of every row in lane order and their exponents.
+  int mfma;
+  void* mfma_table;
+  int* mfma_exp;
 };
 
 namespace vxl {
@@ -208,6 +213,16 @@ int label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, 
 int local_rows(int64_t n, int label_bytes, const void* labels_dev, long long label_lo, int span, const int* slot_host,
                int phases, const double* c21_host, int mode, const double* a_dev, const double* b_dev, int64_t stride,
                double* rows_dev, int64_t local_base, int* bad_host, hipStream_t s);
+// Matrix-core FP32 stencil (mfma.hip): label-grouped dense element matrices
+// on v_mfma_f32_16x16x32_f16.  mfma_prepare builds the per-row tables once;
+// launch_mfma_apply is the plain (partial == nullptr) or CG form of the
+// FP32 apply, as launch_apply / launch_apply_cg.
+int mfma_max_rows();
+int mfma_prepare(vxl_cell* cell);
+void mfma_release(vxl_cell* cell);
+int mfma_blocks(const vxl_cell* cell, int nrhs);
+int launch_mfma_apply(const vxl_cell* cell, int nrhs, const float* u, float* out, const int* active,
+                      double* partial, int* blocks, hipStream_t s);
 // Element-centre fields (any output may be null): strain = E + B0 u_e and
 // stress = C_e strain as planes [r][6][N], energy = strain . stress / 2 as [r][N].
 int launch_center_fields(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
