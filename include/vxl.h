@@ -165,6 +165,21 @@ int vxl_set_stencil_config(vxl_cell* cell, int precision, int tile_y, int varian
  * is 0.1); it is ignored for VXL_ELEMENT_FI. */
 int vxl_cell_set_element(vxl_cell* cell, int element, double hourglass_coefficient);
 int vxl_get_stencil_config(vxl_cell* cell, int precision, int* tile_y, int* variant, int* zmin);
+/* Matrix-core FP32 stencil (enable 1) or the modal VALU stencil (enable 0,
+ * the default).  The matrix-core form is the label-grouped dense element
+ * product f_e = K_L (u_e - u_e0) of the reference's grouped kernel
+ * (_core/runtime/fi/grouped_kernels.cu:12-74): the dense 24x24 FI element
+ * matrix of every coefficient row is built once here (host FP64, 2x2x2
+ * Gauss rule), scaled by a power of two and split into FP16 hi/lo operands;
+ * each element's displacement differences are split the same way, and the
+ * three leading products run on v_mfma_f32_16x16x32_f16 with FP32
+ * accumulation (about 2^-21 relative per element column), one product pass
+ * per distinct row among the 64 elements of a wave.  It replaces every FP32
+ * apply of the cell, plain and CG form (vxl_apply, vxl_solver_run in mixed
+ * mode); FP64 is unchanged.  FI element and at most 4096 coefficient rows
+ * (label-only cells); VXL_ERR_ARG otherwise.  Tile 32 x 7 nodes, schedule
+ * and tile settings of the modal stencil do not apply to it. */
+int vxl_cell_set_mfma(vxl_cell* cell, int enable);
 /* Autotuner: times all 27 (tile, schedule, chunk depth in {8, 4, 2})
  * configurations on the given buffers (reps launches each, HIP events on the
  * cell's stream), keeps the fastest on the cell and returns it.  ms_table

@@ -125,6 +125,9 @@ int cell_create_common(vxl_cell** out, int nz, int ny, int nx, double hx, double
   c->zmin[0] = c->zmin[1] = kDefaultChunkPlanes;
   c->element = VXL_ELEMENT_FI;
   c->hg_coef = 0.1;
+  c->mfma = 0;
+  c->mfma_table = nullptr;
+  c->mfma_exp = nullptr;
   (void)hipGetDevice(&c->device);
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(c->stream);
@@ -328,6 +331,7 @@ int vxl_cell_create_device_rows(vxl_cell** out, int nz, int ny, int nx, double h
 int vxl_cell_destroy(vxl_cell* cell) {
   if (!cell) return VXL_OK;
   (void)hipStreamSynchronize(cell->stream);
+  mfma_release(cell);
   if (cell->c64_owned) dev_free(cell->c64_owned);
   if (cell->c32_owned) dev_free(cell->c32_owned);
   delete cell;
@@ -352,6 +356,18 @@ int vxl_set_stencil_variant(vxl_cell* cell, int precision, int variant) {
   if (!cell || (precision != VXL_F32 && precision != VXL_F64) || variant < 1 || variant > 3)
     return fail("vxl_set_stencil_variant: invalid arguments");
   cell->variant[precision == VXL_F64 ? 1 : 0] = variant;
+  return VXL_OK;
+}
+
+int vxl_cell_set_mfma(vxl_cell* cell, int enable) {
+  g_error.clear();
+  if (!cell || (enable != 0 && enable != 1)) return fail("vxl_cell_set_mfma: invalid arguments");
+  if (enable) {
+    VXL_TRY(mfma_prepare(cell));
+    cell->mfma = 1;
+  } else {
+    cell->mfma = 0;
+  }
   return VXL_OK;
 }
 

@@ -1089,6 +1089,9 @@ constexpr int default_tile_y() {
 
 template <typename T>
 int launch_apply_t(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (cell->mfma) return launch_mfma_apply(cell, nrhs, u, out, nullptr, nullptr, nullptr, s);
+  }
   if (cell->element == VXL_ELEMENT_C3D8R_HG)
     return dispatch_apply<T, 32, default_tile_y<T>(), true>(cell, nrhs, u, out, s);
   return with_tile<T>(cell, [&](auto ty) { return dispatch_apply<T, 32, decltype(ty)::value>(cell, nrhs, u, out, s); });
@@ -1096,6 +1099,9 @@ int launch_apply_t(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream
 template <typename T>
 int launch_apply_cg_t(const vxl_cell* cell, int nrhs, const T* p, T* q, const int* active, double* partial,
                       int* blocks, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (cell->mfma) return launch_mfma_apply(cell, nrhs, p, q, active, partial, blocks, s);
+  }
   if (cell->element == VXL_ELEMENT_C3D8R_HG)
     return dispatch_apply<T, 32, default_tile_y<T>(), true>(cell, nrhs, p, q, s, active, partial, blocks);
   return with_tile<T>(cell, [&](auto ty) {
@@ -1104,6 +1110,9 @@ int launch_apply_cg_t(const vxl_cell* cell, int nrhs, const T* p, T* q, const in
 }
 template <typename T>
 int cg_blocks_t(const vxl_cell* cell, int nrhs, int zmin) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (cell->mfma) return mfma_blocks(cell, nrhs);
+  }
   return with_tile<T>(cell, [&](auto ty) {
     int tx, tiles, chunks, zc;
     apply_grid<T, 32, decltype(ty)::value>(cell, nrhs, tx, tiles, chunks, zc, zmin);
@@ -1149,6 +1158,10 @@ int apply_cg_blocks_max(const vxl_cell* cell, int nrhs, int precision) {
     probe.tile_y[d ? 1 : 0] = d ? kTilesF64[t] : kTilesF32[t];
     const int b = d ? cg_blocks_t<double>(&probe, nrhs, kMinChunkPlanes) : cg_blocks_t<float>(&probe, nrhs, kMinChunkPlanes);
     most = std::max(most, b);
+  }
+  if (!d) {  // the matrix-core form, which the cell may be switched to later
+    probe.zmin[0] = kMinChunkPlanes;
+    most = std::max(most, mfma_blocks(&probe, nrhs));
   }
   return most;
 }

@@ -152,6 +152,13 @@ class Cell:
                                               ctypes.byref(z)), "vxl_get_stencil_config")
         return t.value, v.value, z.value
 
+    def set_mfma(self, enable=True):
+        """Run the cell's FP32 applies (plain and CG form) on the matrix cores:
+        label-grouped dense element matrices on FP16 hi/lo MFMA products
+        (FI element, at most 4096 coefficient rows).  False restores the modal
+        VALU stencil.  Switch before building solvers that should use it."""
+        check(self.lib.vxl_cell_set_mfma(self.handle, 1 if enable else 0), "vxl_cell_set_mfma")
+
     def set_stencil_config(self, precision, tile_y, variant, zmin):
         check(self.lib.vxl_set_stencil_config(self.handle, self._code(precision), int(tile_y), int(variant),
                                               int(zmin)), "vxl_set_stencil_config")

@@ -21,6 +21,11 @@ preconditioner: "fft" (Green operator of the isotropic reference medium, the
 default), "fft_bf16" (same, spectra stored in BF16 between FFT passes; mixed
 precision only) or "jacobi" (1 / diag K, the reference's scalar control in
 repro/bench/chapter_aligned/study.py:84-90).
+stencil: "modal" (the modal VALU stencil, default) or "mfma" (the FP32
+operator of the mixed solve as label-grouped dense element matrices on the
+matrix cores, the reference's grouped-kernel form,
+_core/runtime/fi/grouped_kernels.cu:12-74; FI element, at most 4096
+coefficient rows).  The FP64 operator and the certificate are the same.
 """
 import time
 
@@ -66,7 +71,7 @@ def _field_statistics(cell, u, phase, loads=(0, 5)):
 
 
 def _workload(phase, table, domain, orientation, precision, tol, local, fields=False, preconditioner="fft",
-              element="fi", hourglass_coefficient=0.1, max_iter=10000):
+              element="fi", hourglass_coefficient=0.1, max_iter=10000, stencil="modal"):
     from .gpu import Cell, Solver
     import torch
     torch.cuda.synchronize()
@@ -85,6 +90,8 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
     cell = solver = None
     try:
         cell = Cell(field, phase.shape, h, element=element, hourglass_coefficient=hourglass_coefficient)
+        if stencil == "mfma":
+            cell.set_mfma(True)
         solver = Solver(cell, lam, mu, mode=precision, nrhs=6, preconditioner=preconditioner)
         stage("operator")
         out = solver.run(np.eye(6), tol=tol, return_fields=fields, max_iter=max_iter)
@@ -107,12 +114,13 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
     result = dict(seconds=seconds, CH=np.asarray(out["CH"]).tolist(), calls=calls, stages=stages,
                   field_statistics=field_stats, peak_allocated_bytes=int(torch.cuda.max_memory_allocated()),
                   peak_reserved_bytes=int(torch.cuda.max_memory_reserved()),
-                  device_ms=out["device_ms"], graph_blocks=out["graph_blocks"])
+                  device_ms=out["device_ms"], graph_blocks=out["graph_blocks"], stencil=stencil)
     rows = int(getattr(field, "row_count", len(field.rows)))
     storage = dict(mode="label_rows", n_elements=int(phase.size), pool_rows=rows, index_bytes=index_bytes,
                    coefficient_bytes=rows * 21 * 8,
                    front_end="device" if getattr(field, "index_dev", None) is not None else "host")
-    operator = dict(variant="modal_fi_lds_stencil" if element == "fi" else "modal_c3d8r_hg_lds_stencil",
+    operator = dict(variant=("mfma_label_grouped_dense_element" if stencil == "mfma" else
+                             "modal_fi_lds_stencil" if element == "fi" else "modal_c3d8r_hg_lds_stencil"),
                     element=element, precision=precision, preconditioner=preconditioner, reference_medium=meta)
     if element != "fi":
         operator["hourglass_coefficient"] = float(hourglass_coefficient)
@@ -123,7 +131,7 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
 
 def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="cuda", warmup=False,
                precision="mixed", tol=TOL, fields=False, preconditioner="fft", element="fi",
-               hourglass_coefficient=0.1, max_iter=10000):
+               hourglass_coefficient=0.1, max_iter=10000, stencil="modal"):
     """Return effective stiffness in engineering-Voigt order 11,22,33,23,13,12.
 
     max_iter caps the PCG iterations per load (the reference's 10,000,
@@ -147,6 +155,10 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="c
         raise ValueError("hourglass_coefficient must be finite and positive")
     if isinstance(max_iter, bool) or not isinstance(max_iter, (int, np.integer)) or max_iter < 1:
         raise ValueError("max_iter must be a positive integer")
+    if stencil not in ("modal", "mfma"):
+        raise ValueError("stencil must be 'modal' or 'mfma'")
+    if stencil == "mfma" and element != "fi":
+        raise ValueError("stencil='mfma' implements the FI element only")
     phase, table, domain, orientation, labels, tensors, local = validate(
         phase_id, materials, domain_size, orientation)
     if backend == "cpu":
@@ -165,10 +177,11 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="c
         raise RuntimeError("No GPU is available. This backend runs only on an MI355X (gfx950).")
     if warmup:
         _workload(phase, table, domain, orientation, precision, tol, local, fields, preconditioner, element,
-                  hourglass_coefficient, int(max_iter))
+                  hourglass_coefficient, int(max_iter), stencil)
     start = time.perf_counter()
     result, storage, operator = _workload(phase, table, domain, orientation, precision, tol, local, fields,
-                                          preconditioner, element, hourglass_coefficient, int(max_iter))
+                                          preconditioner, element, hourglass_coefficient, int(max_iter),
+                                          stencil)
     result.update(backend=backend, storage=storage, operator=operator,
                   scope=(("FI HEX8" if element == "fi" else "C3D8R-HG") + " matrix-free stiffness; "
                          + {"fft": "FFT Green preconditioner; ", "fft_bf16": "FFT Green preconditioner (BF16 spectra); ",
