@@ -21,6 +21,15 @@ cpu_baseline: the FP64 NumPy oracle (operator apply + Green preconditioner)
 timed at 64^3 on one host thread (child process, BLAS/OpenMP threads = 1,
 started before the GPU is initialised) and extrapolated to a 256^3 six-load
 solve with the GPU's iteration count (labelled as extrapolated).
+cpu_baseline_all_cores: the same sample in one single-thread process per host
+core at once, rates summed (the oracle's element loop is serial, so this is
+how the host's cores are all used).  cpu_end_to_end: whole oracle solves on one
+host thread, not extrapolated: the 32^3 sphere cell by FP64 FFT-PCG and a 16^3
+sphere cell by sparse direct (the reference's CPU-backend size limit), run in
+a child process while the GPU legs run.
+mfma_stencil: the same six-RHS apply on the matrix cores (vxl_cell_set_mfma),
+same buffers and events.  batch.graph_replay: the 64-cell batch again with the
+check blocks replayed from HIP graphs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 """
@@ -81,18 +90,69 @@ def host_info():
     return dict(cpu_count=os.cpu_count(), affinity=affinity, model=model)
 
 
+def cpu_end_to_end():
+    """Whole oracle solves on one host thread: the FP64 matrix-free FFT-PCG
+    C_eff of the 32^3 sphere cell (r = 3, vf >= 0.30, E 3/70 GPa, tol 1e-8)
+    and the sparse-direct C_eff of a 16^3 sphere cell (4096 voxels, the
+    reference's CPU-backend limit)."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import homogenize as oracle
+    from voxel_lmfem.cells import periodic_spheres
+    (phase, mats, dom, _), _ = periodic_spheres(32, seed=1)
+    t0 = time.perf_counter()
+    r = oracle.homogenize_matfree(phase, mats, dom)
+    t_c1 = time.perf_counter() - t0
+    its = [int(c["iterations"]) for c in r["calls"]] if r.get("calls") and "iterations" in r["calls"][0] else None
+    (phase, mats, dom, _), _ = periodic_spheres(16, radius=1.5, seed=1)
+    t0 = time.perf_counter()
+    oracle.homogenize_direct(phase, mats, dom)
+    t_d16 = time.perf_counter() - t0
+    return dict(c1_seconds=t_c1, c1_iterations=its, direct16_seconds=t_d16)
+
+
+def _child_env(threads):
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+        env[k] = str(threads)
+    return env
+
+
 def cpu_baseline_child(threads=1):
     """Run cpu_sample in a child process with `threads` BLAS/OpenMP threads
     (NumPy's FFT and element loops run on one thread either way).
 
     Called before the GPU is initialised; the child never touches the GPU."""
     import subprocess
-    env = dict(os.environ)
-    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
-        env[k] = str(threads)
-    out = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-sample"], env=env,
+    out = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-sample"], env=_child_env(threads),
                          capture_output=True, text=True, timeout=600, check=True)
     return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline_concurrent(copies):
+    """`copies` single-thread cpu_sample children at once: the host's aggregate
+    oracle throughput with every core busy (NumPy's element loop is serial,
+    so more BLAS threads in one process do not use the other cores)."""
+    import subprocess
+    procs = [subprocess.Popen([sys.executable, str(Path(__file__).resolve()), "--cpu-sample"], env=_child_env(1),
+                              stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True) for _ in range(copies)]
+    outs = []
+    for p in procs:
+        text, _ = p.communicate(timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError("cpu sample child failed")
+        outs.append(json.loads(text.strip().splitlines()[-1]))
+    return outs
+
+
+def cpu_baseline_aggregate(samples, iterations_total, n_target):
+    """Sum of the per-process rates of concurrent single-thread samples."""
+    legs = [cpu_baseline(sm, iterations_total, n_target) for sm in samples]
+    worst = max(sm["t_apply"] + sm["t_green"] for sm in samples)
+    return dict(value=float(sum(leg["value"] for leg in legs)), unit="solves/s", cores=len(samples), kind="port",
+                sample=(f"{len(samples)} concurrent single-thread FP64 NumPy oracle processes, each timing "
+                        f"operator apply + FFT Green apply per RHS-iteration at {samples[0]['n']}^3 (slowest "
+                        f"{worst:.3f} s), each extrapolated to {n_target}^3 x {iterations_total} RHS-iterations; "
+                        f"aggregate throughput, extrapolated"))
 
 
 def cpu_baseline(sample, iterations_total, n_target, threads=1):
@@ -202,12 +262,19 @@ def main():
                     help="skip the 64-cell uneven batch reported beside the headline number")
     ap.add_argument("--batch-only", action="store_true", help="run only the 64-cell batch leg")
     ap.add_argument("--cpu-sample", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-end-to-end", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--skip-mfma", action="store_true", help="skip the matrix-core stencil A/B")
+    ap.add_argument("--stencil", default="modal", choices=["modal", "mfma"],
+                    help="FP32 operator of the timed mixed solve: modal VALU stencil or matrix-core form")
     ap.add_argument("--stencil-only", action="store_true",
                     help="only run 10 six-RHS FP32 stencil applies and a 1 GiB device copy "
                          "(the calibration kernel for FETCH_SIZE/WRITE_SIZE counter runs)")
     args = ap.parse_args()
     if args.cpu_sample:
         print(json.dumps(cpu_sample()), flush=True)
+        return
+    if args.cpu_end_to_end:
+        print(json.dumps(cpu_end_to_end()), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps(dict(error="this benchmark runs on one GPU; sharding across GPUs is not built",
@@ -218,7 +285,8 @@ def main():
         torch.cuda.set_device(0)
         print(json.dumps(dict(batch=batch_leg(steps=args.steps, warmup=args.warmup))), flush=True)
         return
-    sample = sample_all = None
+    sample = samples_all = None
+    e2e_proc = None
     host = host_info()
     all_threads = max(1, min(16, host["affinity"] or host["cpu_count"] or 1))
     if not (args.skip_cpu or args.stencil_only):
@@ -227,9 +295,14 @@ def main():
         print(f"# cpu sample {sample} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
         if all_threads > 1:
             t0 = time.perf_counter()
-            sample_all = cpu_baseline_child(all_threads)
-            print(f"# cpu sample, {all_threads} threads {sample_all} ({time.perf_counter() - t0:.1f} s)",
+            samples_all = cpu_baseline_concurrent(all_threads)
+            print(f"# cpu sample, {all_threads} concurrent processes ({time.perf_counter() - t0:.1f} s)",
                   file=sys.stderr, flush=True)
+        # Whole oracle solves (C1 FFT-PCG, 16^3 sparse direct) on one host core
+        # while the GPU legs run; the child never touches the GPU.
+        import subprocess
+        e2e_proc = subprocess.Popen([sys.executable, str(Path(__file__).resolve()), "--cpu-end-to-end"],
+                                    env=_child_env(1), stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
     import torch
     from voxel_lmfem.cells import voronoi_polycrystal
     from voxel_lmfem import materials as pm
@@ -251,6 +324,8 @@ def main():
         field = pm.build_field_device(labels, mats)
         lam, mu, rmeta = select(field, phase, mats)
         cell = Cell(field, phase.shape, h)
+        if mode == "mixed" and args.stencil == "mfma":
+            cell.set_mfma(True)
         kind = args.preconditioner if mode == "mixed" else "fft"
         return cell, Solver(cell, lam, mu, mode=mode, nrhs=6, preconditioner=kind), rmeta
 
@@ -345,16 +420,28 @@ def main():
           f"{rmeta['status']} lam={rmeta['lam']:.3f} mu={rmeta['mu']:.3f}", file=sys.stderr, flush=True)
     CH_mixed = out["CH"]
 
-    # The same solve on a resident operator (set-up excluded), for the breakdown.
+    # The same solve on a resident operator (set-up excluded), for the breakdown,
+    # launched kernel by kernel and with the check blocks replayed from graphs.
     cell, solver, _ = setup("mixed")
-    solver.run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    solver.run()
-    torch.cuda.synchronize()
-    resident_ms = 1e3 * (time.perf_counter() - t0)
+    resident = {}
+    saved = os.environ.get("VXL_GRAPH")
+    for mode in ("0", "1"):
+        os.environ["VXL_GRAPH"] = mode
+        solver.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            got = solver.run()
+        torch.cuda.synchronize()
+        resident[mode] = (1e3 * (time.perf_counter() - t0) / 3, int(got["graph_blocks"]))
+    if saved is None:
+        del os.environ["VXL_GRAPH"]
+    else:
+        os.environ["VXL_GRAPH"] = saved
+    resident_ms = resident["0"][0]
     solver.close()
-    print(f"# mixed, resident operator: {resident_ms:.1f} ms/solve", file=sys.stderr, flush=True)
+    print(f"# mixed, resident operator: {resident_ms:.1f} ms/solve; graph replay {resident['1'][0]:.1f} ms "
+          f"({resident['1'][1]} blocks)", file=sys.stderr, flush=True)
 
     # Plain FP64 GPU solve of the same cell, same step definition.
     fp64 = None
@@ -373,6 +460,7 @@ def main():
 
     # Stencil roofline: FP32 six-RHS apply timed with HIP events recorded on
     # the stream the kernel is launched on (the cell's stream).
+    cell.set_mfma(False)
     N = cell.n
     R = 6
     stream = torch.cuda.current_stream()
@@ -402,6 +490,36 @@ def main():
     print(f"# stencil fp32 R=6: {t_apply * 1e3:.3f} ms/launch, {achieved:.0f} GB/s algorithmic; "
           f"fp64 R=1: {t_apply64 * 1e3:.3f} ms/launch, {N * 49 / t_apply64 / 1e9:.0f} GB/s",
           file=sys.stderr, flush=True)
+
+    # A/B: the same FP32 six-RHS apply on the matrix cores (label-grouped dense
+    # element matrices, FP16 hi/lo MFMA products), same buffers and events.
+    mfma = None
+    if not args.skip_mfma:
+        cell.set_mfma(True)
+        cell.apply_planes(u, o)
+        e0.record(stream)
+        for _ in range(reps):
+            cell.apply_planes(u, o)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t_mfma = e0.elapsed_time(e1) / reps * 1e-3
+        cell.set_mfma(False)
+        # Issued matrix-core work: every element of a layer (33 x 8 per 32 x 7
+        # output tile, one recomputed layer per z chunk) runs 2 row tiles x 3
+        # products of 16x16x32 (16384 flop each) per 16 element columns.
+        tiles = -(-n // 32) * -(-n // 7)
+        chunks = max(1, min(-(-4096 // (tiles * R)), n // 8))
+        zc = -(-n // chunks)
+        chunks = -(-n // zc)
+        elems = tiles * 33 * 8 * (n + chunks) * R
+        issued = elems * 6 * 16384 / 16
+        mfma = dict(kernel="mfma_apply_kernel<uint8> R=6, 32x7 tile, v_mfma_f32_16x16x32_f16 (3 products)",
+                    ms_per_launch=t_mfma * 1e3, achieved=alg_bytes / t_mfma / 1e9,
+                    frac=alg_bytes / t_mfma / 1e9 / HBM_PEAK_GBS, vs_modal=t_apply / t_mfma,
+                    elements_per_launch=elems, mfma_tflops_issued=issued / t_mfma / 1e12,
+                    mfma_frac_of_fp16_peak=issued / t_mfma / 2.5e15)
+        print(f"# stencil fp32 R=6 on the matrix cores: {t_mfma * 1e3:.3f} ms/launch "
+              f"({t_apply / t_mfma:.2f}x the modal stencil)", file=sys.stderr, flush=True)
     cell.close()
 
     cpu = cpu_all = None
@@ -409,12 +527,38 @@ def main():
         cpu = cpu_baseline(sample, int(sum(its)), n)
         cpu["host"] = host
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
-    if sample_all is not None:
-        cpu_all = cpu_baseline(sample_all, int(sum(its)), n, threads=all_threads)
+    if samples_all is not None:
+        cpu_all = cpu_baseline_aggregate(samples_all, int(sum(its)), n)
 
     batch = None
     if not args.skip_batch:
-        batch = batch_leg(steps=2, warmup=1)
+        # Check blocks launched kernel by kernel (VXL_GRAPH=0) and replayed
+        # from captured HIP graphs (VXL_GRAPH=1), same cells.
+        saved = os.environ.get("VXL_GRAPH")
+        legs = {}
+        for mode in ("0", "1"):
+            os.environ["VXL_GRAPH"] = mode
+            legs[mode] = batch_leg(steps=2, warmup=1)
+        if saved is None:
+            del os.environ["VXL_GRAPH"]
+        else:
+            os.environ["VXL_GRAPH"] = saved
+        batch = legs["0"]
+        batch["graph_replay"] = legs["1"]
+
+    cpu_e2e = None
+    if e2e_proc is not None:
+        text, _ = e2e_proc.communicate(timeout=900)
+        if e2e_proc.returncode == 0:
+            got = json.loads(text.strip().splitlines()[-1])
+            cpu_e2e = dict(
+                c1=dict(value=1.0 / got["c1_seconds"], unit="solves/s", cores=1, kind="port",
+                        sample=("whole FP64 matrix-free FFT-PCG C_eff of the 32^3 sphere cell (six loads, "
+                                f"tol 1e-8) on one host thread: {got['c1_seconds']:.1f} s, not extrapolated")),
+                direct16=dict(value=1.0 / got["direct16_seconds"], unit="solves/s", cores=1, kind="port",
+                              sample=("sparse-direct (SuperLU) C_eff of a 16^3 sphere cell, the reference's "
+                                      f"CPU-backend path, on one host thread: {got['direct16_seconds']:.1f} s")))
+            print(f"# cpu end-to-end: {got}", file=sys.stderr, flush=True)
 
     traffic, traffic_src = measured_traffic()
     line = dict(
@@ -423,19 +567,24 @@ def main():
         ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f32 inner + f64 refinement", data="synthetic",
         config=dict(workload=f"voronoi_polycrystal_{n}^3_200grains_cubic_6loads", grid=[n, n, n], grains=200,
-                    loads=6, precision="mixed", preconditioner=args.preconditioner, tol=1e-8, iterations=its,
+                    loads=6, precision="mixed", preconditioner=args.preconditioner, stencil=args.stencil,
+                    tol=1e-8, iterations=its,
                     step=("device label front end + reference medium + operator/preconditioner set-up + "
                           "RHS + block PCG + FP64 certificate + stress + teardown; labels resident in HBM"),
-                    resident_operator_ms_per_solve=resident_ms, reference_medium=rmeta,
+                    resident_operator_ms_per_solve=resident_ms,
+                    resident_graph_replay_ms_per_solve=resident["1"][0], graph_blocks_per_solve=resident["1"][1],
+                    reference_medium=rmeta,
                     step_ms=[round(t, 2) for t in step_ms], median_ms_per_step=float(np.median(step_ms))),
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_src,
                       kernel="apply_kernel_v2<float,uint8> R=6, 32x14 tile, cubic-voxel form (schedule 3)",
                       ms_per_launch=t_apply * 1e3, algorithmic_bytes=alg_bytes),
         fp64=fp64,
+        mfma_stencil=mfma,
         batch=batch,
         cpu_baseline=cpu,
         cpu_baseline_all_cores=cpu_all,
+        cpu_end_to_end=cpu_e2e,
     )
     print(json.dumps(line), flush=True)
 
