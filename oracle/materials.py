@@ -156,10 +156,24 @@ def bond_matrix(R):
 
 
 def rotate_packed(C, R):
-    """material_field.py:105-114: Bond rotation, 21 upper-triangle outputs."""
+    """material_field.py:105-114: Bond rotation, 21 upper-triangle outputs.
+
+    T C T^T with the sums over k in ascending order and every product
+    rounded on its own: the reference's torch matmul leaves its summation
+    order to the BLAS, so one order is fixed here (pinned by
+    fourth_order_rotate and the diagonal known answers in the tests)."""
     T = bond_matrix(R)
-    TC = T @ C
-    return np.stack([(TC[:, i, :] * T[:, j, :]).sum(-1) for i, j in PAIRS], -1)
+    C = np.asarray(C, dtype=np.float64)
+    TC = T[:, :, 0, None] * C[0][None, None, :]
+    for k in range(1, 6):
+        TC = TC + T[:, :, k, None] * C[k][None, None, :]
+    rows = []
+    for i, j in PAIRS:
+        acc = TC[:, i, 0] * T[:, j, 0]
+        for m in range(1, 6):
+            acc = acc + TC[:, i, m] * T[:, j, m]
+        rows.append(acc)
+    return np.stack(rows, -1)
 
 
 def fourth_order_rotate(C, R):

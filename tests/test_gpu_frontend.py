@@ -120,6 +120,8 @@ def _voxel_tensors_from_device(field):
 
 @pytest.mark.parametrize("kind", ["woven_axis_pairs", "fixture_matrices", "single_matrix"])
 def test_device_local_frames_equal_host_rows(torch_gpu, kind):
+    """Frames and Bond-rotated rows of local-frame voxels built on the device
+    equal the host's bit for bit."""
     torch = torch_gpu
     from test_oracle import gpu_fixture_cell
     from voxel_lmfem.cells import woven_like
@@ -135,8 +137,8 @@ def test_device_local_frames_equal_host_rows(torch_gpu, kind):
     want = pm.unpack21(host.rows[host.index.astype(np.int64)])
     dev = pm.build_field_device(torch.tensor(np.asarray(phase).astype(np.int32), device="cuda"), mats, ori)
     got = _voxel_tensors_from_device(dev)
-    scale = np.abs(want).max()
-    assert np.abs(got - want).max() <= 1e-14 * scale
+    # same expressions in the same order, no fused multiply-add: the same bits
+    np.testing.assert_array_equal(got, want)
     np.testing.assert_array_equal(dev.used_labels(), np.unique(phase))
 
 

@@ -61,6 +61,7 @@ void launch_label_rows(const void* labels, int nx, int ny, int nz, long long lo,
 // order 11, 22, 33, 23, 13, 12; the pairs (i, j) of shear rows/columns are
 // (1, 2), (0, 2), (0, 1).
 __device__ __forceinline__ void bond_matrix(const double (&R)[3][3], double (&T)[6][6]) {
+#pragma clang fp contract(off)
   const int pi[3] = {1, 0, 0}, pj[3] = {2, 2, 1};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -89,6 +90,11 @@ __global__ void __launch_bounds__(128)
 local_rows_kernel(const L* __restrict__ labels, int64_t n, long long lo, int span, const int* __restrict__ slot,
                   const double* __restrict__ cphase, int mode, const double* __restrict__ A, const double* __restrict__ B,
                   int64_t stride, double* __restrict__ rows, int64_t local_base, int* __restrict__ bad) {
+  // Every product is rounded on its own (no fused multiply-add) and every sum
+  // runs in ascending index order: the host front end (materials.frames,
+  // materials.bond_rotate, NumPy elementwise operations) evaluates the same
+  // expressions in the same order, so the rows agree bit for bit.
+#pragma clang fp contract(off)
   for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += int64_t(gridDim.x) * blockDim.x) {
     const long long off = static_cast<long long>(labels[e]) - lo;
     if (off < 0 || off >= span) continue;
@@ -172,15 +178,15 @@ local_rows_kernel(const L* __restrict__ labels, int64_t n, long long lo, int spa
     double TC[6][6];
     for (int i = 0; i < 6; ++i)
       for (int j = 0; j < 6; ++j) {
-        double acc = 0;
-        for (int k = 0; k < 6; ++k) acc = fma(T[i][k], c21[pidx(k, j)], acc);
+        double acc = T[i][0] * c21[pidx(0, j)];
+        for (int k = 1; k < 6; ++k) acc = acc + T[i][k] * c21[pidx(k, j)];
         TC[i][j] = acc;
       }
     int w = 0;
     for (int i = 0; i < 6; ++i)
       for (int j = i; j < 6; ++j, ++w) {
-        double acc = 0;
-        for (int k = 0; k < 6; ++k) acc = fma(TC[i][k], T[j][k], acc);
+        double acc = TC[i][0] * T[j][0];
+        for (int k = 1; k < 6; ++k) acc = acc + TC[i][k] * T[j][k];
         out[w] = acc;
       }
   }

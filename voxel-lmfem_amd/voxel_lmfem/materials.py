@@ -161,15 +161,29 @@ def frames(orientation, shape, flat_indices):
 
 
 def bond_rotate(C, R):
-    """Rotated stiffness rows (k,21) = pack(T C T^T), T the Bond stress matrix."""
+    """Rotated stiffness rows (k,21) = pack(T C T^T), T the Bond stress matrix
+    (the reference's _rotate_packed, material_field.py:105-114).
+
+    Every sum runs over its index in ascending order with each product rounded
+    on its own (no fused multiply-add, no BLAS), the order the device
+    front end (vxl_local_rows) uses, so both give the same bits."""
     R = np.asarray(R, dtype=np.float64).reshape(-1, 3, 3)
+    C = np.asarray(C, dtype=np.float64)
     i, j = [1, 0, 0], [2, 2, 1]
     r1, r2 = R[:, i, :], R[:, j, :]
     T = np.concatenate((np.concatenate((R * R, 2 * R[:, :, i] * R[:, :, j]), -1),
                         np.concatenate((r1 * r2, r1[:, :, i] * r2[:, :, j] + r1[:, :, j] * r2[:, :, i]), -1)),
                        -2)
-    TC = T @ C
-    return np.stack([(TC[:, a, :] * T[:, b, :]).sum(-1) for a, b in PAIRS], -1)
+    TC = T[:, :, 0, None] * C[0][None, None, :]
+    for k in range(1, 6):
+        TC = TC + T[:, :, k, None] * C[k][None, None, :]
+    out = []
+    for a, b in PAIRS:
+        acc = TC[:, a, 0] * T[:, b, 0]
+        for m in range(1, 6):
+            acc = acc + TC[:, a, m] * T[:, b, m]
+        out.append(acc)
+    return np.stack(out, -1)
 
 
 def pack21(C):
