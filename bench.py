@@ -28,7 +28,8 @@ host thread, not extrapolated: the 32^3 sphere cell by FP64 FFT-PCG and a 16^3
 sphere cell by sparse direct (the reference's CPU-backend size limit), run in
 a child process while the GPU legs run.
 mfma_stencil: the same six-RHS apply on the matrix cores (vxl_cell_set_mfma),
-same buffers and events.  batch.graph_replay: the 64-cell batch again with the
+same buffers and events.  block_cg: the resident 256^3 solve (and the batch,
+batch.block_cg) with block CG over the six loads instead of six CG recurrences.  batch.graph_replay: the 64-cell batch again with the
 check blocks replayed from HIP graphs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
@@ -168,7 +169,7 @@ def cpu_baseline(sample, iterations_total, n_target, threads=1):
                         f"{n_target}^3, x{iterations_total} RHS-iterations of the GPU run; extrapolated"))
 
 
-def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8):
+def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg"):
     """Uneven batch: `count` two-phase random-inclusion cells (edges drawn from
     {32, 48, 64, 80, 96}, volume fraction U[0.1, 0.4], radius 2-5 voxels,
     contrast log-uniform in [10, 1000], nu = 0.3), six loads each, mixed
@@ -200,7 +201,7 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8):
             cell = Cell(field, ph.shape, dom / np.array(ph.shape[::-1], dtype=float), stream=pool[k % streams])
             got = tuner.tune(cell, "float32", 6, entries=entries)  # tunes only on a cache miss (warm-up)
             configs[ph.shape[0]] = (got["tile_y"], got["schedule"], got["zmin"])
-            built.append((cell, Solver(cell, lam, mu, mode="mixed", nrhs=6)))
+            built.append((cell, Solver(cell, lam, mu, mode="mixed", nrhs=6, method=method)))
         outs = run_batch([b[1] for b in built], tol=1e-8, workers=workers)
         for cell, solver in built:
             solver.close()
@@ -219,7 +220,8 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8):
     its = [int(i) for o in outs for i in o["iterations"]]
     worst = max(float(r) for o in outs for r in o["true_residuals"])
     voxels = int(sum(ph.size for _, ph, _, _ in resident))
-    out = dict(workload=f"random_inclusion_batch_{count}_cells_edges_32-96_6loads", cells=count, voxels=voxels,
+    out = dict(workload=f"random_inclusion_batch_{count}_cells_edges_32-96_6loads", method=method, cells=count,
+               voxels=voxels,
                cells_per_s=count / dt, ms_per_step=1e3 * dt, voxel_loads_per_s=6 * voxels / dt, steps=steps,
                warmup=warmup, streams=streams, workers=workers, iterations_mean=float(np.mean(its)),
                stencil_config_by_edge={str(k): dict(tile=f"32x{t}", schedule=v, zmin=z)
@@ -264,6 +266,7 @@ def main():
     ap.add_argument("--cpu-sample", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-end-to-end", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--skip-mfma", action="store_true", help="skip the matrix-core stencil A/B")
+    ap.add_argument("--skip-block", action="store_true", help="skip the block-CG A/B")
     ap.add_argument("--stencil", default="modal", choices=["modal", "mfma"],
                     help="FP32 operator of the timed mixed solve: modal VALU stencil or matrix-core form")
     ap.add_argument("--stencil-only", action="store_true",
@@ -440,6 +443,21 @@ def main():
         os.environ["VXL_GRAPH"] = saved
     resident_ms = resident["0"][0]
     solver.close()
+    # Block CG over the six loads on the same resident operator.
+    block = None
+    if not args.skip_block:
+        bsolver = Solver(cell, rmeta["lam"], rmeta["mu"], mode="mixed", nrhs=6, method="block")
+        bsolver.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        got = bsolver.run()
+        torch.cuda.synchronize()
+        block = dict(method="block", resident_ms_per_solve=1e3 * (time.perf_counter() - t0),
+                     iterations=[int(i) for i in got["iterations"]],
+                     ceff_rel_diff_vs_cg=float(np.linalg.norm(got["CH"] - CH_mixed) / np.linalg.norm(CH_mixed)))
+        bsolver.close()
+        print(f"# block CG, resident operator: {block['resident_ms_per_solve']:.1f} ms/solve, iterations "
+              f"{block['iterations']}", file=sys.stderr, flush=True)
     print(f"# mixed, resident operator: {resident_ms:.1f} ms/solve; graph replay {resident['1'][0]:.1f} ms "
           f"({resident['1'][1]} blocks)", file=sys.stderr, flush=True)
 
@@ -545,6 +563,8 @@ def main():
             os.environ["VXL_GRAPH"] = saved
         batch = legs["0"]
         batch["graph_replay"] = legs["1"]
+        if not args.skip_block:
+            batch["block_cg"] = batch_leg(steps=2, warmup=1, method="block")
 
     cpu_e2e = None
     if e2e_proc is not None:
@@ -581,6 +601,7 @@ def main():
                       ms_per_launch=t_apply * 1e3, algorithmic_bytes=alg_bytes),
         fp64=fp64,
         mfma_stencil=mfma,
+        block_cg=block,
         batch=batch,
         cpu_baseline=cpu,
         cpu_baseline_all_cores=cpu_all,

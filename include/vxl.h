@@ -45,6 +45,9 @@ enum { VXL_SOLVE_FP64 = 0, VXL_SOLVE_MIXED = 1 };
  * scalar Jacobi control. */
 enum { VXL_PRECOND_FFT = 0, VXL_PRECOND_JACOBI = 1, VXL_PRECOND_FFT_BF16 = 2 };
 enum { VXL_ELEMENT_FI = 0, VXL_ELEMENT_C3D8R_HG = 1 };
+/* Krylov method of a solver: one CG recurrence per load case sharing every
+ * launch (default), or one block CG over all load cases. */
+enum { VXL_METHOD_CG = 0, VXL_METHOD_BLOCK_CG = 1 };
 
 typedef struct vxl_cell vxl_cell;
 typedef struct vxl_precond vxl_precond;
@@ -280,6 +283,19 @@ int vxl_cell_inverse_diagonal(vxl_cell* cell, double* out_dev);
  * z = project_zero_mean(r / diag K) with the assembled diagonal of K; the
  * diagonal is assembled here on the device (once per solver). */
 int vxl_solver_set_preconditioner(vxl_solver* s, int kind);
+/* Krylov method of the solver's runs.  VXL_METHOD_CG (default): one
+ * preconditioned CG recurrence per load case, all cases sharing every launch
+ * (the reference's batched PCG, femlib/batched/pcg.py:20-252, rewritten).
+ * VXL_METHOD_BLOCK_CG: O'Leary's block PCG over the load cases (at most 8),
+ * one Krylov space for all of them: Q = K P, alpha = (P^T Q)^-1 P^T R,
+ * X += P alpha, R -= Q alpha, Z = M R, beta = -(P^T Q)^-1 Q^T Z,
+ * P = Z + P beta, with the R x R algebra on the host in FP64; directions
+ * whose Cholesky pivot of P^T Q falls below 1e-12 of its diagonal are
+ * dropped from the step (rank deficiency), certified systems leave the
+ * block, and the FP64 certificate and reliable updates are those of the
+ * default method.  FFT preconditioner only.  The reference's batched PCG
+ * solves each case separately; the block method is this build's. */
+int vxl_solver_set_method(vxl_solver* s, int method);
 /* Number of four-iteration blocks the last vxl_solver_run replayed from its
  * captured HIP graph (graph replay is opt-in with VXL_GRAPH=1; 0 without it,
  * with the fused forward pass, or when capture is unavailable).  No reference counterpart (the reference launches

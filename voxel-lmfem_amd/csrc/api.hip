@@ -27,6 +27,7 @@ int solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter
 int solver_release(vxl_solver* s);
 int solver_set_preconditioner(vxl_solver* s, int kind);
 int solver_prepare_graph(vxl_solver* s, double tol);
+int solver_set_method(vxl_solver* s, int method);
 
 namespace {
 
@@ -725,6 +726,13 @@ int vxl_solver_set_preconditioner(vxl_solver* s, int kind) {
   if (kind == VXL_PRECOND_FFT_BF16 && s->mode != VXL_SOLVE_MIXED)
     return fail("the BF16-stored preconditioner is a mixed-mode option");
   return solver_set_preconditioner(s, kind);
+}
+
+int vxl_solver_set_method(vxl_solver* s, int method) {
+  g_error.clear();
+  if (!s || (method != VXL_METHOD_CG && method != VXL_METHOD_BLOCK_CG))
+    return fail("vxl_solver_set_method: invalid arguments");
+  return solver_set_method(s, method);
 }
 
 int vxl_solver_graph_blocks(vxl_solver* s) {

@@ -122,6 +122,11 @@ int free_all(vxl_solver* s) {
   if (s->hist) dev_free(s->hist);
   if (s->gram) dev_free(s->gram);
   if (s->partial_cert) dev_free(s->partial_cert);
+  if (s->blk_partial) dev_free(s->blk_partial);
+  if (s->blk_small) dev_free(s->blk_small);
+  if (s->blk_keep) dev_free(s->blk_keep);
+  s->blk_partial = s->blk_small = nullptr;
+  s->blk_keep = nullptr;
   if (s->active_dev) dev_free(s->active_dev);
   if (s->restart_dev) dev_free(s->restart_dev);
   if (s->stop_dev) dev_free(s->stop_dev);
@@ -321,6 +326,73 @@ int prepare_block_graph(vxl_solver* s, double tol) {
   for (size_t i = 0; i < pend.size(); ++i) s->block_graph_pend[i] = pend[i];
   return VXL_OK;
 }
+
+// End of a run (both methods): mean stress of every system (the zero-mean
+// projection only for returned fields; B0 and the zero-mean RHS both
+// annihilate constants), outputs, per-phase times.  Systems that never started
+// (zero RHS) keep x64 = 0.
+template <typename Upload>
+int finish_run(vxl_solver* s, const double* macro_host, double* ch_host, int* iterations, int* restarts,
+               double* residuals, double* u_out, const std::vector<int>& its, const std::vector<int>& nrest,
+               const std::vector<double>& res_true, bool converged, Upload&& upload_flags,
+               std::vector<int>& active) {
+  vxl_cell* cell = s->cell;
+  hipStream_t st = cell->stream;
+  const int R = s->R;
+  const int64_t len = s->len;
+  const int64_t n = cell->grid.n;
+  double* partial = s->partial;
+  std::fill(active.begin(), active.end(), 1);
+  VXL_TRY(upload_flags());
+  if (u_out) VXL_TRY(launch_project_zero_mean(s->x64, n, R, partial, nullptr, st));
+  // Six unit strains: <sigma_i>(e_k) = <C>_ik - f_i . u_k / |Omega| (f_i = -sum_e
+  // V B0^T C_e e_i, the RHS of load i), two streaming passes instead of the
+  // element gather.
+  bool unit = R == 6;
+  for (int k = 0; unit && k < 6; ++k)
+    for (int i = 0; i < 6; ++i) unit = unit && macro_host[k * 6 + i] == (i == k ? 1.0 : 0.0);
+  std::vector<double> ch(6 * R);
+  double* gp = s->gram;                                 // 36 + 21 rows of kGramBlocks partials
+  double* gsum = s->gram + 57 * size_t(kGramBlocks);    // 36 + 21 totals
+  if (unit) {
+    const double omega = double(n) * cell->volume;
+    VXL_TRY(launch_gram6(s->f, s->x64, len, gp, -1.0 / omega, gsum, st));
+    VXL_TRY(launch_stiffness_sums(cell, gp + 36 * size_t(kGramBlocks), kGramBlocks, st));
+    VXL_TRY(launch_row_totals(gp + 36 * size_t(kGramBlocks), 21, kGramBlocks, 1.0 / double(n), gsum + 36, st));
+    std::vector<double> g(57);
+    VXL_HIP(hipMemcpyAsync(g.data(), gsum, sizeof(double) * 57, hipMemcpyDeviceToHost, st));
+    VXL_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 6; ++k)
+      for (int i = 0; i < 6; ++i) ch[k * 6 + i] = g[36 + pidx(i, k)] + g[i * 6 + k];
+  } else {
+    VXL_TRY(launch_avg_stress(cell, R, s->x64, s->macro_dev, s->stress_partial, s->ch_dev, st));
+    VXL_HIP(hipMemcpyAsync(ch.data(), s->ch_dev, sizeof(double) * 6 * R, hipMemcpyDeviceToHost, st));
+  }
+  if (u_out)
+    VXL_HIP(hipMemcpyAsync(u_out, s->x64, sizeof(double) * len * R, hipMemcpyDeviceToDevice, st));
+  VXL_HIP(hipEventRecord(s->ev[3], st));
+  VXL_HIP(hipStreamSynchronize(st));
+  for (int k = 0; k < R; ++k)
+    for (int i = 0; i < 6; ++i) ch_host[i * R + k] = ch[k * 6 + i];
+  for (int k = 0; k < R; ++k) {
+    if (iterations) iterations[k] = its[k];
+    if (restarts) restarts[k] = std::max(0, nrest[k] - 1);
+    if (residuals) residuals[k] = res_true[k];
+  }
+  float t0, t1, t2;
+  VXL_HIP(hipEventElapsedTime(&t0, s->ev[0], s->ev[1]));
+  VXL_HIP(hipEventElapsedTime(&t1, s->ev[1], s->ev[2]));
+  VXL_HIP(hipEventElapsedTime(&t2, s->ev[2], s->ev[3]));
+  s->ms[0] = t0;
+  s->ms[1] = t1;
+  s->ms[2] = t2;
+  if (!converged) {
+    set_error("CG did not certify ||f - K u||/||f|| <= tol within max_iter");
+    return VXL_ERR_CONVERGENCE;
+  }
+  return VXL_OK;
+}
+
 
 template <typename T>
 int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, double* ch_host,
@@ -572,59 +644,235 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   }
   VXL_TRY(flush());
   VXL_HIP(hipEventRecord(s->ev[2], st));
-  const bool converged = !any_active();
-  // Systems that never started (zero RHS) keep x64 = 0.  Mean stress of
-  // every system; the zero-mean projection only for returned fields (B0 and
-  // the zero-mean RHS both annihilate constants).
-  std::fill(active.begin(), active.end(), 1);
-  VXL_TRY(upload_flags());
-  if (u_out) VXL_TRY(launch_project_zero_mean(s->x64, n, R, partial, nullptr, st));
-  // Six unit strains: <sigma_i>(e_k) = <C>_ik - f_i . u_k / |Omega| (f_i = -sum_e
-  // V B0^T C_e e_i, the RHS of load i), two streaming passes instead of the
-  // element gather.
-  bool unit = R == 6;
-  for (int k = 0; unit && k < 6; ++k)
-    for (int i = 0; i < 6; ++i) unit = unit && macro_host[k * 6 + i] == (i == k ? 1.0 : 0.0);
-  std::vector<double> ch(6 * R);
-  double* gp = s->gram;                                 // 36 + 21 rows of kGramBlocks partials
-  double* gsum = s->gram + 57 * size_t(kGramBlocks);    // 36 + 21 totals
-  if (unit) {
-    const double omega = double(n) * cell->volume;
-    VXL_TRY(launch_gram6(s->f, s->x64, len, gp, -1.0 / omega, gsum, st));
-    VXL_TRY(launch_stiffness_sums(cell, gp + 36 * size_t(kGramBlocks), kGramBlocks, st));
-    VXL_TRY(launch_row_totals(gp + 36 * size_t(kGramBlocks), 21, kGramBlocks, 1.0 / double(n), gsum + 36, st));
-    std::vector<double> g(57);
-    VXL_HIP(hipMemcpyAsync(g.data(), gsum, sizeof(double) * 57, hipMemcpyDeviceToHost, st));
+  return finish_run(s, macro_host, ch_host, iterations, restarts, residuals, u_out, its, nrest, res_true,
+                    !any_active(), upload_flags, active);
+}
+
+// ---- block CG over the load cases ---------------------------------------------
+// Y[idx][j] = G[idx][idx]^-1 B[idx][j] for the columns j with cols[j] != 0 (G
+// symmetric, R x R row major; other entries of Y are 0).  idx starts as the
+// kept directions and loses every direction whose Cholesky pivot falls below
+// 1e-12 of its diagonal (a direction dependent on the others).
+void block_solve(const std::vector<double>& G, int R, std::vector<int>& idx, const std::vector<double>& B,
+                 const std::vector<int>& cols, double sign, std::vector<double>& Y) {
+  std::fill(Y.begin(), Y.end(), 0.0);
+  for (;;) {
+    const int m = int(idx.size());
+    if (m == 0) return;
+    std::vector<double> L(size_t(m) * m, 0.0);
+    int bad = -1;
+    for (int a = 0; a < m && bad < 0; ++a) {
+      for (int b = 0; b <= a; ++b) {
+        double v = 0.5 * (G[idx[a] * R + idx[b]] + G[idx[b] * R + idx[a]]);
+        for (int k = 0; k < b; ++k) v -= L[a * m + k] * L[b * m + k];
+        if (a == b) {
+          if (!(v > 1e-12 * std::fabs(G[idx[a] * R + idx[a]]))) {
+            bad = a;
+            break;
+          }
+          L[a * m + a] = std::sqrt(v);
+        } else {
+          L[a * m + b] = v / L[b * m + b];
+        }
+      }
+    }
+    if (bad >= 0) {
+      idx.erase(idx.begin() + bad);
+      continue;
+    }
+    for (int j = 0; j < R; ++j) {
+      if (!cols[j]) continue;
+      std::vector<double> y(m);
+      for (int a = 0; a < m; ++a) {  // L y = b
+        double v = B[idx[a] * R + j];
+        for (int k = 0; k < a; ++k) v -= L[a * m + k] * y[k];
+        y[a] = v / L[a * m + a];
+      }
+      for (int a = m - 1; a >= 0; --a) {  // L^T x = y
+        double v = y[a];
+        for (int k = a + 1; k < m; ++k) v -= L[k * m + a] * y[k];
+        y[a] = v / L[a * m + a];
+      }
+      for (int a = 0; a < m; ++a) Y[idx[a] * R + j] = sign * y[a];
+    }
+    return;
+  }
+}
+
+// O'Leary's block PCG over the R load cases, conjugate-direction form: with
+// Q = K P, alpha = (P^T Q)^-1 P^T R, X += P alpha, R -= Q alpha, Z = M R,
+// beta = -(P^T Q)^-1 Q^T Z and P = Z + P beta, all R x R matrices formed on the
+// host from FP64 Gram reductions.  A system whose residual certifies leaves
+// the block (its direction is zeroed); a direction that becomes dependent on
+// the others is dropped from the step.  Mixed mode runs the block recurrence
+// in FP32 on a correction x32 with the reliable updates of the batched
+// method (x64 += x32 and r = f - K x64 in FP64 when a residual has dropped
+// 10^3 times or reaches tol; the direction block is kept); FP64 mode updates
+// x64 directly and certifies the same way.
+template <typename T>
+int run_block_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, double* ch_host,
+                   int* iterations, int* restarts, double* residuals, double* u_out) {
+  vxl_cell* cell = s->cell;
+  hipStream_t st = cell->stream;
+  const int R = s->R;
+  const int64_t len = s->len;
+  const int64_t n = cell->grid.n;
+  const bool mixed = s->mode == VXL_SOLVE_MIXED;
+  if (R > block_max_rhs()) return fail("block CG handles at most 8 load cases per solver");
+  if (s->nslot < 2) return fail("block CG needs one more field buffer than could be allocated");
+  if (s->precond_kind == VXL_PRECOND_JACOBI) return fail("block CG runs with the FFT preconditioner");
+  T* X = mixed ? static_cast<T*>(s->x) : reinterpret_cast<T*>(s->x64);
+  T* r = static_cast<T*>(s->r);
+  T* P = static_cast<T*>(s->p);
+  T* Q = static_cast<T*>(s->q);
+  T* Z = static_cast<T*>(s->pring[1]);
+  double* scal = s->scal;
+  double* partial = s->partial;
+  int* act = s->active_dev;
+  double* G = s->blk_small;               // R x R Gram result
+  double* dalpha = s->blk_small + R * R;  // alpha or beta (device)
+  double* drr = s->blk_small + 2 * R * R; // ||r||^2 totals
+  const int gblk = block_gram_blocks(len);
+
+  std::vector<int> active(R, 1), restart_flags(R, 0), its(R, 0), nrest(R, 0);
+  std::vector<double> ff(R), rr(R), rr_ref(R), res_true(R, 0.0);
+  for (auto& t : s->trace) t.clear();
+  int* pin_flags = s->pinned_flags;
+  double* pin_scal = s->pinned_scal;
+  auto upload_flags = [&]() -> int {
+    VXL_HIP(hipEventSynchronize(s->flag_ev));
+    for (int k = 0; k < R; ++k) {
+      pin_flags[k] = active[k];
+      pin_flags[R + k] = 0;
+    }
+    VXL_HIP(hipMemcpyAsync(act, pin_flags, sizeof(int) * R, hipMemcpyHostToDevice, st));
+    VXL_HIP(hipMemcpyAsync(s->restart_dev, pin_flags + R, sizeof(int) * R, hipMemcpyHostToDevice, st));
+    VXL_HIP(hipEventRecord(s->flag_ev, st));
+    return VXL_OK;
+  };
+  auto read_doubles = [&](const double* src, int count, std::vector<double>& out) -> int {
+    out.resize(count);
+    VXL_HIP(hipMemcpyAsync(out.data(), src, sizeof(double) * count, hipMemcpyDeviceToHost, st));
     VXL_HIP(hipStreamSynchronize(st));
-    for (int k = 0; k < 6; ++k)
-      for (int i = 0; i < 6; ++i) ch[k * 6 + i] = g[36 + pidx(i, k)] + g[i * 6 + k];
-  } else {
-    VXL_TRY(launch_avg_stress(cell, R, s->x64, s->macro_dev, s->stress_partial, s->ch_dev, st));
-    VXL_HIP(hipMemcpyAsync(ch.data(), s->ch_dev, sizeof(double) * 6 * R, hipMemcpyDeviceToHost, st));
-  }
-  if (u_out)
-    VXL_HIP(hipMemcpyAsync(u_out, s->x64, sizeof(double) * len * R, hipMemcpyDeviceToDevice, st));
-  VXL_HIP(hipEventRecord(s->ev[3], st));
-  VXL_HIP(hipStreamSynchronize(st));
-  for (int k = 0; k < R; ++k)
-    for (int i = 0; i < 6; ++i) ch_host[i * R + k] = ch[k * 6 + i];
+    return VXL_OK;
+  };
+  auto upload_small = [&](const std::vector<double>& v) -> int {
+    VXL_HIP(hipMemcpyAsync(dalpha, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, st));
+    VXL_HIP(hipStreamSynchronize(st));  // v is host memory of this frame
+    return VXL_OK;
+  };
+  auto upload_keep = [&](const std::vector<int>& keep) -> int {
+    VXL_HIP(hipMemcpyAsync(s->blk_keep, keep.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+    VXL_HIP(hipStreamSynchronize(st));
+    return VXL_OK;
+  };
+  auto any_active = [&]() { return std::any_of(active.begin(), active.end(), [](int a) { return a; }); };
+  auto gram = [&](const T* A, const T* B, std::vector<double>& out) -> int {
+    VXL_TRY(launch_block_gram<T>(A, B, len, R, s->blk_partial, G, st));
+    return read_doubles(G, R * R, out);
+  };
+
+  VXL_HIP(hipEventRecord(s->ev[0], st));
+  VXL_HIP(hipMemcpyAsync(s->macro_dev, macro_host, sizeof(double) * 6 * R, hipMemcpyHostToDevice, st));
+  VXL_HIP(hipMemsetAsync(scal, 0, sizeof(double) * kNumSlots * R, st));
+  VXL_TRY(upload_flags());
+  VXL_TRY(launch_rhs(cell, R, s->macro_dev, s->f, s->rhs_sig, st));
+  int nblk_ff = 0;
+  VXL_TRY(launch_center_copy<T>(s->f, r, n, R, partial, &nblk_ff, st));
+  VXL_TRY(launch_finalize(partial + int64_t(3) * R * (vec_blocks(n) + 1), nblk_ff, kOpFF, scal, R, nullptr, nullptr,
+                          st));
+  VXL_TRY(read_doubles(scal + kFF * R, R, ff));
   for (int k = 0; k < R; ++k) {
-    if (iterations) iterations[k] = its[k];
-    if (restarts) restarts[k] = std::max(0, nrest[k] - 1);
-    if (residuals) residuals[k] = res_true[k];
+    if (!(ff[k] > 0.0)) active[k] = 0;  // zero RHS: exact zero solution (benchmark.py:79)
+    rr_ref[k] = ff[k];
   }
-  float t0, t1, t2;
-  VXL_HIP(hipEventElapsedTime(&t0, s->ev[0], s->ev[1]));
-  VXL_HIP(hipEventElapsedTime(&t1, s->ev[1], s->ev[2]));
-  VXL_HIP(hipEventElapsedTime(&t2, s->ev[2], s->ev[3]));
-  s->ms[0] = t0;
-  s->ms[1] = t1;
-  s->ms[2] = t2;
-  if (!converged) {
-    set_error("CG did not certify ||f - K u||/||f|| <= tol within max_iter");
-    return VXL_ERR_CONVERGENCE;
+  VXL_TRY(upload_flags());
+  VXL_TRY(launch_fill_zero<double>(s->x64, len, R, nullptr, st));
+  if (mixed) VXL_TRY(launch_fill_zero<float>(reinterpret_cast<float*>(s->x), len, R, nullptr, st));
+  VXL_HIP(hipEventRecord(s->ev[1], st));
+
+  std::vector<double> Gpq, Gpr, Gqz, coef(size_t(R) * R), rr_read;
+  std::vector<int> keep(R, 0);
+  bool fresh = true;  // P = M r to be formed (the first block)
+  int it = 0;
+  auto certify = [&]() -> int {
+    std::vector<int> mask = active;
+    if (mixed) VXL_TRY(launch_accumulate(s->x64, reinterpret_cast<float*>(s->x), len, R, act, st));
+    int rblocks = 0;
+    VXL_TRY(launch_residual_apply(cell, R, s->x64, s->f, mixed ? reinterpret_cast<float*>(s->r) : nullptr,
+                                  mixed ? nullptr : s->r64, act, s->partial_cert, &rblocks, st));
+    VXL_TRY(launch_finalize(s->partial_cert, rblocks, kOpStore, scal, R, act, nullptr, st));
+    std::vector<double> t;
+    VXL_TRY(read_doubles(scal + kTmp * R, R, t));
+    for (int k = 0; k < R; ++k) {
+      if (!mask[k]) continue;
+      const double rel = std::sqrt(t[k] / ff[k]);
+      if (!std::isfinite(rel)) return fail("Nonfinite true residual");
+      nrest[k] += 1;
+      s->trace[k].push_back({-it, rel});
+      rr_ref[k] = t[k];
+      if (rel <= tol) {
+        active[k] = 0;
+        res_true[k] = rel;
+      }
+    }
+    return upload_flags();
+  };
+
+  while (any_active() && it < max_iter) {
+    if (fresh) {  // P = Z = M r for the active systems, 0 for the others
+      VXL_TRY(precond_apply_masked(&s->pc, R, r, Z, act, st));
+      keep = active;
+      std::fill(coef.begin(), coef.end(), 0.0);
+      VXL_TRY(upload_small(coef));
+      VXL_TRY(upload_keep(keep));
+      VXL_TRY(launch_block_direction<T>(P, Z, dalpha, s->blk_keep, len, R, st));
+      fresh = false;
+    }
+    VXL_TRY(launch_apply<T>(cell, R, P, Q, st));
+    VXL_TRY(gram(P, Q, Gpq));
+    VXL_TRY(gram(P, r, Gpr));
+    std::vector<int> idx;
+    for (int k = 0; k < R; ++k)
+      if (keep[k]) idx.push_back(k);
+    block_solve(Gpq, R, idx, Gpr, active, 1.0, coef);  // alpha
+    VXL_TRY(upload_small(coef));
+    VXL_TRY(launch_block_update<T>(X, r, P, Q, dalpha, act, len, R, s->blk_partial, st));
+    VXL_TRY(launch_row_totals(s->blk_partial, R, gblk, 1.0, drr, st));
+    ++it;
+    VXL_TRY(read_doubles(drr, R, rr_read));
+    bool check = false;
+    for (int k = 0; k < R; ++k) {
+      if (!active[k]) continue;
+      if (!std::isfinite(rr_read[k])) return fail("Nonfinite residual in block CG recursion");
+      its[k] = it;
+      s->trace[k].push_back({it, std::sqrt(rr_read[k] / ff[k])});
+      if (rr_read[k] <= tol * tol * ff[k]) check = true;                         // reached tol
+      if (mixed && rr_read[k] < kReliable * kReliable * rr_ref[k]) check = true;  // reliable update
+    }
+    if (check) {
+      VXL_TRY(certify());  // r of every active system is now its true residual
+      if (!any_active()) break;
+    }
+    // Next direction block: beta = -(P^T Q)^-1 Q^T Z over the directions kept in
+    // the step, for every active system.  After a reliable update the block
+    // keeps its directions, as the per-case method keeps its search direction:
+    // the new directions are made K-conjugate to the last block from the true
+    // residuals.
+    VXL_TRY(precond_apply_masked(&s->pc, R, r, Z, act, st));
+    VXL_TRY(gram(Q, Z, Gqz));
+    block_solve(Gpq, R, idx, Gqz, active, -1.0, coef);
+    keep = active;
+    VXL_TRY(upload_small(coef));
+    VXL_TRY(upload_keep(keep));
+    VXL_TRY(launch_block_direction<T>(P, Z, dalpha, s->blk_keep, len, R, st));
   }
-  return VXL_OK;
+  VXL_HIP(hipEventRecord(s->ev[2], st));
+  if (!mixed) {
+    // x64 is X itself; nothing to flush.
+  }
+  return finish_run(s, macro_host, ch_host, iterations, restarts, residuals, u_out, its, nrest, res_true,
+                    !any_active(), upload_flags, active);
 }
 
 }  // namespace
@@ -706,11 +954,28 @@ int solver_create(vxl_solver* s) {
 int solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter, double* ch_host,
                int* iterations, int* restarts, double* residuals, double* u_out) {
   if (s->mode == VXL_SOLVE_MIXED)
-    return run_impl<float>(s, macro_host, tol, max_iter, ch_host, iterations, restarts, residuals, u_out);
-  return run_impl<double>(s, macro_host, tol, max_iter, ch_host, iterations, restarts, residuals, u_out);
+    return s->method == VXL_METHOD_BLOCK_CG
+               ? run_block_impl<float>(s, macro_host, tol, max_iter, ch_host, iterations, restarts, residuals, u_out)
+               : run_impl<float>(s, macro_host, tol, max_iter, ch_host, iterations, restarts, residuals, u_out);
+  return s->method == VXL_METHOD_BLOCK_CG
+             ? run_block_impl<double>(s, macro_host, tol, max_iter, ch_host, iterations, restarts, residuals, u_out)
+             : run_impl<double>(s, macro_host, tol, max_iter, ch_host, iterations, restarts, residuals, u_out);
 }
 
 int solver_release(vxl_solver* s) { return free_all(s); }
+
+int solver_set_method(vxl_solver* s, int method) {
+  if (method == VXL_METHOD_BLOCK_CG && !s->blk_partial) {
+    if (s->R > block_max_rhs()) return fail("block CG handles at most 8 load cases per solver");
+    if (s->nslot < 2) return fail("block CG needs one more field buffer than could be allocated");
+    const int64_t gb = block_gram_blocks(s->len);
+    VXL_HIP(dev_malloc(&s->blk_partial, sizeof(double) * size_t(s->R) * s->R * gb));
+    VXL_HIP(dev_malloc(&s->blk_small, sizeof(double) * size_t(2 * s->R * s->R + s->R)));
+    VXL_HIP(dev_malloc(&s->blk_keep, sizeof(int) * size_t(s->R)));
+  }
+  s->method = method;
+  return VXL_OK;
+}
 
 int solver_prepare_graph(vxl_solver* s, double tol) {
   if (!graphs_enabled() || s->graph_failed) return VXL_OK;

@@ -67,6 +67,23 @@ int launch_fill_zero(T* dst, int64_t len, int nrhs, const int* active, hipStream
 int launch_project_zero_mean(double* u, int64_t n, int nrhs, double* partial, const int* active,
                              hipStream_t s);
 
+// Block CG over the load cases (block.hip), at most block_max_rhs() systems.
+// out (R x R, device) = A^T B over len entries per system (partial: R * R *
+// block_gram_blocks(len) doubles); X_j += sum_i P_i alpha_ij (X may be null)
+// and R_j -= sum_i Q_i alpha_ij for active systems with ||R_j||^2 partials
+// rr (R * block_gram_blocks(len)); P_j = Z_j + sum_i P_i beta_ij for keep[j],
+// 0 otherwise.  alpha, beta: R x R row major (device).
+int block_max_rhs();
+int block_gram_blocks(int64_t len);
+template <typename T>
+int launch_block_gram(const T* A, const T* B, int64_t len, int R, double* partial, double* out, hipStream_t s);
+template <typename T>
+int launch_block_update(T* X, T* Rv, const T* P, const T* Q, const double* alpha, const int* active, int64_t len,
+                        int R, double* rr_partial, hipStream_t s);
+template <typename T>
+int launch_block_direction(T* P, const T* Z, const double* beta, const int* keep, int64_t len, int R,
+                           hipStream_t s);
+
 // Scalar Jacobi control preconditioner (jacobi.hip).  dinv = 1 / diag K per
 // DOF (component planes); one CG step forms z = dinv r - mean(dinv r), the
 // r.z partials, beta (kOpBeta) and p = z + beta p.  partial: >= 3 * nrhs *
@@ -185,6 +202,12 @@ struct vxl_solver {
   void* pring[vxl::kMaxDirections];
   double* hist;  // alpha_t per slot use and system, kMaxDirections x R
   double* gram;  // mean-stress partials and totals of the six-unit-strain path
+  // Block CG (method VXL_METHOD_BLOCK_CG): Gram partials, the small R x R
+  // matrices [G | alpha | beta] and ||r||^2 totals, direction flags.
+  int method;
+  double* blk_partial;
+  double* blk_small;
+  int* blk_keep;
   double* partial_cert;  // ||f - K x||^2 partials of the certification stencil
   // Per-system residual history of the last run: (iteration, relative
   // residual); a negative iteration marks an FP64 true-residual evaluation.

@@ -20,6 +20,7 @@ VXL_F32, VXL_F64, VXL_F32_BF16 = 0, 1, 2
 VXL_PRECOND_FFT, VXL_PRECOND_JACOBI, VXL_PRECOND_FFT_BF16 = 0, 1, 2
 VXL_ELEMENT_FI, VXL_ELEMENT_C3D8R_HG = 0, 1
 VXL_SOLVE_FP64, VXL_SOLVE_MIXED = 0, 1
+VXL_METHOD_CG, VXL_METHOD_BLOCK_CG = 0, 1
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -64,6 +65,7 @@ SIGNATURES = {
     "vxl_solver_run_batch": (_I, [_PP, _I, _DP, _D, _I, _DP, _IP, _IP, _DP, _I]),
     "vxl_cell_inverse_diagonal": (_I, [_P, _P]),
     "vxl_solver_set_preconditioner": (_I, [_P, _I]),
+    "vxl_solver_set_method": (_I, [_P, _I]),
     "vxl_solver_graph_blocks": (_I, [_P]),
     "vxl_solver_timings": (_I, [_P, _DP]),
     "vxl_solver_trace": (_I, [_P, _I, _IP, _DP, _I]),

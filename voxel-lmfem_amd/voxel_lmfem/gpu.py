@@ -360,7 +360,9 @@ class Solver:
     PRECONDITIONERS = {"fft": _native.VXL_PRECOND_FFT, "jacobi": _native.VXL_PRECOND_JACOBI,
                        "fft_bf16": _native.VXL_PRECOND_FFT_BF16}
 
-    def __init__(self, cell, lam, mu, *, mode="mixed", nrhs=6, preconditioner="fft"):
+    METHODS = {"cg": _native.VXL_METHOD_CG, "block": _native.VXL_METHOD_BLOCK_CG}
+
+    def __init__(self, cell, lam, mu, *, mode="mixed", nrhs=6, preconditioner="fft", method="cg"):
         self.cell = cell
         self.mode = mode
         code = {"mixed": _native.VXL_SOLVE_MIXED, "fp64": _native.VXL_SOLVE_FP64}.get(mode)
@@ -374,6 +376,18 @@ class Solver:
         self.preconditioner = "fft"
         if preconditioner != "fft":
             self.set_preconditioner(preconditioner)
+        self.method = "cg"
+        if method != "cg":
+            self.set_method(method)
+
+    def set_method(self, method):
+        """'cg' (one CG recurrence per load case, sharing every launch) or
+        'block' (block CG over all load cases, FFT preconditioner, <= 8 cases)."""
+        code = self.METHODS.get(method)
+        if code is None:
+            raise ValueError(f"method must be one of {sorted(self.METHODS)}")
+        check(self.cell.lib.vxl_solver_set_method(self.handle, code), "vxl_solver_set_method")
+        self.method = method
 
     def set_preconditioner(self, kind):
         """'fft' (Green operator of the reference medium), 'fft_bf16' (same,

@@ -21,6 +21,9 @@ preconditioner: "fft" (Green operator of the isotropic reference medium, the
 default), "fft_bf16" (same, spectra stored in BF16 between FFT passes; mixed
 precision only) or "jacobi" (1 / diag K, the reference's scalar control in
 repro/bench/chapter_aligned/study.py:84-90).
+solver: "cg" (one preconditioned CG recurrence per load case, all six sharing
+every launch; default) or "block" (block CG over the six loads: one Krylov
+space, R x R step and direction matrices; FFT preconditioner).
 stencil: "modal" (the modal VALU stencil, default) or "mfma" (the FP32
 operator of the mixed solve as label-grouped dense element matrices on the
 matrix cores, the reference's grouped-kernel form,
@@ -71,7 +74,7 @@ def _field_statistics(cell, u, phase, loads=(0, 5)):
 
 
 def _workload(phase, table, domain, orientation, precision, tol, local, fields=False, preconditioner="fft",
-              element="fi", hourglass_coefficient=0.1, max_iter=10000, stencil="modal"):
+              element="fi", hourglass_coefficient=0.1, max_iter=10000, stencil="modal", solver_method="cg"):
     from .gpu import Cell, Solver
     import torch
     torch.cuda.synchronize()
@@ -92,7 +95,7 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
         cell = Cell(field, phase.shape, h, element=element, hourglass_coefficient=hourglass_coefficient)
         if stencil == "mfma":
             cell.set_mfma(True)
-        solver = Solver(cell, lam, mu, mode=precision, nrhs=6, preconditioner=preconditioner)
+        solver = Solver(cell, lam, mu, mode=precision, nrhs=6, preconditioner=preconditioner, method=solver_method)
         stage("operator")
         out = solver.run(np.eye(6), tol=tol, return_fields=fields, max_iter=max_iter)
         stage("solve")
@@ -114,7 +117,8 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
     result = dict(seconds=seconds, CH=np.asarray(out["CH"]).tolist(), calls=calls, stages=stages,
                   field_statistics=field_stats, peak_allocated_bytes=int(torch.cuda.max_memory_allocated()),
                   peak_reserved_bytes=int(torch.cuda.max_memory_reserved()),
-                  device_ms=out["device_ms"], graph_blocks=out["graph_blocks"], stencil=stencil)
+                  device_ms=out["device_ms"], graph_blocks=out["graph_blocks"], stencil=stencil,
+                  solver=solver_method)
     rows = int(getattr(field, "row_count", len(field.rows)))
     storage = dict(mode="label_rows", n_elements=int(phase.size), pool_rows=rows, index_bytes=index_bytes,
                    coefficient_bytes=rows * 21 * 8,
@@ -131,7 +135,7 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
 
 def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="cuda", warmup=False,
                precision="mixed", tol=TOL, fields=False, preconditioner="fft", element="fi",
-               hourglass_coefficient=0.1, max_iter=10000, stencil="modal"):
+               hourglass_coefficient=0.1, max_iter=10000, stencil="modal", solver="cg"):
     """Return effective stiffness in engineering-Voigt order 11,22,33,23,13,12.
 
     max_iter caps the PCG iterations per load (the reference's 10,000,
@@ -159,6 +163,10 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="c
         raise ValueError("stencil must be 'modal' or 'mfma'")
     if stencil == "mfma" and element != "fi":
         raise ValueError("stencil='mfma' implements the FI element only")
+    if solver not in ("cg", "block"):
+        raise ValueError("solver must be 'cg' or 'block'")
+    if solver == "block" and preconditioner == "jacobi":
+        raise ValueError("solver='block' runs with the FFT preconditioner")
     phase, table, domain, orientation, labels, tensors, local = validate(
         phase_id, materials, domain_size, orientation)
     if backend == "cpu":
@@ -177,11 +185,11 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="c
         raise RuntimeError("No GPU is available. This backend runs only on an MI355X (gfx950).")
     if warmup:
         _workload(phase, table, domain, orientation, precision, tol, local, fields, preconditioner, element,
-                  hourglass_coefficient, int(max_iter), stencil)
+                  hourglass_coefficient, int(max_iter), stencil, solver)
     start = time.perf_counter()
     result, storage, operator = _workload(phase, table, domain, orientation, precision, tol, local, fields,
                                           preconditioner, element, hourglass_coefficient, int(max_iter),
-                                          stencil)
+                                          stencil, solver)
     result.update(backend=backend, storage=storage, operator=operator,
                   scope=(("FI HEX8" if element == "fi" else "C3D8R-HG") + " matrix-free stiffness; "
                          + {"fft": "FFT Green preconditioner; ", "fft_bf16": "FFT Green preconditioner (BF16 spectra); ",
