@@ -373,6 +373,9 @@ def build_field_device(phase_dev, materials, orientation=None):
     if with_local:
         if orientation is None and np.isin(labels[local], used).any():
             raise ValueError("Local-frame anisotropic phases require orientation")
+        if L + n > np.iinfo(np.int32).max:  # rows are addressed by an int32 index and an int row count
+            raise ValueError("Too many voxels for one coefficient row each (int32 row index); use build_field, "
+                             "which shares one row between voxels with equal frames")
         rows_dev = torch.zeros((L + n, 21), dtype=torch.float64, device=phase_dev.device)
         rows_dev[:L] = torch.as_tensor(phase_rows, device=phase_dev.device)
         if orientation is not None and np.isin(labels[local], used).any():
