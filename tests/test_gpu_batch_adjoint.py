@@ -77,6 +77,37 @@ def _J(C, W):
     return float((np.asarray(C) * W).sum())
 
 
+def test_gradients_when_fields_are_solved_again(torch_gpu):
+    """keep_fields=False drops the fields after the forward solve and solves
+    again in backward: the same gradient, bit for bit (density and angle)."""
+    torch = torch_gpu
+    from voxel_lmfem import adjoint, materials as pm
+    from voxel_lmfem.cells import simp_density, woven_like
+    print("case adjoint keep_fields False vs True", flush=True)
+    rho, C1, _, _ = simp_density(12, sigma=2.0, seed=5)
+    grads = []
+    for keep in (True, False):
+        r = torch.tensor(rho, dtype=torch.float64, requires_grad=True)
+        C = adjoint.density_ceff(r, np.ones(3), C1=C1, keep_fields=keep)
+        (C[0, 0] + C[:3, :3].sum() / 9.0).backward()
+        grads.append(r.grad.numpy().copy())
+    assert np.array_equal(grads[0], grads[1])
+    (phase, mats, dom, ori), _ = woven_like(16, section=(2, 4), amplitude=1.0)
+    labels, tensors, local = pm.phase_table(mats)
+    slot = np.searchsorted(labels, phase.reshape(-1))
+    mask = local[slot]
+    idx = np.flatnonzero(mask)
+    frames = pm.frames(ori, phase.shape, idx)
+    iso = pm.pack21(tensors[slot])
+    grads = []
+    for keep in (True, False):
+        theta = torch.zeros(phase.shape, dtype=torch.float64, requires_grad=True)
+        C = adjoint.angle_ceff(theta, dom, tensors[slot[idx[0]]], frames, iso, mask, keep_fields=keep)
+        (C[0, 0] + C[:3, :3].sum() / 9.0).backward()
+        grads.append(theta.grad.numpy().copy())
+    assert np.array_equal(grads[0], grads[1]) and grads[0].any()
+
+
 @pytest.mark.parametrize("n,voxels", [(12, 6), (96, 20)])
 def test_density_gradient_matches_central_differences(torch_gpu, n, voxels):
     """SIMP density cell, J = C_1111 + K_V(C_eff): adjoint vs central FD (step 1e-4) at seeded voxels."""

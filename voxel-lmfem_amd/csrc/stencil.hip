@@ -842,41 +842,49 @@ rhs_direct_kernel(const void* index, const double* table, const float* scale, co
 }
 
 // Node-owned gather of the 8 precomputed element stresses around each node.
+// Grid (ceil(nx ny / 256), min(nz, 65535)): planes come from blockIdx.y (in
+// steps of gridDim.y) and the in-plane position from 32-bit arithmetic
+// (nx ny < 2^31), so no 64-bit division is left.
 template <typename Idx, bool kScale>
 __global__ void __launch_bounds__(256)
 rhs_kernel(const void* index, const double* sig, const float* scale, int nload, double* f, int nx, int ny,
            int nz, double gx, double gy, double gz, double volume) {
   const int64_t nxy = int64_t(nx) * ny, n = nxy * nz;
-  const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (node >= n) return;
-  const int x = int(node % nx), y = int((node / nx) % ny), z = int(node / nxy);
-  int64_t base[8];
-  double sc[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int ax = q & 1, ay = (q >> 1) & 1, az = q >> 2;
-    const int64_t e = int64_t(wrapmod(z - az, nz)) * nxy + int64_t(wrapmod(y - ay, ny)) * nx + wrapmod(x - ax, nx);
-    base[q] = int64_t(load_row<Idx>(index, e)) * nload * 6;
-    sc[q] = kScale ? double(scale[e]) : 1.0;
-  }
-  for (int r = 0; r < nload; ++r) {
-    double fx = 0, fy = 0, fz = 0;
+  const int pxy = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+  if (pxy >= int(nxy)) return;
+  const int y = pxy / nx, x = pxy - y * nx;
+  const int ym = y == 0 ? ny - 1 : y - 1, xm = x == 0 ? nx - 1 : x - 1;
+  for (int z = int(blockIdx.y); z < nz; z += int(gridDim.y)) {
+    const int64_t node = int64_t(z) * nxy + pxy;
+    const int zm = z == 0 ? nz - 1 : z - 1;
+    int64_t base[8];
+    double sc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int ax = q & 1, ay = (q >> 1) & 1, az = q >> 2;
-      const double* sp = sig + base[q] + 6 * r;
-      double s[6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) s[i] = kScale ? sp[i] * sc[q] : sp[i];
-      const double dx = ax ? gx : -gx, dy = ay ? gy : -gy, dz = az ? gz : -gz;
-      fx -= volume * (dx * s[0] + dy * s[5] + dz * s[4]);
-      fy -= volume * (dx * s[5] + dy * s[1] + dz * s[3]);
-      fz -= volume * (dx * s[4] + dy * s[3] + dz * s[2]);
+      const int64_t e = int64_t(az ? zm : z) * nxy + int64_t(ay ? ym : y) * nx + (ax ? xm : x);
+      base[q] = int64_t(load_row<Idx>(index, e)) * nload * 6;
+      sc[q] = kScale ? double(scale[e]) : 1.0;
     }
-    double* fr = f + int64_t(r) * 3 * n;
-    fr[node] = fx;
-    fr[n + node] = fy;
-    fr[2 * n + node] = fz;
+    for (int r = 0; r < nload; ++r) {
+      double fx = 0, fy = 0, fz = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int ax = q & 1, ay = (q >> 1) & 1, az = q >> 2;
+        const double* sp = sig + base[q] + 6 * r;
+        double s[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] = kScale ? sp[i] * sc[q] : sp[i];
+        const double dx = ax ? gx : -gx, dy = ay ? gy : -gy, dz = az ? gz : -gz;
+        fx -= volume * (dx * s[0] + dy * s[5] + dz * s[4]);
+        fy -= volume * (dx * s[5] + dy * s[1] + dz * s[3]);
+        fz -= volume * (dx * s[4] + dy * s[3] + dz * s[2]);
+      }
+      double* fr = f + int64_t(r) * 3 * n;
+      fr[node] = fx;
+      fr[n + node] = fy;
+      fr[2 * n + node] = fz;
+    }
   }
 }
 
@@ -1177,6 +1185,7 @@ int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double*
   const auto& g = cell->grid;
   const int blocks = int((g.n + 255) / 256);
   const double gx = 0.25 / cell->h[0], gy = 0.25 / cell->h[1], gz = 0.25 / cell->h[2];
+  const dim3 gplane(unsigned((int64_t(g.nx) * g.ny + 255) / 256), unsigned(std::min(g.nz, 65535)));
   if (sig) {
     const int64_t work = int64_t(cell->rows) * nload;
     hipLaunchKernelGGL(row_stress_kernel, dim3(unsigned((work + 255) / 256)), dim3(256), 0, s, cell->mat.c64,
@@ -1195,10 +1204,10 @@ int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double*
                            cell->mat.c64, cell->mat.scale, macro_dev, nload, f, g.nx, g.ny, g.nz, gx, gy, gz,
                            cell->volume);
     } else if (cell->mat.scale)
-      hipLaunchKernelGGL((rhs_kernel<Idx, true>), dim3(blocks), dim3(256), 0, s, cell->mat.index, sig,
+      hipLaunchKernelGGL((rhs_kernel<Idx, true>), gplane, dim3(256), 0, s, cell->mat.index, sig,
                          cell->mat.scale, nload, f, g.nx, g.ny, g.nz, gx, gy, gz, cell->volume);
     else
-      hipLaunchKernelGGL((rhs_kernel<Idx, false>), dim3(blocks), dim3(256), 0, s, cell->mat.index, sig,
+      hipLaunchKernelGGL((rhs_kernel<Idx, false>), gplane, dim3(256), 0, s, cell->mat.index, sig,
                          cell->mat.scale, nload, f, g.nx, g.ny, g.nz, gx, gy, gz, cell->volume);
     VXL_HIP(hipGetLastError());
     return VXL_OK;

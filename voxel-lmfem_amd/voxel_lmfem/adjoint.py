@@ -19,6 +19,12 @@ the fields the solve already holds.  Two parameterisations:
 
 Every element gets its own FP64 coefficient row (int32 index), so C_e is exact
 in FP64 and finite differences of C_eff see the same model.
+
+keep_fields: the six FP64 fields (144 B per voxel) and the cell stay on the
+device between forward and backward (True), or only the host rows are kept and
+backward solves again (False; the solve is deterministic, so the gradient is
+bit-identical).  None (default) keeps them when they take at most a quarter of
+the device memory free after the forward solve.
 """
 import ctypes
 
@@ -66,6 +72,14 @@ def _solve(rows, shape, h, precision, tol):
     return cell, out
 
 
+def _keep(cell, keep_fields):
+    """Whether to hold the fields until backward (see the module note)."""
+    if keep_fields is not None:
+        return bool(keep_fields)
+    free, _ = cell.torch.cuda.mem_get_info(cell.device)
+    return 6 * 3 * 8 * cell.n <= free // 4
+
+
 def _contract(cell, u, W, d21=None):
     """out[e] = sum_ij W_ij int_e eps_i^T D_e eps_j (D_e: cell row, or d21[e])."""
     torch = cell.torch
@@ -86,12 +100,17 @@ def _torch_function():
 
     class DensityCeff(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, rho, shape, h, C1, E_min, E_max, p, precision, tol):
+        def forward(ctx, rho, shape, h, C1, E_min, E_max, p, precision, tol, keep_fields):
             r = rho.detach().cpu().numpy().astype(np.float64).reshape(-1)
             E = E_min + r ** p * (E_max - E_min)
             rows = E[:, None] * mat.pack21(C1)[None, :]
             cell, out = _solve(rows, shape, h, precision, tol)
-            ctx.cell, ctx.u = cell, out["fields"]
+            ctx.solve_args = (rows, shape, h, precision, tol)
+            if _keep(cell, keep_fields):
+                ctx.cell, ctx.u = cell, out["fields"]
+            else:  # backward solves again
+                cell.close()
+                ctx.cell = ctx.u = None
             ctx.dE = p * r ** (p - 1) * (E_max - E_min)
             ctx.E = E
             ctx.volume = float(np.prod(h)) * cell.n
@@ -101,43 +120,58 @@ def _torch_function():
         @staticmethod
         def backward(ctx, grad):
             # rows = E_e C1, so the row contraction is E_e int eps^T C1 eps.
+            if ctx.cell is None:
+                ctx.cell, out = _solve(*ctx.solve_args)
+                ctx.u = out["fields"]
             g = _contract(ctx.cell, ctx.u, grad.detach().cpu().numpy()).cpu().numpy()
             d = ctx.dE * g / (ctx.E * ctx.volume)
             ctx.cell.close()
             shape, dtype, device = ctx.rho_meta
-            return (torch.as_tensor(d.reshape(shape), dtype=dtype, device=device),) + (None,) * 8
+            return (torch.as_tensor(d.reshape(shape), dtype=dtype, device=device),) + (None,) * 9
 
     class AngleCeff(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, theta, shape, h, C_loc, frames, iso_rows, local_mask, precision, tol):
+        def forward(ctx, theta, shape, h, C_loc, frames, iso_rows, local_mask, precision, tol, keep_fields):
             t = theta.detach().cpu().numpy().astype(np.float64).reshape(-1)
             rows, drows = angle_rows(t, C_loc, frames, iso_rows, local_mask)
             cell, out = _solve(rows, shape, h, precision, tol)
-            ctx.cell, ctx.u, ctx.drows = cell, out["fields"], drows
+            ctx.drows = drows
+            ctx.solve_args = (rows, shape, h, precision, tol)
+            if _keep(cell, keep_fields):
+                ctx.cell, ctx.u = cell, out["fields"]
+            else:  # backward solves again
+                cell.close()
+                ctx.cell = ctx.u = None
             ctx.volume = float(np.prod(h)) * cell.n
             ctx.meta = (theta.shape, theta.dtype, theta.device)
             return torch.as_tensor(out["CH"], dtype=torch.float64)
 
         @staticmethod
         def backward(ctx, grad):
+            if ctx.cell is None:
+                ctx.cell, out = _solve(*ctx.solve_args)
+                ctx.u = out["fields"]
             g = _contract(ctx.cell, ctx.u, grad.detach().cpu().numpy(), d21=ctx.drows).cpu().numpy()
             ctx.cell.close()
             shape, dtype, device = ctx.meta
-            return (torch.as_tensor((g / ctx.volume).reshape(shape), dtype=dtype, device=device),) + (None,) * 8
+            return (torch.as_tensor((g / ctx.volume).reshape(shape), dtype=dtype, device=device),) + (None,) * 9
 
     return DensityCeff, AngleCeff
 
 
-def density_ceff(rho, domain, *, C1=None, nu=0.3, E_min=1e-3, E_max=70.0, p=3.0, precision="fp64", tol=1e-8):
+def density_ceff(rho, domain, *, C1=None, nu=0.3, E_min=1e-3, E_max=70.0, p=3.0, precision="fp64", tol=1e-8,
+                 keep_fields=None):
     """C_eff (6x6 torch FP64) of a SIMP density cell rho (nz, ny, nx) torch
-    tensor, differentiable with respect to rho through the adjoint."""
+    tensor, differentiable with respect to rho through the adjoint.
+    keep_fields: hold the fields for backward (True), solve again in backward
+    (False), or decide from free device memory (None)."""
     DensityCeff, _ = _torch_function()
     shape = tuple(int(v) for v in rho.shape)
     h = np.asarray(domain, dtype=float) / np.array(shape[::-1], dtype=float)
     if C1 is None:
         C1 = mat.isotropic_voigt(1.0, nu)
     return DensityCeff.apply(rho, shape, h, np.asarray(C1, dtype=float), float(E_min), float(E_max), float(p),
-                             precision, tol)
+                             precision, tol, keep_fields)
 
 
 def _rz(t):
@@ -195,13 +229,14 @@ def angle_rows(theta, C_loc, frames, iso_rows, local_mask):
     return rows, drows
 
 
-def angle_ceff(theta, domain, C_loc, frames, iso_rows, local_mask, *, precision="fp64", tol=1e-8):
+def angle_ceff(theta, domain, C_loc, frames, iso_rows, local_mask, *, precision="fp64", tol=1e-8,
+               keep_fields=None):
     """C_eff (6x6 torch FP64) as a function of per-voxel fiber angles theta
     (nz, ny, nx) torch tensor (radians, 0 = the given frames); voxels outside
-    local_mask keep iso_rows."""
+    local_mask keep iso_rows.  keep_fields as in density_ceff."""
     _, AngleCeff = _torch_function()
     shape = tuple(int(v) for v in theta.shape)
     h = np.asarray(domain, dtype=float) / np.array(shape[::-1], dtype=float)
     return AngleCeff.apply(theta, shape, h, np.asarray(C_loc, dtype=float), np.asarray(frames, dtype=float),
                            np.asarray(iso_rows, dtype=float), np.asarray(local_mask, dtype=bool).reshape(-1),
-                           precision, tol)
+                           precision, tol, keep_fields)
