@@ -26,7 +26,8 @@ core at once, rates summed (the oracle's element loop is serial, so this is
 how the host's cores are all used).  cpu_end_to_end: whole oracle solves on one
 host thread, not extrapolated: the 32^3 sphere cell by FP64 FFT-PCG and a 16^3
 sphere cell by sparse direct (the reference's CPU-backend size limit), run in
-a child process while the GPU legs run.
+a child process while the GPU legs run; gpu_small_cells: the same two cells
+solved on the GPU (set-up included), FP64 and mixed.
 mfma_stencil: the same six-RHS apply on the matrix cores (vxl_cell_set_mfma),
 same buffers and events.  block_cg: the resident 256^3 solve (and the batch,
 batch.block_cg) with block CG over the six loads instead of six CG recurrences.  batch.graph_replay: the 64-cell batch again with the
@@ -267,6 +268,7 @@ def main():
     ap.add_argument("--cpu-end-to-end", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--skip-mfma", action="store_true", help="skip the matrix-core stencil A/B")
     ap.add_argument("--skip-block", action="store_true", help="skip the block-CG A/B")
+    ap.add_argument("--skip-small", action="store_true", help="skip the 32^3 / 16^3 sphere cells on the GPU")
     ap.add_argument("--stencil", default="modal", choices=["modal", "mfma"],
                     help="FP32 operator of the timed mixed solve: modal VALU stencil or matrix-core form")
     ap.add_argument("--stencil-only", action="store_true",
@@ -566,6 +568,36 @@ def main():
         if not args.skip_block:
             batch["block_cg"] = batch_leg(steps=2, warmup=1, method="block")
 
+    # The same small cells on the GPU (set-up included, as a step above): the
+    # 32^3 sphere cell by FP64 and mixed PCG, and the 16^3 sphere cell.
+    small = None
+    if not args.skip_small:
+        from voxel_lmfem.cells import periodic_spheres
+        small = {}
+        for name, (nn, rad) in (("c1_32", (32, 3.0)), ("spheres_16", (16, 1.5))):
+            (sph, smats, sdom, _), _ = periodic_spheres(nn, radius=rad, seed=1)
+            slab = torch.from_numpy(sph.astype(np.int32)).to("cuda")
+            for mode in ("fp64", "mixed"):
+                def small_step():
+                    field = pm.build_field_device(slab, smats)
+                    lam, mu, _ = select(field, sph, smats)
+                    c = Cell(field, sph.shape, sdom / np.array(sph.shape[::-1], dtype=float))
+                    sv = Solver(c, lam, mu, mode=mode, nrhs=6)
+                    o = sv.run()
+                    sv.close()
+                    c.close()
+                    return o
+                small_step()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(10):
+                    o = small_step()
+                torch.cuda.synchronize()
+                dt = (time.perf_counter() - t0) / 10
+                small[f"{name}_{mode}"] = dict(ms_per_solve=1e3 * dt, solves_per_s=1.0 / dt,
+                                               iterations=[int(i) for i in o["iterations"]])
+        print(f"# small cells on the GPU: {small}", file=sys.stderr, flush=True)
+
     cpu_e2e = None
     if e2e_proc is not None:
         text, _ = e2e_proc.communicate(timeout=900)
@@ -606,6 +638,7 @@ def main():
         cpu_baseline=cpu,
         cpu_baseline_all_cores=cpu_all,
         cpu_end_to_end=cpu_e2e,
+        gpu_small_cells=small,
     )
     print(json.dumps(line), flush=True)
 
