@@ -107,9 +107,10 @@ def cpu_end_to_end():
     its = [int(c["iterations"]) for c in r["calls"]] if r.get("calls") and "iterations" in r["calls"][0] else None
     (phase, mats, dom, _), _ = periodic_spheres(16, radius=1.5, seed=1)
     t0 = time.perf_counter()
-    oracle.homogenize_direct(phase, mats, dom)
+    d = oracle.homogenize_direct(phase, mats, dom)
     t_d16 = time.perf_counter() - t0
-    return dict(c1_seconds=t_c1, c1_iterations=its, direct16_seconds=t_d16)
+    return dict(c1_seconds=t_c1, c1_iterations=its, direct16_seconds=t_d16,
+                c1_CH=np.asarray(r["CH"]).tolist(), direct16_CH=np.asarray(d["CH"]).tolist())
 
 
 def _child_env(threads):
@@ -595,8 +596,10 @@ def main():
                 torch.cuda.synchronize()
                 dt = (time.perf_counter() - t0) / 10
                 small[f"{name}_{mode}"] = dict(ms_per_solve=1e3 * dt, solves_per_s=1.0 / dt,
-                                               iterations=[int(i) for i in o["iterations"]])
-        print(f"# small cells on the GPU: {small}", file=sys.stderr, flush=True)
+                                               iterations=[int(i) for i in o["iterations"]],
+                                               CH=np.asarray(o["CH"]).tolist())
+        print("# small cells on the GPU: " + ", ".join(f"{k} {v['ms_per_solve']:.2f} ms" for k, v in small.items()),
+              file=sys.stderr, flush=True)
 
     cpu_e2e = None
     if e2e_proc is not None:
@@ -610,7 +613,16 @@ def main():
                 direct16=dict(value=1.0 / got["direct16_seconds"], unit="solves/s", cores=1, kind="port",
                               sample=("sparse-direct (SuperLU) C_eff of a 16^3 sphere cell, the reference's "
                                       f"CPU-backend path, on one host thread: {got['direct16_seconds']:.1f} s")))
-            print(f"# cpu end-to-end: {got}", file=sys.stderr, flush=True)
+            # C_eff of the GPU solves against the FP64 CPU oracle on the same cells.
+            if small is not None:
+                for name, key in (("c1_32", "c1_CH"), ("spheres_16", "direct16_CH")):
+                    ref = np.asarray(got[key])
+                    for mode in ("fp64", "mixed"):
+                        leg = small[f"{name}_{mode}"]
+                        leg["ceff_rel_err_vs_cpu_oracle"] = float(np.linalg.norm(np.asarray(leg["CH"]) - ref) /
+                                                                  np.linalg.norm(ref))
+            print(f"# cpu end-to-end: c1 {got['c1_seconds']:.1f} s, direct16 {got['direct16_seconds']:.1f} s",
+                  file=sys.stderr, flush=True)
 
     traffic, traffic_src = measured_traffic()
     line = dict(
