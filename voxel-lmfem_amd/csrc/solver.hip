@@ -965,20 +965,20 @@ int solver_run(vxl_solver* s, const double* macro_host, double tol, int max_iter
 int solver_release(vxl_solver* s) { return free_all(s); }
 
 int solver_set_method(vxl_solver* s, int method) {
-  if (method == VXL_METHOD_BLOCK_CG && !s->blk_partial) {
+  if (method == VXL_METHOD_BLOCK_CG && !(s->blk_partial && s->blk_small && s->blk_keep)) {
     if (s->R > block_max_rhs()) return fail("block CG handles at most 8 load cases per solver");
     if (s->nslot < 2) return fail("block CG needs one more field buffer than could be allocated");
     const int64_t gb = block_gram_blocks(s->len);
-    VXL_HIP(dev_malloc(&s->blk_partial, sizeof(double) * size_t(s->R) * s->R * gb));
-    VXL_HIP(dev_malloc(&s->blk_small, sizeof(double) * size_t(2 * s->R * s->R + s->R)));
-    VXL_HIP(dev_malloc(&s->blk_keep, sizeof(int) * size_t(s->R)));
+    if (!s->blk_partial) VXL_HIP(dev_malloc(&s->blk_partial, sizeof(double) * size_t(s->R) * s->R * gb));
+    if (!s->blk_small) VXL_HIP(dev_malloc(&s->blk_small, sizeof(double) * size_t(2 * s->R * s->R + s->R)));
+    if (!s->blk_keep) VXL_HIP(dev_malloc(&s->blk_keep, sizeof(int) * size_t(s->R)));
   }
   s->method = method;
   return VXL_OK;
 }
 
 int solver_prepare_graph(vxl_solver* s, double tol) {
-  if (!graphs_enabled() || s->graph_failed) return VXL_OK;
+  if (!graphs_enabled() || s->graph_failed || s->method != VXL_METHOD_CG) return VXL_OK;
   return s->mode == VXL_SOLVE_MIXED ? prepare_block_graph<float>(s, tol) : prepare_block_graph<double>(s, tol);
 }
 
