@@ -195,8 +195,11 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg"):
     torch.cuda.synchronize()
     configs = {}
 
+    split = {"setup": 0.0, "solve": 0.0, "teardown": 0.0}
+
     def step():
         built = []
+        t_a = time.perf_counter()
         for k, (lab, ph, mats, dom) in enumerate(resident):
             field = pm.build_field_device(lab, mats)
             lam, mu, _ = select(field, ph, mats)
@@ -204,14 +207,22 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg"):
             got = tuner.tune(cell, "float32", 6, entries=entries)  # tunes only on a cache miss (warm-up)
             configs[ph.shape[0]] = (got["tile_y"], got["schedule"], got["zmin"])
             built.append((cell, Solver(cell, lam, mu, mode="mixed", nrhs=6, method=method)))
+        t_b = time.perf_counter()
         outs = run_batch([b[1] for b in built], tol=1e-8, workers=workers)
+        t_c = time.perf_counter()
         for cell, solver in built:
             solver.close()
             cell.close()
+        t_d = time.perf_counter()
+        split["setup"] += t_b - t_a
+        split["solve"] += t_c - t_b
+        split["teardown"] += t_d - t_c
         return outs
 
     for _ in range(warmup):
         step()
+    for key in split:
+        split[key] = 0.0
     tuner.save(entries, strict=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -229,9 +240,12 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg"):
                stencil_config_by_edge={str(k): dict(tile=f"32x{t}", schedule=v, zmin=z)
                                        for k, (t, v, z) in sorted(configs.items())},
                graph_blocks=int(sum(o["graph_blocks"] for o in outs)),
+               host_split_ms={k: 1e3 * v / steps for k, v in split.items()},
                iterations_max=max(its), max_true_residual=worst)
     print(f"# batch: {1e3 * dt:.1f} ms per {count}-cell batch ({count / dt:.1f} cells/s), iterations mean "
-          f"{np.mean(its):.1f} max {max(its)}, worst residual {worst:.2e}", file=sys.stderr, flush=True)
+          f"{np.mean(its):.1f} max {max(its)}, worst residual {worst:.2e}; set-up "
+          f"{1e3 * split['setup'] / steps:.0f} ms, solves {1e3 * split['solve'] / steps:.0f} ms, teardown "
+          f"{1e3 * split['teardown'] / steps:.0f} ms", file=sys.stderr, flush=True)
     return out
 
 

@@ -178,13 +178,19 @@ def test_center_fields_match_oracle(torch_gpu, shape):
 
 
 PC_SHAPES = [(4, 4, 4), (8, 8, 8), (6, 5, 4), (3, 7, 5), (16, 12, 10), (17, 2, 9), (32, 48, 20),
-             (5, 3, 16), (3, 6, 64), (2, 5, 128), (3, 3, 256), (2, 3, 512)]
+             (5, 3, 16), (3, 6, 64), (2, 5, 128), (3, 3, 256), (2, 3, 512),
+             # four-step passes of 2^a times 3 or 5 on every axis
+             (24, 40, 48), (48, 24, 80), (96, 5, 40), (160, 3, 24), (8, 80, 96), (192, 2, 8), (2, 192, 8),
+             (3, 4, 160), (2, 3, 192)]
+FOUR_STEP = {8, 16, 32, 64, 128, 256, 512, 24, 40, 48, 80, 96, 160, 192}
 
 
 @pytest.mark.parametrize("shape", PC_SHAPES)
 def test_precond_matches_oracle(torch_gpu, shape, monkeypatch):
-    """Power-of-two nx >= 8 runs the four-step x passes; they are also checked
-    against the Stockham x passes (VXL_FFT_XFOURSTEP=0) on the same input."""
+    """nx with a four-step factorisation (2^a >= 8, or 2^a times 3 or 5 from 24
+    to 192) runs the four-step x passes; they are also checked against the
+    Stockham x passes (VXL_FFT_XFOURSTEP=0) on the same input.  y and z axes
+    with a factorisation run the four-step y and z passes."""
     torch = torch_gpu
     from voxel_lmfem.gpu import Cell, Preconditioner
     print(f"case precond shape={shape}", flush=True)
@@ -196,8 +202,7 @@ def test_precond_matches_oracle(torch_gpu, shape, monkeypatch):
     R -= R.mean(axis=1, keepdims=True)
     ref = np.stack([planes(spectral.green_apply(r, shape, h, lam, mu)) for r in R])
     r = torch.tensor(np.stack([planes(x) for x in R]), device="cuda")
-    pow2 = shape[2] >= 8 and shape[2] & (shape[2] - 1) == 0
-    for xfour in (("1", "0") if pow2 else ("1",)):
+    for xfour in (("1", "0") if shape[2] in FOUR_STEP else ("1",)):
         monkeypatch.setenv("VXL_FFT_XFOURSTEP", xfour)
         for precision, tol in (("float64", 1e-11), ("float32", 5e-6)):
             pc = Preconditioner(cell, lam, mu, precision=precision, nrhs_max=3)

@@ -129,11 +129,11 @@ def test_bf16_spectra_apply(torch_gpu, shape):
     cell.close()
 
 
-def test_bf16_spectra_rejects_non_power_of_two(torch_gpu):
+def test_bf16_spectra_rejects_axes_without_four_step(torch_gpu):
     from voxel_lmfem.gpu import Cell, Preconditioner
     field, _, h = random_cell((8, 8, 12), seed=1)
     cell = Cell(field, (8, 8, 12), h)
-    with pytest.raises(ValueError, match="power-of-two"):
+    with pytest.raises(ValueError, match="four-step factorisation"):
         Preconditioner(cell, 2.0, 1.0, precision="bf16")
     cell.close()
 

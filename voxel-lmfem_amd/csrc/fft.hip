@@ -153,6 +153,25 @@ __device__ __forceinline__ void dft3(C* v) {
   v[1] = cadd(m, jb);
   v[2] = csub(m, jb);
 }
+// Radix 5: with a1 = x1 + x4, b1 = x1 - x4, a2 = x2 + x3, b2 = x2 - x3 and
+// theta = 2 pi / 5, X1,4 = x0 + a1 cos(theta) + a2 cos(2 theta) -+ i (b1 sin(theta)
+// + b2 sin(2 theta)), X2,3 = x0 + a1 cos(2 theta) + a2 cos(theta) -+ i (b1 sin(2 theta)
+// - b2 sin(theta)) (signs of i flipped for the inverse).
+template <bool INV, typename T, typename C>
+__device__ __forceinline__ void dft5(C* v) {
+  const T c1 = T(0.30901699437494742410229341718281905886), c2 = T(-0.80901699437494742410229341718281905886);
+  const T s1 = T(0.95105651629515357211643933337938214340), s2 = T(0.58778525229247312916870595463907276860);
+  const C a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]), a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+  const C t1 = {v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y};
+  const C t2 = {v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y};
+  const C u1 = rot90<INV>(C{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y});
+  const C u2 = rot90<INV>(C{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y});
+  v[0] = cadd(v[0], cadd(a1, a2));
+  v[1] = cadd(t1, u1);
+  v[4] = csub(t1, u1);
+  v[2] = cadd(t2, u2);
+  v[3] = csub(t2, u2);
+}
 // Odd prime radix R via the n-point twiddle table (w[k] = exp(-2 pi i k/n)).
 template <bool INV, int R, typename C>
 __device__ __forceinline__ void dft_prime(C* v, const C* w, int n) {
@@ -590,7 +609,7 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
 }
 
 
-// ---- four-step pencil FFTs for power-of-two axes -----------------------------
+// ---- four-step pencil FFTs for axes of 2^a (times 3 or 5) -----------------------
 // n = N1 * N2.  Step 1: thread t < N2 of a column holds x[t + N2 k1] (k1 < N1),
 // transforms over k1, multiplies by w_n^(j1 t) and writes to LDS; step 2:
 // thread t < N1 reads the N2 values of row t, transforms over k2 and holds
@@ -599,18 +618,24 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
 // global row segment is TJ contiguous complex values.  Twiddles come from the
 // axis table w[k] = exp(-2 pi i k / n).
 
-// In-register DFT of power-of-two size N; w_N^k = w[k * wstride].
+// In-register DFT of size N = 2^a, 3 * 2^a or 5 * 2^a (even sizes split into
+// even and odd halves); w_N^k = w[k * wstride].
 template <int N, bool INV, typename T, typename C>
 __device__ __forceinline__ void dft_reg(C* v, const C* w, int wstride) {
   if constexpr (N == 1) {
     return;
   } else if constexpr (N == 2) {
     dft2<INV, T>(v);
+  } else if constexpr (N == 3) {
+    dft3<INV, T>(v);
   } else if constexpr (N == 4) {
     dft4<INV, T>(v);
+  } else if constexpr (N == 5) {
+    dft5<INV, T>(v);
   } else if constexpr (N == 8) {
     dft8<INV, T>(v);
   } else {
+    static_assert(N % 2 == 0, "in-register DFT sizes are 3, 5 or 2^a times them");
     C e[N / 2], o[N / 2];
 #pragma unroll
     for (int k = 0; k < N / 2; ++k) {
@@ -777,7 +802,7 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
     four_step_2<true, T, N1, N2>(L, TJ, p, t, w, [&](int e, C val) { sstore(col + e * zstride + p, val); });
 }
 
-// ---- four-step x passes for power-of-two nx -------------------------------
+// ---- four-step x passes for nx with a four-step factorisation ----------------
 // The x pencils are contiguous in memory, so lanes run along x: pencil p of a
 // workgroup is worked by `lanes` consecutive threads (t fastest), every global
 // access is a run of consecutive floats, and no integer division by runtime
@@ -1041,7 +1066,7 @@ struct PrecondImpl {
   int64_t spec_stride;  // complex entries per RHS
   int pairs_per_block, tj, tk;
   size_t lds_x, lds_y, lds_z;
-  // four-step kernels for power-of-two y / z axes (0: use the Stockham pass)
+  // four-step kernels for y / z axes with a factorisation (0: use the Stockham pass)
   int bf16;          // spectrum stored as BF16 pairs (FP32 arithmetic, four-step passes only)
   int x4;            // nx handled by fft_x4_r2c / fft_x4_c2r (0: Stockham x passes)
   int pp4, thr_x4;   // row pairs and threads per x4 workgroup
@@ -1052,7 +1077,8 @@ struct PrecondImpl {
   size_t lds_y4, lds_z4;
 };
 
-// Power-of-two lengths with a four-step kernel; lanes = max(N1, N2).
+// Lengths with a four-step kernel (powers of two, and 24, 40, 48, 80, 96, 160,
+// 192 = 2^a times 3 or 5); lanes = max(N1, N2) = N1.
 inline int four_step_lanes(int n) {
   switch (n) {
     case 2: return 2;
@@ -1064,6 +1090,14 @@ inline int four_step_lanes(int n) {
     case 128: return 16;
     case 256: return 16;
     case 512: return 32;
+    // 2^a times 3 or 5 (N1 x N2 below; lanes = N1)
+    case 24: return 8;
+    case 40: return 8;
+    case 48: return 8;
+    case 80: return 16;
+    case 96: return 16;
+    case 160: return 16;
+    case 192: return 16;
   }
   return 0;
 }
@@ -1085,6 +1119,13 @@ void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int 
     VXL_Y4(128, 16, 8)
     VXL_Y4(256, 16, 16)
     VXL_Y4(512, 32, 16)
+    VXL_Y4(24, 8, 3)
+    VXL_Y4(40, 8, 5)
+    VXL_Y4(48, 8, 6)
+    VXL_Y4(80, 16, 5)
+    VXL_Y4(96, 16, 6)
+    VXL_Y4(160, 16, 10)
+    VXL_Y4(192, 16, 12)
   }
 #undef VXL_Y4
 }
@@ -1108,11 +1149,20 @@ void launch_z4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int 
     VXL_Z4(128, 16, 8)
     VXL_Z4(256, 16, 16)
     VXL_Z4(512, 32, 16)
+    VXL_Z4(24, 8, 3)
+    VXL_Z4(40, 8, 5)
+    VXL_Z4(48, 8, 6)
+    VXL_Z4(80, 16, 5)
+    VXL_Z4(96, 16, 6)
+    VXL_Z4(160, 16, 10)
+    VXL_Z4(192, 16, 12)
   }
 #undef VXL_Z4
 }
 
-#define VXL_X4_CASES(M) M(8, 4, 2) M(16, 4, 4) M(32, 8, 4) M(64, 8, 8) M(128, 16, 8) M(256, 16, 16) M(512, 32, 16)
+#define VXL_X4_CASES(M)                                                                                      \
+  M(8, 4, 2) M(16, 4, 4) M(32, 8, 4) M(64, 8, 8) M(128, 16, 8) M(256, 16, 16) M(512, 32, 16) M(24, 8, 3) M(40, 8, 5) \
+  M(48, 8, 6) M(80, 16, 5) M(96, 16, 6) M(160, 16, 10) M(192, 16, 12)
 
 template <typename T>
 size_t x4_stride(int n) {
@@ -1266,7 +1316,7 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
     }
   }
   {
-    // Four-step x passes for power-of-two nx >= 8: 64 / lanes row pairs per
+    // Four-step x passes for nx >= 8 with a factorisation: 64 / lanes row pairs per
     // workgroup (3 * 64 threads).
     const int lx = four_step_lanes(im->nx);
     const char* env = std::getenv("VXL_FFT_XFOURSTEP");
@@ -1289,7 +1339,7 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   }
   im->bf16 = pc->precision == VXL_F32_BF16 ? 1 : 0;
   if (im->bf16 && !(im->x4 && im->y4 && im->z4))
-    return fail("BF16 spectrum storage needs power-of-two axes of at least 8 (four-step passes)");
+    return fail("BF16 spectrum storage needs axes of at least 8 with a four-step factorisation (2^a, or 2^a times 3 or 5 from 24 to 192)");
   const size_t elem = im->bf16 ? sizeof(bf16x2) : sizeof(C);
   VXL_HIP(dev_malloc(&im->spec, elem * size_t(im->spec_stride) * pc->nrhs_max));
   return VXL_OK;
