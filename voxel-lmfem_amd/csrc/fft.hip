@@ -1078,7 +1078,7 @@ struct PrecondImpl {
 };
 
 // Lengths with a four-step kernel (powers of two, and 24, 40, 48, 80, 96, 160,
-// 192 = 2^a times 3 or 5); lanes = max(N1, N2) = N1.
+// 192 = 2^a times 3 or 5); lanes = max(N1, N2).
 inline int four_step_lanes(int n) {
   switch (n) {
     case 2: return 2;
@@ -1090,7 +1090,7 @@ inline int four_step_lanes(int n) {
     case 128: return 16;
     case 256: return 16;
     case 512: return 32;
-    // 2^a times 3 or 5 (N1 x N2 below; lanes = N1)
+    // 2^a times 3 or 5 (lanes = the larger factor)
     case 24: return 8;
     case 40: return 8;
     case 48: return 8;
@@ -1164,14 +1164,31 @@ void launch_z4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int 
   M(8, 4, 2) M(16, 4, 4) M(32, 8, 4) M(64, 8, 8) M(128, 16, 8) M(256, 16, 16) M(512, 32, 16) M(24, 8, 3) M(40, 8, 5) \
   M(48, 8, 6) M(80, 16, 5) M(96, 16, 6) M(160, 16, 10) M(192, 16, 12)
 
+// The r2c pass loads in step 1, whose threads per pencil are N2, and the c2r
+// pass stores in step 2, whose threads are N1: r2c takes the larger factor as
+// N2 and c2r as N1, so every lane of a pencil moves data (at 96 = 16 x 6,
+// r2c with 16 x 6 had 6 of 16 lanes loading).
+#define VXL_X4R_CASES(M)                                                                                     \
+  M(8, 2, 4) M(16, 4, 4) M(32, 4, 8) M(64, 8, 8) M(128, 8, 16) M(256, 16, 16) M(512, 16, 32) M(24, 3, 8) M(40, 5, 8) \
+  M(48, 6, 8) M(80, 5, 16) M(96, 6, 16) M(160, 10, 16) M(192, 12, 16)
+
+// LDS entries per pencil: the larger of the two passes' tiles.
 template <typename T>
 size_t x4_stride(int n) {
+  size_t sr = 0, sc = 0;
 #define VXL_X4S(N, A, B) \
   case N:                \
-    return XTile<A, B>::stride;
+    sc = XTile<A, B>::stride; \
+    break;
   switch (n) { VXL_X4_CASES(VXL_X4S) }
 #undef VXL_X4S
-  return 0;
+#define VXL_X4S(N, A, B) \
+  case N:                \
+    sr = XTile<A, B>::stride; \
+    break;
+  switch (n) { VXL_X4R_CASES(VXL_X4S) }
+#undef VXL_X4S
+  return sr > sc ? sr : sc;
 }
 
 template <typename T, bool kUpd, typename SC>
@@ -1183,7 +1200,7 @@ void launch_x4_r2c(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const T
     hipLaunchKernelGGL((fft_x4_r2c<T, A, B, kUpd, SC>), grid, dim3(thr), lds, s, r, S, rows, PP, w, fstride, \
                        sstride, active, upd);                                                              \
     break;
-  switch (n) { VXL_X4_CASES(VXL_X4R) }
+  switch (n) { VXL_X4R_CASES(VXL_X4R) }
 #undef VXL_X4R
 }
 
