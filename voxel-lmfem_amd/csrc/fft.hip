@@ -1110,22 +1110,22 @@ void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int 
     hipLaunchKernelGGL((fft_y4<INV, T, A, B, SC>), grid, dim3(thr), lds, s, S, W, TJ, w, stride, active); \
     break;
   switch (n) {
-    VXL_Y4(2, 2, 1)
+    VXL_Y4(2, 1, 2)
     VXL_Y4(4, 2, 2)
-    VXL_Y4(8, 4, 2)
+    VXL_Y4(8, 2, 4)
     VXL_Y4(16, 4, 4)
-    VXL_Y4(32, 8, 4)
+    VXL_Y4(32, 4, 8)
     VXL_Y4(64, 8, 8)
-    VXL_Y4(128, 16, 8)
+    VXL_Y4(128, 8, 16)
     VXL_Y4(256, 16, 16)
-    VXL_Y4(512, 32, 16)
-    VXL_Y4(24, 8, 3)
-    VXL_Y4(40, 8, 5)
-    VXL_Y4(48, 8, 6)
-    VXL_Y4(80, 16, 5)
-    VXL_Y4(96, 16, 6)
-    VXL_Y4(160, 16, 10)
-    VXL_Y4(192, 16, 12)
+    VXL_Y4(512, 16, 32)
+    VXL_Y4(24, 3, 8)
+    VXL_Y4(40, 5, 8)
+    VXL_Y4(48, 6, 8)
+    VXL_Y4(80, 5, 16)
+    VXL_Y4(96, 6, 16)
+    VXL_Y4(160, 10, 16)
+    VXL_Y4(192, 12, 16)
   }
 #undef VXL_Y4
 }
@@ -1140,22 +1140,22 @@ void launch_z4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int 
                        inv_vn, stride, active, nyq, partial);                                      \
     break;
   switch (n) {
-    VXL_Z4(2, 2, 1)
+    VXL_Z4(2, 1, 2)
     VXL_Z4(4, 2, 2)
-    VXL_Z4(8, 4, 2)
+    VXL_Z4(8, 2, 4)
     VXL_Z4(16, 4, 4)
-    VXL_Z4(32, 8, 4)
+    VXL_Z4(32, 4, 8)
     VXL_Z4(64, 8, 8)
-    VXL_Z4(128, 16, 8)
+    VXL_Z4(128, 8, 16)
     VXL_Z4(256, 16, 16)
-    VXL_Z4(512, 32, 16)
-    VXL_Z4(24, 8, 3)
-    VXL_Z4(40, 8, 5)
-    VXL_Z4(48, 8, 6)
-    VXL_Z4(80, 16, 5)
-    VXL_Z4(96, 16, 6)
-    VXL_Z4(160, 16, 10)
-    VXL_Z4(192, 16, 12)
+    VXL_Z4(512, 16, 32)
+    VXL_Z4(24, 3, 8)
+    VXL_Z4(40, 5, 8)
+    VXL_Z4(48, 6, 8)
+    VXL_Z4(80, 5, 16)
+    VXL_Z4(96, 6, 16)
+    VXL_Z4(160, 10, 16)
+    VXL_Z4(192, 12, 16)
   }
 #undef VXL_Z4
 }
