@@ -171,6 +171,62 @@ def cpu_baseline(sample, iterations_total, n_target, threads=1):
                         f"{n_target}^3, x{iterations_total} RHS-iterations of the GPU run; extrapolated"))
 
 
+def frames_leg(n=256, R=6, reps=20):
+    """FP32 six-RHS stencil on an n^3 single-phase cell whose every voxel has
+    its own random frame (uniform rotations, seed 1), for a cubic crystal
+    (C11 168.4, C12 121.4, C44 75.4 GPa: the closed-form rotation) and an
+    orthotropic tow (the Bond rotation): per-voxel coefficient rows (the
+    device front end's i32 index and one FP64 + FP32 row per voxel) against
+    phase rows rotated in the stencil by a per-voxel FP32 quaternion
+    (vxl_cell_set_frames).  HIP events on the stream the kernels run on."""
+    import torch
+    from scipy.spatial.transform import Rotation
+    from voxel_lmfem import materials as pm
+    from voxel_lmfem.gpu import Cell
+    cubic = [[168.4, 121.4, 121.4, 0, 0, 0], [121.4, 168.4, 121.4, 0, 0, 0], [121.4, 121.4, 168.4, 0, 0, 0],
+             [0, 0, 0, 75.4, 0, 0], [0, 0, 0, 0, 75.4, 0], [0, 0, 0, 0, 0, 75.4]]
+    tow = dict(kind="orthotropic", frame="local", E1=230., E2=15., E3=15., nu12=.2, nu13=.2, nu23=.3,
+               G12=27., G13=27., G23=15. / 2.6)
+    shape = (n, n, n)
+    N = n ** 3
+    rot = Rotation.random(N, random_state=1).as_matrix().reshape(shape + (3, 3))
+    phase = np.ones(shape, np.int32)
+    labels = torch.from_numpy(phase).cuda()
+    stream = torch.cuda.current_stream()
+
+    def timed(cell):
+        u = torch.randn(R, 3, N, dtype=torch.float32, device="cuda")
+        o = torch.empty_like(u)
+        for _ in range(3):
+            cell.apply_planes(u, o)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            cell.apply_planes(u, o)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = dict(workload=f"single-phase {n}^3 cell, random frame per voxel", n=n, rhs=R,
+               bytes_per_voxel_as_scheduled=dict(rows=R * (24 + 84) + 4, frames=R * (24 + 17)),
+               bytes_per_voxel_minimal=dict(rows=24 * R + 4 + 84, frames=24 * R + 17))
+    for name, spec in (("cubic", dict(kind="anisotropic", frame="local", C=cubic)), ("orthotropic", tow)):
+        mats = {1: spec}
+        field = pm.build_field_device(labels, mats, rot)
+        cell = Cell(field, shape, (1.0 / n,) * 3)
+        try:
+            ms_rows = timed(cell)
+            cell.set_frames(*pm.phase_frames(phase, mats, rot))
+            ms_frames = timed(cell)
+        finally:
+            cell.close()
+        del field
+        out[name] = dict(rows_ms=ms_rows, frames_ms=ms_frames, frames_vs_rows=ms_frames / ms_rows)
+        print(f"# frames {name} {n}^3 R={R}: per-voxel rows {ms_rows:.3f} ms, frames in the stencil "
+              f"{ms_frames:.3f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg"):
     """Uneven batch: `count` two-phase random-inclusion cells (edges drawn from
     {32, 48, 64, 80, 96}, volume fraction U[0.1, 0.4], radius 2-5 voxels,
@@ -286,6 +342,9 @@ def main():
     ap.add_argument("--skip-small", action="store_true", help="skip the 32^3 / 16^3 sphere cells on the GPU")
     ap.add_argument("--stencil", default="modal", choices=["modal", "mfma"],
                     help="FP32 operator of the timed mixed solve: modal VALU stencil or matrix-core form")
+    ap.add_argument("--frames-only", action="store_true",
+                    help="only time the FP32 six-RHS stencil at 256^3 on random per-voxel frames: "
+                         "per-voxel coefficient rows against frames rebuilt in the stencil")
     ap.add_argument("--stencil-only", action="store_true",
                     help="only run 10 six-RHS FP32 stencil applies and a 1 GiB device copy "
                          "(the calibration kernel for FETCH_SIZE/WRITE_SIZE counter runs)")
@@ -295,6 +354,9 @@ def main():
         return
     if args.cpu_end_to_end:
         print(json.dumps(cpu_end_to_end()), flush=True)
+        return
+    if args.frames_only:
+        print(json.dumps(dict(frames_stencil=frames_leg(args.n))), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps(dict(error="this benchmark runs on one GPU; sharding across GPUs is not built",

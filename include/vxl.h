@@ -183,6 +183,27 @@ int vxl_get_stencil_config(vxl_cell* cell, int precision, int* tile_y, int* vari
  * (label-only cells); VXL_ERR_ARG otherwise.  Tile 32 x 7 nodes, schedule
  * and tile settings of the modal stencil do not apply to it. */
 int vxl_cell_set_mfma(vxl_cell* cell, int enable);
+/* Per-voxel material frames for the FP32 stencil: element stiffness rebuilt
+ * in the stencil from each voxel's phase and orientation, in place of its
+ * coefficient row.  Replaces the per-voxel rows that the reference's hybrid
+ * pool gives every local-frame voxel (hybrid_material.py:49-61, rotated by
+ * _rotate_packed, material_field.py:105-114) in every FP32 apply of the cell
+ * (vxl_apply, the mixed solve's inner iterations); FP64 applies, the RHS, the
+ * certificate and the stress keep the cell's FP64 rows.
+ * phase_rows: nphase x 21 FP64 (host), each phase's C21 in its own frame;
+ * phase_local[p] != 0 marks local-frame phases (rotated per voxel; a phase
+ * that is exactly cubic in its frame takes a closed form, any other the Bond
+ * rotation), 0 global-frame phases (used as given).  phase_index_dev: uint8
+ * phase slot of every voxel (device); quat_dev: 4 floats (w, x, y, z) per
+ * voxel (device), the unit quaternion of the voxel's rotation R (columns =
+ * local axes in cell coordinates), read for local-frame voxels.  Both are
+ * copied (17 B per voxel).  phase_index_dev == NULL turns the frames off.
+ * FI element, no scale field, not with the matrix-core stencil; 1 to 256
+ * phases; VXL_ERR_ARG if an index is >= nphase or a quaternion is not finite
+ * with |q|^2 within 1e-5 of 1.  No counterpart in the reference, which stores
+ * one rotated row per local-frame voxel. */
+int vxl_cell_set_frames(vxl_cell* cell, int nphase, const double* phase_rows, const int* phase_local,
+                        const void* phase_index_dev, const float* quat_dev);
 /* Autotuner: times all 27 (tile, schedule, chunk depth in {8, 4, 2})
  * configurations on the given buffers (reps launches each, HIP events on the
  * cell's stream), keeps the fastest on the cell and returns it.  ms_table

@@ -112,3 +112,38 @@ def test_objective_weights():
     C = C + C.T
     KV = (C[0, 0] + C[1, 1] + C[2, 2] + 2 * (C[0, 1] + C[0, 2] + C[1, 2])) / 9
     assert float((adjoint.objective_weights() * C).sum()) == pytest.approx(C[0, 0] + KV)
+
+
+def test_rotation_quaternions_match_scipy():
+    """Quaternions of the per-voxel frames stencil: scipy's rotation built from
+    them (its (x, y, z, w) order) gives back R, including the half-turns where
+    w = 0 and Shepperd's method takes another component from the diagonal."""
+    from scipy.spatial.transform import Rotation
+    R = Rotation.random(500, random_state=4).as_matrix()
+    R = np.concatenate([R, np.eye(3)[None], np.diag([1., -1., -1.])[None], np.diag([-1., 1., -1.])[None],
+                        np.diag([-1., -1., 1.])[None]])
+    q = pm.rotation_quaternions(R)
+    assert np.allclose(np.linalg.norm(q, axis=1), 1.0, rtol=0, atol=1e-15)
+    back = Rotation.from_quat(q[:, [1, 2, 3, 0]]).as_matrix()
+    assert np.abs(back - R).max() < 1e-14
+
+
+def test_phase_frames_bookkeeping():
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(1)
+    phase = rng.choice([3, 7, 9], size=(4, 5, 6)).astype(np.int32)
+    mats = {3: dict(kind="isotropic", E=2.0, nu=.3),
+            7: dict(kind="orthotropic", frame="local", E1=230., E2=15., E3=15., nu12=.2, nu13=.2, nu23=.3,
+                    G12=27., G13=27., G23=5.769),
+            9: dict(kind="isotropic", E=70.0, nu=.2)}
+    R = Rotation.random(phase.size, random_state=2).as_matrix().reshape(phase.shape + (3, 3))
+    rows, local, slot, quat = pm.phase_frames(phase, mats, R)
+    labels, tensors, flags = pm.phase_table(mats)
+    assert np.array_equal(rows, pm.pack21(tensors)) and np.array_equal(local, flags)
+    assert slot.dtype == np.uint8 and np.array_equal(labels[slot], phase.reshape(-1))
+    loc = phase.reshape(-1) == 7
+    assert np.array_equal(quat[~loc], np.tile([1.0, 0, 0, 0], ((~loc).sum(), 1)))
+    back = Rotation.from_quat(quat[loc][:, [1, 2, 3, 0]]).as_matrix()
+    assert np.abs(back - R.reshape(-1, 3, 3)[loc]).max() < 1e-14
+    with pytest.raises(ValueError, match="require orientation"):
+        pm.phase_frames(phase, mats)

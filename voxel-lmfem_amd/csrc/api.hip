@@ -129,6 +129,10 @@ int cell_create_common(vxl_cell** out, int nz, int ny, int nx, double hx, double
   c->mfma = 0;
   c->mfma_table = nullptr;
   c->mfma_exp = nullptr;
+  c->frame_index = nullptr;
+  c->frame_table = nullptr;
+  c->frame_quat = nullptr;
+  c->frame_cubic_only = 0;
   (void)hipGetDevice(&c->device);
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(c->stream);
@@ -333,6 +337,7 @@ int vxl_cell_destroy(vxl_cell* cell) {
   if (!cell) return VXL_OK;
   (void)hipStreamSynchronize(cell->stream);
   mfma_release(cell);
+  frames_release(cell);
   if (cell->c64_owned) dev_free(cell->c64_owned);
   if (cell->c32_owned) dev_free(cell->c32_owned);
   delete cell;
@@ -363,12 +368,82 @@ int vxl_set_stencil_variant(vxl_cell* cell, int precision, int variant) {
 int vxl_cell_set_mfma(vxl_cell* cell, int enable) {
   g_error.clear();
   if (!cell || (enable != 0 && enable != 1)) return fail("vxl_cell_set_mfma: invalid arguments");
+  if (enable && cell->frame_index) return fail("the matrix-core stencil does not take per-voxel frames");
   if (enable) {
     VXL_TRY(mfma_prepare(cell));
     cell->mfma = 1;
   } else {
     cell->mfma = 0;
   }
+  return VXL_OK;
+}
+
+int vxl_cell_set_frames(vxl_cell* cell, int nphase, const double* phase_rows, const int* phase_local,
+                        const void* phase_index_dev, const float* quat_dev) {
+  g_error.clear();
+  if (!cell) return fail("vxl_cell_set_frames: invalid arguments");
+  if (!phase_index_dev) {
+    (void)hipStreamSynchronize(cell->stream);
+    frames_release(cell);
+    return VXL_OK;
+  }
+  if (nphase < 1 || nphase > 256 || !phase_rows || !phase_local || !quat_dev)
+    return fail("vxl_cell_set_frames: 1 to 256 phase rows, their frame flags, a phase index and quaternions");
+  if (cell->element != VXL_ELEMENT_FI) return fail("per-voxel frames need the FI element");
+  if (cell->mat.scale) return fail("per-voxel frames do not combine with a scale field");
+  if (cell->mfma) return fail("the matrix-core stencil does not take per-voxel frames");
+  std::vector<float> table(size_t(nphase) * kRowWidth, 0.0f);
+  bool cubic_only = true;
+  for (int p = 0; p < nphase; ++p) {
+    const double* c = phase_rows + size_t(p) * 21;
+    for (int k = 0; k < 21; ++k)
+      if (!std::isfinite(c[k])) return fail("phase rows contain nonfinite values");
+    int kind = 0;
+    if (phase_local[p]) {
+      // Cubic in its local frame: equal normal diagonal, equal normal
+      // off-diagonal, equal shear diagonal, zeros elsewhere (exactly).
+      const double d = c[pidx(0, 0)], o = c[pidx(0, 1)], g = c[pidx(3, 3)];
+      bool cubic = c[pidx(1, 1)] == d && c[pidx(2, 2)] == d && c[pidx(0, 2)] == o && c[pidx(1, 2)] == o &&
+                   c[pidx(4, 4)] == g && c[pidx(5, 5)] == g;
+      for (int i = 0; i < 6; ++i)
+        for (int j = i + 1; j < 6; ++j)
+          if (j >= 3 && c[pidx(i, j)] != 0.0) cubic = false;
+      kind = cubic ? 2 : 1;
+      if (!cubic) cubic_only = false;
+    }
+    for (int k = 0; k < 21; ++k) table[size_t(p) * kRowWidth + k] = float(c[k]);
+    table[size_t(p) * kRowWidth + 21] = float(kind);
+  }
+  (void)hipStreamSynchronize(cell->stream);
+  frames_release(cell);
+  const int64_t n = cell->grid.n;
+  auto undo = [&](int rc) {
+    frames_release(cell);
+    return rc;
+  };
+  if (dev_malloc(&cell->frame_table, table.size() * sizeof(float)) != hipSuccess ||
+      dev_malloc(&cell->frame_index, size_t(n)) != hipSuccess ||
+      dev_malloc(&cell->frame_quat, size_t(n) * 4 * sizeof(float)) != hipSuccess)
+    return undo(fail("out of device memory for the frames"));
+  int* flag = nullptr;
+  if (dev_malloc(&flag, 2 * sizeof(int)) != hipSuccess) return undo(fail("out of device memory for the frames"));
+  int bad[2] = {0, 0};
+  const int rc = [&]() -> int {
+    VXL_HIP(hipMemcpyAsync(cell->frame_table, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice,
+                           cell->stream));
+    VXL_HIP(hipMemcpyAsync(cell->frame_index, phase_index_dev, size_t(n), hipMemcpyDeviceToDevice, cell->stream));
+    VXL_HIP(hipMemcpyAsync(cell->frame_quat, quat_dev, size_t(n) * 4 * sizeof(float), hipMemcpyDeviceToDevice,
+                           cell->stream));
+    VXL_TRY(launch_frames_check(cell->frame_index, cell->frame_quat, n, nphase, flag, cell->stream));
+    VXL_HIP(hipMemcpyAsync(bad, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, cell->stream));
+    VXL_HIP(hipStreamSynchronize(cell->stream));
+    return VXL_OK;
+  }();
+  dev_free(flag);
+  if (rc != VXL_OK) return undo(rc);
+  if (bad[0]) return undo(fail("phase index out of range of the phase rows"));
+  if (bad[1]) return undo(fail("frames must be finite unit quaternions (|q|^2 within 1e-5 of 1)"));
+  cell->frame_cubic_only = cubic_only ? 1 : 0;
   return VXL_OK;
 }
 

@@ -227,6 +227,7 @@ struct ApplyArgs {
   const void* index;
   const T* table;      // rows x 24
   const float* scale;  // optional
+  const float4* frames;  // kFrames: unit quaternion (w, x, y, z) of every voxel's material frame
   int nx, ny, nz;
   int zchunk;
   int tiles_x;
@@ -407,6 +408,104 @@ apply_kernel(ApplyArgs<T> a) {
   }
 }
 
+// ---- material frames rebuilt in the stencil ---------------------------------
+// With per-voxel frames the FP32 stencil reads a phase row (the phase's C21 in
+// its local frame; slot 21 holds the kind: 0 global frame, 1 local frame, 2
+// local-frame cubic) and the voxel's unit quaternion q = (w, x, y, z), 16 B,
+// and rotates the row in registers: kind 1 by the Bond stress matrix,
+// C' = T C T^T (the reference's _rotate_packed, material_field.py:105-114,
+// in engineering Voigt order 11, 22, 33, 23, 13, 12); kind 2 by the cubic
+// closed form C' = C12 d(x)d + C44 (shear part) + a sum_m p_m p_m^T, where p_m
+// holds the Voigt pair products of local axis m (column m of R) and
+// a = C11 - C12 - 2 C44, which equals T C T^T for a cubic C.
+__device__ __forceinline__ void frame_matrix(float4 q, float (&R)[3][3]) {
+  const float w = q.x, x = q.y, y = q.z, z = q.w;
+  R[0][0] = 1.f - 2.f * (y * y + z * z);
+  R[0][1] = 2.f * (x * y - w * z);
+  R[0][2] = 2.f * (x * z + w * y);
+  R[1][0] = 2.f * (x * y + w * z);
+  R[1][1] = 1.f - 2.f * (x * x + z * z);
+  R[1][2] = 2.f * (y * z - w * x);
+  R[2][0] = 2.f * (x * z - w * y);
+  R[2][1] = 2.f * (y * z + w * x);
+  R[2][2] = 1.f - 2.f * (x * x + y * y);
+}
+
+// Row i of the Bond stress matrix of R; shear rows/columns pair (1, 2),
+// (0, 2), (0, 1).
+__device__ __forceinline__ void bond_row(const float (&R)[3][3], int i, float (&t)[6]) {
+  constexpr int pi[3] = {1, 0, 0}, pj[3] = {2, 2, 1};
+  if (i < 3) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) t[b] = R[i][b] * R[i][b];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[3 + k] = 2.f * R[i][pi[k]] * R[i][pj[k]];
+  } else {
+    const int m = i - 3;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) t[b] = R[pi[m]][b] * R[pj[m]][b];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[3 + k] = R[pi[m]][pi[k]] * R[pj[m]][pj[k]] + R[pi[m]][pj[k]] * R[pj[m]][pi[k]];
+  }
+}
+
+// kCubicOnly: every local-frame phase of the cell is cubic (the Bond branch
+// is not compiled, which keeps the register count down).
+template <bool kCubicOnly>
+__device__ __forceinline__ void rotate_row(float (&c)[21], float4 q, int kind) {
+  float R[3][3];
+  frame_matrix(q, R);
+  if (kCubicOnly || kind == 2) {
+    const float c12 = c[pidx(0, 1)], c44 = c[pidx(3, 3)];
+    const float an = c[pidx(0, 0)] - c12 - 2.f * c44;
+    float p[3][6];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      p[m][0] = R[0][m] * R[0][m];
+      p[m][1] = R[1][m] * R[1][m];
+      p[m][2] = R[2][m] * R[2][m];
+      p[m][3] = R[1][m] * R[2][m];
+      p[m][4] = R[0][m] * R[2][m];
+      p[m][5] = R[0][m] * R[1][m];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = i; j < 6; ++j) {
+        float v = (i < 3 && j < 3) ? c12 : 0.f;
+        if (i == j) v += i < 3 ? 2.f * c44 : c44;
+        const float sum = p[0][i] * p[0][j] + p[1][i] * p[1][j] + p[2][i] * p[2][j];
+        c[pidx(i, j)] = fmaf(an, sum, v);
+      }
+    return;
+  }
+  if constexpr (kCubicOnly) return;
+  float out[21];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    float ti[6], w[6];
+    bond_row(R, i, ti);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc = fmaf(ti[k], c[pidx(k, l)], acc);
+      w[l] = acc;
+    }
+#pragma unroll
+    for (int j = i; j < 6; ++j) {
+      float tj[6];
+      bond_row(R, j, tj);
+      float acc = 0.f;
+#pragma unroll
+      for (int l = 0; l < 6; ++l) acc = fmaf(w[l], tj[l], acc);
+      out[pidx(i, j)] = acc;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 21; ++k) c[k] = out[k];
+}
+
 // Stencil v2: element columns persist across the z march.  A column keeps the
 // face (xy) Hadamard modes of its bottom node face and the force its previous
 // layer put on that face in registers, so each layer transforms one new face
@@ -418,7 +517,7 @@ apply_kernel(ApplyArgs<T> a) {
 // FP32: the register budget lets two workgroups share a CU (waves per SIMD).
 // kResid: the certification form r = f - K u with ||r||^2 (FP64 operator).
 template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse, bool kPF, bool kIso, bool kHG = false,
-          bool kResid = false>
+          bool kResid = false, int kFrames = 0>
 __global__ void __launch_bounds__((Tile<TX, TY>::NT), (sizeof(T) == 4 ? (2 * Tile<TX, TY>::NT / 64 + 3) / 4 : 0))
 apply_kernel_v2(ApplyArgs<T> a) {
   using TL = Tile<TX, TY>;
@@ -478,10 +577,14 @@ apply_kernel_v2(ApplyArgs<T> a) {
   // Element of layer z from its bottom (Bb) and top (Tf) face modes; returns the
   // bottom-face force (Fb) and top-face force (Ft) in face-mode form.
   auto row_of = [&](int z) { return load_row<Idx>(a.index, int64_t(wrapmod(z, nz)) * nxy + e_xy); };
-  auto element = [&](int z, int row, const T (&Bb)[3][4], const T (&Tf)[3][4], T (&Fb)[3][4], T (&Ft)[3][4]) {
+  static_assert(kFrames == 0 || (std::is_same<T, float>::value && !kScale && !kHG && !kResid),
+                "per-voxel frames: FP32 FI operator without a scale field");
+  // Coefficients of the element of layer z: its row, times the voxel's scale
+  // (kScale), rotated into the voxel's material frame (kFrames: 1 any
+  // local-frame phase, 2 cubic local-frame phases only).
+  auto coeffs = [&](int z, int row, T (&c)[21]) {
     const int64_t voxel = int64_t(wrapmod(z, nz)) * nxy + e_xy;
     const T* crow = a.table + int64_t(row) * kRowWidth;
-    T c[21];
 #pragma unroll
     for (int k = 0; k < 21; ++k) c[k] = crow[k];
     if constexpr (kScale) {
@@ -489,6 +592,12 @@ apply_kernel_v2(ApplyArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < 21; ++k) c[k] *= sc_;
     }
+    if constexpr (kFrames != 0) {
+      const int kind = int(crow[21]);
+      if (kind != 0) rotate_row<kFrames == 2>(c, a.frames[voxel], kind);
+    }
+  };
+  auto element = [&](const T (&c)[21], const T (&Bb)[3][4], const T (&Tf)[3][4], T (&Fb)[3][4], T (&Ft)[3][4]) {
     T q[3][8], P[3][8];
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc)
@@ -561,7 +670,9 @@ apply_kernel_v2(ApplyArgs<T> a) {
   if (is_elem) {
     T Tf[3][4], Fb[3][4];
     face_modes(Tf);
-    element(z0 - 1, row_of(z0 - 1), Bf, Tf, Fb, Ftp);
+    T c[21];
+    coeffs(z0 - 1, row_of(z0 - 1), c);
+    element(c, Bf, Tf, Fb, Ftp);
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
@@ -599,6 +710,12 @@ apply_kernel_v2(ApplyArgs<T> a) {
         for (int cc = 0; cc < 3; ++cc) fpre[cc] = a.f[base + cc * n];
       }
     }
+    // kFrames: the rotation runs before this step's face modes and prefetch
+    // are live, which keeps the register count inside the launch bound.
+    T c[21];
+    if constexpr (kFrames != 0) {
+      if (is_elem) coeffs(z, row_cur, c);
+    }
     if constexpr (!kPF) {
       load_plane(z + 1);
       __syncthreads();
@@ -611,7 +728,8 @@ apply_kernel_v2(ApplyArgs<T> a) {
     if (more) prefetch(z + 2);
     if (is_elem) {
       T Fb[3][4];
-      element(z, row_cur, Bc, Bn, Fb, Fn);
+      if constexpr (kFrames == 0) coeffs(z, row_cur, c);
+      element(c, Bc, Bn, Fb, Fn);
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
         T F[4];
@@ -762,6 +880,73 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
     VXL_V2(false, false);
   }
 #undef VXL_V2
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
+}
+
+// FP32 stencil with per-voxel material frames (vxl_cell_set_frames): phase
+// index (u8), phase rows and quaternions; always the schedule-3 form.  Tile:
+// the cell's with cubic local-frame phases only; otherwise 32 x 8 (32 x 6 if
+// the cell is set to it), since with the Bond rotation's registers the 32 x 14
+// tile spills (128 VGPRs, 26 spilled in the CG form) and 32 x 8 takes 146
+// VGPRs without spilling.
+template <typename F>
+int frames_tile(const vxl_cell* cell, F&& f) {
+  const int ty = cell->tile_y[0];
+  if (cell->frame_cubic_only && ty == kTilesF32[0]) return f(std::integral_constant<int, kTilesF32[0]>());
+  if (ty == kTilesF32[2]) return f(std::integral_constant<int, kTilesF32[2]>());
+  return f(std::integral_constant<int, kTilesF32[1]>());
+}
+
+template <int TX, int TY>
+int launch_apply_frames(const vxl_cell* cell, int nrhs, const float* u, float* out, hipStream_t s,
+                        const int* active = nullptr, double* partial = nullptr, int* blocks = nullptr) {
+  using TL = Tile<TX, TY>;
+  ApplyArgs<float> a{};
+  a.u = u;
+  a.out = out;
+  a.active = active;
+  a.partial = partial;
+  a.rhs_stride = 3 * cell->grid.n;
+  a.index = cell->frame_index;
+  a.table = cell->frame_table;
+  a.frames = reinterpret_cast<const float4*>(cell->frame_quat);
+  a.nx = cell->grid.nx;
+  a.ny = cell->grid.ny;
+  a.nz = cell->grid.nz;
+  int tiles = 0, chunks = 0;
+  apply_grid<float, TX, TY>(cell, nrhs, a.tiles_x, tiles, chunks, a.zchunk);
+  const ModalFactors mf = modal_factors(cell->h);
+  a.g = cast_g<float>(mf);
+  dim3 grid(tiles, chunks, nrhs);
+  if (blocks) *blocks = tiles * chunks;
+  const bool iso = cell->h[0] == cell->h[1] && cell->h[1] == cell->h[2];
+  if (iso)
+    for (int k = 0; k < 3; ++k)
+      for (int d = 0; d < 3; ++d) a.g.g[k][d] = float(mf.g[k][d] * mf.g[k][d]);
+#define VXL_FR(FUSE, ISO, KIND)                                                                              \
+  hipLaunchKernelGGL((apply_kernel_v2<float, uint8_t, false, TX, TY, FUSE, true, ISO, false, false, KIND>), grid, \
+                     dim3(TL::NT), 0, s, a)
+#define VXL_FR2(FUSE, ISO)       \
+  do {                           \
+    if (cell->frame_cubic_only)  \
+      VXL_FR(FUSE, ISO, 2);      \
+    else                         \
+      VXL_FR(FUSE, ISO, 1);      \
+  } while (0)
+  if (partial) {
+    if (iso)
+      VXL_FR2(true, true);
+    else
+      VXL_FR2(true, false);
+  } else {
+    if (iso)
+      VXL_FR2(false, true);
+    else
+      VXL_FR2(false, false);
+  }
+#undef VXL_FR2
+#undef VXL_FR
   VXL_HIP(hipGetLastError());
   return VXL_OK;
 }
@@ -1099,6 +1284,10 @@ template <typename T>
 int launch_apply_t(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (cell->mfma) return launch_mfma_apply(cell, nrhs, u, out, nullptr, nullptr, nullptr, s);
+    if (cell->frame_index)
+      return frames_tile(cell, [&](auto ty) {
+        return launch_apply_frames<32, decltype(ty)::value>(cell, nrhs, u, out, s);
+      });
   }
   if (cell->element == VXL_ELEMENT_C3D8R_HG)
     return dispatch_apply<T, 32, default_tile_y<T>(), true>(cell, nrhs, u, out, s);
@@ -1109,6 +1298,10 @@ int launch_apply_cg_t(const vxl_cell* cell, int nrhs, const T* p, T* q, const in
                       int* blocks, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     if (cell->mfma) return launch_mfma_apply(cell, nrhs, p, q, active, partial, blocks, s);
+    if (cell->frame_index)
+      return frames_tile(cell, [&](auto ty) {
+        return launch_apply_frames<32, decltype(ty)::value>(cell, nrhs, p, q, s, active, partial, blocks);
+      });
   }
   if (cell->element == VXL_ELEMENT_C3D8R_HG)
     return dispatch_apply<T, 32, default_tile_y<T>(), true>(cell, nrhs, p, q, s, active, partial, blocks);
@@ -1120,12 +1313,54 @@ template <typename T>
 int cg_blocks_t(const vxl_cell* cell, int nrhs, int zmin) {
   if constexpr (std::is_same<T, float>::value) {
     if (cell->mfma) return mfma_blocks(cell, nrhs);
+    if (cell->frame_index)
+      return frames_tile(cell, [&](auto ty) {
+        int tx, tiles, chunks, zc;
+        apply_grid<T, 32, decltype(ty)::value>(cell, nrhs, tx, tiles, chunks, zc, zmin);
+        return tiles * chunks;
+      });
   }
   return with_tile<T>(cell, [&](auto ty) {
     int tx, tiles, chunks, zc;
     apply_grid<T, 32, decltype(ty)::value>(cell, nrhs, tx, tiles, chunks, zc, zmin);
     return tiles * chunks;
   });
+}
+
+namespace {
+__global__ void zero_flags_kernel(int* flag, int n) {
+  if (int(threadIdx.x) < n) flag[threadIdx.x] = 0;
+}
+
+__global__ void __launch_bounds__(256)
+frames_check_kernel(const uint8_t* __restrict__ index, const float4* __restrict__ quat, int64_t n, int nphase,
+                    int* __restrict__ flag) {
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += int64_t(gridDim.x) * blockDim.x) {
+    if (int(index[e]) >= nphase) flag[0] = 1;
+    const float4 q = quat[e];
+    const float nq = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+    if (!(fabsf(nq - 1.f) <= 1e-5f)) flag[1] = 1;
+  }
+}
+}  // namespace
+
+void frames_release(vxl_cell* cell) {
+  if (cell->frame_index) dev_free(cell->frame_index);
+  if (cell->frame_table) dev_free(cell->frame_table);
+  if (cell->frame_quat) dev_free(cell->frame_quat);
+  cell->frame_index = nullptr;
+  cell->frame_table = nullptr;
+  cell->frame_quat = nullptr;
+}
+
+int launch_frames_check(const uint8_t* index, const float* quat, int64_t n, int nphase, int* flag, hipStream_t s) {
+  hipLaunchKernelGGL(zero_flags_kernel, dim3(1), dim3(64), 0, s, flag, 2);
+  VXL_HIP(hipGetLastError());
+  const int blocks = int(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)));
+  hipLaunchKernelGGL(frames_check_kernel, dim3(blocks), dim3(256), 0, s, index,
+                     reinterpret_cast<const float4*>(quat), n, nphase, flag);
+  VXL_HIP(hipGetLastError());
+  return VXL_OK;
 }
 
 template <>

@@ -154,6 +154,14 @@ struct vxl_cell {
   int mfma;
   void* mfma_table;
   int* mfma_exp;
+  // Per-voxel material frames for the FP32 stencil (vxl_cell_set_frames): on
+  // when frame_index != nullptr.  Phase slot per voxel, FP32 phase rows x 24
+  // (slot 21: 0 global frame, 1 local frame, 2 local-frame cubic) and the
+  // unit quaternion (w, x, y, z) of every voxel.
+  uint8_t* frame_index;
+  float* frame_table;
+  float* frame_quat;
+  int frame_cubic_only;  // every local-frame phase is cubic in its frame
 };
 
 namespace vxl {
@@ -220,6 +228,11 @@ int local_rows(int64_t n, int label_bytes, const void* labels_dev, long long lab
 int mfma_max_rows();
 int mfma_prepare(vxl_cell* cell);
 void mfma_release(vxl_cell* cell);
+// Per-voxel frames (stencil.hip): frees the cell's frame buffers (frames off);
+// flag[0] = 1 if a phase index is >= nphase, flag[1] = 1 if a quaternion is
+// not finite or |q|^2 is off 1 by more than 1e-5 (flag cleared here).
+void frames_release(vxl_cell* cell);
+int launch_frames_check(const uint8_t* index, const float* quat, int64_t n, int nphase, int* flag, hipStream_t s);
 int mfma_blocks(const vxl_cell* cell, int nrhs);
 int launch_mfma_apply(const vxl_cell* cell, int nrhs, const float* u, float* out, const int* active,
                       double* partial, int* blocks, hipStream_t s);

@@ -50,6 +50,7 @@ SIGNATURES = {
     "vxl_cell_set_element": (_I, [_P, _I, _D]),
     "vxl_get_stencil_config": (_I, [_P, _I, _IP, _IP, _IP]),
     "vxl_cell_set_mfma": (_I, [_P, _I]),
+    "vxl_cell_set_frames": (_I, [_P, _I, _DP, _IP, _P, _P]),
     "vxl_autotune_stencil": (_I, [_P, _I, _I, _P, _P, _I, _IP, _IP, _IP, _DP]),
     "vxl_rhs": (_I, [_P, _I, _DP, _P]),
     "vxl_avg_stress": (_I, [_P, _I, _P, _DP, _DP]),

@@ -159,6 +159,35 @@ class Cell:
         VALU stencil.  Switch before building solvers that should use it."""
         check(self.lib.vxl_cell_set_mfma(self.handle, 1 if enable else 0), "vxl_cell_set_mfma")
 
+    def set_frames(self, phase_rows, phase_local, phase_index, quaternions):
+        """Rebuild element stiffness in the FP32 stencil from each voxel's
+        phase and frame instead of reading its coefficient row.
+
+        phase_rows: (P, 21) FP64 phase rows in their own frames; phase_local:
+        (P,) flags of the local-frame phases; phase_index: (N,) uint8 phase
+        slot per voxel (z, y, x order; host or device); quaternions: (N, 4)
+        unit quaternions (w, x, y, z) of the voxel rotations.  FP64 applies,
+        RHS, certificate and stress keep the cell's rows.  None as
+        phase_index turns the frames off."""
+        torch = self.torch
+        if phase_index is None:
+            check(self.lib.vxl_cell_set_frames(self.handle, 0, None, None, None, None), "vxl_cell_set_frames")
+            return
+        rows = np.ascontiguousarray(phase_rows, dtype=np.float64)
+        flags = np.ascontiguousarray(np.asarray(phase_local).astype(np.int32))
+        if rows.ndim != 2 or rows.shape[1] != 21 or flags.shape != (rows.shape[0],):
+            raise ValueError("phase_rows must be (P, 21) with one frame flag per phase")
+        index = torch.as_tensor(phase_index).reshape(-1)
+        quat = torch.as_tensor(quaternions).reshape(-1, 4)
+        if index.numel() != self.n or quat.shape[0] != self.n:
+            raise ValueError("phase_index and quaternions need one entry per voxel")
+        index = index.to(device=self.device, dtype=torch.uint8).contiguous()
+        quat = quat.to(device=self.device, dtype=torch.float32).contiguous()
+        self.torch.cuda.current_stream(self.device).synchronize()
+        check(self.lib.vxl_cell_set_frames(self.handle, rows.shape[0], rows.ctypes.data_as(_native._DP),
+                                           flags.ctypes.data_as(_native._IP), dptr(index), dptr(quat)),
+              "vxl_cell_set_frames")
+
     def set_stencil_config(self, precision, tile_y, variant, zmin):
         check(self.lib.vxl_set_stencil_config(self.handle, self._code(precision), int(tile_y), int(variant),
                                               int(zmin)), "vxl_set_stencil_config")

@@ -186,6 +186,62 @@ def bond_rotate(C, R):
     return np.stack(out, -1)
 
 
+def rotation_quaternions(R):
+    """Unit quaternions (k,4) = (w, x, y, z) of rotations R (k,3,3), with
+    R = [[1-2(y2+z2), 2(xy-wz), 2(xz+wy)], [2(xy+wz), 1-2(x2+z2), 2(yz-wx)],
+    [2(xz-wy), 2(yz+wx), 1-2(x2+y2)]]: Shepperd's method (the largest of
+    w, x, y, z is taken from the diagonal, the others from the off-diagonal
+    sums), then normalised."""
+    R = np.asarray(R, dtype=np.float64).reshape(-1, 3, 3)
+    d0, d1, d2 = R[:, 0, 0], R[:, 1, 1], R[:, 2, 2]
+    cand = np.stack([1 + d0 + d1 + d2, 1 + d0 - d1 - d2, 1 - d0 + d1 - d2, 1 - d0 - d1 + d2], -1)
+    pick = np.argmax(cand, axis=-1)
+    big = 0.5 * np.sqrt(np.maximum(cand[np.arange(len(R)), pick], 0.0))
+    f = 0.25 / np.where(big > 0, big, 1.0)
+    s10, s01 = R[:, 1, 0] + R[:, 0, 1], R[:, 1, 0] - R[:, 0, 1]
+    s20, s02 = R[:, 2, 0] + R[:, 0, 2], R[:, 0, 2] - R[:, 2, 0]
+    s21, s12 = R[:, 2, 1] + R[:, 1, 2], R[:, 2, 1] - R[:, 1, 2]
+    q = np.empty((len(R), 4))
+    forms = [
+        (big, s12 * f, s02 * f, s01 * f),
+        (s12 * f, big, s10 * f, s20 * f),
+        (s02 * f, s10 * f, big, s21 * f),
+        (s01 * f, s20 * f, s21 * f, big),
+    ]
+    for k, form in enumerate(forms):
+        sel = pick == k
+        for c in range(4):
+            q[sel, c] = form[c][sel]
+    return q / np.linalg.norm(q, axis=-1, keepdims=True)
+
+
+def phase_frames(phase, materials, orientation=None):
+    """Per-voxel phase slots and frames for the stencil that rotates each
+    voxel's phase row itself (Cell.set_frames): (phase rows (P,21) FP64 in
+    their own frames, local-frame flags (P,), slot per voxel (N,) uint8, unit
+    quaternion per voxel (N,4) FP64, identity outside local-frame phases).
+    The frames are the ones build_field rotates by (same validation)."""
+    phase = np.asarray(phase)
+    labels, tensors, local = phase_table(materials)
+    if len(labels) > 256:
+        raise ValueError("Per-voxel frames take at most 256 material labels")
+    flat = phase.reshape(-1)
+    slot = np.searchsorted(labels, flat)
+    if (slot >= len(labels)).any() or not np.array_equal(labels[np.minimum(slot, len(labels) - 1)], flat):
+        used = np.unique(flat)
+        raise ValueError(f"Undefined material labels: {used[~np.isin(used, labels)].tolist()}")
+    quat = np.zeros((flat.size, 4))
+    quat[:, 0] = 1.0
+    for p in np.flatnonzero(local):
+        selected = np.flatnonzero(slot == p)
+        if not selected.size:
+            continue
+        if orientation is None:
+            raise ValueError("Local-frame anisotropic phases require orientation")
+        quat[selected] = rotation_quaternions(frames(orientation, phase.shape, selected))
+    return pack21(tensors), local, slot.astype(np.uint8), quat
+
+
 def pack21(C):
     return np.asarray(C)[..., _PI, _PJ]
 

@@ -28,7 +28,10 @@ stencil: "modal" (the modal VALU stencil, default) or "mfma" (the FP32
 operator of the mixed solve as label-grouped dense element matrices on the
 matrix cores, the reference's grouped-kernel form,
 _core/runtime/fi/grouped_kernels.cu:12-74; FI element, at most 4096
-coefficient rows).  The FP64 operator and the certificate are the same.
+coefficient rows) or "frames" (the modal stencil rebuilding each element's
+stiffness from its phase row and a per-voxel FP32 quaternion of its frame
+instead of reading a pre-rotated row; FI element, at most 256 labels).  The
+FP64 operator and the certificate are the same.
 """
 import time
 
@@ -95,6 +98,9 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
         cell = Cell(field, phase.shape, h, element=element, hourglass_coefficient=hourglass_coefficient)
         if stencil == "mfma":
             cell.set_mfma(True)
+        elif stencil == "frames":
+            rows_p, local_p, slot_p, quat_p = mat.phase_frames(phase, table, orientation)
+            cell.set_frames(rows_p, local_p, slot_p, quat_p)
         solver = Solver(cell, lam, mu, mode=precision, nrhs=6, preconditioner=preconditioner, method=solver_method)
         stage("operator")
         out = solver.run(np.eye(6), tol=tol, return_fields=fields, max_iter=max_iter)
@@ -123,7 +129,10 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
     storage = dict(mode="label_rows", n_elements=int(phase.size), pool_rows=rows, index_bytes=index_bytes,
                    coefficient_bytes=rows * 21 * 8,
                    front_end="device" if getattr(field, "index_dev", None) is not None else "host")
+    if stencil == "frames":
+        storage["fp32_operator"] = dict(mode="phase_frames", frame_bytes=int(phase.size) * 17)
     operator = dict(variant=("mfma_label_grouped_dense_element" if stencil == "mfma" else
+                             "modal_fi_lds_stencil_frames" if stencil == "frames" else
                              "modal_fi_lds_stencil" if element == "fi" else "modal_c3d8r_hg_lds_stencil"),
                     element=element, precision=precision, preconditioner=preconditioner, reference_medium=meta)
     if element != "fi":
@@ -159,10 +168,10 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="c
         raise ValueError("hourglass_coefficient must be finite and positive")
     if isinstance(max_iter, bool) or not isinstance(max_iter, (int, np.integer)) or max_iter < 1:
         raise ValueError("max_iter must be a positive integer")
-    if stencil not in ("modal", "mfma"):
-        raise ValueError("stencil must be 'modal' or 'mfma'")
-    if stencil == "mfma" and element != "fi":
-        raise ValueError("stencil='mfma' implements the FI element only")
+    if stencil not in ("modal", "mfma", "frames"):
+        raise ValueError("stencil must be 'modal', 'mfma' or 'frames'")
+    if stencil in ("mfma", "frames") and element != "fi":
+        raise ValueError(f"stencil='{stencil}' implements the FI element only")
     if solver not in ("cg", "block"):
         raise ValueError("solver must be 'cg' or 'block'")
     if solver == "block" and preconditioner == "jacobi":
