@@ -22,6 +22,7 @@
 //   4. y: inverse complex
 //   5. x: complex-to-real (two rows per complex FFT)
 #include <functional>
+#include <type_traits>
 
 #include "vxl_cg.h"
 
@@ -493,8 +494,9 @@ __device__ __forceinline__ double green_point(C& q0, C& q1, C& q2, const AxisF<T
   const C o0 = {scale * (a0.x - w0 * sdot.x), scale * (a0.y - w0 * sdot.y)};
   const C o1 = {scale * (a1.x - w1 * sdot.x), scale * (a1.y - w1 * sdot.y)};
   const C o2 = {scale * (a2.x - w2 * sdot.x), scale * (a2.y - w2 * sdot.y)};
-  const double d = double(q0.x) * o0.x + double(q0.y) * o0.y + double(q1.x) * o1.x + double(q1.y) * o1.y +
-                   double(q2.x) * o2.x + double(q2.y) * o2.y;
+  // Re(conj(q) . out) = q^H G q / (t N) >= 0 (G positive definite): the six
+  // products are summed in T and only the wavevector sum runs in FP64.
+  const double d = double(q0.x * o0.x + q0.y * o0.y + q1.x * o1.x + q1.y * o1.y + q2.x * o2.x + q2.y * o2.y);
   q0 = o0;
   q1 = o1;
   q2 = o2;
@@ -534,8 +536,9 @@ __device__ __forceinline__ double green_point_hg(C& q0, C& q1, C& q2, const Axis
   const C o0 = {sc * (q0.x - g * wx * wd.x), sc * (q0.y - g * wx * wd.y)};
   const C o1 = {sc * (q1.x - g * wy * wd.x), sc * (q1.y - g * wy * wd.y)};
   const C o2 = {sc * (q2.x - g * wz * wd.x), sc * (q2.y - g * wz * wd.y)};
-  const double d = double(q0.x) * o0.x + double(q0.y) * o0.y + double(q1.x) * o1.x + double(q1.y) * o1.y +
-                   double(q2.x) * o2.x + double(q2.y) * o2.y;
+  // Re(conj(q) . out) = q^H G q / (t N) >= 0 (G positive definite): the six
+  // products are summed in T and only the wavevector sum runs in FP64.
+  const double d = double(q0.x * o0.x + q0.y * o0.y + q1.x * o1.x + q1.y * o1.y + q2.x * o2.x + q2.y * o2.y);
   q0 = o0;
   q1 = o1;
   q2 = o2;
@@ -618,8 +621,64 @@ fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int T
 // global row segment is TJ contiguous complex values.  Twiddles come from the
 // axis table w[k] = exp(-2 pi i k / n).
 
+// cos and sin of x in [0, pi), evaluated at compile time: reduced to
+// [0, pi/4] by the quarter-turn symmetries, then Taylor series (x^2 <= 0.62,
+// 14 terms, far below double rounding).
+constexpr double kPiD = 3.14159265358979323846264338327950288;
+constexpr double taylor_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i < 15; ++i) {
+    term *= -x * x / double((2 * i - 1) * (2 * i));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double taylor_sin(double x) {
+  double term = x, sum = x;
+  for (int i = 1; i < 15; ++i) {
+    term *= -x * x / double((2 * i) * (2 * i + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double const_cos(double x) {
+  if (x > kPiD / 2) return -const_cos(kPiD - x);
+  if (x > kPiD / 4) return taylor_sin(kPiD / 2 - x);
+  return taylor_cos(x);
+}
+constexpr double const_sin(double x) {
+  if (x > kPiD / 2) return const_sin(kPiD - x);
+  if (x > kPiD / 4) return taylor_cos(kPiD / 2 - x);
+  return taylor_sin(x);
+}
+
+// a * exp(-+2 pi i K / N) with the root a compile-time constant (0 <= K < N/2);
+// K = 0 and K = N/4 take no multiplies.
+template <int N, int K, bool INV, typename T, typename C>
+__device__ __forceinline__ C twiddle_const(C a) {
+  if constexpr (K == 0) {
+    return a;
+  } else if constexpr (4 * K == N) {
+    return rot90<INV>(a);
+  } else {
+    constexpr double ang = 2.0 * kPiD * K / N;
+    constexpr double c = const_cos(ang), sn = const_sin(ang);
+    const C tw = {T(c), T(INV ? sn : -sn)};
+    return cmul(a, tw);
+  }
+}
+
+template <int K, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (K < N) {
+    f(std::integral_constant<int, K>());
+    static_for<K + 1, N>(f);
+  }
+}
+
 // In-register DFT of size N = 2^a, 3 * 2^a or 5 * 2^a (even sizes split into
-// even and odd halves); w_N^k = w[k * wstride].
+// even and odd halves, combined with compile-time roots of unity).  w and
+// wstride are not read (the four-step callers pass the axis table).
 template <int N, bool INV, typename T, typename C>
 __device__ __forceinline__ void dft_reg(C* v, const C* w, int wstride) {
   if constexpr (N == 1) {
@@ -644,14 +703,12 @@ __device__ __forceinline__ void dft_reg(C* v, const C* w, int wstride) {
     }
     dft_reg<N / 2, INV, T>(e, w, 2 * wstride);
     dft_reg<N / 2, INV, T>(o, w, 2 * wstride);
-#pragma unroll
-    for (int k = 0; k < N / 2; ++k) {
-      C tw = w[k * wstride];
-      if (INV) tw = cconj(tw);
-      const C t = cmul(o[k], tw);
+    static_for<0, N / 2>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      const C t = twiddle_const<N, k, INV, T>(o[k]);
       v[k] = cadd(e[k], t);
       v[k + N / 2] = csub(e[k], t);
-    }
+    });
   }
 }
 
@@ -662,8 +719,8 @@ struct FourStep {
 };
 
 // Step 1 for column p, lane t: v[] = x[t + N2 k1] -> DFT_N1 -> twiddle -> L.
-template <bool INV, typename T, int N1, int N2, typename C, typename Load>
-__device__ __forceinline__ void four_step_1(C* L, int TJ, int p, int t, const C* w, Load load) {
+template <bool INV, typename T, int N1, int N2, int TS, typename C, typename Load>
+__device__ __forceinline__ void four_step_1(C* L, int p, int t, const C* w, Load load) {
   constexpr int n = N1 * N2;
   C v[N1];
 #pragma unroll
@@ -676,16 +733,16 @@ __device__ __forceinline__ void four_step_1(C* L, int TJ, int p, int t, const C*
       if (INV) tw = cconj(tw);
       v[j1] = cmul(v[j1], tw);
     }
-    L[(j1 * N2 + t) * TJ + p] = v[j1];
+    L[(j1 * N2 + t) * TS + p] = v[j1];
   }
 }
 
 // Step 2 for column p, lane t: u[] = L[t N2 + k2] -> DFT_N2 -> store(t + N1 j2).
-template <bool INV, typename T, int N1, int N2, typename C, typename Store>
-__device__ __forceinline__ void four_step_2(const C* L, int TJ, int p, int t, const C* w, Store store) {
+template <bool INV, typename T, int N1, int N2, int TS, typename C, typename Store>
+__device__ __forceinline__ void four_step_2(const C* L, int p, int t, const C* w, Store store) {
   C u[N2];
 #pragma unroll
-  for (int k2 = 0; k2 < N2; ++k2) u[k2] = L[(t * N2 + k2) * TJ + p];
+  for (int k2 = 0; k2 < N2; ++k2) u[k2] = L[(t * N2 + k2) * TS + p];
   dft_reg<N2, INV, T>(u, w, N1);
 #pragma unroll
   for (int j2 = 0; j2 < N2; ++j2) store(t + N1 * j2, u[j2]);
@@ -693,6 +750,31 @@ __device__ __forceinline__ void four_step_2(const C* L, int TJ, int p, int t, co
 
 // Threads per four-step y / z workgroup, at most.
 constexpr int kThreads4 = 512;
+constexpr size_t kMaxLds4 = 65536;  // dynamic LDS per y / z workgroup, at most
+
+// Columns per workgroup of the y pass and kx per workgroup of the z pass for
+// an axis of N1 x N2: they are also the LDS row strides, fixed at compile time
+// so every LDS access of the transforms is an immediate offset from one base
+// (the host launches with these or, for short rows, fewer live columns).
+// FP32 y: 32 columns (256 B row segments, measured 2.5% faster per apply
+// than 16); z: 10 wavenumbers x 3 components (240 B; 0.5% faster per solve
+// than 5, while 3 and 4 are slower); FP64 >= 128 B.
+template <typename T, int N1, int N2>
+constexpr int y4_cols() {
+  constexpr int lanes = N1 > N2 ? N1 : N2;
+  int c = sizeof(T) == 4 ? 32 : 16;
+  if (c > kThreads4 / lanes) c = kThreads4 / lanes;
+  while (c > 1 && 2 * sizeof(T) * size_t(N1 * N2) * c > kMaxLds4) --c;
+  return c;
+}
+template <typename T, int N1, int N2>
+constexpr int z4_kx() {
+  constexpr int lanes = N1 > N2 ? N1 : N2;
+  int k = sizeof(T) == 4 ? 10 : 3;
+  if (k > kThreads4 / (3 * lanes)) k = kThreads4 / (3 * lanes);
+  while (k > 1 && 2 * sizeof(T) * size_t(N1 * N2) * 3 * k + 4 * sizeof(T) * size_t(k) > kMaxLds4) --k;
+  return k < 1 ? 1 : k;
+}
 
 template <bool INV, typename T, int N1, int N2, typename SC>
 __global__ void __launch_bounds__(kThreads4)
@@ -704,18 +786,22 @@ fft_y4(SC* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict
   const uint3 b = xcd_block3();
   if (active && !active[b.z]) return;
   C* L = reinterpret_cast<C*>(smem_raw);
+  constexpr int TS = y4_cols<T, N1, N2>();  // LDS row stride (TJ <= TS)
   const int j0 = b.x * TJ;
   const int tj = min(TJ, W - j0);
   SC* plane = S + int64_t(b.z) * spec_stride + int64_t(b.y) * n * W + j0;
-  const int p = threadIdx.x % TJ, t = threadIdx.x / TJ;
+  const int p = threadIdx.x % TS, t = threadIdx.x / TS;
   const bool col = p < tj;
+  // Offsets inside one (rhs, z) plane of n x W entries stay below 2^31.
+  SC* const base = plane + (t * W + p);
+  const int step = N2 * W;
   if (t < N2)
-    four_step_1<INV, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C {
-      return col ? sload<C>(plane + int64_t(t + N2 * k1) * W + p) : C{T(0), T(0)};
+    four_step_1<INV, T, N1, N2, TS>(L, p, t, w, [&](int k1) -> C {
+      return col ? sload<C>(base + k1 * step) : C{T(0), T(0)};
     });
   __syncthreads();
   if (t < N1 && col)
-    four_step_2<INV, T, N1, N2>(L, TJ, p, t, w, [&](int e, C v) { sstore(plane + int64_t(e) * W + p, v); });
+    four_step_2<INV, T, N1, N2, TS>(L, p, t, w, [&](int e, C v) { sstore(plane + (e * W + p), v); });
 }
 
 // Launched with a multiple of 64 threads; threads past 3 TK lanes-per-column
@@ -731,12 +817,15 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
   const uint3 b = xcd_block3();
   if (active && !active[b.z]) return;
   C* L = reinterpret_cast<C*>(smem_raw);
-  const int TJ = 3 * TK;
+  constexpr int TKC = z4_kx<T, N1, N2>();  // kx per workgroup (TK <= TKC)
+  constexpr int TJ = 3 * TKC;              // columns, also the LDS row stride
   const int k0 = b.x * TK;
   const int y = b.y;
   const int tk = min(TK, nxh - k0);
   const int W = 3 * nxh;
-  const int64_t zstride = int64_t(ny) * W;
+  // Offsets inside one right-hand side's spectrum (nz * ny * W entries, below
+  // 2^31: the host checks) in 32-bit arithmetic.
+  const int zstride = ny * W;
   SC* col = S + int64_t(b.z) * spec_stride + int64_t(y) * W + 3 * k0;
   const int p = threadIdx.x % TJ, t = threadIdx.x / TJ;
   const bool live = p < 3 * tk;
@@ -754,8 +843,8 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
   const AxisF<T> ay = axis_factors(tb.gy, ny, y, hg);
   // forward z transform: global -> L (natural kz order)
   if (t < N2)
-    four_step_1<false, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C {
-      return live ? sload<C>(col + (t + N2 * k1) * zstride + p) : C{T(0), T(0)};
+    four_step_1<false, T, N1, N2, TJ>(L, p, t, w, [&](int k1) -> C {
+      return live ? sload<C>(col + ((t + N2 * k1) * zstride + p)) : C{T(0), T(0)};
     });
   __syncthreads();
   C u[N2];
@@ -796,10 +885,10 @@ fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T
   }
   __syncthreads();
   if (t < N2)
-    four_step_1<true, T, N1, N2>(L, TJ, p, t, w, [&](int k1) -> C { return v[k1]; });
+    four_step_1<true, T, N1, N2, TJ>(L, p, t, w, [&](int k1) -> C { return v[k1]; });
   __syncthreads();
   if (t < N1 && live)
-    four_step_2<true, T, N1, N2>(L, TJ, p, t, w, [&](int e, C val) { sstore(col + e * zstride + p, val); });
+    four_step_2<true, T, N1, N2, TJ>(L, p, t, w, [&](int e, C val) { sstore(col + (e * zstride + p), val); });
 }
 
 // ---- four-step x passes for nx with a four-step factorisation ----------------
@@ -1102,6 +1191,30 @@ inline int four_step_lanes(int n) {
   return 0;
 }
 
+// Compile-time column counts of the y and z kernels for a runtime axis length
+// (the factorisations of launch_y4 / launch_z4).
+#define VXL_YZ4_FACTORS(M)                                                                                 \
+  M(2, 1, 2) M(4, 2, 2) M(8, 2, 4) M(16, 4, 4) M(32, 4, 8) M(64, 8, 8) M(128, 8, 16) M(256, 16, 16) M(512, 16, 32) \
+  M(24, 3, 8) M(40, 5, 8) M(48, 6, 8) M(80, 5, 16) M(96, 6, 16) M(160, 10, 16) M(192, 12, 16)
+template <typename T>
+int y4_cols_of(int n) {
+#define VXL_C(N, A, B) \
+  case N:              \
+    return y4_cols<T, A, B>();
+  switch (n) { VXL_YZ4_FACTORS(VXL_C) }
+#undef VXL_C
+  return 0;
+}
+template <typename T>
+int z4_kx_of(int n) {
+#define VXL_C(N, A, B) \
+  case N:              \
+    return z4_kx<T, A, B>();
+  switch (n) { VXL_YZ4_FACTORS(VXL_C) }
+#undef VXL_C
+  return 0;
+}
+
 template <bool INV, typename T, typename SC>
 void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int W, int TJ,
                const typename CT<T>::type* w, int64_t stride, const int* active) {
@@ -1302,33 +1415,26 @@ int create_impl(vxl_precond* pc, PrecondImpl* im) {
   if (im->lds_x > kMaxLds || im->lds_y > kMaxLds || im->lds_z > kMaxLds)
     return fail("grid axis too long for the LDS pencil FFT");
   {
-    const int cols = sizeof(T) == 4 ? 16 : 8;  // >= 128 B row segments (z pass)
-    // y pass: 256 B row segments (measured 2.5 % faster per apply than
-    // 128 B at 256^3, FP32)
-    const int ycols = sizeof(T) == 4 ? 32 : 16;
     const int ly = four_step_lanes(im->ny), lz = four_step_lanes(im->nz);
     const char* env = std::getenv("VXL_FFT_FOURSTEP");
     const bool enable = !(env && env[0] == '0');
     im->y4 = (enable && ly) ? im->ny : 0;
     im->z4 = (enable && lz) ? im->nz : 0;
     if (im->y4) {
-      im->tj4 = std::max(1, std::min({ycols, kThreads4 / ly, 3 * im->nxh}));
-      if (const int v = env_int("VXL_FFT_YCOLS")) im->tj4 = std::min({v, kThreads4 / ly, 3 * im->nxh});
-      while (im->tj4 > 1 && sizeof(C) * size_t(im->ny) * im->tj4 > kMaxLds) --im->tj4;
-      im->thr_y4 = im->tj4 * ly;
-      im->lds_y4 = sizeof(C) * size_t(im->ny) * im->tj4;
-      if (im->lds_y4 > kMaxLds) im->y4 = 0;
+      // The kernel's compile-time column count sets the LDS stride and the
+      // thread layout; rows shorter than it run with fewer live columns.
+      const int cols4 = y4_cols_of<T>(im->ny);
+      im->tj4 = std::max(1, std::min(cols4, 3 * im->nxh));
+      im->thr_y4 = cols4 * ly;
+      im->lds_y4 = sizeof(C) * size_t(im->ny) * cols4;
+      if (im->lds_y4 > kMaxLds || im->thr_y4 > kThreads4) im->y4 = 0;
     }
     if (im->z4) {
-      // FP32: 10 wavenumbers (240 B row segments, 512 threads at nz = 256;
-      // 0.5% faster per solve than 5 at 256^3), FP64: >= 128 B.
-      const int zk = sizeof(T) == 4 ? 10 : (cols + 2) / 3;
-      im->tk4 = std::max(1, std::min({zk, kThreads4 / (3 * lz), im->nxh}));
-      if (const int v = env_int("VXL_FFT_ZCOLS")) im->tk4 = std::min({v, kThreads4 / (3 * lz), im->nxh});
-      while (im->tk4 > 1 && sizeof(C) * size_t(im->nz) * 3 * im->tk4 + 4 * sizeof(T) * size_t(im->tk4) > kMaxLds)
-        --im->tk4;
-      im->thr_z4 = (3 * im->tk4 * lz + 63) / 64 * 64;  // whole waves for the block reduction
-      im->lds_z4 = sizeof(C) * size_t(im->nz) * 3 * im->tk4 + 4 * sizeof(T) * size_t(im->tk4);  // + x factors
+      const int kx4 = z4_kx_of<T>(im->nz);
+      if (int64_t(im->nz) * im->ny * 3 * im->nxh >= (int64_t(1) << 31)) im->z4 = 0;  // 32-bit offsets
+      im->tk4 = std::max(1, std::min(kx4, im->nxh));
+      im->thr_z4 = (3 * kx4 * lz + 63) / 64 * 64;  // whole waves for the block reduction
+      im->lds_z4 = sizeof(C) * size_t(im->nz) * 3 * kx4 + 4 * sizeof(T) * size_t(kx4);  // + x factors
       if (im->lds_z4 > kMaxLds || im->thr_z4 > kThreads4 || im->thr_z4 < 1) im->z4 = 0;
     }
   }
