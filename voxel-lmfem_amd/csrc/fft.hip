@@ -494,9 +494,11 @@ __device__ __forceinline__ double green_point(C& q0, C& q1, C& q2, const AxisF<T
   const C o0 = {scale * (a0.x - w0 * sdot.x), scale * (a0.y - w0 * sdot.y)};
   const C o1 = {scale * (a1.x - w1 * sdot.x), scale * (a1.y - w1 * sdot.y)};
   const C o2 = {scale * (a2.x - w2 * sdot.x), scale * (a2.y - w2 * sdot.y)};
-  // Re(conj(q) . out) = q^H G q / (t N) >= 0 (G positive definite): the six
-  // products are summed in T and only the wavevector sum runs in FP64.
-  const double d = double(q0.x * o0.x + q0.y * o0.y + q1.x * o1.x + q1.y * o1.y + q2.x * o2.x + q2.y * o2.y);
+  // Re(conj(q) . out) in FP64 products: a right-hand side at rounding-noise
+  // level (a load whose exact fluctuation is zero) has FP32 products below
+  // the normal range, which would stop the recurrence.
+  const double d = double(q0.x) * o0.x + double(q0.y) * o0.y + double(q1.x) * o1.x + double(q1.y) * o1.y +
+                   double(q2.x) * o2.x + double(q2.y) * o2.y;
   q0 = o0;
   q1 = o1;
   q2 = o2;
@@ -536,9 +538,11 @@ __device__ __forceinline__ double green_point_hg(C& q0, C& q1, C& q2, const Axis
   const C o0 = {sc * (q0.x - g * wx * wd.x), sc * (q0.y - g * wx * wd.y)};
   const C o1 = {sc * (q1.x - g * wy * wd.x), sc * (q1.y - g * wy * wd.y)};
   const C o2 = {sc * (q2.x - g * wz * wd.x), sc * (q2.y - g * wz * wd.y)};
-  // Re(conj(q) . out) = q^H G q / (t N) >= 0 (G positive definite): the six
-  // products are summed in T and only the wavevector sum runs in FP64.
-  const double d = double(q0.x * o0.x + q0.y * o0.y + q1.x * o1.x + q1.y * o1.y + q2.x * o2.x + q2.y * o2.y);
+  // Re(conj(q) . out) in FP64 products: a right-hand side at rounding-noise
+  // level (a load whose exact fluctuation is zero) has FP32 products below
+  // the normal range, which would stop the recurrence.
+  const double d = double(q0.x) * o0.x + double(q0.y) * o0.y + double(q1.x) * o1.x + double(q1.y) * o1.y +
+                   double(q2.x) * o2.x + double(q2.y) * o2.y;
   q0 = o0;
   q1 = o1;
   q2 = o2;
