@@ -521,8 +521,11 @@ template <typename T, typename Idx, bool kScale, int TX, int TY, bool kFuse, boo
 __global__ void __launch_bounds__((Tile<TX, TY>::NT), (sizeof(T) == 4 ? (2 * Tile<TX, TY>::NT / 64 + 3) / 4 : 0))
 apply_kernel_v2(ApplyArgs<T> a) {
   using TL = Tile<TX, TY>;
-  __shared__ T su[3][TL::PLANE];
-  __shared__ T sc[12][TL::EL];
+  // kPF: two staged planes and two force buffers (layer parity), so one
+  // barrier per layer separates the writers of one buffer from its readers.
+  constexpr int NB = kPF ? 2 : 1;
+  __shared__ T su[NB][3][TL::PLANE];
+  __shared__ T sc[NB][12][TL::EL];
   __shared__ double red[TL::NT / 64];
 
   const int tid = threadIdx.x;
@@ -551,25 +554,25 @@ apply_kernel_v2(ApplyArgs<T> a) {
     const int px = idx % TL::PX, py = idx / TL::PX;
     ld_off[k] = wrapmod(y0 - 1 + py, ny) * nx + wrapmod(x0 - 1 + px, nx);
   }
-  auto load_plane = [&](int z) {
+  auto load_plane = [&](int z, int buf) {
     const int64_t base = int64_t(wrapmod(z, nz)) * nxy;
 #pragma unroll
     for (int k = 0; k < TL::ITEMS; ++k) {
       if (ld_lds[k] < 0) continue;
       const int64_t node = base + ld_off[k];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) su[c][ld_lds[k]] = u[c * n + node];
+      for (int c = 0; c < 3; ++c) su[buf][c][ld_lds[k]] = u[c * n + node];
     }
   };
 
   const bool is_elem = tid < TL::EL;
   const int exl = tid % TL::EX, eyl = tid / TL::EX;
   const int e_xy = wrapmod(y0 - 1 + eyl, ny) * nx + wrapmod(x0 - 1 + exl, nx);
-  auto face_modes = [&](T (&f)[3][4]) {
+  auto face_modes = [&](T (&f)[3][4], int buf) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
-      for (int a2 = 0; a2 < 4; ++a2) f[c][a2] = su[c][(eyl + (a2 >> 1)) * TL::PX + exl + (a2 & 1)];
+      for (int a2 = 0; a2 < 4; ++a2) f[c][a2] = su[buf][c][(eyl + (a2 >> 1)) * TL::PX + exl + (a2 & 1)];
       face_forward(f[c]);
     }
   };
@@ -628,9 +631,9 @@ apply_kernel_v2(ApplyArgs<T> a) {
   // plane is resident (one layer earlier) and kept in registers.
   double rr = 0.0;  // kResid: sum of r^2 over the nodes this thread writes
   T pq = T(0);  // at most zchunk nodes per thread; FP64 from the block sum on
-  auto node_value = [&](T (&v)[3]) {
+  auto node_value = [&](T (&v)[3], int buf) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] = su[c][(yl + 1) * TL::PX + xl + 1];
+    for (int c = 0; c < 3; ++c) v[c] = su[buf][c][(yl + 1) * TL::PX + xl + 1];
   };
 
   // Next-plane prefetch: raw values of the plane after next are loaded into
@@ -647,29 +650,29 @@ apply_kernel_v2(ApplyArgs<T> a) {
       for (int c = 0; c < 3; ++c) pfz[k][c] = u[c * n + node];
     }
   };
-  auto commit = [&]() {
+  auto commit = [&](int buf) {
 #pragma unroll
     for (int k = 0; k < TL::ITEMS; ++k) {
       if (ld_lds[k] < 0) continue;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) su[c][ld_lds[k]] = pfz[k][c];
+      for (int c = 0; c < 3; ++c) su[buf][c][ld_lds[k]] = pfz[k][c];
     }
   };
 
   // Prologue: bottom face of layer z0-1, layer z0-1 itself, then plane z0+1.
-  load_plane(z0 - 1);
+  load_plane(z0 - 1, 0);
   __syncthreads();
-  if (is_elem) face_modes(Bf);
+  if (is_elem) face_modes(Bf, 0);
   __syncthreads();
-  load_plane(z0);
+  load_plane(z0, 0);
   __syncthreads();
   T pnA[3] = {T(0), T(0), T(0)}, pnB[3] = {T(0), T(0), T(0)};  // kFuse: u at this thread's node
   if constexpr (kFuse) {
-    if (node_valid) node_value(pnA);
+    if (node_valid) node_value(pnA, 0);
   }
   if (is_elem) {
     T Tf[3][4], Fb[3][4];
-    face_modes(Tf);
+    face_modes(Tf, 0);
     T c[21];
     coeffs(z0 - 1, row_of(z0 - 1), c);
     element(c, Bf, Tf, Fb, Ftp);
@@ -680,7 +683,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
   }
   __syncthreads();
   if constexpr (kPF) {
-    load_plane(z0 + 1);
+    load_plane(z0 + 1, 0);  // the first layer's buffer
     __syncthreads();
   }
   // The material row of the next layer is fetched one step ahead, so the
@@ -691,8 +694,11 @@ apply_kernel_v2(ApplyArgs<T> a) {
   // below put on that face; Bn/Fn receive the top face modes and top force,
   // which are the next layer's Bc/Fp.  The loop alternates two register sets,
   // so no copies are needed between layers.
-  auto layer = [&](int z, const T (&Bc)[3][4], const T (&Fp)[3][4], T (&Bn)[3][4], T (&Fn)[3][4],
+  // par: layer parity (kPF: this layer's plane and force buffer; the plane
+  // after next goes to the other one).
+  auto layer = [&](auto par, int z, const T (&Bc)[3][4], const T (&Fp)[3][4], T (&Bn)[3][4], T (&Fn)[3][4],
                    const T (&pc)[3], T (&pn)[3]) {
+    constexpr int P = kPF ? decltype(par)::value : 0;
     int row_cur = row_next;
     if constexpr (kFuse) {
       if (is_elem) row_cur = row_of(z);
@@ -717,13 +723,13 @@ apply_kernel_v2(ApplyArgs<T> a) {
       if (is_elem) coeffs(z, row_cur, c);
     }
     if constexpr (!kPF) {
-      load_plane(z + 1);
+      load_plane(z + 1, 0);
       __syncthreads();
     }
     // LDS holds plane z+1: read everything this step needs from it first.
-    if (is_elem) face_modes(Bn);
+    if (is_elem) face_modes(Bn, P);
     if constexpr (kFuse) {
-      if (node_valid && z + 1 < z1) node_value(pn);
+      if (node_valid && z + 1 < z1) node_value(pn, P);
     }
     if (more) prefetch(z + 2);
     if (is_elem) {
@@ -737,11 +743,13 @@ apply_kernel_v2(ApplyArgs<T> a) {
         for (int m = 0; m < 4; ++m) F[m] = Fp[cc][m] + Fb[cc][m];
         face_transpose(F);
 #pragma unroll
-        for (int a2 = 0; a2 < 4; ++a2) sc[a2 * 3 + cc][tid] = F[a2];
+        for (int a2 = 0; a2 < 4; ++a2) sc[P][a2 * 3 + cc][tid] = F[a2];
       }
     }
+    if constexpr (kPF) {
+      if (more) commit(1 - P);  // the other plane buffer: last read one layer ago
+    }
     __syncthreads();
-    if (more) commit();
     if (node_valid) {
       T acc[3] = {T(0), T(0), T(0)};
 #pragma unroll
@@ -751,7 +759,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
           const int e = (yl + dy) * TL::EX + xl + dx;
           const int corner = (1 - dx) + 2 * (1 - dy);
 #pragma unroll
-          for (int cc = 0; cc < 3; ++cc) acc[cc] += sc[corner * 3 + cc][e];
+          for (int cc = 0; cc < 3; ++cc) acc[cc] += sc[P][corner * 3 + cc][e];
         }
       const int64_t node = int64_t(z) * nxy + n_xy;
       if constexpr (kResid) {
@@ -771,12 +779,14 @@ apply_kernel_v2(ApplyArgs<T> a) {
         if constexpr (kFuse) pq = fma(pc[2], acc[2], fma(pc[1], acc[1], fma(pc[0], acc[0], pq)));
       }
     }
-    if constexpr (kPF) __syncthreads();
+    // kPF: no barrier here.  The next layer writes the other force buffer
+    // and reads the plane committed above; this layer's buffers are written
+    // again two layers on, after the next layer's barrier.
   };
   T Bf2[3][4], Ftp2[3][4];
   for (int z = z0; z < z1; z += 2) {
-    layer(z, Bf, Ftp, Bf2, Ftp2, pnA, pnB);
-    if (z + 1 < z1) layer(z + 1, Bf2, Ftp2, Bf, Ftp, pnB, pnA);
+    layer(std::integral_constant<int, 0>(), z, Bf, Ftp, Bf2, Ftp2, pnA, pnB);
+    if (z + 1 < z1) layer(std::integral_constant<int, 1>(), z + 1, Bf2, Ftp2, Bf, Ftp, pnB, pnA);
   }
   if constexpr (kFuse || kResid) {
     const double total = block_reduce_sum<TL::NT>(kResid ? rr : double(pq), red);
