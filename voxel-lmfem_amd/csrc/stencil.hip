@@ -510,8 +510,11 @@ __device__ __forceinline__ void rotate_row(float (&c)[21], float4 q, int kind) {
 // face (xy) Hadamard modes of its bottom node face and the force its previous
 // layer put on that face in registers, so each layer transforms one new face
 // (4 nodes), runs the modal element action, and exchanges only the 4 corner
-// forces of one face through LDS.  One staged node plane; two barriers per
-// plane.  Same arithmetic as the full 8-point transform, regrouped.
+// forces of one face through LDS.  kPF (schedule 3): the plane after next is
+// prefetched into registers and the staged planes and forces are double
+// buffered by layer parity, one barrier per layer; otherwise one staged plane
+// and two barriers per layer.  Same arithmetic as the full 8-point
+// transform, regrouped.
 // kIso: cubic voxels, a.g holds the squared modal factors (strain_mode).
 // kHG: the C3D8R-HG element instead of the fully integrated one.
 // FP32: the register budget lets two workgroups share a CU (waves per SIMD).
