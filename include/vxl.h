@@ -165,7 +165,8 @@ int vxl_set_stencil_config(vxl_cell* cell, int precision, int tile_y, int varian
  * and schedule 3; preconditioners built afterwards use its symbol (FFT) or
  * diagonal (Jacobi).  Set it before creating solvers or preconditioners.
  * hourglass_coefficient must be finite and positive (the reference default
- * is 0.1); it is ignored for VXL_ELEMENT_FI. */
+ * is 0.1); it is ignored for VXL_ELEMENT_FI.  VXL_ERR_ARG for the HG element
+ * while the matrix-core stencil or per-voxel frames are on (FI only). */
 int vxl_cell_set_element(vxl_cell* cell, int element, double hourglass_coefficient);
 int vxl_get_stencil_config(vxl_cell* cell, int precision, int* tile_y, int* variant, int* zmin);
 /* Matrix-core FP32 stencil (enable 1) or the modal VALU stencil (enable 0,

@@ -126,6 +126,12 @@ def test_frames_homogenize_matches_oracle(torch_gpu, axis_pairs, cubic_only):
     assert err < 1e-7
 
 
+def _set_hg(cell):
+    from voxel_lmfem import _native
+    _native.check(cell.lib.vxl_cell_set_element(cell.handle, _native.VXL_ELEMENT_C3D8R_HG, 0.1),
+                  "vxl_cell_set_element")
+
+
 def test_frames_refusals(torch_gpu):
     torch = torch_gpu
     from voxel_lmfem import homogenize
@@ -144,6 +150,8 @@ def test_frames_refusals(torch_gpu):
         cell.set_frames(rows_p, local_p, slot_p, quat_p)
         with pytest.raises(ValueError, match="per-voxel frames"):
             cell.set_mfma(True)
+        with pytest.raises(ValueError, match="FI element only"):
+            _set_hg(cell)
         cell.set_frames(None, None, None, None)
     finally:
         cell.close()

@@ -496,6 +496,8 @@ int vxl_cell_set_element(vxl_cell* cell, int element, double hourglass_coefficie
     return fail("vxl_cell_set_element: invalid arguments");
   if (element == VXL_ELEMENT_C3D8R_HG && !(std::isfinite(hourglass_coefficient) && hourglass_coefficient > 0.0))
     return fail("hourglass_coefficient must be finite and positive");
+  if (element == VXL_ELEMENT_C3D8R_HG && (cell->mfma || cell->frame_index))
+    return fail("the matrix-core stencil and per-voxel frames implement the FI element only; turn them off first");
   cell->element = element;
   if (element == VXL_ELEMENT_C3D8R_HG) cell->hg_coef = hourglass_coefficient;
   return VXL_OK;
