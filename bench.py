@@ -227,15 +227,18 @@ def frames_leg(n=256, R=6, reps=20):
     return out
 
 
-def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg"):
+def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg", stack=1, stack_voxels=64 ** 3,
+              sizes=(32, 48, 64, 80, 96)):
     """Uneven batch: `count` two-phase random-inclusion cells (edges drawn from
     {32, 48, 64, 80, 96}, volume fraction U[0.1, 0.4], radius 2-5 voxels,
     contrast log-uniform in [10, 1000], nu = 0.3), six loads each, mixed
     precision, certified <= 1e-8.  Labels are resident in HBM before timing; a
     step is every cell's device front end, reference medium, operator and
     preconditioner set-up, the concurrent solves (vxl_solver_run_batch,
-    largest cells first, `streams` HIP streams, `workers` host threads) and
-    teardown."""
+    largest first, `streams` HIP streams, `workers` host threads) and
+    teardown.  stack > 1: cells of the same edge with at most `stack_voxels`
+    voxels run up to `stack` at a time as one stacked cell (every launch
+    covers all of them)."""
     import torch
     from voxel_lmfem import autotune as tuner
     from voxel_lmfem.cells import random_inclusion_batch
@@ -243,7 +246,7 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg"):
     from voxel_lmfem.gpu import Cell, Solver, run_batch
     from voxel_lmfem.reference_medium import select
     t0 = time.perf_counter()
-    cells = random_inclusion_batch(count=count, seed=7)
+    cells = random_inclusion_batch(count=count, seed=7, sizes=tuple(sizes))
     print(f"# generated {count} inclusion cells in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     resident = [(torch.from_numpy(ph.astype(np.int32)).to("cuda"), ph, mats, dom) for (ph, mats, dom, _), _ in cells]
     pool = [torch.cuda.Stream() for _ in range(streams)]
@@ -253,16 +256,42 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg"):
 
     split = {"setup": 0.0, "solve": 0.0, "teardown": 0.0}
 
+    by_edge = {}
+    for k, (_, ph, _, _) in enumerate(resident):
+        by_edge.setdefault(ph.shape, []).append(k)
+    # A stack runs until its slowest cell certifies: stack cells of similar
+    # stiffness contrast (iteration counts follow it).
+    contrast = [max(m["E"] for m in mats.values()) / min(m["E"] for m in mats.values())
+                for _, _, mats, _ in resident]
+    for ks in by_edge.values():
+        ks.sort(key=lambda k: contrast[k])
+    groups = []
+    for shape, ks in by_edge.items():
+        width = max(1, stack) if int(np.prod(shape)) <= stack_voxels else 1
+        groups += [ks[i:i + width] for i in range(0, len(ks), width)]
+
     def step():
         built = []
         t_a = time.perf_counter()
-        for k, (lab, ph, mats, dom) in enumerate(resident):
-            field = pm.build_field_device(lab, mats)
-            lam, mu, _ = select(field, ph, mats)
-            cell = Cell(field, ph.shape, dom / np.array(ph.shape[::-1], dtype=float), stream=pool[k % streams])
-            got = tuner.tune(cell, "float32", 6, entries=entries)  # tunes only on a cache miss (warm-up)
-            configs[ph.shape[0]] = (got["tile_y"], got["schedule"], got["zmin"])
-            built.append((cell, Solver(cell, lam, mu, mode="mixed", nrhs=6, method=method)))
+        for j, ks in enumerate(groups):
+            fields, lams, mus = [], [], []
+            for k in ks:
+                lab, ph, mats, dom = resident[k]
+                field = pm.build_field_device(lab, mats)
+                lam, mu, _ = select(field, ph, mats)
+                fields.append(field)
+                lams.append(lam)
+                mus.append(mu)
+            h = dom / np.array(ph.shape[::-1], dtype=float)
+            if len(ks) == 1:
+                cell = Cell(fields[0], ph.shape, h, stream=pool[j % streams])
+                lam, mu = lams[0], mus[0]
+            else:
+                cell = Cell.stack(fields, ph.shape, h, stream=pool[j % streams])
+                lam, mu = lams, mus
+            got = tuner.tune(cell, "float32", 6 * len(ks), entries=entries)  # tunes only on a cache miss
+            configs[(ph.shape[0], len(ks))] = (got["tile_y"], got["schedule"], got["zmin"])
+            built.append((cell, Solver(cell, lam, mu, mode="mixed", method=method)))
         t_b = time.perf_counter()
         outs = run_batch([b[1] for b in built], tol=1e-8, workers=workers)
         t_c = time.perf_counter()
@@ -289,16 +318,19 @@ def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg"):
     its = [int(i) for o in outs for i in o["iterations"]]
     worst = max(float(r) for o in outs for r in o["true_residuals"])
     voxels = int(sum(ph.size for _, ph, _, _ in resident))
-    out = dict(workload=f"random_inclusion_batch_{count}_cells_edges_32-96_6loads", method=method, cells=count,
-               voxels=voxels,
+    edges = "-".join(str(v) for v in (min(sizes), max(sizes))) if len(sizes) > 1 else str(sizes[0])
+    out = dict(workload=f"random_inclusion_batch_{count}_cells_edges_{edges}_6loads", method=method, cells=count,
+               voxels=voxels, stack=stack, stack_voxels=stack_voxels, solvers=len(groups),
                cells_per_s=count / dt, ms_per_step=1e3 * dt, voxel_loads_per_s=6 * voxels / dt, steps=steps,
                warmup=warmup, streams=streams, workers=workers, iterations_mean=float(np.mean(its)),
-               stencil_config_by_edge={str(k): dict(tile=f"32x{t}", schedule=v, zmin=z)
+               stencil_config_by_edge={f"{k[0]}x{k[1]}" if k[1] > 1 else str(k[0]):
+                                       dict(tile=f"32x{t}", schedule=v, zmin=z)
                                        for k, (t, v, z) in sorted(configs.items())},
                graph_blocks=int(sum(o["graph_blocks"] for o in outs)),
                host_split_ms={k: 1e3 * v / steps for k, v in split.items()},
                iterations_max=max(its), max_true_residual=worst)
-    print(f"# batch: {1e3 * dt:.1f} ms per {count}-cell batch ({count / dt:.1f} cells/s), iterations mean "
+    print(f"# batch ({len(groups)} solvers, stack {stack}): {1e3 * dt:.1f} ms per {count}-cell batch "
+          f"({count / dt:.1f} cells/s), iterations mean "
           f"{np.mean(its):.1f} max {max(its)}, worst residual {worst:.2e}; set-up "
           f"{1e3 * split['setup'] / steps:.0f} ms, solves {1e3 * split['solve'] / steps:.0f} ms, teardown "
           f"{1e3 * split['teardown'] / steps:.0f} ms", file=sys.stderr, flush=True)
@@ -644,6 +676,10 @@ def main():
         batch["graph_replay"] = legs["1"]
         if not args.skip_block:
             batch["block_cg"] = batch_leg(steps=2, warmup=1, method="block")
+        # Many small cells: 64 inclusion cells of 32^3, one solver per cell
+        # against stacks of ten cells per launch.
+        batch["small_cells"] = dict(per_cell=batch_leg(steps=3, warmup=1, sizes=(32,)),
+                                    stacked=batch_leg(steps=3, warmup=1, sizes=(32,), stack=10))
 
     # The same small cells on the GPU (set-up included, as a step above): the
     # 32^3 sphere cell by FP64 and mixed PCG, and the 16^3 sphere cell.

@@ -205,6 +205,21 @@ int vxl_cell_set_mfma(vxl_cell* cell, int enable);
  * one rotated row per local-frame voxel. */
 int vxl_cell_set_frames(vxl_cell* cell, int nphase, const double* phase_rows, const int* phase_local,
                         const void* phase_index_dev, const float* quat_dev);
+/* Stacked cells: declare that the cell's index array holds `groups` index
+ * blocks of n entries each (cells of this grid and voxel size, one after the
+ * other, indexing the one row table passed at creation), so one launch runs
+ * all of them: right-hand side r of an nrhs-RHS call belongs to cell
+ * r / (nrhs / groups), and nrhs must be a multiple of groups.  The
+ * reference runs one cell per call (solver.py:9-85) and batches only
+ * same-shape cells of one material (_core/runtime/repro/source/femlib/
+ * batched/driver.py:13-60); here the cells may differ in material and each
+ * keeps its own reference medium (vxl_solver_set_media).  Stencil, RHS,
+ * mean stress, FFT preconditioner and solver (per-case CG, mixed or FP64)
+ * take stacked cells; centre fields, the energy gradient, the Jacobi
+ * diagonal, block CG, per-voxel frames, the matrix-core form, a scale field
+ * and the HG element do not (VXL_ERR_ARG).  1 to 64 cells; set before
+ * creating solvers. */
+int vxl_cell_set_groups(vxl_cell* cell, int groups);
 /* Autotuner: times all 27 (tile, schedule, chunk depth in {8, 4, 2})
  * configurations on the given buffers (reps launches each, HIP events on the
  * cell's stream), keeps the fastest on the cell and returns it.  ms_table
@@ -318,6 +333,11 @@ int vxl_solver_set_preconditioner(vxl_solver* s, int kind);
  * default method.  FFT preconditioner only.  The reference's batched PCG
  * solves each case separately; the block method is this build's. */
 int vxl_solver_set_method(vxl_solver* s, int method);
+/* Stacked cells: the isotropic reference medium (lam[g], mu[g]) of each of
+ * the cell's groups (host arrays of vxl_cell_set_groups' count), used by the
+ * FFT preconditioner for that cell's right-hand sides; a reliable update
+ * re-certifies the systems of the cell that triggered it. */
+int vxl_solver_set_media(vxl_solver* s, const double* lam, const double* mu);
 /* Number of four-iteration blocks the last vxl_solver_run replayed from its
  * captured HIP graph (graph replay is opt-in with VXL_GRAPH=1; 0 without it,
  * with the fused forward pass, or when capture is unavailable).  No reference counterpart (the reference launches

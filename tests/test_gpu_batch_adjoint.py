@@ -22,7 +22,7 @@ def test_batch_equals_individual_solves(torch_gpu, tmp_path):
     cells = random_inclusion_batch(count=5, seed=11, sizes=(16, 24, 32))
     print("case batch", [m["n"] for _, m in cells], flush=True)
     inputs = [(phase, mats, dom) for (phase, mats, dom, _), _ in cells]
-    batch = homogenize_batch(inputs, streams=3, workers=3, autotune=False)
+    batch = homogenize_batch(inputs, streams=3, workers=3, autotune=False, stack=1)
     tuned = homogenize_batch(inputs, streams=3, workers=3, tune_cache=tmp_path / "tuning.json")
     assert (tmp_path / "tuning.json").is_file()
     for (phase, mats, dom), got, alt in zip(inputs, batch, tuned):
@@ -48,7 +48,7 @@ def test_batch_graph_replay_equals_single_solves(torch_gpu, monkeypatch):
     cells = random_inclusion_batch(count=4, seed=12, sizes=(16, 24, 32))
     print("case batch graph replay", [m["n"] for _, m in cells], flush=True)
     inputs = [(phase, mats, dom) for (phase, mats, dom, _), _ in cells]
-    batch = homogenize_batch(inputs, streams=2, workers=2, autotune=False)
+    batch = homogenize_batch(inputs, streams=2, workers=2, autotune=False, stack=1)
     assert sum(got["graph_blocks"] for got in batch) > 0
     for (phase, mats, dom), got in zip(inputs, batch):
         one = homogenize(phase, mats, dom)

@@ -28,6 +28,7 @@ int solver_release(vxl_solver* s);
 int solver_set_preconditioner(vxl_solver* s, int kind);
 int solver_prepare_graph(vxl_solver* s, double tol);
 int solver_set_method(vxl_solver* s, int method);
+int solver_set_media(vxl_solver* s, const double* lam, const double* mu);
 
 namespace {
 
@@ -133,6 +134,7 @@ int cell_create_common(vxl_cell** out, int nz, int ny, int nx, double hx, double
   c->frame_table = nullptr;
   c->frame_quat = nullptr;
   c->frame_cubic_only = 0;
+  c->groups = 1;
   (void)hipGetDevice(&c->device);
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(c->stream);
@@ -347,6 +349,7 @@ int vxl_cell_destroy(vxl_cell* cell) {
 int vxl_apply(vxl_cell* cell, int precision, int nrhs, const void* u_dev, void* out_dev) {
   g_error.clear();
   if (!cell || !u_dev || !out_dev || nrhs < 1) return fail("vxl_apply: invalid arguments");
+  if (nrhs % cell->groups) return fail("stacked cells: nrhs must be a multiple of the cell count");
   if (u_dev == out_dev) return fail("vxl_apply: input and output must not alias");
   if (precision == VXL_F64)
     return launch_apply<double>(cell, nrhs, static_cast<const double*>(u_dev), static_cast<double*>(out_dev),
@@ -365,9 +368,19 @@ int vxl_set_stencil_variant(vxl_cell* cell, int precision, int variant) {
   return VXL_OK;
 }
 
+int vxl_cell_set_groups(vxl_cell* cell, int groups) {
+  g_error.clear();
+  if (!cell || groups < 1 || groups > 64) return fail("vxl_cell_set_groups: 1 to 64 cells");
+  if (groups > 1 && (cell->mfma || cell->frame_index || cell->mat.scale || cell->element != VXL_ELEMENT_FI))
+    return fail("stacked cells take the modal FI stencil without a scale field, frames or the matrix-core form");
+  cell->groups = groups;
+  return VXL_OK;
+}
+
 int vxl_cell_set_mfma(vxl_cell* cell, int enable) {
   g_error.clear();
   if (!cell || (enable != 0 && enable != 1)) return fail("vxl_cell_set_mfma: invalid arguments");
+  if (enable && cell->groups > 1) return fail("the matrix-core stencil takes one cell, not a stacked cell");
   if (enable && cell->frame_index) return fail("the matrix-core stencil does not take per-voxel frames");
   if (enable) {
     VXL_TRY(mfma_prepare(cell));
@@ -390,6 +403,7 @@ int vxl_cell_set_frames(vxl_cell* cell, int nphase, const double* phase_rows, co
   if (nphase < 1 || nphase > 256 || !phase_rows || !phase_local || !quat_dev)
     return fail("vxl_cell_set_frames: 1 to 256 phase rows, their frame flags, a phase index and quaternions");
   if (cell->element != VXL_ELEMENT_FI) return fail("per-voxel frames need the FI element");
+  if (cell->groups > 1) return fail("per-voxel frames take one cell, not a stacked cell");
   if (cell->mat.scale) return fail("per-voxel frames do not combine with a scale field");
   if (cell->mfma) return fail("the matrix-core stencil does not take per-voxel frames");
   std::vector<float> table(size_t(nphase) * kRowWidth, 0.0f);
@@ -496,7 +510,7 @@ int vxl_cell_set_element(vxl_cell* cell, int element, double hourglass_coefficie
     return fail("vxl_cell_set_element: invalid arguments");
   if (element == VXL_ELEMENT_C3D8R_HG && !(std::isfinite(hourglass_coefficient) && hourglass_coefficient > 0.0))
     return fail("hourglass_coefficient must be finite and positive");
-  if (element == VXL_ELEMENT_C3D8R_HG && (cell->mfma || cell->frame_index))
+  if (element == VXL_ELEMENT_C3D8R_HG && (cell->mfma || cell->frame_index || cell->groups > 1))
     return fail("the matrix-core stencil and per-voxel frames implement the FI element only; turn them off first");
   cell->element = element;
   if (element == VXL_ELEMENT_C3D8R_HG) cell->hg_coef = hourglass_coefficient;
@@ -630,6 +644,7 @@ int vxl_energy_gradient(vxl_cell* cell, int nload, const double* u_dev, const do
   if (!cell || nload < 1 || nload > 64 || !u_dev || !macro_host || !w_host || !out_dev)
     return fail("vxl_energy_gradient: invalid arguments");
   if (cell->element != VXL_ELEMENT_FI) return fail("vxl_energy_gradient: the contraction is for the FI element");
+  if (cell->groups > 1) return fail("vxl_energy_gradient takes one cell, not a stacked cell");
   for (int i = 0; i < nload; ++i)
     for (int j = 0; j < nload; ++j)
       if (!std::isfinite(w_host[i * nload + j]) || w_host[i * nload + j] != w_host[j * nload + i])
@@ -702,6 +717,7 @@ int vxl_solver_create(vxl_solver** out, vxl_cell* cell, int mode, int nrhs, doub
   g_error.clear();
   if (!out || !cell || nrhs < 1 || nrhs > 64) return fail("vxl_solver_create: invalid arguments");
   if (mode != VXL_SOLVE_FP64 && mode != VXL_SOLVE_MIXED) return fail("mode must be VXL_SOLVE_FP64 or VXL_SOLVE_MIXED");
+  if (nrhs % cell->groups) return fail("stacked cells: nrhs must be a multiple of the cell count");
   *out = nullptr;
   auto* s = new vxl_solver();  // value-initialised: pointers null, counters zero
   s->cell = cell;
@@ -791,6 +807,7 @@ int vxl_solver_run_batch(vxl_solver* const* solvers, int count, const double* ma
 int vxl_cell_inverse_diagonal(vxl_cell* cell, double* out_dev) {
   g_error.clear();
   if (!cell || !out_dev) return fail("vxl_cell_inverse_diagonal: invalid arguments");
+  if (cell->groups > 1) return fail("vxl_cell_inverse_diagonal takes one cell, not a stacked cell");
   VXL_TRY(launch_jacobi_diag(cell, out_dev, nullptr, cell->stream));
   VXL_HIP(hipStreamSynchronize(cell->stream));
   return VXL_OK;
@@ -802,6 +819,8 @@ int vxl_solver_set_preconditioner(vxl_solver* s, int kind) {
     return fail("vxl_solver_set_preconditioner: invalid arguments");
   if (kind == VXL_PRECOND_FFT_BF16 && s->mode != VXL_SOLVE_MIXED)
     return fail("the BF16-stored preconditioner is a mixed-mode option");
+  if (kind == VXL_PRECOND_JACOBI && s->cell->groups > 1)
+    return fail("stacked cells run with the FFT preconditioner");
   return solver_set_preconditioner(s, kind);
 }
 
@@ -809,7 +828,16 @@ int vxl_solver_set_method(vxl_solver* s, int method) {
   g_error.clear();
   if (!s || (method != VXL_METHOD_CG && method != VXL_METHOD_BLOCK_CG))
     return fail("vxl_solver_set_method: invalid arguments");
+  if (method == VXL_METHOD_BLOCK_CG && s->cell->groups > 1)
+    return fail("block CG couples the load cases of one cell; stacked cells run per-case CG");
   return solver_set_method(s, method);
+}
+
+int vxl_solver_set_media(vxl_solver* s, const double* lam, const double* mu) {
+  g_error.clear();
+  if (!s || !lam || !mu) return fail("vxl_solver_set_media: invalid arguments");
+  if (s->cell->groups < 2) return fail("vxl_solver_set_media: the cell is not a stacked cell");
+  return solver_set_media(s, lam, mu);
 }
 
 int vxl_solver_graph_blocks(vxl_solver* s) {

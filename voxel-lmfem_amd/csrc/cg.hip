@@ -100,6 +100,12 @@ flush_kernel(double* x64, DirectionSet<T> d, const double* alpha, int nrhs, int6
   double a[kMaxDirections];
 #pragma unroll
   for (int t = 0; t < kMaxDirections; ++t) a[t] = t < d.count ? alpha[t * nrhs + k] : 0.0;
+  if constexpr (!kAssign) {  // no step pending for this system: x64 stays
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < kMaxDirections; ++t) any = any || a[t] != 0.0;
+    if (!any) return;
+  }
   for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len; i += int64_t(gridDim.x) * kBlock) {
     double acc = kAssign ? 0.0 : x64[off + i];
 #pragma unroll
@@ -134,6 +140,12 @@ flush4_kernel(double* x64, DirectionSet<T> d, const double* alpha, int nrhs, int
   double a[kMaxDirections];
 #pragma unroll
   for (int t = 0; t < kMaxDirections; ++t) a[t] = t < d.count ? alpha[t * nrhs + k] : 0.0;
+  if constexpr (!kAssign) {  // no step pending for this system (converged or inactive): x64 stays
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < kMaxDirections; ++t) any = any || a[t] != 0.0;
+    if (!any) return;
+  }
   const int64_t len4 = len / 4;
   for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < len4; i += int64_t(gridDim.x) * kBlock) {
     double2* xp = reinterpret_cast<double2*>(x64 + off + 4 * i);

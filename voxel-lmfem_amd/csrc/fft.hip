@@ -567,11 +567,15 @@ template <typename T, bool kDot>
 __global__ void __launch_bounds__(kThreads)
 fft_z_green(typename CT<T>::type* __restrict__ S, int ny, int nz, int nxh, int TK, Axes ax,
             Tables<T> tb, T lam, T mu, T inv_vn, int64_t spec_stride, const int* active, int nyq,
-            double* partial) {
+            double* partial, const double* media, int group_rhs) {
   using C = typename CT<T>::type;
   extern __shared__ double2 smem_raw[];
   __shared__ double red[kThreads / 64];
   if (active && !active[blockIdx.z]) return;
+  if (media) {  // stacked cells: this right-hand side's reference medium
+    lam = T(media[2 * (blockIdx.z / group_rhs)]);
+    mu = T(media[2 * (blockIdx.z / group_rhs) + 1]);
+  }
   C* A = reinterpret_cast<C*>(smem_raw);
   const int TJ = 3 * TK;
   C* B = A + TJ * nz;
@@ -813,13 +817,18 @@ fft_y4(SC* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict
 template <typename T, int N1, int N2, bool kDot, typename SC>
 __global__ void __launch_bounds__(kThreads4)
 fft_z4_green(SC* __restrict__ S, int ny, int nxh, int TK, Tables<T> tb, T lam, T mu, T inv_vn,
-             int64_t spec_stride, const int* active, int nyq, double* partial) {
+             int64_t spec_stride, const int* active, int nyq, double* partial, const double* media,
+             int group_rhs) {
   using C = typename CT<T>::type;
   constexpr int nz = N1 * N2;
   extern __shared__ double2 smem_raw[];
   __shared__ double red[kThreads4 / 64];
   const uint3 b = xcd_block3();
   if (active && !active[b.z]) return;
+  if (media) {  // stacked cells: this right-hand side's reference medium
+    lam = T(media[2 * (b.z / group_rhs)]);
+    mu = T(media[2 * (b.z / group_rhs) + 1]);
+  }
   C* L = reinterpret_cast<C*>(smem_raw);
   constexpr int TKC = z4_kx<T, N1, N2>();  // kx per workgroup (TK <= TKC)
   constexpr int TJ = 3 * TKC;              // columns, also the LDS row stride
@@ -1250,11 +1259,11 @@ void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int 
 template <typename T, bool kDot, typename SC>
 void launch_z4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int ny, int nxh,
                int TK, Tables<T> tb, T lam, T mu, T inv_vn, int64_t stride, const int* active, int nyq,
-               double* partial) {
+               double* partial, const double* media, int group_rhs) {
 #define VXL_Z4(N, A, B)                                                                              \
   case N:                                                                                          \
     hipLaunchKernelGGL((fft_z4_green<T, A, B, kDot, SC>), grid, dim3(thr), lds, s, S, ny, nxh, TK, tb, lam, mu, \
-                       inv_vn, stride, active, nyq, partial);                                      \
+                       inv_vn, stride, active, nyq, partial, media, group_rhs);                    \
     break;
   switch (n) {
     VXL_Z4(2, 1, 2)
@@ -1538,24 +1547,26 @@ int apply_stages(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, h
   const double V = pc->cell->volume;
   const double N = double(pc->cell->grid.n);
   const T lam = T(pc->lam), mu = T(pc->mu), inv_vn = T(1.0 / (V * N));
+  const double* media = pc->groups > 1 ? pc->media : nullptr;
+  const int grhs = pc->groups > 1 ? std::max(1, nrhs / pc->groups) : std::max(1, nrhs);
   const int nyq = im->nx % 2 == 0 ? im->nx / 2 : -1;
   double* part = cg ? cg->rz_partial : nullptr;
   if (im->z4) {
     const dim3 g((im->nxh + im->tk4 - 1) / im->tk4, im->ny, nrhs);
     if (cg)
       launch_z4<T, true, SC>(im->z4, g, im->thr_z4, im->lds_z4, s, S, im->ny, im->nxh, im->tk4, tb, lam, mu, inv_vn,
-                         im->spec_stride, active, nyq, part);
+                         im->spec_stride, active, nyq, part, media, grhs);
     else
       launch_z4<T, false, SC>(im->z4, g, im->thr_z4, im->lds_z4, s, S, im->ny, im->nxh, im->tk4, tb, lam, mu, inv_vn,
-                          im->spec_stride, active, nyq, part);
+                          im->spec_stride, active, nyq, part, media, grhs);
   } else if constexpr (kFull) {
     const dim3 g((im->nxh + im->tk - 1) / im->tk, im->ny, nrhs);
     if (cg)
       hipLaunchKernelGGL((fft_z_green<T, true>), g, dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh,
-                         im->tk, im->axes, tb, lam, mu, inv_vn, im->spec_stride, active, nyq, part);
+                         im->tk, im->axes, tb, lam, mu, inv_vn, im->spec_stride, active, nyq, part, media, grhs);
     else
       hipLaunchKernelGGL((fft_z_green<T, false>), g, dim3(kThreads), im->lds_z, s, S, im->ny, im->nz, im->nxh,
-                         im->tk, im->axes, tb, lam, mu, inv_vn, im->spec_stride, active, nyq, part);
+                         im->tk, im->axes, tb, lam, mu, inv_vn, im->spec_stride, active, nyq, part, media, grhs);
   }
   VXL_HIP(hipGetLastError());
   if (cg) VXL_TRY(cg->mid());

@@ -105,7 +105,7 @@ int free_all(vxl_solver* s) {
   s->capture_stream = nullptr;
   double* bufs[] = {s->f, s->x64, s->r64, s->q64, s->scal, s->partial, s->macro_dev,
                     s->stress_partial, s->ch_dev, s->partial_st, s->partial_rz, s->rhs_sig, s->partial_fx,
-                    s->dinv64, s->jmean, s->jrz};
+                    s->dinv64, s->jmean, s->jrz, s->media};
   for (double* b : bufs)
     if (b) dev_free(b);
   if (s->mode == VXL_SOLVE_MIXED) {
@@ -626,8 +626,14 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
         }
       }
       if (anyw) {
-        if (mixed) {  // reliable update of every active system keeps them in step
-          for (int k = 0; k < R; ++k) which[k] = active[k];
+        if (mixed) {  // reliable update of every active system of a cell keeps them in step
+          const int per = R / std::max(1, cell->groups);
+          for (int g0 = 0; g0 < R; g0 += per) {
+            bool any = false;
+            for (int k = g0; k < g0 + per; ++k) any = any || which[k];
+            if (any)
+              for (int k = g0; k < g0 + per; ++k) which[k] = active[k];
+          }
         }
         VXL_TRY(certify(which));
         forward_done = false;  // r was replaced
@@ -974,6 +980,29 @@ int solver_set_method(vxl_solver* s, int method) {
     if (!s->blk_keep) VXL_HIP(dev_malloc(&s->blk_keep, sizeof(int) * size_t(s->R)));
   }
   s->method = method;
+  return VXL_OK;
+}
+
+int solver_set_media(vxl_solver* s, const double* lam, const double* mu) {
+  const int G = s->cell->groups;
+  for (int g = 0; g < G; ++g)
+    if (!(lam[g] >= 0.0) || !(mu[g] > 0.0) || !std::isfinite(lam[g]) || !std::isfinite(mu[g]))
+      return fail("reference medium requires lam >= 0, mu > 0");
+  if (s->block_graph) {  // captured without the per-cell media
+    VXL_HIP(hipStreamSynchronize(s->cell->stream));
+    VXL_HIP(hipGraphExecDestroy(s->block_graph));
+    s->block_graph = nullptr;
+  }
+  std::vector<double> m(2 * size_t(G));
+  for (int g = 0; g < G; ++g) {
+    m[2 * g] = lam[g];
+    m[2 * g + 1] = mu[g];
+  }
+  if (!s->media) VXL_HIP(dev_malloc(&s->media, sizeof(double) * m.size()));
+  VXL_HIP(hipMemcpyAsync(s->media, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, s->cell->stream));
+  VXL_HIP(hipStreamSynchronize(s->cell->stream));
+  s->pc.media = s->media;
+  s->pc.groups = G;
   return VXL_OK;
 }
 

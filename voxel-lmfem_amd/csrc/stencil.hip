@@ -228,6 +228,10 @@ struct ApplyArgs {
   const T* table;      // rows x 24
   const float* scale;  // optional
   const float4* frames;  // kFrames: unit quaternion (w, x, y, z) of every voxel's material frame
+  // Stacked cells: right-hand side r reads index block r / group_rhs, which
+  // starts group_stride entries after the previous one (0 for one cell).
+  int64_t group_stride;
+  int group_rhs;
   int nx, ny, nz;
   int zchunk;
   int tiles_x;
@@ -242,6 +246,13 @@ struct ApplyArgs {
   const int* active;
   double* partial;
 };
+
+template <typename T>
+void set_groups(ApplyArgs<T>& a, const vxl_cell* cell, int nrhs) {
+  const int groups = cell->groups > 1 ? cell->groups : 1;
+  a.group_stride = groups > 1 ? cell->grid.n : 0;
+  a.group_rhs = groups > 1 ? std::max(1, nrhs / groups) : std::max(1, nrhs);
+}
 
 __device__ __forceinline__ int wrapmod(int v, int n) {
   v %= n;
@@ -338,7 +349,7 @@ apply_kernel(ApplyArgs<T> a) {
   auto compute_layer = [&](int z, int bottom, int top) {
     if (!is_elem) return;
     const int64_t voxel = int64_t(wrapmod(z, nz)) * nxy + e_xy;
-    const int row = load_row<Idx>(a.index, voxel);
+    const int row = load_row<Idx>(a.index, int64_t(bid.rhs / a.group_rhs) * a.group_stride + voxel);
     const T* crow = a.table + int64_t(row) * kRowWidth;
     T c[21];
 #pragma unroll
@@ -582,7 +593,8 @@ apply_kernel_v2(ApplyArgs<T> a) {
   T Bf[3][4], Ftp[3][4];
   // Element of layer z from its bottom (Bb) and top (Tf) face modes; returns the
   // bottom-face force (Fb) and top-face force (Ft) in face-mode form.
-  auto row_of = [&](int z) { return load_row<Idx>(a.index, int64_t(wrapmod(z, nz)) * nxy + e_xy); };
+  const int64_t gbase = int64_t(bid.rhs / a.group_rhs) * a.group_stride;  // stacked cells
+  auto row_of = [&](int z) { return load_row<Idx>(a.index, gbase + int64_t(wrapmod(z, nz)) * nxy + e_xy); };
   static_assert(kFrames == 0 || (std::is_same<T, float>::value && !kScale && !kHG && !kResid),
                 "per-voxel frames: FP32 FI operator without a scale field");
   // Coefficients of the element of layer z: its row, times the voxel's scale
@@ -839,6 +851,7 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
                       const int* active = nullptr, double* partial = nullptr, int* blocks = nullptr) {
   using TL = Tile<TX, TY>;
   ApplyArgs<T> a{};
+  set_groups(a, cell, nrhs);
   a.u = u;
   a.out = out;
   a.active = active;
@@ -916,6 +929,7 @@ int launch_apply_frames(const vxl_cell* cell, int nrhs, const float* u, float* o
                         const int* active = nullptr, double* partial = nullptr, int* blocks = nullptr) {
   using TL = Tile<TX, TY>;
   ApplyArgs<float> a{};
+  set_groups(a, cell, nrhs);
   a.u = u;
   a.out = out;
   a.active = active;
@@ -1211,6 +1225,7 @@ int launch_resid_impl(const vxl_cell* cell, int nrhs, const double* u, const dou
   constexpr int TY = kTilesF64[0];
   using TL = Tile<32, TY>;
   ApplyArgs<double> a{};
+  set_groups(a, cell, nrhs);
   a.u = u;
   a.out = nullptr;
   a.f = f;
@@ -1429,7 +1444,25 @@ bool stencil_tile_supported(int precision, int tile_y) {
 int default_stencil_tile(int precision) { return precision == VXL_F64 ? kTilesF64[0] : kTilesF32[0]; }
 int stencil_tile_option(int precision, int k) { return precision == VXL_F64 ? kTilesF64[k] : kTilesF32[k]; }
 
+// One cell of a stacked cell: the same grid and rows, index block g.
+vxl_cell group_view(const vxl_cell* cell, int g) {
+  vxl_cell v = *cell;
+  const size_t es = cell->mat.index_kind == VXL_INDEX_U8 ? 1 : cell->mat.index_kind == VXL_INDEX_U16 ? 2 : 4;
+  v.mat.index = static_cast<const char*>(cell->mat.index) + size_t(g) * size_t(cell->grid.n) * es;
+  v.groups = 1;
+  return v;
+}
+
 int launch_rhs(const vxl_cell* cell, int nload, const double* macro_dev, double* f, double* sig, hipStream_t s) {
+  if (cell->groups > 1) {  // stacked cells: each group's loads over its own index block
+    if (nload % cell->groups) return fail("stacked cells: the load count must be a multiple of the cell count");
+    const int per = nload / cell->groups;
+    for (int g = 0; g < cell->groups; ++g) {
+      const vxl_cell v = group_view(cell, g);
+      VXL_TRY(launch_rhs(&v, per, macro_dev + int64_t(6) * per * g, f + int64_t(3) * cell->grid.n * per * g, sig, s));
+    }
+    return VXL_OK;
+  }
   const auto& g = cell->grid;
   const int blocks = int((g.n + 255) / 256);
   const double gx = 0.25 / cell->h[0], gy = 0.25 / cell->h[1], gz = 0.25 / cell->h[2];
@@ -1506,6 +1539,17 @@ int64_t stress_partial_size(const vxl_cell* cell, int nload) {
 static int stress_common(const vxl_cell* cell, int nload, const double* u, const double* macro_dev,
                          double* partial, double* out_dev, double* strain, double* stress, double* energy,
                          hipStream_t s) {
+  if (cell->groups > 1) {  // stacked cells: mean stress per group (no centre fields)
+    if (strain || stress || energy) return fail("centre fields take one cell, not a stacked cell");
+    if (nload % cell->groups) return fail("stacked cells: the load count must be a multiple of the cell count");
+    const int per = nload / cell->groups;
+    for (int g = 0; g < cell->groups; ++g) {
+      const vxl_cell v = group_view(cell, g);
+      VXL_TRY(stress_common(&v, per, u + int64_t(3) * cell->grid.n * per * g, macro_dev + int64_t(6) * per * g,
+                            partial, out_dev ? out_dev + int64_t(6) * per * g : nullptr, nullptr, nullptr, nullptr, s));
+    }
+    return VXL_OK;
+  }
   const auto& g = cell->grid;
   const int64_t blocks = (g.n + kStressBlock - 1) / kStressBlock;
   const double gx = 0.25 / cell->h[0], gy = 0.25 / cell->h[1], gz = 0.25 / cell->h[2];

@@ -104,6 +104,10 @@ struct vxl_precond {
   int nrhs_max;
   double lam, mu;
   void* impl;
+  // Stacked cells: device (lam, mu) per cell, right-hand side r taking cell
+  // r / (nrhs / groups); nullptr: lam, mu for every right-hand side.
+  double* media;
+  int groups;
 };
 
 namespace vxl {
@@ -209,6 +213,9 @@ struct vxl_solver {
   double* blk_small;
   int* blk_keep;
   double* partial_cert;  // ||f - K x||^2 partials of the certification stencil
+  // Stacked cells: (lam, mu) of each cell's reference medium on the device
+  // (the preconditioner reads it through pc.media).
+  double* media;
   // Per-system residual history of the last run: (iteration, relative
   // residual); a negative iteration marks an FP64 true-residual evaluation.
   std::vector<std::vector<std::pair<int, double>>> trace;

@@ -162,6 +162,11 @@ struct vxl_cell {
   float* frame_table;
   float* frame_quat;
   int frame_cubic_only;  // every local-frame phase is cubic in its frame
+  // Stacked cells (vxl_cell_set_groups): `groups` cells of this grid and
+  // voxel size whose index blocks follow each other in mat.index (n entries
+  // each, into one shared row table); right-hand side r of an nrhs launch
+  // belongs to cell r / (nrhs / groups).
+  int groups;
 };
 
 namespace vxl {
@@ -232,6 +237,7 @@ void mfma_release(vxl_cell* cell);
 // flag[0] = 1 if a phase index is >= nphase, flag[1] = 1 if a quaternion is
 // not finite or |q|^2 is off 1 by more than 1e-5 (flag cleared here).
 void frames_release(vxl_cell* cell);
+vxl_cell group_view(const vxl_cell* cell, int g);  // one cell of a stacked cell (stencil.hip)
 int launch_frames_check(const uint8_t* index, const float* quat, int64_t n, int nphase, int* flag, hipStream_t s);
 int mfma_blocks(const vxl_cell* cell, int nrhs);
 int launch_mfma_apply(const vxl_cell* cell, int nrhs, const float* u, float* out, const int* active,

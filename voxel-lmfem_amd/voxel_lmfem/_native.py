@@ -51,6 +51,8 @@ SIGNATURES = {
     "vxl_get_stencil_config": (_I, [_P, _I, _IP, _IP, _IP]),
     "vxl_cell_set_mfma": (_I, [_P, _I]),
     "vxl_cell_set_frames": (_I, [_P, _I, _DP, _IP, _P, _P]),
+    "vxl_cell_set_groups": (_I, [_P, _I]),
+    "vxl_solver_set_media": (_I, [_P, _DP, _DP]),
     "vxl_autotune_stencil": (_I, [_P, _I, _I, _P, _P, _I, _IP, _IP, _IP, _DP]),
     "vxl_rhs": (_I, [_P, _I, _DP, _P]),
     "vxl_avg_stress": (_I, [_P, _I, _P, _DP, _DP]),

@@ -83,6 +83,49 @@ class Cell:
                                     n_elem=self.n, n_nodes=self.n, n_dofs=3 * self.n,
                                     V_elem=float(np.prod(self.h)))
         self.B = 1
+        self.groups = 1
+
+    @classmethod
+    def stack(cls, fields, shape, h, *, stream=None):
+        """One cell that runs several cells of the same grid and voxel size in
+        every launch (vxl_cell_set_groups): their row tables concatenated,
+        their index blocks one after the other (shifted to the concatenated
+        rows).  Label-only fields (no per-voxel local-frame rows), FI element.
+        Solvers on it take 6 right-hand sides per cell and one reference
+        medium per cell (Solver(cell, lams, mus))."""
+        from .materials import index_dtype
+        torch = _torch()
+        fields = list(fields)
+        if not 1 <= len(fields) <= 64:
+            raise ValueError("a stacked cell takes 1 to 64 cells")
+        if any(getattr(f, "rows_dev", None) is not None for f in fields):
+            raise ValueError("stacked cells take label-only fields (no per-voxel local-frame rows)")
+        n = int(np.prod(shape))
+        rows = [np.asarray(f.rows, dtype=np.float64).reshape(-1, 21) for f in fields]
+        total = sum(len(r) for r in rows)
+        dtype = index_dtype(total)
+        kind = {np.uint8: _native.VXL_INDEX_U8, np.uint16: _native.VXL_INDEX_U16, np.int32: _native.VXL_INDEX_I32}[dtype]
+        device = torch.device("cuda", torch.cuda.current_device())
+        parts, offset = [], 0
+        for f, r in zip(fields, rows):
+            idx = getattr(f, "index_dev", None)
+            if idx is None:
+                idx = torch.from_numpy(np.ascontiguousarray(np.asarray(f.index).astype(np.int64))).to(device)
+            idx = idx.reshape(-1).to(torch.int64)
+            if idx.numel() != n:
+                raise ValueError("every stacked field needs one index per voxel of the shared grid")
+            parts.append(idx + offset)
+            offset += len(r)
+        index = torch.cat(parts).to(torch.int32)
+        index = {np.uint8: lambda t: t.to(torch.uint8), np.uint16: lambda t: t.to(torch.int16),
+                 np.int32: lambda t: t}[dtype](index)
+        torch.cuda.current_stream().synchronize()
+        stacked = SimpleNamespace(index_dev=index, rows=np.concatenate(rows), index_kind=kind, rows_dev=None,
+                                  fields=fields)
+        cell = cls(stacked, shape, h, stream=stream)
+        check(cell.lib.vxl_cell_set_groups(cell.handle, len(fields)), "vxl_cell_set_groups")
+        cell.groups = len(fields)
+        return cell
 
     def close(self):
         pc = getattr(self, "_preconditioner", None)
@@ -391,23 +434,42 @@ class Solver:
 
     METHODS = {"cg": _native.VXL_METHOD_CG, "block": _native.VXL_METHOD_BLOCK_CG}
 
-    def __init__(self, cell, lam, mu, *, mode="mixed", nrhs=6, preconditioner="fft", method="cg"):
+    def __init__(self, cell, lam, mu, *, mode="mixed", nrhs=None, preconditioner="fft", method="cg"):
+        """lam, mu: the isotropic reference medium; for a stacked cell
+        (Cell.stack) one value per cell.  nrhs: default 6 per cell."""
         self.cell = cell
         self.mode = mode
         code = {"mixed": _native.VXL_SOLVE_MIXED, "fp64": _native.VXL_SOLVE_FP64}.get(mode)
         if code is None:
             raise ValueError("precision must be 'mixed' or 'fp64'")
+        groups = getattr(cell, "groups", 1)
+        lams, mus = np.atleast_1d(np.asarray(lam, dtype=np.float64)), np.atleast_1d(np.asarray(mu, dtype=np.float64))
+        if groups > 1 and (lams.shape != (groups,) or mus.shape != (groups,)):
+            raise ValueError("a stacked cell needs one lam and one mu per cell")
+        nrhs = 6 * groups if nrhs is None else int(nrhs)
         handle = ctypes.c_void_p()
-        check(cell.lib.vxl_solver_create(ctypes.byref(handle), cell.handle, code, int(nrhs), float(lam),
-                                         float(mu)), "vxl_solver_create")
+        check(cell.lib.vxl_solver_create(ctypes.byref(handle), cell.handle, code, nrhs, float(lams[0]),
+                                         float(mus[0])), "vxl_solver_create")
         self.handle = handle
         self.nrhs = nrhs
+        self.groups = groups
+        if groups > 1:
+            lp, mp = np.ascontiguousarray(lams), np.ascontiguousarray(mus)
+            check(cell.lib.vxl_solver_set_media(handle, lp.ctypes.data_as(_native._DP), mp.ctypes.data_as(_native._DP)),
+                  "vxl_solver_set_media")
         self.preconditioner = "fft"
         if preconditioner != "fft":
             self.set_preconditioner(preconditioner)
         self.method = "cg"
         if method != "cg":
             self.set_method(method)
+
+    def default_macro(self):
+        """The six unit engineering strains, once per cell of a stacked cell
+        (the first nrhs of them for one cell)."""
+        if self.groups > 1:
+            return np.tile(np.eye(6), (self.groups, 1))[: self.nrhs]
+        return np.eye(6)[: self.nrhs]
 
     def set_method(self, method):
         """'cg' (one CG recurrence per load case, sharing every launch) or
@@ -429,7 +491,7 @@ class Solver:
 
     def run(self, macro=None, *, tol=1e-8, max_iter=10000, return_fields=False):
         torch = self.cell.torch
-        macro = np.eye(6)[: self.nrhs] if macro is None else np.atleast_2d(macro)
+        macro = self.default_macro() if macro is None else np.atleast_2d(macro)
         if macro.shape != (self.nrhs, 6):
             raise ValueError("macro strains must be (nrhs, 6)")
         m, mp = darray(macro)
@@ -494,7 +556,7 @@ def run_batch(solvers, macro=None, *, tol=1e-8, max_iter=10000, workers=4):
     Rs = [s.nrhs for s in solvers]
     total = sum(Rs)
     if macro is None:
-        macro = np.concatenate([np.eye(6)[:R] for R in Rs])
+        macro = np.concatenate([s.default_macro() for s in solvers])
     macro, mp = darray(macro)
     if macro.shape != (total, 6):
         raise ValueError("macro strains must stack (R_i, 6) blocks of every solver")
