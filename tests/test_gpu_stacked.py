@@ -112,6 +112,32 @@ def test_stacked_solve_matches_single_and_oracle(torch_gpu, mode):
         assert rel(ch, ref) < 1e-7
 
 
+def test_stacked_graph_replay_equals_eager(torch_gpu, monkeypatch):
+    """Check blocks of a stacked solve replayed from a captured graph (ring and
+    fused update on): the same bits and iterations as launched kernel by
+    kernel, and blocks really replayed."""
+    from voxel_lmfem.gpu import Cell, Solver
+    cells = three_cells()
+    fields, media = build(cells)
+    shape, h = cells[0][0].shape, (1 / 16,) * 3
+    print("case stacked graph replay", flush=True)
+    outs = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("VXL_GRAPH", mode)
+        stacked = Cell.stack(fields, shape, h)
+        try:
+            sv = Solver(stacked, [m[0] for m in media], [m[1] for m in media])
+            try:
+                outs[mode] = sv.run()
+            finally:
+                sv.close()
+        finally:
+            stacked.close()
+    assert outs["1"]["graph_blocks"] > 0 and outs["0"]["graph_blocks"] == 0
+    assert np.array_equal(outs["1"]["CH"], outs["0"]["CH"])
+    assert outs["1"]["iterations"] == outs["0"]["iterations"]
+
+
 def test_homogenize_batch_stacks_same_shape_cells(torch_gpu):
     from voxel_lmfem.cells import random_inclusion_batch
     from voxel_lmfem.solver import homogenize_batch
