@@ -138,7 +138,10 @@ def test_stacked_graph_replay_equals_eager(torch_gpu, monkeypatch):
     assert outs["1"]["iterations"] == outs["0"]["iterations"]
 
 
-def test_homogenize_batch_stacks_same_shape_cells(torch_gpu):
+def test_homogenize_batch_stacks_same_shape_cells(torch_gpu, monkeypatch):
+    """Stacked against one solver per cell; and the stacked batch with its
+    check blocks captured before the worker threads start and replayed
+    (VXL_GRAPH=1) against the same batch launched kernel by kernel."""
     from voxel_lmfem.cells import random_inclusion_batch
     from voxel_lmfem.solver import homogenize_batch
     cells = random_inclusion_batch(count=7, seed=13, sizes=(16, 24))
@@ -147,11 +150,15 @@ def test_homogenize_batch_stacks_same_shape_cells(torch_gpu):
     print("case batch stacking", edges, flush=True)
     stacked = homogenize_batch(inputs, streams=2, workers=2, autotune=False, stack=10)
     single = homogenize_batch(inputs, streams=2, workers=2, autotune=False, stack=1)
-    for k, (a, b) in enumerate(zip(stacked, single)):
+    monkeypatch.setenv("VXL_GRAPH", "1")
+    replayed = homogenize_batch(inputs, streams=2, workers=2, autotune=False, stack=10)
+    assert sum(r["graph_blocks"] for r in replayed) > 0
+    for k, (a, b, c) in enumerate(zip(stacked, single, replayed)):
         assert a["stacked_with"] == edges.count(edges[k]) and b["stacked_with"] == 1
         assert a["max_true_residual"] <= 1e-8
         assert a["grid_shape"] == [edges[k]] * 3
         assert rel(a["CH"], b["CH"]) < 1e-9, k
+        assert np.array_equal(np.asarray(c["CH"]), np.asarray(a["CH"])), k
 
 
 def test_stacked_refusals(torch_gpu):
