@@ -174,11 +174,13 @@ def cpu_baseline(sample, iterations_total, n_target, threads=1):
 def frames_leg(n=256, R=6, reps=20):
     """FP32 six-RHS stencil on an n^3 single-phase cell whose every voxel has
     its own random frame (uniform rotations, seed 1), for a cubic crystal
-    (C11 168.4, C12 121.4, C44 75.4 GPa: the closed-form rotation) and an
-    orthotropic tow (the Bond rotation): per-voxel coefficient rows (the
-    device front end's i32 index and one FP64 + FP32 row per voxel) against
-    phase rows rotated in the stencil by a per-voxel FP32 quaternion
-    (vxl_cell_set_frames).  HIP events on the stream the kernels run on."""
+    (C11 168.4, C12 121.4, C44 75.4 GPa: the cubic closed form), an
+    orthotropic tow (E1 230, E2 = E3 15 GPa: the orthotropic closed form, on
+    the 32 x 14 and the 32 x 8 tile) and a general anisotropic phase (the
+    Bond rotation): per-voxel coefficient rows (the device front end's i32
+    index and one FP64 + FP32 row per voxel) against phase rows rotated in
+    the stencil by a per-voxel FP32 quaternion (vxl_cell_set_frames).  HIP
+    events on the stream the kernels run on."""
     import torch
     from scipy.spatial.transform import Rotation
     from voxel_lmfem import materials as pm
@@ -187,6 +189,8 @@ def frames_leg(n=256, R=6, reps=20):
              [0, 0, 0, 75.4, 0, 0], [0, 0, 0, 0, 75.4, 0], [0, 0, 0, 0, 0, 75.4]]
     tow = dict(kind="orthotropic", frame="local", E1=230., E2=15., E3=15., nu12=.2, nu13=.2, nu23=.3,
                G12=27., G13=27., G23=15. / 2.6)
+    X = np.random.default_rng(3).normal(size=(6, 6))
+    general = (X @ X.T + 6 * np.eye(6)) * 10.0
     shape = (n, n, n)
     N = n ** 3
     rot = Rotation.random(N, random_state=1).as_matrix().reshape(shape + (3, 3))
@@ -210,7 +214,8 @@ def frames_leg(n=256, R=6, reps=20):
     out = dict(workload=f"single-phase {n}^3 cell, random frame per voxel", n=n, rhs=R,
                bytes_per_voxel_as_scheduled=dict(rows=R * (24 + 84) + 4, frames=R * (24 + 17)),
                bytes_per_voxel_minimal=dict(rows=24 * R + 4 + 84, frames=24 * R + 17))
-    for name, spec in (("cubic", dict(kind="anisotropic", frame="local", C=cubic)), ("orthotropic", tow)):
+    for name, spec in (("cubic", dict(kind="anisotropic", frame="local", C=cubic)), ("orthotropic", tow),
+                       ("anisotropic", dict(kind="anisotropic", frame="local", C=general.tolist()))):
         mats = {1: spec}
         field = pm.build_field_device(labels, mats, rot)
         cell = Cell(field, shape, (1.0 / n,) * 3)
@@ -218,10 +223,17 @@ def frames_leg(n=256, R=6, reps=20):
             ms_rows = timed(cell)
             cell.set_frames(*pm.phase_frames(phase, mats, rot))
             ms_frames = timed(cell)
+            res = dict(rows_ms=ms_rows, frames_ms=ms_frames, frames_vs_rows=ms_frames / ms_rows,
+                       frames_tile=[32, cell.stencil_config()[0]])
+            if name == "orthotropic":  # the closed form on the 32 x 8 tile too
+                ty, sched, zmin = cell.stencil_config()
+                cell.set_stencil_config("float32", 8, sched, zmin)
+                res["frames_ms_tile_32x8"] = timed(cell)
+                cell.set_stencil_config("float32", ty, sched, zmin)
         finally:
             cell.close()
         del field
-        out[name] = dict(rows_ms=ms_rows, frames_ms=ms_frames, frames_vs_rows=ms_frames / ms_rows)
+        out[name] = res
         print(f"# frames {name} {n}^3 R={R}: per-voxel rows {ms_rows:.3f} ms, frames in the stencil "
               f"{ms_frames:.3f} ms", file=sys.stderr, flush=True)
     return out

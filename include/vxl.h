@@ -193,7 +193,9 @@ int vxl_cell_set_mfma(vxl_cell* cell, int enable);
  * certificate and the stress keep the cell's FP64 rows.
  * phase_rows: nphase x 21 FP64 (host), each phase's C21 in its own frame;
  * phase_local[p] != 0 marks local-frame phases (rotated per voxel; a phase
- * that is exactly cubic in its frame takes a closed form, any other the Bond
+ * that is exactly cubic in its frame takes the cubic closed form, one that
+ * is orthotropic in its frame (exact zeros outside the normal block and the
+ * shear diagonal) the orthotropic closed form, any other the Bond
  * rotation), 0 global-frame phases (used as given).  phase_index_dev: uint8
  * phase slot of every voxel (device); quat_dev: 4 floats (w, x, y, z) per
  * voxel (device), the unit quaternion of the voxel's rotation R (columns =

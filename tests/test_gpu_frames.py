@@ -1,8 +1,10 @@
 """Per-voxel material frames rebuilt in the FP32 stencil (vxl_cell_set_frames,
 homogenize(stencil="frames")): K.u against the oracle's per-voxel tensors
 (its own fourth-order rotation) on cells whose every local-frame voxel has a
-random frame -- cubic phases take the closed form, orthotropic and general
-anisotropic phases the Bond rotation, global-frame phases none -- given as
+random frame -- cubic phases take the cubic closed form, orthotropic phases
+the orthotropic closed form, general anisotropic phases the Bond rotation,
+global-frame phases none; cells with cubic phases only, with cubic and
+orthotropic phases, and with all kinds (the three kernel forms) -- given as
 rotation matrices and as axis pairs; the same cell's row-based FP32 stencil;
 C_eff of mixed solves against the oracle; refusals."""
 import numpy as np
@@ -26,13 +28,14 @@ def planes(a):
     return np.ascontiguousarray(a.T)
 
 
-def frame_cell(shape, seed, *, axis_pairs=False, cubic_only=False):
+def frame_cell(shape, seed, *, axis_pairs=False, kinds=5):
     """Labels 0 (isotropic, global), 1 (cubic, local), 2 (orthotropic tow,
     local), 3 (general anisotropic, local), 4 (anisotropic, global) with a
-    random frame at every voxel; labels 0 and 1 only with cubic_only (the
-    cubic-only kernels, which run the cell's own tile)."""
+    random frame at every voxel; the first `kinds` labels only: 2 runs the
+    cubic-only kernels and 3 the cubic-and-orthotropic ones (both on the
+    cell's own tile), 5 the kernels with the Bond rotation."""
     rng = np.random.default_rng(seed)
-    phase = rng.integers(0, 2 if cubic_only else 5, size=shape).astype(np.int32)
+    phase = rng.integers(0, kinds, size=shape).astype(np.int32)
     X = rng.normal(size=(6, 6))
     Y = rng.normal(size=(6, 6))
     mats = {0: dict(kind="isotropic", E=3.5, nu=.35),
@@ -76,15 +79,15 @@ def apply_pair(torch, shape, h, phase, mats, orientation):
     return want, got, rows
 
 
-@pytest.mark.parametrize("shape,cubic,axis_pairs,cubic_only",
-                         [((2, 2, 2), True, False, False), ((6, 5, 7), False, False, False),
-                          ((10, 12, 9), True, True, False), ((17, 9, 40), True, False, False),
-                          ((8, 20, 33), False, True, False), ((9, 30, 35), True, False, True),
-                          ((5, 16, 20), False, True, True)])
-def test_frames_apply_matches_oracle(torch_gpu, shape, cubic, axis_pairs, cubic_only):
-    print(f"case frames apply shape={shape} cubic={cubic} axis_pairs={axis_pairs} cubic_only={cubic_only}",
-          flush=True)
-    phase, mats, orientation = frame_cell(shape, sum(shape), axis_pairs=axis_pairs, cubic_only=cubic_only)
+@pytest.mark.parametrize("shape,cubic,axis_pairs,kinds",
+                         [((2, 2, 2), True, False, 5), ((6, 5, 7), False, False, 5),
+                          ((10, 12, 9), True, True, 5), ((17, 9, 40), True, False, 5),
+                          ((8, 20, 33), False, True, 5), ((9, 30, 35), True, False, 2),
+                          ((5, 16, 20), False, True, 2), ((7, 31, 18), True, False, 3),
+                          ((12, 15, 29), False, True, 3)])
+def test_frames_apply_matches_oracle(torch_gpu, shape, cubic, axis_pairs, kinds):
+    print(f"case frames apply shape={shape} cubic={cubic} axis_pairs={axis_pairs} kinds={kinds}", flush=True)
+    phase, mats, orientation = frame_cell(shape, sum(shape), axis_pairs=axis_pairs, kinds=kinds)
     rng = np.random.default_rng(7)
     h = (0.5, 0.5, 0.5) if cubic else tuple(rng.uniform(.3, 1.5, size=3))
     want, got, rows = apply_pair(torch_gpu, shape, h, phase, mats, orientation)
@@ -108,12 +111,12 @@ def test_frames_global_only_cell_equals_rows(torch_gpu):
     assert np.array_equal(got, rows)
 
 
-@pytest.mark.parametrize("axis_pairs,cubic_only", [(False, False), (True, False), (False, True)])
-def test_frames_homogenize_matches_oracle(torch_gpu, axis_pairs, cubic_only):
+@pytest.mark.parametrize("axis_pairs,kinds", [(False, 5), (True, 5), (False, 2), (True, 3)])
+def test_frames_homogenize_matches_oracle(torch_gpu, axis_pairs, kinds):
     from voxel_lmfem import homogenize
     shape = (12, 12, 12)
-    print(f"case frames homogenize {shape} axis_pairs={axis_pairs} cubic_only={cubic_only}", flush=True)
-    phase, mats, orientation = frame_cell(shape, 21, axis_pairs=axis_pairs, cubic_only=cubic_only)
+    print(f"case frames homogenize {shape} axis_pairs={axis_pairs} kinds={kinds}", flush=True)
+    phase, mats, orientation = frame_cell(shape, 21, axis_pairs=axis_pairs, kinds=kinds)
     dom = [1.0, 1.0, 1.0]
     ref = oracle.homogenize_matfree(phase, mats, dom, orientation=orientation)["CH"]
     out = homogenize(phase, mats, dom, orientation=orientation, stencil="frames")

@@ -133,7 +133,7 @@ int cell_create_common(vxl_cell** out, int nz, int ny, int nx, double hx, double
   c->frame_index = nullptr;
   c->frame_table = nullptr;
   c->frame_quat = nullptr;
-  c->frame_cubic_only = 0;
+  c->frame_forms = 0;
   c->groups = 1;
   (void)hipGetDevice(&c->device);
   auto cleanup = [&]() {
@@ -407,23 +407,26 @@ int vxl_cell_set_frames(vxl_cell* cell, int nphase, const double* phase_rows, co
   if (cell->mat.scale) return fail("per-voxel frames do not combine with a scale field");
   if (cell->mfma) return fail("the matrix-core stencil does not take per-voxel frames");
   std::vector<float> table(size_t(nphase) * kRowWidth, 0.0f);
-  bool cubic_only = true;
+  bool any_bond = false, any_ortho = false;
   for (int p = 0; p < nphase; ++p) {
     const double* c = phase_rows + size_t(p) * 21;
     for (int k = 0; k < 21; ++k)
       if (!std::isfinite(c[k])) return fail("phase rows contain nonfinite values");
     int kind = 0;
     if (phase_local[p]) {
-      // Cubic in its local frame: equal normal diagonal, equal normal
-      // off-diagonal, equal shear diagonal, zeros elsewhere (exactly).
-      const double d = c[pidx(0, 0)], o = c[pidx(0, 1)], g = c[pidx(3, 3)];
-      bool cubic = c[pidx(1, 1)] == d && c[pidx(2, 2)] == d && c[pidx(0, 2)] == o && c[pidx(1, 2)] == o &&
-                   c[pidx(4, 4)] == g && c[pidx(5, 5)] == g;
+      // Orthotropic in its local frame: no normal-shear and no shear-shear
+      // coupling (exact zeros); cubic besides: equal normal diagonal, equal
+      // normal off-diagonal, equal shear diagonal (exactly).
+      bool ortho = true;
       for (int i = 0; i < 6; ++i)
         for (int j = i + 1; j < 6; ++j)
-          if (j >= 3 && c[pidx(i, j)] != 0.0) cubic = false;
-      kind = cubic ? 2 : 1;
-      if (!cubic) cubic_only = false;
+          if (j >= 3 && c[pidx(i, j)] != 0.0) ortho = false;
+      const double d = c[pidx(0, 0)], o = c[pidx(0, 1)], g = c[pidx(3, 3)];
+      const bool cubic = ortho && c[pidx(1, 1)] == d && c[pidx(2, 2)] == d && c[pidx(0, 2)] == o &&
+                         c[pidx(1, 2)] == o && c[pidx(4, 4)] == g && c[pidx(5, 5)] == g;
+      kind = cubic ? 2 : ortho ? 3 : 1;
+      any_bond = any_bond || kind == 1;
+      any_ortho = any_ortho || kind == 3;
     }
     for (int k = 0; k < 21; ++k) table[size_t(p) * kRowWidth + k] = float(c[k]);
     table[size_t(p) * kRowWidth + 21] = float(kind);
@@ -457,7 +460,7 @@ int vxl_cell_set_frames(vxl_cell* cell, int nphase, const double* phase_rows, co
   if (rc != VXL_OK) return undo(rc);
   if (bad[0]) return undo(fail("phase index out of range of the phase rows"));
   if (bad[1]) return undo(fail("frames must be finite unit quaternions (|q|^2 within 1e-5 of 1)"));
-  cell->frame_cubic_only = cubic_only ? 1 : 0;
+  cell->frame_forms = any_bond ? 1 : any_ortho ? 3 : 2;
   return VXL_OK;
 }
 

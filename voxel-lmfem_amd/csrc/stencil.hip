@@ -422,13 +422,17 @@ apply_kernel(ApplyArgs<T> a) {
 // ---- material frames rebuilt in the stencil ---------------------------------
 // With per-voxel frames the FP32 stencil reads a phase row (the phase's C21 in
 // its local frame; slot 21 holds the kind: 0 global frame, 1 local frame, 2
-// local-frame cubic) and the voxel's unit quaternion q = (w, x, y, z), 16 B,
-// and rotates the row in registers: kind 1 by the Bond stress matrix,
-// C' = T C T^T (the reference's _rotate_packed, material_field.py:105-114,
-// in engineering Voigt order 11, 22, 33, 23, 13, 12); kind 2 by the cubic
-// closed form C' = C12 d(x)d + C44 (shear part) + a sum_m p_m p_m^T, where p_m
-// holds the Voigt pair products of local axis m (column m of R) and
-// a = C11 - C12 - 2 C44, which equals T C T^T for a cubic C.
+// local-frame cubic, 3 local-frame orthotropic) and the voxel's unit
+// quaternion q = (w, x, y, z), 16 B, and rotates the row in registers: kind 1
+// by the Bond stress matrix, C' = T C T^T (the reference's _rotate_packed,
+// material_field.py:105-114, in engineering Voigt order 11, 22, 33, 23, 13,
+// 12); kind 2 by the cubic closed form C' = C12 d(x)d + C44 (shear part) +
+// a sum_m p_m p_m^T, where p_m holds the Voigt pair products of local axis m
+// (column m of R) and a = C11 - C12 - 2 C44, which equals T C T^T for a cubic
+// C; kind 3 by T C T^T restricted to the nine orthotropic constants: the
+// columns of T are p_0, p_1, p_2 (normal) and q_s = (r_a r_b^T + r_b r_a^T)
+// in Voigt form for the shear pairs s = (1, 2), (0, 2), (0, 1), so
+// C' = sum_{m,n<3} C_mn p_m p_n^T + sum_s G_s q_s q_s^T.
 __device__ __forceinline__ void frame_matrix(float4 q, float (&R)[3][3]) {
   const float w = q.x, x = q.y, y = q.z, z = q.w;
   R[0][0] = 1.f - 2.f * (y * y + z * z);
@@ -460,13 +464,14 @@ __device__ __forceinline__ void bond_row(const float (&R)[3][3], int i, float (&
   }
 }
 
-// kCubicOnly: every local-frame phase of the cell is cubic (the Bond branch
-// is not compiled, which keeps the register count down).
-template <bool kCubicOnly>
+// kForms: the kinds compiled in, 1 every kind, 2 cubic only, 3 cubic and
+// orthotropic (the branches a cell does not need are left out, which keeps
+// the register count down).
+template <int kForms>
 __device__ __forceinline__ void rotate_row(float (&c)[21], float4 q, int kind) {
   float R[3][3];
   frame_matrix(q, R);
-  if (kCubicOnly || kind == 2) {
+  if (kForms == 2 || kind == 2) {
     const float c12 = c[pidx(0, 1)], c44 = c[pidx(3, 3)];
     const float an = c[pidx(0, 0)] - c12 - 2.f * c44;
     float p[3][6];
@@ -490,7 +495,48 @@ __device__ __forceinline__ void rotate_row(float (&c)[21], float4 q, int kind) {
       }
     return;
   }
-  if constexpr (kCubicOnly) return;
+  if constexpr (kForms == 2) return;
+  if (kForms == 3 || kind == 3) {
+    constexpr int sa[3] = {1, 0, 0}, sb[3] = {2, 2, 1};  // shear pair s = (sa, sb)
+    constexpr int vi[6] = {0, 1, 2, 1, 0, 0}, vj[6] = {0, 1, 2, 2, 2, 1};  // Voigt I = (vi, vj)
+    float p[3][6], qs[3][6];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int I = 0; I < 6; ++I) p[m][I] = R[vi[I]][m] * R[vj[I]][m];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+      for (int I = 0; I < 6; ++I)
+        qs[s][I] = fmaf(R[vi[I]][sa[s]], R[vj[I]][sb[s]], R[vi[I]][sb[s]] * R[vj[I]][sa[s]]);
+    float nrm[3][3], g[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+#pragma unroll
+      for (int n = 0; n < 3; ++n) nrm[m][n] = c[pidx(m, n)];
+      g[m] = c[pidx(3 + m, 3 + m)];
+    }
+#pragma unroll
+    for (int I = 0; I < 6; ++I) {
+      float u[3], v[3];
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        u[n] = fmaf(nrm[2][n], p[2][I], fmaf(nrm[1][n], p[1][I], nrm[0][n] * p[0][I]));
+        v[n] = g[n] * qs[n][I];
+      }
+#pragma unroll
+      for (int J = I; J < 6; ++J) {
+        float acc = u[0] * p[0][J];
+        acc = fmaf(u[1], p[1][J], acc);
+        acc = fmaf(u[2], p[2][J], acc);
+        acc = fmaf(v[0], qs[0][J], acc);
+        acc = fmaf(v[1], qs[1][J], acc);
+        c[pidx(I, J)] = fmaf(v[2], qs[2][J], acc);
+      }
+    }
+    return;
+  }
+  if constexpr (kForms == 3) return;
   float out[21];
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
@@ -599,7 +645,8 @@ apply_kernel_v2(ApplyArgs<T> a) {
                 "per-voxel frames: FP32 FI operator without a scale field");
   // Coefficients of the element of layer z: its row, times the voxel's scale
   // (kScale), rotated into the voxel's material frame (kFrames: 1 any
-  // local-frame phase, 2 cubic local-frame phases only).
+  // local-frame phase, 2 cubic local-frame phases only, 3 cubic and
+  // orthotropic local-frame phases only).
   auto coeffs = [&](int z, int row, T (&c)[21]) {
     const int64_t voxel = int64_t(wrapmod(z, nz)) * nxy + e_xy;
     const T* crow = a.table + int64_t(row) * kRowWidth;
@@ -612,7 +659,7 @@ apply_kernel_v2(ApplyArgs<T> a) {
     }
     if constexpr (kFrames != 0) {
       const int kind = int(crow[21]);
-      if (kind != 0) rotate_row<kFrames == 2>(c, a.frames[voxel], kind);
+      if (kind != 0) rotate_row<kFrames>(c, a.frames[voxel], kind);
     }
   };
   auto element = [&](const T (&c)[21], const T (&Bb)[3][4], const T (&Tf)[3][4], T (&Fb)[3][4], T (&Ft)[3][4]) {
@@ -911,15 +958,15 @@ int launch_apply_impl(const vxl_cell* cell, int nrhs, const T* u, T* out, hipStr
 }
 
 // FP32 stencil with per-voxel material frames (vxl_cell_set_frames): phase
-// index (u8), phase rows and quaternions; always the schedule-3 form.  Tile:
-// the cell's with cubic local-frame phases only; otherwise 32 x 8 (32 x 6 if
-// the cell is set to it), since with the Bond rotation's registers the 32 x 14
-// tile spills (128 VGPRs, 26 spilled in the CG form) and 32 x 8 takes 146
-// VGPRs without spilling.
+// index (u8), phase rows and quaternions; always the schedule-3 form, on the
+// cell's tile.  With the Bond rotation the 32 x 14 tile spills (128 VGPRs,
+// 18-27 registers to scratch) and is still faster than the spill-free 32 x 8
+// (146 VGPRs, fewer waves per CU and more halo elements): 3.8 against 5.2 ms
+// for six right-hand sides at 256^3, random frames.
 template <typename F>
 int frames_tile(const vxl_cell* cell, F&& f) {
   const int ty = cell->tile_y[0];
-  if (cell->frame_cubic_only && ty == kTilesF32[0]) return f(std::integral_constant<int, kTilesF32[0]>());
+  if (ty == kTilesF32[0]) return f(std::integral_constant<int, kTilesF32[0]>());
   if (ty == kTilesF32[2]) return f(std::integral_constant<int, kTilesF32[2]>());
   return f(std::integral_constant<int, kTilesF32[1]>());
 }
@@ -954,12 +1001,14 @@ int launch_apply_frames(const vxl_cell* cell, int nrhs, const float* u, float* o
 #define VXL_FR(FUSE, ISO, KIND)                                                                              \
   hipLaunchKernelGGL((apply_kernel_v2<float, uint8_t, false, TX, TY, FUSE, true, ISO, false, false, KIND>), grid, \
                      dim3(TL::NT), 0, s, a)
-#define VXL_FR2(FUSE, ISO)       \
-  do {                           \
-    if (cell->frame_cubic_only)  \
-      VXL_FR(FUSE, ISO, 2);      \
-    else                         \
-      VXL_FR(FUSE, ISO, 1);      \
+#define VXL_FR2(FUSE, ISO)          \
+  do {                              \
+    if (cell->frame_forms == 2)     \
+      VXL_FR(FUSE, ISO, 2);         \
+    else if (cell->frame_forms == 3) \
+      VXL_FR(FUSE, ISO, 3);         \
+    else                            \
+      VXL_FR(FUSE, ISO, 1);         \
   } while (0)
   if (partial) {
     if (iso)

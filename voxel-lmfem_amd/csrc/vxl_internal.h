@@ -161,7 +161,7 @@ struct vxl_cell {
   uint8_t* frame_index;
   float* frame_table;
   float* frame_quat;
-  int frame_cubic_only;  // every local-frame phase is cubic in its frame
+  int frame_forms;  // rotations the cell needs: 1 any (Bond), 2 cubic only, 3 cubic and orthotropic only
   // Stacked cells (vxl_cell_set_groups): `groups` cells of this grid and
   // voxel size whose index blocks follow each other in mat.index (n entries
   // each, into one shared row table); right-hand side r of an nrhs launch
