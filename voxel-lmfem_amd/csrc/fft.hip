@@ -784,9 +784,11 @@ constexpr int z4_kx() {
   return k < 1 ? 1 : k;
 }
 
+// src: the spectrum read (S itself: in place; another buffer of the same
+// layout: the pass writes S from it).
 template <bool INV, typename T, int N1, int N2, typename SC>
 __global__ void __launch_bounds__(kThreads4)
-fft_y4(SC* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict__ w, int64_t spec_stride,
+fft_y4(SC* S, const SC* src, int W, int TJ, const typename CT<T>::type* __restrict__ w, int64_t spec_stride,
        const int* active) {
   using C = typename CT<T>::type;
   constexpr int n = N1 * N2;
@@ -797,11 +799,12 @@ fft_y4(SC* __restrict__ S, int W, int TJ, const typename CT<T>::type* __restrict
   constexpr int TS = y4_cols<T, N1, N2>();  // LDS row stride (TJ <= TS)
   const int j0 = b.x * TJ;
   const int tj = min(TJ, W - j0);
-  SC* plane = S + int64_t(b.z) * spec_stride + int64_t(b.y) * n * W + j0;
+  const int64_t off = int64_t(b.z) * spec_stride + int64_t(b.y) * n * W + j0;
+  SC* plane = S + off;
   const int p = threadIdx.x % TS, t = threadIdx.x / TS;
   const bool col = p < tj;
   // Offsets inside one (rhs, z) plane of n x W entries stay below 2^31.
-  SC* const base = plane + (t * W + p);
+  const SC* const base = src + off + (t * W + p);
   const int step = N2 * W;
   if (t < N2)
     four_step_1<INV, T, N1, N2, TS>(L, p, t, w, [&](int k1) -> C {
@@ -935,7 +938,12 @@ struct XTile {
 // is loaded (r and x written back; upd.x == nullptr leaves x alone), the
 // updated r is transformed, and the FP64 ||r||^2 of the workgroup goes to
 // upd.partial[rhs * gridDim.x + blockIdx.x].
-template <typename T, int N1, int N2, bool kUpd, typename SC>
+// kSpec (spectral residual): the input is q and S holds the x spectrum of the
+// residual, R^; the transform q^ of each row updates it in place,
+// R^ -= alpha q^, and the workgroup's FP64 ||r||^2 = (1/n) sum_k w(k) |R^(k)|^2
+// (w = 1 at k = 0 and k = n/2, 2 elsewhere; Parseval over the half spectrum)
+// goes to upd.partial as above.  S is then not written otherwise.
+template <typename T, int N1, int N2, bool kUpd, typename SC, bool kSpec = false>
 __global__ void __launch_bounds__(256)
 fft_x4_r2c(const T* __restrict__ r, SC* __restrict__ S, int64_t rows, int PP,
            const typename CT<T>::type* __restrict__ w, int64_t field_stride, int64_t spec_stride,
@@ -943,6 +951,8 @@ fft_x4_r2c(const T* __restrict__ r, SC* __restrict__ S, int64_t rows, int PP,
   using C = typename CT<T>::type;
   using XT = XTile<N1, N2>;
   constexpr int n = XT::n, nxh = n / 2 + 1, lanes = FourStep<N1, N2>::lanes;
+  static_assert(!(kUpd && kSpec), "one update form per launch");
+  static_assert(!kSpec || std::is_same<SC, C>::value, "the residual spectrum is stored at full precision");
   extern __shared__ double2 smem_raw[];
   __shared__ double red[4];
   if (active && !active[blockIdx.y]) return;
@@ -1025,6 +1035,8 @@ fft_x4_r2c(const T* __restrict__ r, SC* __restrict__ S, int64_t rows, int PP,
   }
   __syncthreads();
   // Two-for-one split, as in fft_x_r2c.
+  double spec_rr = 0.0;  // kSpec: sum of w(k) |R^(k)|^2 over this thread's entries
+  const T spec_alpha = kSpec ? T(upd.alpha[blockIdx.y]) : T(0);
   for (int idx = threadIdx.x; idx < PP * nxh; idx += blockDim.x) {
     const int a = idx / nxh, k = idx - a * nxh;
     const int64_t rr = 2 * (int64_t(blockIdx.x) * PP + a);
@@ -1037,6 +1049,27 @@ fft_x4_r2c(const T* __restrict__ r, SC* __restrict__ S, int64_t rows, int PP,
       const C sm = cadd(x1, x2), df = csub(x1, x2);
       sp[qq][0] = C{T(0.5) * sm.x, T(0.5) * sm.y};
       sp[qq][1] = C{T(0.5) * df.y, T(-0.5) * df.x};
+    }
+    if constexpr (kSpec) {
+      const double wk = (k == 0 || 2 * k == n) ? 1.0 : 2.0;
+      auto sub = [&](C* o, C v) {
+        C x = *o;
+        x.x = fma(-spec_alpha, v.x, x.x);
+        x.y = fma(-spec_alpha, v.y, x.y);
+        *o = x;
+        return double(x.x) * double(x.x) + double(x.y) * double(x.y);
+      };
+      double e = 0.0;
+      if (rr < rows) {
+        C* o = reinterpret_cast<C*>(out) + (rr * nxh + k) * 3;
+        e += sub(o + 0, sp[0][0]) + sub(o + 1, sp[0][1]) + sub(o + 2, sp[2][0]);
+      }
+      if (rr + 1 < rows) {
+        C* o = reinterpret_cast<C*>(out) + ((rr + 1) * nxh + k) * 3;
+        e += sub(o + 0, sp[1][0]) + sub(o + 1, sp[1][1]) + sub(o + 2, sp[2][1]);
+      }
+      spec_rr += wk * e;
+      continue;
     }
     if (rr < rows) {
       SC* o = out + (rr * nxh + k) * 3;
@@ -1051,8 +1084,8 @@ fft_x4_r2c(const T* __restrict__ r, SC* __restrict__ S, int64_t rows, int PP,
       sstore(o + 2, sp[2][1]);
     }
   }
-  if constexpr (kUpd) {
-    const double total = block_reduce_dyn(rr, red);
+  if constexpr (kUpd || kSpec) {
+    const double total = block_reduce_dyn(kSpec ? spec_rr * (1.0 / n) : rr, red);
     if (threadIdx.x == 0) upd.partial[int64_t(blockIdx.y) * gridDim.x + blockIdx.x] = total;
   }
 }
@@ -1166,6 +1199,7 @@ struct PrecondImpl {
   void* tab[3];
   void* spec;
   int64_t spec_stride;  // complex entries per RHS
+  void* rhat;           // x spectrum of the CG residual (spectral-residual form), allocated on demand
   int pairs_per_block, tj, tk;
   size_t lds_x, lds_y, lds_z;
   // four-step kernels for y / z axes with a factorisation (0: use the Stockham pass)
@@ -1229,11 +1263,11 @@ int z4_kx_of(int n) {
 }
 
 template <bool INV, typename T, typename SC>
-void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, int W, int TJ,
+void launch_y4(int n, dim3 grid, int thr, size_t lds, hipStream_t s, SC* S, const SC* src, int W, int TJ,
                const typename CT<T>::type* w, int64_t stride, const int* active) {
 #define VXL_Y4(N, A, B)                                                                         \
   case N:                                                                                     \
-    hipLaunchKernelGGL((fft_y4<INV, T, A, B, SC>), grid, dim3(thr), lds, s, S, W, TJ, w, stride, active); \
+    hipLaunchKernelGGL((fft_y4<INV, T, A, B, SC>), grid, dim3(thr), lds, s, S, src, W, TJ, w, stride, active); \
     break;
   switch (n) {
     VXL_Y4(2, 1, 2)
@@ -1317,14 +1351,14 @@ size_t x4_stride(int n) {
   return sr > sc ? sr : sc;
 }
 
-template <typename T, bool kUpd, typename SC>
+template <typename T, bool kUpd, typename SC, bool kSpec = false>
 void launch_x4_r2c(int n, dim3 grid, int thr, size_t lds, hipStream_t s, const T* r, SC* S,
                    int64_t rows, int PP, const typename CT<T>::type* w, int64_t fstride, int64_t sstride,
                    const int* active, const XUpdate<T>& upd) {
 #define VXL_X4R(N, A, B)                                                                                     \
   case N:                                                                                                  \
-    hipLaunchKernelGGL((fft_x4_r2c<T, A, B, kUpd, SC>), grid, dim3(thr), lds, s, r, S, rows, PP, w, fstride, \
-                       sstride, active, upd);                                                              \
+    hipLaunchKernelGGL((fft_x4_r2c<T, A, B, kUpd, SC, kSpec>), grid, dim3(thr), lds, s, r, S, rows, PP, w,  \
+                       fstride, sstride, active, upd);                                                     \
     break;
   switch (n) { VXL_X4R_CASES(VXL_X4R) }
 #undef VXL_X4R
@@ -1498,9 +1532,14 @@ enum Stages { kForwardX = 1, kRest = 2, kAllStages = 3 };
 // Runs the x forward pass (kForwardX, optionally fused with the CG update upd,
 // four-step x passes only) and/or the remaining passes (kRest).
 // SC is the spectrum storage type: CT<T>::type, or bf16x2 (four-step passes only).
+// sr (spectral residual; FP32 storage, four-step x and y passes): the CG
+// residual lives as its x spectrum in im->rhat.  The forward pass writes the
+// x spectrum of r there (no upd) or updates it, R^ -= alpha q^, from the
+// transform of upd->q (with upd); the y forward pass reads R^ and writes the
+// working spectrum, which the later passes use in place.
 template <typename T, typename SC>
 int apply_stages(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s, const int* active,
-                 const CgHooks* cg, int stages, const XUpdate<T>* upd) {
+                 const CgHooks* cg, int stages, const XUpdate<T>* upd, bool sr) {
   using C = typename CT<T>::type;
   constexpr bool kFull = std::is_same<SC, C>::value;
   const Tables<T> tb = tables_of<T>(im);
@@ -1510,9 +1549,28 @@ int apply_stages(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, h
   const unsigned bx = unsigned((pairs + im->pairs_per_block - 1) / im->pairs_per_block);
   const int W = 3 * im->nxh;
   const unsigned bx4 = im->x4 ? unsigned((pairs + im->pp4 - 1) / im->pp4) : 0u;
+  if (sr) {
+    if constexpr (!kFull) {
+      return fail("the spectral residual needs full-precision spectrum storage");
+    } else {
+      if (!im->x4 || !im->y4 || !im->rhat) return fail("the spectral residual needs the four-step x and y passes");
+    }
+  }
+  SC* const Rh = static_cast<SC*>(im->rhat);
   if (stages & kForwardX) {
     if (upd && !im->x4) return fail("fused CG update needs the four-step x pass");
-    if (upd)
+    bool done = false;
+    if constexpr (kFull) {
+      if (sr && upd)
+        launch_x4_r2c<T, false, SC, true>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, upd->q, Rh, im->rows,
+                                          im->pp4, tb.wx, fstride, im->spec_stride, active, *upd);
+      else if (sr)
+        launch_x4_r2c<T, false, SC>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, Rh, im->rows, im->pp4,
+                                    tb.wx, fstride, im->spec_stride, active, XUpdate<T>{});
+      done = sr;
+    }
+    if (done) {
+    } else if (upd)
       launch_x4_r2c<T, true, SC>(im->x4, dim3(bx4, nrhs), im->thr_x4, im->lds_x4, s, r, S, im->rows, im->pp4, tb.wx,
                              fstride, im->spec_stride, active, *upd);
     else if (im->x4)
@@ -1528,9 +1586,11 @@ int apply_stages(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, h
     if (im->y4) {
       const dim3 g((W + im->tj4 - 1) / im->tj4, im->nz, nrhs);
       if (inverse)
-        launch_y4<true, T, SC>(im->y4, g, im->thr_y4, im->lds_y4, s, S, W, im->tj4, tb.wy, im->spec_stride, active);
+        launch_y4<true, T, SC>(im->y4, g, im->thr_y4, im->lds_y4, s, S, S, W, im->tj4, tb.wy, im->spec_stride,
+                               active);
       else
-        launch_y4<false, T, SC>(im->y4, g, im->thr_y4, im->lds_y4, s, S, W, im->tj4, tb.wy, im->spec_stride, active);
+        launch_y4<false, T, SC>(im->y4, g, im->thr_y4, im->lds_y4, s, S, sr ? Rh : S, W, im->tj4, tb.wy,
+                                im->spec_stride, active);
     } else if constexpr (!kFull) {
       return fail("BF16 spectrum storage needs power-of-two axes");
     } else if (inverse) {
@@ -1593,11 +1653,12 @@ int apply_stages(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, h
 
 template <typename T>
 int apply_impl(vxl_precond* pc, PrecondImpl* im, int nrhs, const T* r, T* z, hipStream_t s,
-               const int* active, const CgHooks* cg, int stages = kAllStages, const XUpdate<T>* upd = nullptr) {
+               const int* active, const CgHooks* cg, int stages = kAllStages, const XUpdate<T>* upd = nullptr,
+               bool sr = false) {
   if constexpr (std::is_same<T, float>::value) {
-    if (im->bf16) return apply_stages<T, bf16x2>(pc, im, nrhs, r, z, s, active, cg, stages, upd);
+    if (im->bf16) return apply_stages<T, bf16x2>(pc, im, nrhs, r, z, s, active, cg, stages, upd, sr);
   }
-  return apply_stages<T, typename CT<T>::type>(pc, im, nrhs, r, z, s, active, cg, stages, upd);
+  return apply_stages<T, typename CT<T>::type>(pc, im, nrhs, r, z, s, active, cg, stages, upd, sr);
 }
 
 }  // namespace
@@ -1607,6 +1668,7 @@ int precond_create(vxl_precond* pc) {
   im->tw[0] = im->tw[1] = im->tw[2] = nullptr;
   im->tab[0] = im->tab[1] = im->tab[2] = nullptr;
   im->spec = nullptr;
+  im->rhat = nullptr;
   pc->impl = im;
   if (!(pc->lam >= 0.0) || !(pc->mu > 0.0)) return fail("reference medium requires lam >= 0, mu > 0");
   return pc->precision == VXL_F64 ? create_impl<double>(pc, im) : create_impl<float>(pc, im);
@@ -1632,32 +1694,47 @@ template int precond_apply_cg<double>(vxl_precond*, int, const double*, double*,
 
 template <typename T>
 int precond_cg_forward(vxl_precond* pc, int nrhs, T* r, T* x, const T* p, const T* q, const double* alpha,
-                       double* rr_partial, const int* active, hipStream_t s) {
+                       double* rr_partial, const int* active, hipStream_t s, bool sr) {
   auto* im = static_cast<PrecondImpl*>(pc->impl);
   if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
   if ((pc->precision == VXL_F64) != std::is_same<T, double>::value) return fail("precond precision mismatch");
-  if (!q) return apply_impl<T>(pc, im, nrhs, r, nullptr, s, active, nullptr, kForwardX);
+  if (sr && (x || p)) return fail("the spectral residual takes no x update in the forward pass");
+  if (!q) return apply_impl<T>(pc, im, nrhs, r, nullptr, s, active, nullptr, kForwardX, nullptr, sr);
   const XUpdate<T> upd{r, x, p, q, alpha, rr_partial};
-  return apply_impl<T>(pc, im, nrhs, r, nullptr, s, active, nullptr, kForwardX, &upd);
+  return apply_impl<T>(pc, im, nrhs, r, nullptr, s, active, nullptr, kForwardX, &upd, sr);
 }
 template int precond_cg_forward<float>(vxl_precond*, int, float*, float*, const float*, const float*, const double*,
-                                       double*, const int*, hipStream_t);
+                                       double*, const int*, hipStream_t, bool);
 template int precond_cg_forward<double>(vxl_precond*, int, double*, double*, const double*, const double*,
-                                        const double*, double*, const int*, hipStream_t);
+                                        const double*, double*, const int*, hipStream_t, bool);
 
 template <typename T>
 int precond_cg_rest(vxl_precond* pc, int nrhs, T* p, double* rz_partial, const double* beta,
-                    const std::function<int()>& mid, const int* active, hipStream_t s, const T* p_old) {
+                    const std::function<int()>& mid, const int* active, hipStream_t s, const T* p_old, bool sr) {
   auto* im = static_cast<PrecondImpl*>(pc->impl);
   if (nrhs < 1 || nrhs > pc->nrhs_max) return fail("precond nrhs out of range");
   if ((pc->precision == VXL_F64) != std::is_same<T, double>::value) return fail("precond precision mismatch");
   const CgHooks cg{rz_partial, beta, mid, p_old};
-  return apply_impl<T>(pc, im, nrhs, static_cast<const T*>(nullptr), p, s, active, &cg, kRest);
+  return apply_impl<T>(pc, im, nrhs, static_cast<const T*>(nullptr), p, s, active, &cg, kRest, nullptr, sr);
 }
 template int precond_cg_rest<float>(vxl_precond*, int, float*, double*, const double*, const std::function<int()>&,
-                                    const int*, hipStream_t, const float*);
+                                    const int*, hipStream_t, const float*, bool);
 template int precond_cg_rest<double>(vxl_precond*, int, double*, double*, const double*,
-                                     const std::function<int()>&, const int*, hipStream_t, const double*);
+                                     const std::function<int()>&, const int*, hipStream_t, const double*, bool);
+
+bool precond_spectral_residual_ok(const vxl_precond* pc) {
+  const auto* im = static_cast<const PrecondImpl*>(pc->impl);
+  const char* env = std::getenv("VXL_SPECTRAL_RESIDUAL");
+  return pc->precision == VXL_F32 && im->x4 && im->y4 && !im->bf16 && !(env && env[0] == '0');
+}
+
+int precond_residual_spectrum(vxl_precond* pc) {
+  auto* im = static_cast<PrecondImpl*>(pc->impl);
+  if (im->rhat) return VXL_OK;
+  if (!precond_spectral_residual_ok(pc)) return fail("the spectral residual needs the FP32 four-step x and y passes");
+  VXL_HIP(dev_malloc(&im->rhat, sizeof(float2) * size_t(im->spec_stride) * pc->nrhs_max));
+  return VXL_OK;
+}
 
 int precond_fused_update_blocks(const vxl_precond* pc) {
   const auto* im = static_cast<const PrecondImpl*>(pc->impl);
@@ -1699,6 +1776,7 @@ void precond_release(vxl_precond* pc) {
     if (im->tab[a]) dev_free(im->tab[a]);
   }
   if (im->spec) dev_free(im->spec);
+  if (im->rhat) dev_free(im->rhat);
   delete im;
   pc->impl = nullptr;
 }

@@ -128,14 +128,25 @@ int precond_rz_blocks(const vxl_precond* pc);
 // x += alpha p when x != nullptr) to every entry it loads and writes FP64
 // ||r||^2 partials (rr_partial[rhs * precond_fused_update_blocks + b]); that
 // form needs the four-step x pass (precond_fused_update_blocks > 0).
+// sr (spectral residual, precond_residual_spectrum first): the residual is
+// kept as its x spectrum R^ inside the preconditioner.  Without q the pass
+// forms R^ from r; with q it updates R^ -= alpha q^ and writes the ||r||^2
+// partials (Parseval), r is not read and x, p must be null.
 template <typename T>
 int precond_cg_forward(vxl_precond* pc, int nrhs, T* r, T* x, const T* p, const T* q, const double* alpha,
-                       double* rr_partial, const int* active, hipStream_t s);
+                       double* rr_partial, const int* active, hipStream_t s, bool sr = false);
 // y, z (+ Green, r.z), mid(), y^-1 and the x inverse pass writing
-// p = z + beta p_old (p_old == nullptr: p itself, in place).
+// p = z + beta p_old (p_old == nullptr: p itself, in place).  sr: the y pass
+// reads R^.
 template <typename T>
 int precond_cg_rest(vxl_precond* pc, int nrhs, T* p, double* rz_partial, const double* beta,
-                    const std::function<int()>& mid, const int* active, hipStream_t s, const T* p_old = nullptr);
+                    const std::function<int()>& mid, const int* active, hipStream_t s, const T* p_old = nullptr,
+                    bool sr = false);
+// Whether the spectral-residual form is available (FP32 spectrum storage,
+// four-step x and y passes; VXL_SPECTRAL_RESIDUAL=0 turns it off), and the
+// allocation of its R^ buffer (nrhs_max spectra; once, outside any capture).
+bool precond_spectral_residual_ok(const vxl_precond* pc);
+int precond_residual_spectrum(vxl_precond* pc);
 // Workgroups per RHS of the fused forward pass, 0 if it is not available.
 int precond_fused_update_blocks(const vxl_precond* pc);
 // Whether the CG r update runs inside the x pass (VXL_FUSED_UPDATE=0 turns
@@ -193,6 +204,7 @@ struct vxl_solver {
   // and the slots whose alpha_t p_t the block leaves pending, in order.
   bool block_graph_ring;
   bool block_graph_fused;  // the block starts after a fused r update (spectrum of r formed)
+  bool block_graph_sr;     // the block keeps the residual as its x spectrum
   int block_graph_cur0, block_graph_cur1;
   int block_graph_pend[8];
   int block_graph_npend;
