@@ -232,11 +232,22 @@ struct Steps {
 
 // Whether a solve with these steps keeps the residual as its x spectrum: mixed
 // mode (FP32 spectra), direction ring, fused forward pass, four-step x and y
-// passes.  At 256^3 x 6 this takes the forward pass from 4.8 to 3.6 GB.
+// passes, and room for the R^ buffer (allocated here, once; if it does not
+// fit, the solver keeps the real-space residual from then on, so eager runs
+// and captured blocks always agree).  At 256^3 x 6 this takes the forward
+// pass from 4.8 to 3.6 GB.
 template <typename T>
 bool spectral_on(vxl_solver* s, const Steps<T>& steps) {
-  return std::is_same<T, float>::value && steps.ring && steps.nblk_fx > 0 && !steps.jacobi &&
-         precond_spectral_residual_ok(&s->pc);
+  if (!(std::is_same<T, float>::value && steps.ring && steps.nblk_fx > 0 && !steps.jacobi && !s->sr_failed &&
+        precond_spectral_residual_ok(&s->pc)))
+    return false;
+  if (precond_residual_spectrum(&s->pc) != VXL_OK) {
+    (void)hipGetLastError();
+    set_error("");
+    s->sr_failed = true;
+    return false;
+  }
+  return true;
 }
 
 // Blocks of kCheckStride iterations between host checks replay one captured
@@ -283,7 +294,6 @@ int prepare_block_graph(vxl_solver* s, double tol) {
     s->capture_stream = nullptr;
     return give_up();
   }
-  if (sr) VXL_TRY(precond_residual_spectrum(&s->pc));  // before the capture (a batch prepares ahead of its run)
   hipStream_t st = s->capture_stream;
   Steps<T> steps(s, tol, st);
   steps.ring = ring;
@@ -501,7 +511,6 @@ int run_impl(vxl_solver* s, const double* macro_host, double tol, int max_iter, 
   const bool flush_each_check = ring && gmode != 0;
   steps.ring = ring;
   steps.sr = spectral_on<T>(s, steps);
-  if (steps.sr) VXL_TRY(precond_residual_spectrum(&s->pc));
   // x = 0.  With the ring, x64 is first written by the first flush (an
   // assignment) and the FP32 x is not used.
   bool x64_set = !ring;

@@ -205,6 +205,7 @@ struct vxl_solver {
   bool block_graph_ring;
   bool block_graph_fused;  // the block starts after a fused r update (spectrum of r formed)
   bool block_graph_sr;     // the block keeps the residual as its x spectrum
+  bool sr_failed;          // no room for the residual spectrum: real-space residual from then on
   int block_graph_cur0, block_graph_cur1;
   int block_graph_pend[8];
   int block_graph_npend;
