@@ -202,3 +202,42 @@ def test_preconditioner_protocol_matches_oracle(torch_gpu, precision):
     with pytest.raises(ValueError):
         cell.precond_packed(planes.reshape(2, 3, -1))
     cell.close()
+
+
+def test_side_stream_cell_orders_against_the_current_stream(torch_gpu):
+    """A cell on a stream of its own (homogenize_batch builds them so): inputs
+    produced on the current stream right before each call and outputs read
+    there right after give the default-stream cell's results bit for bit
+    (apply, RHS, preconditioner, solver fields)."""
+    import torch
+    from voxel_lmfem import materials as pm
+    from voxel_lmfem.gpu import Cell, Preconditioner, Solver
+    print("case side-stream cell 32^3", flush=True)
+    rng = np.random.default_rng(12)
+    phase = (rng.random((32, 32, 32)) < 0.3).astype(np.uint8)
+    mats = {0: dict(kind="isotropic", E=3000.0, nu=0.35), 1: dict(kind="isotropic", E=70000.0, nu=0.22)}
+    h = np.full(3, 1.0 / 32)
+    field = pm.build_field(phase, mats)
+    side = torch.cuda.Stream()
+    ref_cell = Cell(field, phase.shape, h)
+    cell = Cell(field, phase.shape, h, stream=side)
+    n = phase.size
+    for dtype in (torch.float32, torch.float64):
+        for _ in range(3):
+            u = torch.randn(6, 3, n, dtype=dtype, device="cuda")
+            for _ in range(20):  # a chain of producer kernels on the current stream
+                u = u * 1.0001 + 0.5
+            assert torch.equal(cell.apply_planes(u), ref_cell.apply_planes(u))
+    assert torch.equal(cell.rhs_planes(np.eye(6)), ref_cell.rhs_planes(np.eye(6)))
+    pc, ref_pc = Preconditioner(cell, 8000.0, 9000.0), Preconditioner(ref_cell, 8000.0, 9000.0)
+    r = torch.randn(6, 3, n, dtype=torch.float32, device="cuda") * 3.0 + 1.0
+    assert torch.equal(pc.apply_planes(r), ref_pc.apply_planes(r))
+    pc.close()
+    ref_pc.close()
+    solver, ref_solver = Solver(cell, 8000.0, 9000.0), Solver(ref_cell, 8000.0, 9000.0)
+    got = solver.run(return_fields=True)
+    ref = ref_solver.run(return_fields=True)
+    assert torch.equal(got["fields"], ref["fields"]) and np.array_equal(got["CH"], ref["CH"])
+    assert max(got["true_residuals"]) <= 1e-8
+    for obj in (solver, ref_solver, cell, ref_cell):
+        obj.close()

@@ -7,6 +7,7 @@ operator protocol (src/voxel_lmfem/_core/materials/hybrid_adapter.py:39-125:
 ``project_zero_mean``, ``apply_precond``) on (1, 3N) FP64 node-major tensors,
 and component-plane batched entry points used by the solver.
 """
+import contextlib
 import ctypes
 from types import SimpleNamespace
 
@@ -142,6 +143,24 @@ class Cell:
         except Exception:
             pass
 
+    @contextlib.contextmanager
+    def ordered(self):
+        """libvxl launches on the cell's stream; tensors handed to it are made
+        and read on the caller's current stream.  Inside this block the cell's
+        stream waits for the work queued so far on the current stream (inputs
+        written, freed blocks reused), and on leaving the current stream waits
+        for the cell's (outputs read there).  Device-side waits only; nothing
+        to do when the cell runs on the current stream."""
+        cur = self.torch.cuda.current_stream(self.device)
+        same = cur.cuda_stream == self.stream.cuda_stream
+        if not same:
+            self.stream.wait_stream(cur)
+        try:
+            yield
+        finally:
+            if not same:
+                cur.wait_stream(self.stream)
+
     # -- component-plane batched entry points ---------------------------------
     def apply_planes(self, u, out=None):
         """u: (R, 3, N) float32/float64 device tensor -> K u (same layout)."""
@@ -153,7 +172,8 @@ class Cell:
             raise ValueError("apply supports float32 and float64")
         u = u.contiguous()
         out = torch.empty_like(u) if out is None else out
-        check(self.lib.vxl_apply(self.handle, precision, u.shape[0], dptr(u), dptr(out)), "vxl_apply")
+        with self.ordered():
+            check(self.lib.vxl_apply(self.handle, precision, u.shape[0], dptr(u), dptr(out)), "vxl_apply")
         return out
 
     def _timing_buffers(self, nrhs, dtype):
@@ -252,7 +272,8 @@ class Cell:
     def rhs_planes(self, macro):
         macro, p = darray(np.atleast_2d(macro))
         out = self.torch.empty((len(macro), 3, self.n), dtype=self.torch.float64, device=self.device)
-        check(self.lib.vxl_rhs(self.handle, len(macro), p, dptr(out)), "vxl_rhs")
+        with self.ordered():
+            check(self.lib.vxl_rhs(self.handle, len(macro), p, dptr(out)), "vxl_rhs")
         return out
 
     def avg_stress_planes(self, u, macro):
@@ -261,8 +282,9 @@ class Cell:
         if u.dtype != self.torch.float64 or u.shape != (len(macro), 3, self.n):
             raise ValueError("expected FP64 (R, 3, N) displacements, one per macro strain")
         out = np.empty((len(macro), 6))
-        check(self.lib.vxl_avg_stress(self.handle, len(macro), dptr(u), p,
-                                      out.ctypes.data_as(_native._DP)), "vxl_avg_stress")
+        with self.ordered():
+            check(self.lib.vxl_avg_stress(self.handle, len(macro), dptr(u), p,
+                                          out.ctypes.data_as(_native._DP)), "vxl_avg_stress")
         return out
 
     def center_fields_planes(self, u, macro, *, strain=True, stress=True, energy=True):
@@ -277,19 +299,25 @@ class Cell:
         sig = torch.empty(R, 6, self.n, dtype=torch.float64, device=self.device) if stress else None
         w = torch.empty(R, self.n, dtype=torch.float64, device=self.device) if energy else None
         null = ctypes.c_void_p()
-        check(self.lib.vxl_center_fields(self.handle, R, dptr(u), p, dptr(eps) if strain else null,
-                                         dptr(sig) if stress else null, dptr(w) if energy else null),
-              "vxl_center_fields")
+        with self.ordered():
+            check(self.lib.vxl_center_fields(self.handle, R, dptr(u), p, dptr(eps) if strain else null,
+                                             dptr(sig) if stress else null, dptr(w) if energy else null),
+                  "vxl_center_fields")
         return eps, sig, w
 
     def inverse_diagonal_planes(self):
         """(3, N) FP64: 1 / diag K, assembled from the 8 elements around each node."""
         out = self.torch.empty((3, self.n), dtype=self.torch.float64, device=self.device)
-        check(self.lib.vxl_cell_inverse_diagonal(self.handle, dptr(out)), "vxl_cell_inverse_diagonal")
+        with self.ordered():
+            check(self.lib.vxl_cell_inverse_diagonal(self.handle, dptr(out)), "vxl_cell_inverse_diagonal")
         return out
 
     def project_planes(self, u):
-        check(self.lib.vxl_project_zero_mean(self.handle, u.shape[0], dptr(u)), "vxl_project_zero_mean")
+        """Subtract each (R, 3) component plane's mean in place (FP64)."""
+        if u.dtype != self.torch.float64 or u.dim() != 3 or u.shape[1:] != (3, self.n) or not u.is_contiguous():
+            raise ValueError("expected contiguous FP64 (R, 3, N) planes")
+        with self.ordered():
+            check(self.lib.vxl_project_zero_mean(self.handle, u.shape[0], dptr(u)), "vxl_project_zero_mean")
         return u
 
     # -- reference operator protocol on (1, 3N) node-major FP64 -----------------
@@ -394,7 +422,8 @@ class Preconditioner:
             raise ValueError("expected (R, 3, N) in the preconditioner's precision")
         r = r.contiguous()
         out = torch.empty_like(r) if out is None else out
-        check(self.cell.lib.vxl_precond_apply(self.handle, r.shape[0], dptr(r), dptr(out)), "vxl_precond_apply")
+        with self.cell.ordered():
+            check(self.cell.lib.vxl_precond_apply(self.handle, r.shape[0], dptr(r), dptr(out)), "vxl_precond_apply")
         return out
 
     def apply_precond(self, residual):
@@ -502,9 +531,10 @@ class Solver:
         fields = None
         if return_fields:
             fields = torch.empty((self.nrhs, 3, self.cell.n), dtype=torch.float64, device=self.cell.device)
-        rc = self.cell.lib.vxl_solver_run(self.handle, mp, float(tol), int(max_iter),
-                                          ch.ctypes.data_as(_native._DP), its, rst, res,
-                                          dptr(fields) if fields is not None else None)
+        with self.cell.ordered():
+            rc = self.cell.lib.vxl_solver_run(self.handle, mp, float(tol), int(max_iter),
+                                              ch.ctypes.data_as(_native._DP), its, rst, res,
+                                              dptr(fields) if fields is not None else None)
         if rc not in (_native.VXL_OK, _native.VXL_ERR_CONVERGENCE):
             check(rc, "vxl_solver_run")  # before any other call replaces the error text
         error = None if rc == _native.VXL_OK else self.cell.lib.vxl_last_error()

@@ -270,10 +270,15 @@ def homogenize_batch(cells, *, precision="mixed", tol=TOL, streams=4, workers=4,
             else:
                 cell = Cell.stack([prepared[k][1] for k in ks], shape, h, stream=stream)
                 lam, mu = [prepared[k][3] for k in ks], [prepared[k][4] for k in ks]
-            if autotune:
-                prec = "float32" if precision == "mixed" else "float64"
-                tuned_any |= tuner.tune(cell, prec, 6 * len(ks), entries=entries)["tuned"]
-            built.append((cell, Solver(cell, lam, mu, mode=precision), ks))
+            try:
+                if autotune:
+                    prec = "float32" if precision == "mixed" else "float64"
+                    tuned_any |= tuner.tune(cell, prec, 6 * len(ks), entries=entries)["tuned"]
+                solver = Solver(cell, lam, mu, mode=precision)
+            except BaseException:
+                cell.close()  # not yet in `built`, which the finally below closes
+                raise
+            built.append((cell, solver, ks))
         if tuned_any:
             tuner.save(entries, tune_cache, strict=False)
         outs = run_batch([b[1] for b in built], tol=tol, workers=workers)
