@@ -32,6 +32,10 @@ mfma_stencil: the same six-RHS apply on the matrix cores (vxl_cell_set_mfma),
 same buffers and events.  block_cg: the resident 256^3 solve (and the batch,
 batch.block_cg) with block CG over the six loads instead of six CG recurrences.  batch.graph_replay: the 64-cell batch again with the
 check blocks replayed from HIP graphs.
+woven_reference_workload: the reference's own published headline workload, its
+analytic woven cell at 192^3 and 256^3, as complete `homogenize` calls (mixed
+and FP64) beside the reference's published complete-call times on its GPU, with
+C_eff against the reference's certified record of the same cell.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 """
@@ -239,6 +243,59 @@ def frames_leg(n=256, R=6, reps=20):
     return out
 
 
+# The reference's published complete-call times of its analytic woven cell
+# (RTX 5090; FP64 operator and PCG vectors, FP32 FFT preconditioner, FP64
+# true residual <= 1e-8; material construction, operator preparation, RHS,
+# solves, residual checks, stress recovery and cleanup included; median of
+# three after a same-grid warm-up): repro/generated/figure_9_optimization_chain.csv:7
+# (192^3) and repro/generated/figure_10_grid_cost.csv:11 (256^3).
+REFERENCE_WOVEN_SECONDS = {192: (1.8897, "repro/generated/figure_9_optimization_chain.csv:7"),
+                           256: (4.3867, "repro/generated/figure_10_grid_cost.csv:11")}
+
+
+def woven_leg(grids=(192, 256), repeats=3):
+    """The reference's own headline workload on this GPU: the analytic woven
+    cell (four orthotropic yarns in local frames, matrix), six unit loads,
+    through `homogenize` (the drop-in call) in mixed and FP64 precision.
+    Complete-call wall time (host validation and material front end
+    included, geometry generation excluded), median of `repeats` calls after
+    one warm-up call, beside the reference's published time; C_eff against
+    the reference's certified record of the same cell (tests/golden)."""
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import reference_weave
+    records = json.loads((Path(__file__).resolve().parent / "tests" / "golden" / "woven_records.json").read_text())
+    out = {}
+    for g in grids:
+        (phase, mats, dom, ori), meta = reference_weave(g)
+        rec = records[f"certify_{g}"]
+        ref_s, ref_src = REFERENCE_WOVEN_SECONDS[g]
+        row = dict(input_hash_matches_reference=meta["input_hash"] == rec["input_hash"],
+                   reference_complete_call_s=ref_s, reference_source=ref_src,
+                   reference_hardware="RTX 5090 (FP64 operator, FP32 FFT preconditioner)")
+        for precision in ("mixed", "fp64"):
+            homogenize(phase, mats, dom, orientation=ori, precision=precision)
+            walls, inner, res = [], [], None
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                res = homogenize(phase, mats, dom, orientation=ori, precision=precision)
+                walls.append(time.perf_counter() - t0)
+                inner.append(res["seconds"])
+            wall = float(np.median(walls))
+            ch = np.asarray(res["CH"])
+            row[precision] = dict(
+                complete_call_s=wall, complete_call_s_all=[round(w, 4) for w in walls],
+                workload_s=float(np.median(inner)), speedup_vs_reference=ref_s / wall,
+                stages_s={st["name"]: round(st["seconds"], 4) for st in res["stages"]},
+                iterations=[c["iterations"] for c in res["calls"]], max_true_residual=res["max_true_residual"],
+                ceff_rel_diff_vs_reference_record=float(np.linalg.norm(ch - np.asarray(rec["CH"])) /
+                                                        np.linalg.norm(np.asarray(rec["CH"]))))
+            print(f"# woven {g}^3 {precision}: {wall:.3f} s per complete call (reference {ref_s:.3f} s on its "
+                  f"GPU), iterations {row[precision]['iterations']}, C_eff vs the reference's certified record "
+                  f"{row[precision]['ceff_rel_diff_vs_reference_record']:.1e}", file=sys.stderr, flush=True)
+        out[f"{g}"] = row
+    return out
+
+
 def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg", stack=1, stack_voxels=64 ** 3,
               sizes=(32, 48, 64, 80, 96)):
     """Uneven batch: `count` two-phase random-inclusion cells (edges drawn from
@@ -384,6 +441,9 @@ def main():
     ap.add_argument("--skip-mfma", action="store_true", help="skip the matrix-core stencil A/B")
     ap.add_argument("--skip-block", action="store_true", help="skip the block-CG A/B")
     ap.add_argument("--skip-small", action="store_true", help="skip the 32^3 / 16^3 sphere cells on the GPU")
+    ap.add_argument("--skip-woven", action="store_true",
+                    help="skip the reference's woven 192^3 / 256^3 cells (its published headline workload)")
+    ap.add_argument("--woven-only", action="store_true", help="run only the woven reference-workload leg")
     ap.add_argument("--stencil", default="modal", choices=["modal", "mfma"],
                     help="FP32 operator of the timed mixed solve: modal VALU stencil or matrix-core form")
     ap.add_argument("--frames-only", action="store_true",
@@ -410,6 +470,11 @@ def main():
         import torch
         torch.cuda.set_device(0)
         print(json.dumps(dict(batch=batch_leg(steps=args.steps, warmup=args.warmup))), flush=True)
+        return
+    if args.woven_only:
+        import torch
+        torch.cuda.set_device(0)
+        print(json.dumps(dict(woven_reference_workload=woven_leg())), flush=True)
         return
     sample = samples_all = None
     e2e_proc = None
@@ -748,6 +813,8 @@ def main():
             print(f"# cpu end-to-end: c1 {got['c1_seconds']:.1f} s, direct16 {got['direct16_seconds']:.1f} s",
                   file=sys.stderr, flush=True)
 
+    woven = None if args.skip_woven else woven_leg()
+
     traffic, traffic_src = measured_traffic()
     line = dict(
         metric="256^3 polycrystal C_eff solves/s (mixed FP32/FP64, 6 loads, FP64 residual <= 1e-8)",
@@ -775,6 +842,7 @@ def main():
         cpu_baseline_all_cores=cpu_all,
         cpu_end_to_end=cpu_e2e,
         gpu_small_cells=small,
+        woven_reference_workload=woven,
     )
     print(json.dumps(line), flush=True)
 

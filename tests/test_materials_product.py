@@ -147,3 +147,101 @@ def test_phase_frames_bookkeeping():
     assert np.abs(back - R.reshape(-1, 3, 3)[loc]).max() < 1e-14
     with pytest.raises(ValueError, match="require orientation"):
         pm.phase_frames(phase, mats)
+
+
+def _first_error(fn):
+    try:
+        fn()
+    except Exception as exc:  # noqa: BLE001 - the type and text are what is compared
+        return type(exc), str(exc)
+    return None
+
+
+def _reference_loop(orientation, shape, selections, chunk):
+    """The reference's frame validation loop (src/voxel_lmfem/inputs.py:41-46)
+    over the oracle's restatement of its frame routine."""
+    for sel in selections:
+        for first in range(0, len(sel), chunk):
+            om.orientation_values(orientation, shape, sel[first:first + chunk])
+
+
+def _axes(shape, rng):
+    a = rng.normal(size=shape + (3,))
+    b = rng.normal(size=shape + (3,))
+    return a, b
+
+
+def _case(name, rng):
+    shape = (18, 20, 24)
+    phase = rng.integers(0, 3, size=shape)
+    sels = [np.flatnonzero(phase.ravel() == k) for k in (1, 2)]
+    a, b = _axes(shape, rng)
+    flat_a, flat_b = a.reshape(-1, 3), b.reshape(-1, 3)
+    s1, s2 = sels
+    if name == "valid":
+        pass
+    elif name == "nan_late_collinear_early":
+        flat_b[s1[400]] = np.nan
+        flat_b[s1[60]] = 3.0 * flat_a[s1[60]]
+    elif name == "collinear_late_zero_early":
+        flat_b[s1[300]] = -2.0 * flat_a[s1[300]]
+        flat_a[s1[120]] = 0.0
+    elif name == "nan_and_collinear_same_chunk":
+        flat_b[s1[5]] = flat_a[s1[5]]
+        flat_a[s1[7]] = np.inf
+    elif name == "second_label_only":
+        flat_b[s2[200]] = 0.0
+    elif name == "broadcast_collinear":
+        return shape, dict(axis1=np.array([1.0, 0.0, 0.0]), axis2=np.array([2.0, 0.0, 0.0])), sels
+    elif name == "broadcast_valid":
+        return shape, dict(axis1=np.array([1.0, 0.0, 0.0]), axis2=np.array([0.0, 1.0, 0.0])), sels
+    elif name == "not_broadcastable":
+        return shape, dict(axis1=a[:, :, :5], axis2=b), sels
+    elif name == "float32_axes":
+        flat_b[s1[350]] = flat_a[s1[350]]
+        return shape, dict(axis1=a.astype(np.float32), axis2=b.astype(np.float32)), sels
+    elif name in ("rot_valid", "rot_det_early_skew_late", "rot_nan"):
+        q = rng.normal(size=shape + (4,))
+        q /= np.linalg.norm(q, axis=-1, keepdims=True)
+        w, x, y, z = np.moveaxis(q, -1, 0)
+        R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                      np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                      np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+        flat = R.reshape(-1, 3, 3)
+        if name == "rot_det_early_skew_late":
+            flat[s1[420]] *= 1.01
+            flat[s1[90]] = -flat[s1[90]]
+        if name == "rot_nan":
+            flat[s2[10], 1, 2] = np.nan
+        return shape, R, sels
+    return shape, dict(axis1=a, axis2=b), sels
+
+
+@pytest.mark.parametrize("name", ["valid", "nan_late_collinear_early", "collinear_late_zero_early",
+                                  "nan_and_collinear_same_chunk", "second_label_only", "broadcast_collinear",
+                                  "broadcast_valid", "not_broadcastable", "float32_axes", "rot_valid",
+                                  "rot_det_early_skew_late", "rot_nan"])
+@pytest.mark.parametrize("workers", [1, 4])
+def test_check_frames_raises_what_the_chunked_reference_loop_raises(name, workers):
+    """The pooled frame check (pieces on threads, no frames built) raises the
+    exception the reference's piece-by-piece loop raises first, or none."""
+    shape, orientation, sels = _case(name, np.random.default_rng(17))
+    chunk = 64  # dozens of pieces per label at this size
+    expect = _first_error(lambda: _reference_loop(orientation, shape, sels, chunk))
+    got = _first_error(lambda: pm.check_frames(orientation, shape, sels, chunk=chunk, workers=workers))
+    assert got == expect
+    if name in ("valid", "broadcast_valid", "rot_valid"):
+        assert got is None
+    elif name != "not_broadcastable":
+        assert got is not None and got[0] is ValueError
+
+
+@pytest.mark.parametrize("dtype,lo,hi", [(np.uint8, 0, 255), (np.uint8, 3, 9), (np.int8, -128, 127),
+                                         (np.int16, -300, 2000), (np.int32, -7, 40), (np.int64, -(2 ** 40), 2 ** 40),
+                                         (np.uint64, 2 ** 63 + 5, 2 ** 63 + 90), (np.uint16, 0, 65535)])
+def test_used_labels_equal_numpy_unique(dtype, lo, hi):
+    rng = np.random.default_rng(3)
+    phase = rng.integers(lo, hi, size=(7, 6, 5), dtype=dtype, endpoint=True)
+    got = inputs.used_labels(phase)
+    ref = np.unique(phase)
+    assert got.dtype == ref.dtype and np.array_equal(got, ref)

@@ -239,10 +239,9 @@ def import_model(input_path, materials_path, out, *, axes="zyx", spacing=None, d
         phase, checked, domain, directions, labels, _, local = validate(phase, table, size, directions)
     except (TypeError, KeyError, OverflowError) as exc:
         raise ValueError(f"Invalid material definition: {exc}") from exc
-    for slot in np.flatnonzero(local):   # every local frame must be a proper rotation
-        voxels = np.flatnonzero(phase.reshape(-1) == labels[slot])
-        for first in range(0, voxels.size, 65536):
-            mat.frames(directions, phase.shape, voxels[first:first + 65536])
+    if local.any():   # every local frame must be a proper rotation
+        flat = phase.reshape(-1)
+        mat.check_frames(directions, phase.shape, [np.flatnonzero(flat == labels[s]) for s in np.flatnonzero(local)])
 
     arrays = {"phase_id": phase, "domain_size": domain}
     if isinstance(directions, dict):

@@ -12,6 +12,18 @@ import numpy as np
 from . import materials as mat
 
 
+def used_labels(phase):
+    """np.unique(phase) of an integer array: a count over the label range when
+    that range is narrow (one pass), a sort otherwise.  Same values and dtype."""
+    flat = phase.reshape(-1)
+    if flat.size:
+        lo, hi = int(flat.min()), int(flat.max())
+        if hi - lo < (1 << 22) and hi <= np.iinfo(np.int64).max:
+            shifted = flat if lo == 0 else flat.astype(np.int64) - lo
+            return (np.flatnonzero(np.bincount(shifted, minlength=hi - lo + 1)) + lo).astype(phase.dtype)
+    return np.unique(flat)
+
+
 def validate(phase_id, materials, domain_size, orientation=None):
     """Return (phase, table, domain, orientation, labels, tensors, local)."""
     phase = np.asarray(phase_id)
@@ -35,7 +47,7 @@ def validate(phase_id, materials, domain_size, orientation=None):
             raise ValueError(f"Duplicate material label after conversion: {label}")
         table[label] = value
     labels, tensors, local = mat.phase_table(table)
-    used = np.unique(phase)
+    used = used_labels(phase)
     if not np.isin(used, labels).all():
         raise ValueError(f"Undefined material labels: {used[~np.isin(used, labels)].tolist()}")
     if orientation is None:

@@ -122,42 +122,106 @@ def phase_table(materials):
     return labels, tensors, local
 
 
+def _as_float(values):
+    """Orientation values as float64 (one conversion per array)."""
+    return np.asarray(values, dtype=float)
+
+
+def _pick(values, shape, tail, flat_indices, coordinates=None):
+    """Entries of `values` (float64, broadcastable to shape + tail) at the
+    flat voxel indices, checked finite.  A full-shape C-ordered array is read
+    through its flat view, a broadcast one at the unravelled coordinates;
+    both give the same values."""
+    try:
+        view = np.broadcast_to(values, tuple(shape) + tail)
+    except ValueError as exc:
+        raise ValueError("Orientation shape is not broadcastable to phase_id") from exc
+    if values.shape == view.shape and values.flags.c_contiguous:
+        out = values.reshape((-1,) + tail)[flat_indices]
+    else:
+        if coordinates is None:
+            coordinates = np.unravel_index(flat_indices, shape)
+        out = np.array(view[coordinates], copy=True)
+    if not np.isfinite(out).all():
+        raise ValueError("Active local-frame orientations must be finite")
+    return out
+
+
+def _axis_frames(a, b, build=True):
+    """Frames from axis pairs: columns a, c x a, c with c = a x b, all unit;
+    the checks (nonzero, not collinear) raise the reference's messages."""
+    na, nb = np.linalg.norm(a, axis=-1), np.linalg.norm(b, axis=-1)
+    if (na <= 0).any() or (nb <= 0).any():
+        raise ValueError("Orientation axes must be nonzero")
+    a, b = a / na[:, None], b / nb[:, None]
+    c = np.cross(a, b)
+    nc = np.linalg.norm(c, axis=-1)
+    if (nc <= 1e-12).any():
+        raise ValueError("Orientation axes must not be collinear")
+    if not build:
+        return None
+    c /= nc[:, None]
+    return np.stack([a, np.cross(c, a), c], axis=-1)
+
+
+def _matrix_frames(R):
+    if not np.allclose(R.swapaxes(-1, -2) @ R, np.eye(3), rtol=0, atol=1e-10):
+        raise ValueError("Orientation must be an orthonormal rotation matrix")
+    if not np.allclose(np.linalg.det(R), 1, rtol=0, atol=1e-10):
+        raise ValueError("Orientation must be right-handed (det=+1)")
+    return R
+
+
 def frames(orientation, shape, flat_indices):
     """Material frames (k,3,3) at the selected voxels; columns = local axes.
 
     Accepts a rotation array broadcastable to shape+(3,3) or a mapping with
     axis1/axis2 broadcastable to shape+(3,).
     """
-    coordinates = np.unravel_index(flat_indices, shape)
-
-    def pick(values, tail):
-        try:
-            view = np.broadcast_to(np.asarray(values, dtype=float), tuple(shape) + tail)
-        except ValueError as exc:
-            raise ValueError("Orientation shape is not broadcastable to phase_id") from exc
-        out = np.array(view[coordinates], copy=True)
-        if not np.isfinite(out).all():
-            raise ValueError("Active local-frame orientations must be finite")
-        return out
-
     if isinstance(orientation, Mapping):
-        a, b = pick(orientation["axis1"], (3,)), pick(orientation["axis2"], (3,))
-        na, nb = np.linalg.norm(a, axis=-1), np.linalg.norm(b, axis=-1)
-        if (na <= 0).any() or (nb <= 0).any():
-            raise ValueError("Orientation axes must be nonzero")
-        a, b = a / na[:, None], b / nb[:, None]
-        c = np.cross(a, b)
-        nc = np.linalg.norm(c, axis=-1)
-        if (nc <= 1e-12).any():
-            raise ValueError("Orientation axes must not be collinear")
-        c /= nc[:, None]
-        return np.stack([a, np.cross(c, a), c], axis=-1)
-    R = pick(orientation, (3, 3))
-    if not np.allclose(R.swapaxes(-1, -2) @ R, np.eye(3), rtol=0, atol=1e-10):
-        raise ValueError("Orientation must be an orthonormal rotation matrix")
-    if not np.allclose(np.linalg.det(R), 1, rtol=0, atol=1e-10):
-        raise ValueError("Orientation must be right-handed (det=+1)")
-    return R
+        a = _pick(_as_float(orientation["axis1"]), shape, (3,), flat_indices)
+        b = _pick(_as_float(orientation["axis2"]), shape, (3,), flat_indices)
+        return _axis_frames(a, b)
+    return _matrix_frames(_pick(_as_float(orientation), shape, (3, 3), flat_indices))
+
+
+def check_frames(orientation, shape, selections, chunk=65536, workers=None):
+    """Validate the frames of the selected voxels without building them.
+
+    selections: flat voxel indices per local-frame phase, in the order they
+    are checked.  Each is cut into `chunk`-voxel pieces checked in turn, as
+    the reference validates (src/voxel_lmfem/inputs.py:41-46), and the error
+    raised is the one the first failing piece raises there: the same checks
+    in the same order (finite axis1, finite axis2, nonzero, not collinear;
+    or finite, orthonormal, right-handed), on the same values.  The pieces
+    run on a thread pool (NumPy releases the GIL in these loops); the
+    orientation arrays are converted to float64 once up front."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
+    pieces = [sel[first:first + chunk] for sel in selections for first in range(0, len(sel), chunk)]
+    if not pieces:
+        return
+    if isinstance(orientation, Mapping):
+        arrays = (_as_float(orientation["axis1"]), _as_float(orientation["axis2"]))
+
+        def one(idx):
+            a = _pick(arrays[0], shape, (3,), idx)
+            b = _pick(arrays[1], shape, (3,), idx)
+            _axis_frames(a, b, build=False)
+    else:
+        rot = _as_float(orientation)
+
+        def one(idx):
+            _matrix_frames(_pick(rot, shape, (3, 3), idx))
+    if workers is None:
+        workers = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
+    if workers <= 1 or len(pieces) == 1:
+        for idx in pieces:
+            one(idx)
+        return
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for _ in pool.map(one, pieces):  # results in piece order: the first failing piece raises
+            pass
 
 
 def bond_rotate(C, R):

@@ -182,10 +182,8 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="c
         raise ValueError("backend='cpu' is the reference's sparse-direct oracle and is not part of this "
                          "GPU build; use backend='gpu'")
     if local.any():  # every local frame is validated before any GPU work
-        for k in np.flatnonzero(local):
-            selected = np.flatnonzero(phase.reshape(-1) == labels[k])
-            for first in range(0, len(selected), 65536):
-                mat.frames(orientation, phase.shape, selected[first:first + 65536])
+        flat = phase.reshape(-1)
+        mat.check_frames(orientation, phase.shape, [np.flatnonzero(flat == labels[k]) for k in np.flatnonzero(local)])
     try:
         import torch
     except ImportError as exc:
