@@ -114,8 +114,10 @@ def test_homogenize_field_statistics(torch_gpu):
 
 def _voxel_tensors_from_device(field):
     rows = field.rows_dev.cpu().numpy()
-    idx = field.index_dev.cpu().numpy().astype(np.int64)
-    return pm.unpack21(rows[idx])
+    idx = field.index_dev.cpu().numpy()
+    if idx.dtype == np.int16:   # u16 row numbers stored as int16
+        idx = idx.view(np.uint16)
+    return pm.unpack21(rows[idx.astype(np.int64)])
 
 
 @pytest.mark.parametrize("kind", ["woven_axis_pairs", "fixture_matrices", "single_matrix"])
@@ -174,3 +176,38 @@ def test_woven_solve_device_front_end_matches_host_field(torch_gpu):
     solver.close()
     cell.close()
     assert np.linalg.norm(np.asarray(out["CH"]) - CH) <= 1e-11 * np.linalg.norm(CH)
+
+
+def test_device_rows_shared_between_equal_frames(torch_gpu):
+    """Voxels whose Bond-rotated rows are bitwise equal share one table row
+    (the reference weave at 64^3: a few hundred rows instead of one per yarn
+    voxel).  Every voxel sees the same coefficients bit for bit, the Zeman
+    lattice samples follow the new row numbers, and the solve matches the
+    one-row-per-voxel table's."""
+    torch = torch_gpu
+    from voxel_lmfem.cells import reference_weave
+    from voxel_lmfem.gpu import Cell, Solver
+    from voxel_lmfem.reference_medium import select
+    print("case shared device rows, reference weave 64^3", flush=True)
+    (phase, mats, dom, ori), _ = reference_weave(64)
+    labels = torch.tensor(phase.astype(np.int32), device="cuda")
+    full = pm.build_field_device(labels, mats, ori, deduplicate=False)
+    shared = pm.build_field_device(labels, mats, ori)
+    assert full.row_count == 2 + phase.size and full.index_kind == 4
+    assert shared.row_count < 2000 and shared.index_kind in (1, 2)
+    np.testing.assert_array_equal(_voxel_tensors_from_device(shared), _voxel_tensors_from_device(full))
+    np.testing.assert_array_equal(shared.lattice_rows(phase.shape), full.lattice_rows(phase.shape))
+    h = dom / np.array(phase.shape[::-1], dtype=float)
+    out = []
+    for field in (full, shared):
+        lam, mu, _ = select(field, phase, mats)
+        cell = Cell(field, phase.shape, h)
+        solver = Solver(cell, lam, mu, mode="mixed", nrhs=6)
+        got = solver.run()
+        out.append((lam, mu, np.asarray(got["CH"]), got["iterations"], max(got["true_residuals"])))
+        solver.close()
+        cell.close()
+    assert out[0][:2] == out[1][:2]
+    assert np.linalg.norm(out[1][2] - out[0][2]) <= 1e-12 * np.linalg.norm(out[0][2])
+    assert all(abs(a - b) <= 1 for a, b in zip(out[0][3], out[1][3]))
+    assert out[1][4] <= 1e-8

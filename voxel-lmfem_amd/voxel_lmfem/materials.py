@@ -439,7 +439,7 @@ def _device_frames(orientation, shape, torch, device):
     return 2, R, None, sr
 
 
-def build_field_device(phase_dev, materials, orientation=None):
+def build_field_device(phase_dev, materials, orientation=None, deduplicate=True):
     """Label and orientation bookkeeping of a cell whose int32/int64 labels are
     resident on the GPU.
 
@@ -449,7 +449,9 @@ def build_field_device(phase_dev, materials, orientation=None):
     phase gets its own Bond-rotated row, built on the device by
     ``vxl_local_rows`` after the phase rows (int32 index), with the host
     path's frame checks and messages.  Raises the reference's ValueError for
-    labels without a material.
+    labels without a material.  deduplicate: voxels whose rows are bitwise
+    equal share one row (index narrowed to u16 / u8 where it fits); False
+    keeps one row per voxel (row L + voxel).
     """
     import ctypes
     import torch
@@ -512,7 +514,59 @@ def build_field_device(phase_dev, materials, orientation=None):
             for flag, message in zip(bad, _FRAME_ERRORS):
                 if flag:
                     raise ValueError(message)
-    return DeviceField(phase_rows, index, kind, labels, tensors, local, used, samples, rows_dev)
+    field = DeviceField(phase_rows, index, kind, labels, tensors, local, used, samples, rows_dev)
+    if rows_dev is not None and deduplicate:
+        _share_equal_rows(field, L, torch)
+    return field
+
+
+def _share_equal_rows(field, L, torch):
+    """One table row per distinct Bond-rotated row instead of one per
+    local-frame voxel.  Equal frames give bitwise equal rows (the reference
+    weave repeats about a thousand frames over millions of yarn voxels), and
+    those then share one entry, as the host path's de-duplication does; the
+    coefficients every voxel sees are unchanged bit for bit.  Rows are keyed
+    by a 64-bit mix of their words; every row is then compared word for word
+    with the row of its key, and a collision keeps one row per voxel.  The
+    index narrows to u16 / u8 when the rows allow, and the Zeman lattice
+    samples follow the new row numbers."""
+    idx = field.index_dev.to(torch.int64)
+    vox = torch.nonzero(idx >= L).squeeze(1)
+    count = int(vox.numel())
+    dev = idx.device
+    new_idx = idx.clone()
+    if count == 0:   # no voxel of a local-frame phase: the phase rows alone
+        table = field.rows_dev[:L]
+    else:
+        rows = field.rows_dev.index_select(0, idx.index_select(0, vox))
+        words = rows.view(torch.int64)
+        key = words[:, 0].clone()
+        for k in range(1, 21):   # wraps modulo 2^64; only groups rows, the comparison below decides
+            w = words[:, k]
+            key = key * 0x100000001B3 + (w ^ (w >> 31))
+        keys, inverse = torch.unique(key, return_inverse=True)
+        first = torch.full((keys.numel(),), count, dtype=torch.int64, device=dev)
+        first.scatter_reduce_(0, inverse, torch.arange(count, dtype=torch.int64, device=dev), reduce="amin")
+        if not torch.equal(words.index_select(0, first).index_select(0, inverse), words):
+            return
+        new_idx[vox] = L + inverse
+        table = torch.cat([field.rows_dev[:L], rows.index_select(0, first)])
+    total = int(table.shape[0])
+    dtype = index_dtype(total)
+    if dtype == np.uint8:
+        index = new_idx.to(torch.uint8)
+    elif dtype == np.uint16:   # stored as int16 with the u16 bit pattern
+        index = torch.where(new_idx >= 32768, new_idx - 65536, new_idx).to(torch.int16)
+    else:
+        index = new_idx.to(torch.int32)
+    pick = torch.as_tensor(field.samples.astype(np.int64), device=dev)
+    local_pick = pick >= L
+    pick[local_pick] = new_idx.index_select(0, pick[local_pick] - L)
+    field.samples = pick.cpu().numpy().astype(np.int32)
+    field.index_dev = index.contiguous()
+    field.kind = {np.uint8: 1, np.uint16: 2, np.int32: 4}[dtype]
+    field.rows_dev = table.contiguous()
+    field._lattice = None
 
 
 def build_field(phase, materials, orientation=None, *, deduplicate=True):
