@@ -444,6 +444,9 @@ def main():
     ap.add_argument("--skip-woven", action="store_true",
                     help="skip the reference's woven 192^3 / 256^3 cells (its published headline workload)")
     ap.add_argument("--woven-only", action="store_true", help="run only the woven reference-workload leg")
+    ap.add_argument("--woven-grids", type=int, nargs="+", default=[192, 256], choices=[192, 256],
+                    help="woven cells of the reference-workload leg")
+    ap.add_argument("--woven-repeats", type=int, default=3, help="timed calls per woven cell and precision")
     ap.add_argument("--stencil", default="modal", choices=["modal", "mfma"],
                     help="FP32 operator of the timed mixed solve: modal VALU stencil or matrix-core form")
     ap.add_argument("--frames-only", action="store_true",
@@ -474,7 +477,7 @@ def main():
     if args.woven_only:
         import torch
         torch.cuda.set_device(0)
-        print(json.dumps(dict(woven_reference_workload=woven_leg())), flush=True)
+        print(json.dumps(dict(woven_reference_workload=woven_leg(args.woven_grids, args.woven_repeats))), flush=True)
         return
     sample = samples_all = None
     e2e_proc = None
@@ -813,7 +816,7 @@ def main():
             print(f"# cpu end-to-end: c1 {got['c1_seconds']:.1f} s, direct16 {got['direct16_seconds']:.1f} s",
                   file=sys.stderr, flush=True)
 
-    woven = None if args.skip_woven else woven_leg()
+    woven = None if args.skip_woven else woven_leg(args.woven_grids, args.woven_repeats)
 
     traffic, traffic_src = measured_traffic()
     line = dict(

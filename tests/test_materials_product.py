@@ -245,3 +245,66 @@ def test_used_labels_equal_numpy_unique(dtype, lo, hi):
     got = inputs.used_labels(phase)
     ref = np.unique(phase)
     assert got.dtype == ref.dtype and np.array_equal(got, ref)
+
+
+def _fake_device_field(torch, L=3, n=4000, distinct=40, seed=5):
+    """A DeviceField-shaped object on CPU tensors: L phase rows, local voxels
+    (every third voxel) with one of `distinct` rows each, at row L + voxel."""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(seed)
+    table = np.zeros((L + n, 21))
+    table[:L] = rng.normal(size=(L, 21))
+    index = rng.integers(0, L, size=n).astype(np.int32)
+    local = np.arange(n) % 3 == 0
+    choices = rng.normal(size=(distinct, 21))
+    choices[1::2] = choices[0::2]
+    choices[1::2, [4, 9]] *= -1   # rows of frames +theta / -theta: two words differ in the sign bit
+    table[L + np.flatnonzero(local)] = choices[rng.integers(0, distinct, size=int(local.sum()))]
+    index[local] = L + np.flatnonzero(local)
+    samples = np.arange(0, n, 7).astype(np.int32)
+    samples = index[samples]
+    return SimpleNamespace(index_dev=torch.as_tensor(index), rows_dev=torch.as_tensor(table), samples=samples,
+                           kind=4, _lattice=None)
+
+
+def _effective(field):
+    idx = field.index_dev.numpy()
+    if idx.dtype == np.int16:
+        idx = idx.view(np.uint16)
+    return field.rows_dev.numpy()[idx.astype(np.int64)]
+
+
+def test_equal_rows_share_one_table_row():
+    """Device row sharing (run here on CPU tensors): every voxel keeps its
+    coefficients bit for bit, the table shrinks to the distinct rows, the
+    index narrows, and the lattice samples point at rows with the same
+    coefficients."""
+    import torch
+    field = _fake_device_field(torch)
+    before = _effective(field).copy()
+    sampled = field.rows_dev.numpy()[field.samples.astype(np.int64)].copy()
+    pm._share_equal_rows(field, 3, torch)
+    assert field.rows_dev.shape[0] <= 3 + 40 and field.kind == 1
+    assert np.array_equal(_effective(field).view(np.int64), before.view(np.int64))
+    assert np.array_equal(field.rows_dev.numpy()[field.samples.astype(np.int64)], sampled)
+
+
+def test_row_key_collisions_keep_one_row_per_voxel(monkeypatch):
+    """With every key equal (a forced collision) the word-for-word check
+    fails and the one-row-per-voxel table stays as built."""
+    import torch
+    field = _fake_device_field(torch)
+    rows_before = field.rows_dev.clone()
+    monkeypatch.setattr(pm, "_ROW_KEY_MIX", (0, 0))
+    pm._share_equal_rows(field, 3, torch)
+    assert torch.equal(field.rows_dev, rows_before) and field.kind == 4
+
+
+def test_no_local_voxels_keep_the_phase_rows_only():
+    import torch
+    field = _fake_device_field(torch)
+    field.index_dev = torch.as_tensor(np.arange(4000, dtype=np.int32) % 3)
+    field.samples = np.array([0, 1, 2], dtype=np.int32)
+    pm._share_equal_rows(field, 3, torch)
+    assert field.rows_dev.shape[0] == 3 and field.kind == 1
+    assert np.array_equal(field.index_dev.numpy(), np.arange(4000) % 3)

@@ -520,6 +520,12 @@ def build_field_device(phase_dev, materials, orientation=None, deduplicate=True)
     return field
 
 
+# Odd multipliers of the row key (fixed, below 2^63): the two word-mixing
+# steps, then one weight per coefficient word.
+_ROW_KEY_MIX = (0x3F58476D1CE4E5B9, 0x14D049BB133111EB)
+_ROW_KEY_MULTIPLIERS = [(0x9E3779B97F4A7C15 * (2 * k + 1) + 0x632BE59BD9B4E019 * k) % (1 << 63) | 1 for k in range(21)]
+
+
 def _share_equal_rows(field, L, torch):
     """One table row per distinct Bond-rotated row instead of one per
     local-frame voxel.  Equal frames give bitwise equal rows (the reference
@@ -540,13 +546,23 @@ def _share_equal_rows(field, L, torch):
     else:
         rows = field.rows_dev.index_select(0, idx.index_select(0, vox))
         words = rows.view(torch.int64)
-        key = words[:, 0].clone()
-        for k in range(1, 21):   # wraps modulo 2^64; only groups rows, the comparison below decides
-            w = words[:, k]
-            key = key * 0x100000001B3 + (w ^ (w >> 31))
-        keys, inverse = torch.unique(key, return_inverse=True)
-        first = torch.full((keys.numel(),), count, dtype=torch.int64, device=dev)
-        first.scatter_reduce_(0, inverse, torch.arange(count, dtype=torch.int64, device=dev), reduce="amin")
+        # Key: every word mixed on its own (xor-shift-multiply, so that sign
+        # flips of two words, as frames +theta and -theta give, do not cancel),
+        # then summed with odd per-word weights, wrapping modulo 2^64.  It
+        # only groups rows; the word-for-word comparison decides.
+        z = words ^ (words >> 30)
+        z = z * _ROW_KEY_MIX[0]
+        z = z ^ (z >> 27)
+        z = z * _ROW_KEY_MIX[1]
+        mult = torch.tensor(_ROW_KEY_MULTIPLIERS, dtype=torch.int64, device=dev)
+        key = (z * mult).sum(dim=1)
+        sorted_keys, order = torch.sort(key)
+        starts = torch.ones(count, dtype=torch.bool, device=dev)
+        starts[1:] = sorted_keys[1:] != sorted_keys[:-1]
+        group = torch.cumsum(starts, 0) - 1
+        first = order[starts]                   # one voxel per group (its members are checked equal)
+        inverse = torch.empty_like(order)
+        inverse[order] = group
         if not torch.equal(words.index_select(0, first).index_select(0, inverse), words):
             return
         new_idx[vox] = L + inverse
