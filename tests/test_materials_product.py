@@ -158,7 +158,7 @@ def _first_error(fn):
 
 
 def _reference_loop(orientation, shape, selections, chunk):
-    """The reference's frame validation loop (src/voxel_lmfem/inputs.py:41-46)
+    """The reference's frame validation loop (src/voxel_lmfem/inputs.py:42-47)
     over the oracle's restatement of its frame routine."""
     for sel in selections:
         for first in range(0, len(sel), chunk):

@@ -190,7 +190,7 @@ def check_frames(orientation, shape, selections, chunk=65536, workers=None):
 
     selections: flat voxel indices per local-frame phase, in the order they
     are checked.  Each is cut into `chunk`-voxel pieces checked in turn, as
-    the reference validates (src/voxel_lmfem/inputs.py:41-46), and the error
+    the reference validates (src/voxel_lmfem/inputs.py:42-47), and the error
     raised is the one the first failing piece raises there: the same checks
     in the same order (finite axis1, finite axis2, nonzero, not collinear;
     or finite, orthonormal, right-handed), on the same values.  The pieces
