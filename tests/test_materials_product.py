@@ -238,13 +238,23 @@ def test_check_frames_raises_what_the_chunked_reference_loop_raises(name, worker
 
 @pytest.mark.parametrize("dtype,lo,hi", [(np.uint8, 0, 255), (np.uint8, 3, 9), (np.int8, -128, 127),
                                          (np.int16, -300, 2000), (np.int32, -7, 40), (np.int64, -(2 ** 40), 2 ** 40),
-                                         (np.uint64, 2 ** 63 + 5, 2 ** 63 + 90), (np.uint16, 0, 65535)])
+                                         (np.uint64, 2 ** 63 + 5, 2 ** 63 + 90), (np.uint64, 0, 90),
+                                         (np.uint16, 0, 65535)])
 def test_used_labels_equal_numpy_unique(dtype, lo, hi):
     rng = np.random.default_rng(3)
-    phase = rng.integers(lo, hi, size=(7, 6, 5), dtype=dtype, endpoint=True)
-    got = inputs.used_labels(phase)
-    ref = np.unique(phase)
-    assert got.dtype == ref.dtype and np.array_equal(got, ref)
+    for shape in ((7, 6, 5), (40, 180, 300)):   # one piece; several pieces on the thread pool
+        phase = rng.integers(lo, hi, size=shape, dtype=dtype, endpoint=True)
+        got = inputs.used_labels(phase)
+        ref = np.unique(phase)
+        assert got.dtype == ref.dtype and np.array_equal(got, ref)
+
+
+def test_voxels_with_label_equal_flatnonzero():
+    rng = np.random.default_rng(4)
+    flat = rng.integers(0, 5, size=3_000_001).astype(np.uint8)
+    for label in (0, 3, 7):
+        assert np.array_equal(pm.voxels_with_label(flat, label), np.flatnonzero(flat == label))
+    assert pm.voxels_with_label(flat[:0], 1).size == 0
 
 
 def _fake_device_field(torch, L=3, n=4000, distinct=40, seed=5):

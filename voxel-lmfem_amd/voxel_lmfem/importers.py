@@ -241,7 +241,7 @@ def import_model(input_path, materials_path, out, *, axes="zyx", spacing=None, d
         raise ValueError(f"Invalid material definition: {exc}") from exc
     if local.any():   # every local frame must be a proper rotation
         flat = phase.reshape(-1)
-        mat.check_frames(directions, phase.shape, [np.flatnonzero(flat == labels[s]) for s in np.flatnonzero(local)])
+        mat.check_frames(directions, phase.shape, [mat.voxels_with_label(flat, labels[s]) for s in np.flatnonzero(local)])
 
     arrays = {"phase_id": phase, "domain_size": domain}
     if isinstance(directions, dict):

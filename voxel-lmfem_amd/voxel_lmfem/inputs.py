@@ -13,14 +13,20 @@ from . import materials as mat
 
 
 def used_labels(phase):
-    """np.unique(phase) of an integer array: a count over the label range when
-    that range is narrow (one pass), a sort otherwise.  Same values and dtype."""
+    """np.unique(phase) of an integer array: counts over the label range when
+    that range is narrow (pieces counted on a thread pool), a sort otherwise.
+    Same values and dtype."""
     flat = phase.reshape(-1)
     if flat.size:
         lo, hi = int(flat.min()), int(flat.max())
         if hi - lo < (1 << 22) and hi <= np.iinfo(np.int64).max:
-            shifted = flat if lo == 0 else flat.astype(np.int64) - lo
-            return (np.flatnonzero(np.bincount(shifted, minlength=hi - lo + 1)) + lo).astype(phase.dtype)
+            shift = lo != 0 or flat.dtype == np.uint64
+
+            def count(a, b):
+                piece = flat[a:b]
+                return np.bincount(piece.astype(np.int64) - lo if shift else piece, minlength=hi - lo + 1)
+            counts = sum(mat.map_pieces(count, flat.size))
+            return (np.flatnonzero(counts) + lo).astype(phase.dtype)
     return np.unique(flat)
 
 

@@ -49,8 +49,13 @@ def _prepare_device(phase, table, domain, orientation=None):
     """Device bookkeeping: labels uploaded once; label -> row, and the frames and
     Bond-rotated rows of local-frame phases, computed on the GPU."""
     import torch
-    small = phase.min() >= np.iinfo(np.int32).min and phase.max() <= np.iinfo(np.int32).max
-    labels = torch.from_numpy(np.ascontiguousarray(phase, dtype=np.int32 if small else np.int64)).to("cuda")
+    if phase.dtype in (np.uint8, np.int8, np.int16, np.int32):   # uploaded as is, widened on the device
+        labels = torch.from_numpy(np.ascontiguousarray(phase)).to("cuda").to(torch.int32)
+    elif phase.dtype == np.uint16:
+        labels = torch.from_numpy(np.ascontiguousarray(phase).view(np.int16)).to("cuda").to(torch.int32) & 0xFFFF
+    else:
+        small = phase.min() >= np.iinfo(np.int32).min and phase.max() <= np.iinfo(np.int32).max
+        labels = torch.from_numpy(np.ascontiguousarray(phase, dtype=np.int32 if small else np.int64)).to("cuda")
     field = mat.build_field_device(labels, table, orientation)
     h = domain / np.array(phase.shape[::-1], dtype=float)
     lam, mu, meta = select(field, phase, table)
@@ -183,7 +188,8 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="c
                          "GPU build; use backend='gpu'")
     if local.any():  # every local frame is validated before any GPU work
         flat = phase.reshape(-1)
-        mat.check_frames(orientation, phase.shape, [np.flatnonzero(flat == labels[k]) for k in np.flatnonzero(local)])
+        mat.check_frames(orientation, phase.shape,
+                         [mat.voxels_with_label(flat, labels[k]) for k in np.flatnonzero(local)])
     try:
         import torch
     except ImportError as exc:
