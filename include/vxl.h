@@ -97,7 +97,9 @@ int vxl_label_rows(int nz, int ny, int nx, int label_bytes, const void* labels_d
  * row major.  stride is 0 (one frame for every voxel) or 3 (mode 1) / 9
  * (mode 2) doubles per voxel.  bad_host[5] flags, as the host checks raise:
  * 0 nonfinite, 1 zero axis, 2 collinear axes, 3 not orthonormal (1e-10),
- * 4 det != +1 (1e-10).  Synchronises `stream`. */
+ * 4 det != +1 (1e-10).  Only the voxel count nz ny nx matters here, so a
+ * list of selected voxels may be passed as a 1 x 1 x m "grid" with their
+ * labels and frames.  Synchronises `stream`. */
 int vxl_local_rows(int nz, int ny, int nx, int label_bytes, const void* labels_dev, long long label_lo, int span,
                    const int* slot_host, int phases, const double* c21_host, int mode, const double* a_dev,
                    const double* b_dev, long long stride, double* rows_dev, long long local_base, int* bad_host,

@@ -211,3 +211,39 @@ def test_device_rows_shared_between_equal_frames(torch_gpu):
     assert np.linalg.norm(out[1][2] - out[0][2]) <= 1e-12 * np.linalg.norm(out[0][2])
     assert all(abs(a - b) <= 1 for a, b in zip(out[0][3], out[1][3]))
     assert out[1][4] <= 1e-8
+
+
+@pytest.mark.parametrize("kind", ["woven_axis_pairs", "fixture_matrices", "two_local_phases"])
+def test_compact_frames_equal_grid_frames(torch_gpu, kind):
+    """The device front end fed only the local voxels' frames, as homogenize
+    feeds it after its host check (check_frames(..., keep=True)), gives every
+    voxel the same coefficients, bit for bit, and the same Zeman lattice rows
+    as the one reading the whole grid's orientation."""
+    torch = torch_gpu
+    from test_oracle import gpu_fixture_cell
+    from voxel_lmfem.cells import woven_like
+    print(f"case compact frames {kind}", flush=True)
+    if kind == "fixture_matrices":
+        phase, mats, dom, ori = gpu_fixture_cell()
+    else:
+        (phase, mats, dom, ori), _ = woven_like(32, section=(4, 8), amplitude=3.0)
+        if kind == "two_local_phases":
+            phase = phase.copy()
+            phase[:, :, :16][phase[:, :, :16] == 1] = 2
+            mats = dict(mats)
+            mats[2] = dict(kind="anisotropic", frame="local",
+                           C=(np.diag([90000., 8000., 7000., 3000., 4000., 4500.]) + 500.).tolist())
+    labels_dev = torch.tensor(np.asarray(phase).astype(np.int32), device="cuda")
+    labels, _, local = pm.phase_table(mats)
+    flat = np.asarray(phase).reshape(-1)
+    sels = [pm.voxels_with_label(flat, labels[k]) for k in np.flatnonzero(local)]
+    compact = (sels, pm.check_frames(ori, np.asarray(phase).shape, sels, keep=True))
+    for dedup in (False, True):
+        full = pm.build_field_device(labels_dev, mats, ori, deduplicate=dedup)
+        comp = pm.build_field_device(labels_dev, mats, ori, deduplicate=dedup, compact=compact)
+        np.testing.assert_array_equal(_voxel_tensors_from_device(comp).view(np.int64),
+                                      _voxel_tensors_from_device(full).view(np.int64))
+        np.testing.assert_array_equal(comp.lattice_rows(phase.shape), full.lattice_rows(phase.shape))
+        np.testing.assert_array_equal(comp.used_labels(), full.used_labels())
+        if not dedup:
+            assert comp.row_count == len(labels) + sum(len(s) for s in sels)

@@ -257,22 +257,24 @@ def test_voxels_with_label_equal_flatnonzero():
     assert pm.voxels_with_label(flat[:0], 1).size == 0
 
 
-def _fake_device_field(torch, L=3, n=4000, distinct=40, seed=5):
+def _fake_device_field(torch, L=3, n=4000, distinct=40, seed=5, compact=False):
     """A DeviceField-shaped object on CPU tensors: L phase rows, local voxels
-    (every third voxel) with one of `distinct` rows each, at row L + voxel."""
+    (every third voxel) with one of `distinct` rows each, at row L + voxel
+    (or, compact, at row L + their rank among the local voxels)."""
     from types import SimpleNamespace
     rng = np.random.default_rng(seed)
-    table = np.zeros((L + n, 21))
-    table[:L] = rng.normal(size=(L, 21))
     index = rng.integers(0, L, size=n).astype(np.int32)
     local = np.arange(n) % 3 == 0
+    vox = np.flatnonzero(local)
+    rows_of_local = L + (np.arange(len(vox)) if compact else vox)
+    table = np.zeros((L + (len(vox) if compact else n), 21))
+    table[:L] = rng.normal(size=(L, 21))
     choices = rng.normal(size=(distinct, 21))
     choices[1::2] = choices[0::2]
     choices[1::2, [4, 9]] *= -1   # rows of frames +theta / -theta: two words differ in the sign bit
-    table[L + np.flatnonzero(local)] = choices[rng.integers(0, distinct, size=int(local.sum()))]
-    index[local] = L + np.flatnonzero(local)
-    samples = np.arange(0, n, 7).astype(np.int32)
-    samples = index[samples]
+    table[rows_of_local] = choices[rng.integers(0, distinct, size=len(vox))]
+    index[vox] = rows_of_local
+    samples = index[np.arange(0, n, 7)].astype(np.int32)
     return SimpleNamespace(index_dev=torch.as_tensor(index), rows_dev=torch.as_tensor(table), samples=samples,
                            kind=4, _lattice=None)
 
@@ -284,13 +286,15 @@ def _effective(field):
     return field.rows_dev.numpy()[idx.astype(np.int64)]
 
 
-def test_equal_rows_share_one_table_row():
+@pytest.mark.parametrize("compact", [False, True])
+def test_equal_rows_share_one_table_row(compact):
     """Device row sharing (run here on CPU tensors): every voxel keeps its
     coefficients bit for bit, the table shrinks to the distinct rows, the
     index narrows, and the lattice samples point at rows with the same
-    coefficients."""
+    coefficients, for a table with one row per voxel and for a compact one
+    (one row per local voxel, numbered by rank)."""
     import torch
-    field = _fake_device_field(torch)
+    field = _fake_device_field(torch, compact=compact)
     before = _effective(field).copy()
     sampled = field.rows_dev.numpy()[field.samples.astype(np.int64)].copy()
     pm._share_equal_rows(field, 3, torch)

@@ -277,7 +277,7 @@ int vxl_local_rows(int nz, int ny, int nx, int label_bytes, const void* labels_d
                    const double* b_dev, long long stride, double* rows_dev, long long local_base, int* bad_host,
                    void* stream) {
   g_error.clear();
-  if (nz < 2 || ny < 2 || nx < 2 || (label_bytes != 4 && label_bytes != 8) || !labels_dev || span < 1 ||
+  if (nz < 1 || ny < 1 || nx < 1 || (label_bytes != 4 && label_bytes != 8) || !labels_dev || span < 1 ||
       span > (1 << 24) || !slot_host || phases < 1 || !c21_host || (mode != 1 && mode != 2) || !a_dev ||
       (mode == 1 && !b_dev) || (stride != 0 && stride != (mode == 1 ? 3 : 9)) || !rows_dev || local_base < 0 ||
       !bad_host)

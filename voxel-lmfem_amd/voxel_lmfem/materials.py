@@ -221,7 +221,7 @@ def frames(orientation, shape, flat_indices):
     return _matrix_frames(_pick(_as_float(orientation), shape, (3, 3), flat_indices))
 
 
-def check_frames(orientation, shape, selections, chunk=65536, workers=None):
+def check_frames(orientation, shape, selections, chunk=65536, workers=None, keep=False):
     """Validate the frames of the selected voxels without building them.
 
     selections: flat voxel indices per local-frame phase, in the order they
@@ -231,33 +231,44 @@ def check_frames(orientation, shape, selections, chunk=65536, workers=None):
     in the same order (finite axis1, finite axis2, nonzero, not collinear;
     or finite, orthonormal, right-handed), on the same values.  The pieces
     run on a thread pool (NumPy releases the GIL in these loops); the
-    orientation arrays are converted to float64 once up front."""
+    orientation arrays are converted to float64 once up front.
+
+    keep: also return the values read, one row per selected voxel in
+    selection order, as (axis1 (m, 3), axis2 (m, 3)) or (rotation (m, 3, 3),)
+    float64 arrays, for the device front end to upload instead of the whole
+    grid's orientation (``build_field_device(..., compact=...)``)."""
     from concurrent.futures import ThreadPoolExecutor
-    import os
     pieces = [sel[first:first + chunk] for sel in selections for first in range(0, len(sel), chunk)]
+    total = sum(len(p) for p in pieces)
+    mapping = isinstance(orientation, Mapping)
+    tails = ((3,), (3,)) if mapping else ((3, 3),)
+    kept = tuple(np.empty((total,) + tail) for tail in tails) if keep else None
     if not pieces:
-        return
-    if isinstance(orientation, Mapping):
-        arrays = (_as_float(orientation["axis1"]), _as_float(orientation["axis2"]))
+        return kept
+    sources = ((_as_float(orientation["axis1"]), _as_float(orientation["axis2"])) if mapping
+               else (_as_float(orientation),))
+    starts = np.concatenate(([0], np.cumsum([len(p) for p in pieces])))
 
-        def one(idx):
-            a = _pick(arrays[0], shape, (3,), idx)
-            b = _pick(arrays[1], shape, (3,), idx)
-            _axis_frames(a, b, build=False)
-    else:
-        rot = _as_float(orientation)
-
-        def one(idx):
-            _matrix_frames(_pick(rot, shape, (3, 3), idx))
+    def one(k):
+        idx = pieces[k]
+        got = [_pick(src, shape, tail, idx) for src, tail in zip(sources, tails)]
+        if mapping:
+            _axis_frames(got[0], got[1], build=False)
+        else:
+            _matrix_frames(got[0])
+        if keep:
+            for out, values in zip(kept, got):
+                out[starts[k]:starts[k + 1]] = values
     if workers is None:
-        workers = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
+        workers = host_workers()
     if workers <= 1 or len(pieces) == 1:
-        for idx in pieces:
-            one(idx)
-        return
+        for k in range(len(pieces)):
+            one(k)
+        return kept
     with ThreadPoolExecutor(max_workers=workers) as pool:
-        for _ in pool.map(one, pieces):  # results in piece order: the first failing piece raises
+        for _ in pool.map(one, range(len(pieces))):  # results in piece order: the first failing piece raises
             pass
+    return kept
 
 
 def bond_rotate(C, R):
@@ -475,7 +486,7 @@ def _device_frames(orientation, shape, torch, device):
     return 2, R, None, sr
 
 
-def build_field_device(phase_dev, materials, orientation=None, deduplicate=True):
+def build_field_device(phase_dev, materials, orientation=None, deduplicate=True, compact=None):
     """Label and orientation bookkeeping of a cell whose int32/int64 labels are
     resident on the GPU.
 
@@ -487,7 +498,11 @@ def build_field_device(phase_dev, materials, orientation=None, deduplicate=True)
     path's frame checks and messages.  Raises the reference's ValueError for
     labels without a material.  deduplicate: voxels whose rows are bitwise
     equal share one row (index narrowed to u16 / u8 where it fits); False
-    keeps one row per voxel (row L + voxel).
+    keeps one row per voxel (row L + voxel).  compact: (selections,
+    values) with the flat voxel lists of every local-frame phase in label
+    order and their frames as ``check_frames(..., keep=True)`` returns them;
+    only those values are uploaded and their rows built, instead of the
+    whole grid's orientation (the rows are the same bits).
     """
     import ctypes
     import torch
@@ -534,22 +549,55 @@ def build_field_device(phase_dev, materials, orientation=None, deduplicate=True)
         if L + n > np.iinfo(np.int32).max:  # rows are addressed by an int32 index and an int row count
             raise ValueError("Too many voxels for one coefficient row each (int32 row index); use build_field, "
                              "which shares one row between voxels with equal frames")
-        rows_dev = torch.zeros((L + n, 21), dtype=torch.float64, device=phase_dev.device)
+        present_local = orientation is not None and np.isin(labels[local], used).any()
+        if present_local and compact is not None:
+            sels, values = compact
+            vox = np.concatenate(sels) if len(sels) else np.zeros(0, dtype=np.int64)
+            if len(sels) != int(local.sum()) or len(vox) != len(values[0]):
+                raise ValueError("compact frames must hold every local-frame phase's voxels and their values")
+            m = len(vox)
+        else:
+            m = n
+        rows_dev = torch.zeros((L + m, 21), dtype=torch.float64, device=phase_dev.device)
         rows_dev[:L] = torch.as_tensor(phase_rows, device=phase_dev.device)
-        if orientation is not None and np.isin(labels[local], used).any():
+        if present_local:
             slot = np.full(span, -1, dtype=np.int32)
             slot[labels[local] - lo] = np.arange(int(local.sum()), dtype=np.int32)
             cloc = np.ascontiguousarray(phase_rows[local])
-            mode, a, b, stride = _device_frames(orientation, (nz, ny, nx), torch, phase_dev.device)
             bad = np.zeros(5, dtype=np.int32)
-            check(lib.vxl_local_rows(nz, ny, nx, phase_dev.element_size(), dptr(phase_dev), lo, span,
-                                     slot.ctypes.data_as(ip), int(local.sum()),
-                                     cloc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), mode, dptr(a),
-                                     dptr(b) if b is not None else None, stride, dptr(rows_dev), L,
-                                     bad.ctypes.data_as(ip), stream), "vxl_local_rows")
+            if compact is None:   # every voxel's frame from the grid's orientation
+                mode, a, b, stride = _device_frames(orientation, (nz, ny, nx), torch, phase_dev.device)
+                grid, lab, lab_bytes = (nz, ny, nx), phase_dev, phase_dev.element_size()
+            else:   # the selected voxels' frames only, as one row of m voxels
+                dev = phase_dev.device
+                counts = [len(sel) for sel in sels]
+                lab = torch.as_tensor(np.repeat(labels[local], counts).astype(np.int32), device=dev)
+                a = torch.as_tensor(np.ascontiguousarray(values[0]).reshape(m, -1), device=dev)
+                b = torch.as_tensor(np.ascontiguousarray(values[1]), device=dev) if len(values) > 1 else None
+                mode, stride = (1, 3) if len(values) > 1 else (2, 9)
+                grid, lab_bytes = (1, 1, m), 4
+            if m:
+                check(lib.vxl_local_rows(grid[0], grid[1], grid[2], lab_bytes, dptr(lab), lo, span,
+                                         slot.ctypes.data_as(ip), int(local.sum()),
+                                         cloc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), mode, dptr(a),
+                                         dptr(b) if b is not None else None, stride, dptr(rows_dev), L,
+                                         bad.ctypes.data_as(ip), stream), "vxl_local_rows")
             for flag, message in zip(bad, _FRAME_ERRORS):
                 if flag:
                     raise ValueError(message)
+            if compact is not None and m:
+                # label_rows gave voxel v of a local phase row L + v; its row is
+                # now L + (its position in the selections).
+                idx = index.to(torch.int64)
+                vox_dev = torch.as_tensor(vox.astype(np.int64), device=phase_dev.device)
+                idx[vox_dev] = L + torch.arange(m, dtype=torch.int64, device=phase_dev.device)
+                pick = samples.astype(np.int64)
+                local_pick = pick >= L
+                if local_pick.any():
+                    pick[local_pick] = idx.index_select(
+                        0, torch.as_tensor(pick[local_pick] - L, device=phase_dev.device)).cpu().numpy()
+                samples = pick.astype(np.int32)
+                index = idx.to(torch.int32)
     field = DeviceField(phase_rows, index, kind, labels, tensors, local, used, samples, rows_dev)
     if rows_dev is not None and deduplicate:
         _share_equal_rows(field, L, torch)
@@ -611,10 +659,12 @@ def _share_equal_rows(field, L, torch):
         index = torch.where(new_idx >= 32768, new_idx - 65536, new_idx).to(torch.int16)
     else:
         index = new_idx.to(torch.int32)
+    # Lattice samples hold row numbers: map every old row to its new one.
+    row_map = torch.arange(int(field.rows_dev.shape[0]), dtype=torch.int64, device=dev)
+    if count:
+        row_map[idx.index_select(0, vox)] = new_idx.index_select(0, vox)
     pick = torch.as_tensor(field.samples.astype(np.int64), device=dev)
-    local_pick = pick >= L
-    pick[local_pick] = new_idx.index_select(0, pick[local_pick] - L)
-    field.samples = pick.cpu().numpy().astype(np.int32)
+    field.samples = row_map.index_select(0, pick).cpu().numpy().astype(np.int32)
     field.index_dev = index.contiguous()
     field.kind = {np.uint8: 1, np.uint16: 2, np.int32: 4}[dtype]
     field.rows_dev = table.contiguous()

@@ -45,9 +45,10 @@ GPU_BACKENDS = ("gpu", "hip", "cuda")
 TOL = 1e-8
 
 
-def _prepare_device(phase, table, domain, orientation=None):
+def _prepare_device(phase, table, domain, orientation=None, compact=None):
     """Device bookkeeping: labels uploaded once; label -> row, and the frames and
-    Bond-rotated rows of local-frame phases, computed on the GPU."""
+    Bond-rotated rows of local-frame phases, computed on the GPU (from the
+    local voxels' frames alone when `compact` holds them)."""
     import torch
     if phase.dtype in (np.uint8, np.int8, np.int16, np.int32):   # uploaded as is, widened on the device
         labels = torch.from_numpy(np.ascontiguousarray(phase)).to("cuda").to(torch.int32)
@@ -56,7 +57,7 @@ def _prepare_device(phase, table, domain, orientation=None):
     else:
         small = phase.min() >= np.iinfo(np.int32).min and phase.max() <= np.iinfo(np.int32).max
         labels = torch.from_numpy(np.ascontiguousarray(phase, dtype=np.int32 if small else np.int64)).to("cuda")
-    field = mat.build_field_device(labels, table, orientation)
+    field = mat.build_field_device(labels, table, orientation, compact=compact)
     h = domain / np.array(phase.shape[::-1], dtype=float)
     lam, mu, meta = select(field, phase, table)
     return field, h, lam, mu, meta
@@ -82,7 +83,8 @@ def _field_statistics(cell, u, phase, loads=(0, 5)):
 
 
 def _workload(phase, table, domain, orientation, precision, tol, local, fields=False, preconditioner="fft",
-              element="fi", hourglass_coefficient=0.1, max_iter=10000, stencil="modal", solver_method="cg"):
+              element="fi", hourglass_coefficient=0.1, max_iter=10000, stencil="modal", solver_method="cg",
+              compact=None):
     from .gpu import Cell, Solver
     import torch
     torch.cuda.synchronize()
@@ -96,7 +98,7 @@ def _workload(phase, table, domain, orientation, precision, tol, local, fields=F
         stage.t = now
     stage.t = start
     torch.cuda.reset_peak_memory_stats()
-    field, h, lam, mu, meta = _prepare_device(phase, table, domain, orientation)
+    field, h, lam, mu, meta = _prepare_device(phase, table, domain, orientation, compact)
     stage("material")
     cell = solver = None
     try:
@@ -186,10 +188,11 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="c
     if backend == "cpu":
         raise ValueError("backend='cpu' is the reference's sparse-direct oracle and is not part of this "
                          "GPU build; use backend='gpu'")
-    if local.any():  # every local frame is validated before any GPU work
+    compact = None
+    if local.any():  # every local frame is validated before any GPU work; the values read are kept
         flat = phase.reshape(-1)
-        mat.check_frames(orientation, phase.shape,
-                         [mat.voxels_with_label(flat, labels[k]) for k in np.flatnonzero(local)])
+        sels = [mat.voxels_with_label(flat, labels[k]) for k in np.flatnonzero(local)]
+        compact = (sels, mat.check_frames(orientation, phase.shape, sels, keep=True))
     try:
         import torch
     except ImportError as exc:
@@ -198,11 +201,11 @@ def homogenize(phase_id, materials, domain_size, *, orientation=None, backend="c
         raise RuntimeError("No GPU is available. This backend runs only on an MI355X (gfx950).")
     if warmup:
         _workload(phase, table, domain, orientation, precision, tol, local, fields, preconditioner, element,
-                  hourglass_coefficient, int(max_iter), stencil, solver)
+                  hourglass_coefficient, int(max_iter), stencil, solver, compact)
     start = time.perf_counter()
     result, storage, operator = _workload(phase, table, domain, orientation, precision, tol, local, fields,
                                           preconditioner, element, hourglass_coefficient, int(max_iter),
-                                          stencil, solver)
+                                          stencil, solver, compact)
     result.update(backend=backend, storage=storage, operator=operator,
                   scope=(("FI HEX8" if element == "fi" else "C3D8R-HG") + " matrix-free stiffness; "
                          + {"fft": "FFT Green preconditioner; ", "fft_bf16": "FFT Green preconditioner (BF16 spectra); ",
