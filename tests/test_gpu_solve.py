@@ -262,3 +262,28 @@ def test_spectral_residual_matches_real_space(torch_gpu, monkeypatch, shape):
     assert err < 1e-9
     assert max(abs(x - y) for x, y in zip(its_a, its_b)) <= 1
     assert a["max_true_residual"] <= 1e-8 and b["max_true_residual"] <= 1e-8
+
+
+@pytest.mark.parametrize("form", ["pair_vectors", "pair_mixed_shapes", "matrix_grid"])
+def test_one_frame_for_every_voxel_through_homogenize(torch_gpu, form):
+    """One frame for every voxel, given as a broadcast axis pair, as a
+    per-voxel axis1 with one axis2, or as a per-voxel rotation array: the
+    front end (host check keeping the frames it reads, device rows of the
+    selected voxels) gives a one-phase local-frame cell the rotated tensor,
+    C_eff = diag(20, 10, 30, 5, 4, 6) for diag(10, 20, 30, 4, 5, 6) turned
+    90 degrees about z (the reference's tests/test_solver.py:49-56)."""
+    from voxel_lmfem import homogenize
+    print(f"case one frame for every voxel, {form}", flush=True)
+    shape = (2, 3, 4)
+    a1, a2 = np.array([0., 1., 0.]), np.array([-1., 0., 0.])   # R = [a1, c x a1, c], c = e_z
+    if form == "pair_vectors":
+        ori = dict(axis1=a1, axis2=a2)
+    elif form == "pair_mixed_shapes":
+        ori = dict(axis1=np.broadcast_to(a1, shape + (3,)).copy(), axis2=a2)
+    else:
+        R = np.array([[0., -1., 0.], [1., 0., 0.], [0., 0., 1.]])
+        ori = np.broadcast_to(R, shape + (3, 3)).copy()
+    mats = {0: dict(kind="anisotropic", C=np.diag([10., 20., 30., 4., 5., 6.]), frame="local")}
+    for precision in ("fp64", "mixed"):
+        r = homogenize(np.zeros(shape, dtype=np.uint8), mats, [1., 2., 3.], orientation=ori, precision=precision)
+        np.testing.assert_allclose(r["CH"], np.diag([20., 10., 30., 5., 4., 6.]), rtol=2e-13, atol=1e-12)
