@@ -36,6 +36,9 @@ woven_reference_workload: the reference's own published headline workload, its
 analytic woven cell at 192^3 and 256^3, as complete `homogenize` calls (mixed
 and FP64) beside the reference's published complete-call times on its GPU, with
 C_eff against the reference's certified record of the same cell.
+adjoint: forward C_eff and backward gradient of the 96^3 SIMP density cell and
+of the tow angles of the 64^3 woven copy (FP64), beside the central-difference
+cost the adjoint replaces.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 """
@@ -296,6 +299,69 @@ def woven_leg(grids=(192, 256), repeats=3):
     return out
 
 
+def adjoint_leg(repeats=3):
+    """dC_eff/d(per-voxel parameter) through the adjoint (torch.autograd):
+    the SIMP density cell at 96^3 (Gaussian random field, sigma 4 voxels,
+    E = 1e-3 + rho^3 (70 - 1e-3), nu 0.3) and the fiber angles of the tow
+    voxels of the 64^3 woven copy, J = C_1111 + K_V(C_eff), FP64 solves.
+    Forward (C_eff) and backward (the whole gradient field from the fields
+    the forward solve kept) timed separately, medians of `repeats` after one
+    untimed pass; beside them the central-difference cost the adjoint
+    replaces, two forward solves per voxel (not run)."""
+    import torch
+    from voxel_lmfem import adjoint, materials as pm
+    from voxel_lmfem.cells import simp_density, woven_like
+
+    def timed(make):
+        fw, bw = [], []
+        for k in range(repeats + 1):
+            x = make()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            C, param = x()
+            t1 = time.perf_counter()
+            J = C[0, 0] + C[:3, :3].sum() / 9.0
+            J.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if k:
+                fw.append(t1 - t0)
+                bw.append(t2 - t1)
+        return float(np.median(fw)), float(np.median(bw)), param.grad
+
+    out = {}
+    rho, C1, _, _ = simp_density(96, sigma=4.0, seed=5)
+
+    def density():
+        r = torch.tensor(rho, dtype=torch.float64, requires_grad=True)
+        return lambda: (adjoint.density_ceff(r, np.ones(3), C1=C1), r)
+    fw, bw, g = timed(density)
+    n = rho.size
+    out["density_96"] = dict(voxels=n, forward_s=fw, backward_s=bw, gradient_entries_per_s=n / bw,
+                             central_difference_s=2 * n * fw, adjoint_vs_fd=2 * n * fw / (fw + bw),
+                             gradient_finite=bool(torch.isfinite(g).all()))
+    (phase, mats, dom, ori), _ = woven_like(64, section=(2, 6), amplitude=3.0)
+    labels, tensors, local = pm.phase_table(mats)
+    slot = np.searchsorted(labels, phase.reshape(-1))
+    mask = local[slot]
+    idx = np.flatnonzero(mask)
+    frames = pm.frames(ori, phase.shape, idx)
+    iso = pm.pack21(tensors[slot])
+    C_loc = tensors[slot[idx[0]]]
+
+    def angle():
+        t = torch.zeros(phase.shape, dtype=torch.float64, requires_grad=True)
+        return lambda: (adjoint.angle_ceff(t, dom, C_loc, frames, iso, mask), t)
+    fw, bw, g = timed(angle)
+    out["angle_woven_64"] = dict(voxels=int(phase.size), tow_voxels=int(len(idx)), forward_s=fw, backward_s=bw,
+                                 central_difference_s=2 * len(idx) * fw, adjoint_vs_fd=2 * len(idx) * fw / (fw + bw),
+                                 gradient_finite=bool(torch.isfinite(g).all()))
+    for name, leg in out.items():
+        print(f"# adjoint {name}: forward {leg['forward_s'] * 1e3:.1f} ms, backward {leg['backward_s'] * 1e3:.1f} ms "
+              f"(central differences would take {leg['central_difference_s']:.0f} s)", file=sys.stderr, flush=True)
+    return out
+
+
 def batch_leg(steps=2, warmup=1, count=64, streams=4, workers=8, method="cg", stack=1, stack_voxels=64 ** 3,
               sizes=(32, 48, 64, 80, 96)):
     """Uneven batch: `count` two-phase random-inclusion cells (edges drawn from
@@ -444,6 +510,8 @@ def main():
     ap.add_argument("--skip-woven", action="store_true",
                     help="skip the reference's woven 192^3 / 256^3 cells (its published headline workload)")
     ap.add_argument("--woven-only", action="store_true", help="run only the woven reference-workload leg")
+    ap.add_argument("--skip-adjoint", action="store_true", help="skip the adjoint-gradient timing leg")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint-gradient timing leg")
     ap.add_argument("--woven-grids", type=int, nargs="+", default=[192, 256], choices=[192, 256],
                     help="woven cells of the reference-workload leg")
     ap.add_argument("--woven-repeats", type=int, default=3, help="timed calls per woven cell and precision")
@@ -478,6 +546,11 @@ def main():
         import torch
         torch.cuda.set_device(0)
         print(json.dumps(dict(woven_reference_workload=woven_leg(args.woven_grids, args.woven_repeats))), flush=True)
+        return
+    if args.adjoint_only:
+        import torch
+        torch.cuda.set_device(0)
+        print(json.dumps(dict(adjoint=adjoint_leg())), flush=True)
         return
     sample = samples_all = None
     e2e_proc = None
@@ -817,6 +890,7 @@ def main():
                   file=sys.stderr, flush=True)
 
     woven = None if args.skip_woven else woven_leg(args.woven_grids, args.woven_repeats)
+    adj = None if args.skip_adjoint else adjoint_leg()
 
     traffic, traffic_src = measured_traffic()
     line = dict(
@@ -846,6 +920,7 @@ def main():
         cpu_end_to_end=cpu_e2e,
         gpu_small_cells=small,
         woven_reference_workload=woven,
+        adjoint=adj,
     )
     print(json.dumps(line), flush=True)
 
