@@ -38,7 +38,8 @@ and FP64) beside the reference's published complete-call times on its GPU, with
 C_eff against the reference's certified record of the same cell.
 adjoint: forward C_eff and backward gradient of the 96^3 SIMP density cell and
 of the tow angles of the 64^3 woven copy (FP64), beside the central-difference
-cost the adjoint replaces.
+cost the adjoint replaces.  woven_like_128: the 128^3 woven-like cell with
+per-voxel tow tangents, mixed against FP64.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 """
@@ -299,6 +300,34 @@ def woven_leg(grids=(192, 256), repeats=3):
     return out
 
 
+def c2_leg(n=128, repeats=3):
+    """The 128^3 woven-like cell (three labels: matrix and warp / weft tows,
+    transversely isotropic tows with one path tangent per voxel) as complete
+    `homogenize` calls, mixed and FP64, medians of `repeats` after one untimed
+    call, with the mixed C_eff's error against the FP64 one."""
+    from voxel_lmfem import homogenize
+    from voxel_lmfem.cells import woven_like
+    (phase, mats, dom, ori), meta = woven_like(n)
+    out = dict(workload=f"woven_like_{n}^3_ti_tows_per_voxel_tangents_6loads", tow_fraction=meta.get("tow_fraction"))
+    CH = {}
+    for precision in ("mixed", "fp64"):
+        homogenize(phase, mats, dom, orientation=ori, precision=precision)
+        walls, res = [], None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            res = homogenize(phase, mats, dom, orientation=ori, precision=precision)
+            walls.append(time.perf_counter() - t0)
+        CH[precision] = np.asarray(res["CH"])
+        out[precision] = dict(complete_call_s=float(np.median(walls)), workload_s=float(res["seconds"]),
+                              iterations=[c["iterations"] for c in res["calls"]],
+                              max_true_residual=res["max_true_residual"])
+    out["ceff_rel_err_mixed_vs_fp64"] = float(np.linalg.norm(CH["mixed"] - CH["fp64"]) / np.linalg.norm(CH["fp64"]))
+    print(f"# woven-like {n}^3: mixed {out['mixed']['complete_call_s'] * 1e3:.1f} ms, fp64 "
+          f"{out['fp64']['complete_call_s'] * 1e3:.1f} ms per complete call, mixed vs fp64 "
+          f"{out['ceff_rel_err_mixed_vs_fp64']:.1e}", file=sys.stderr, flush=True)
+    return out
+
+
 def adjoint_leg(repeats=3):
     """dC_eff/d(per-voxel parameter) through the adjoint (torch.autograd):
     the SIMP density cell at 96^3 (Gaussian random field, sigma 4 voxels,
@@ -511,7 +540,9 @@ def main():
                     help="skip the reference's woven 192^3 / 256^3 cells (its published headline workload)")
     ap.add_argument("--woven-only", action="store_true", help="run only the woven reference-workload leg")
     ap.add_argument("--skip-adjoint", action="store_true", help="skip the adjoint-gradient timing leg")
-    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint-gradient timing leg")
+    ap.add_argument("--skip-c2", action="store_true", help="skip the 128^3 woven-like cell")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint-gradient leg and the 128^3 woven-like cell")
     ap.add_argument("--woven-grids", type=int, nargs="+", default=[192, 256], choices=[192, 256],
                     help="woven cells of the reference-workload leg")
     ap.add_argument("--woven-repeats", type=int, default=3, help="timed calls per woven cell and precision")
@@ -550,7 +581,7 @@ def main():
     if args.adjoint_only:
         import torch
         torch.cuda.set_device(0)
-        print(json.dumps(dict(adjoint=adjoint_leg())), flush=True)
+        print(json.dumps(dict(adjoint=adjoint_leg(), woven_like_128=c2_leg())), flush=True)
         return
     sample = samples_all = None
     e2e_proc = None
@@ -891,6 +922,7 @@ def main():
 
     woven = None if args.skip_woven else woven_leg(args.woven_grids, args.woven_repeats)
     adj = None if args.skip_adjoint else adjoint_leg()
+    c2 = None if args.skip_c2 else c2_leg()
 
     traffic, traffic_src = measured_traffic()
     line = dict(
@@ -921,6 +953,7 @@ def main():
         gpu_small_cells=small,
         woven_reference_workload=woven,
         adjoint=adj,
+        woven_like_128=c2,
     )
     print(json.dumps(line), flush=True)
 
