@@ -24,7 +24,7 @@ int launch_finalize(const double* partial, int nblk, int op, double* scal, int n
                     int* stop = nullptr, double* hist = nullptr);
 // Stored search directions p_t (nrhs fields each) whose steps alpha_t are
 // still to be added to x64.
-constexpr int kMaxDirections = 8;
+constexpr int kMaxDirections = 16;
 template <typename T>
 struct DirectionSet {
   const T* p[kMaxDirections];
