@@ -94,6 +94,28 @@ __device__ __forceinline__ void strain_mode(const T* c, const G9<T>& g, const T 
   else if constexpr (HX) e[5] = cx[1];
   else if constexpr (HY) e[5] = cy[0];
   T s[6];
+  if constexpr (kIso && CLS == 0) {
+    // Constant mode of a cubic voxel: one factor b0 = g^2 on every axis.
+    // Scaling the six strains by it (6 products) instead of the nine
+    // back-projected stresses makes the back-projection plain assignments;
+    // only this mode writes the slots MX = 1, MY = 2, MZ = 4, so P starts at
+    // zero there.
+    static_assert(MX == 1 && MY == 2 && MZ == 4, "the constant mode");
+    const T b0 = bx;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) e[i] *= b0;
+    cmul<MASK>(c, e, s);
+    P[0][MX] = s[0];
+    P[1][MX] = s[5];
+    P[2][MX] = s[4];
+    P[0][MY] = s[5];
+    P[1][MY] = s[1];
+    P[2][MY] = s[3];
+    P[0][MZ] = s[4];
+    P[1][MZ] = s[3];
+    P[2][MZ] = s[2];
+    return;
+  }
   cmul<MASK>(c, e, s);
   // Back-projection: P[m_j] += g_j * (sigma_0j, sigma_1j, sigma_2j).
   if constexpr (HX) {
@@ -676,12 +698,16 @@ apply_kernel_v2(ApplyArgs<T> a) {
     else
       element_modal<kIso>(q, c, a.g, P);
 #pragma unroll
-    for (int cc = 0; cc < 3; ++cc)
+    for (int cc = 0; cc < 3; ++cc) {
+      // Mode 0 (a rigid translation) carries no force: P[cc][0] is 0.
+      Fb[cc][0] = -P[cc][4];
+      Ft[cc][0] = P[cc][4];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
+      for (int m = 1; m < 4; ++m) {
         Fb[cc][m] = P[cc][m] - P[cc][m + 4];
         Ft[cc][m] = P[cc][m] + P[cc][m + 4];
       }
+    }
   };
 
   const bool is_node = tid < TL::NODES;
