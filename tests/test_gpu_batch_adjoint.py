@@ -121,7 +121,7 @@ def test_density_gradient_matches_central_differences(torch_gpu, n, voxels):
     J = C[0, 0] + (C[:3, :3].sum()) / 9.0
     J.backward()
     g = r.grad.numpy().reshape(-1)
-    assert float(J) == pytest.approx(_J(C.detach().numpy(), W), rel=1e-14)
+    assert float(J.detach()) == pytest.approx(_J(C.detach().numpy(), W), rel=1e-14)
     rng = np.random.default_rng(n)
     picks = rng.choice(rho.size, size=voxels, replace=False)
     step = 1e-4

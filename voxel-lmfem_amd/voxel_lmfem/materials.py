@@ -15,6 +15,7 @@ voxels whose frames are bit-identical share one row (an exact de-duplication:
 the same R gives the same rotated row bit for bit), so per-grain or
 per-layer frames cost one row each instead of one row per voxel.
 """
+import warnings
 from collections.abc import Mapping
 from numbers import Integral
 
@@ -465,15 +466,22 @@ def _device_frames(orientation, shape, torch, device):
     """(mode, a_dev, b_dev, stride) for vxl_local_rows: one frame for all
     voxels (stride 0) or one per voxel, from a rotation array or an axis pair
     broadcastable to the grid (the host path's accepted forms)."""
+    def upload(host):
+        # The host array is only read by the copy to the device; a read-only
+        # one (a full-size broadcast view, a memory map) needs no host copy.
+        with warnings.catch_warnings():
+            warnings.filterwarnings("ignore", message="The given NumPy array is not writable")
+            return torch.as_tensor(host, device=device)
+
     def grid_array(values, tail):
         a = np.asarray(values, dtype=np.float64)
         if a.shape == tail:
-            return torch.as_tensor(np.ascontiguousarray(a), device=device), 0
+            return upload(np.ascontiguousarray(a)), 0
         try:
             full = np.broadcast_to(a, tuple(shape) + tail)
         except ValueError as exc:
             raise ValueError("Orientation shape is not broadcastable to phase_id") from exc
-        return torch.as_tensor(np.ascontiguousarray(full.reshape(-1, *tail)), device=device), int(np.prod(tail))
+        return upload(np.ascontiguousarray(full.reshape(-1, *tail))), int(np.prod(tail))
 
     if isinstance(orientation, Mapping):
         a, sa = grid_array(orientation["axis1"], (3,))
